@@ -1,0 +1,542 @@
+/*
+ * kmc_api.hip -- host side of the C ABI (include/kmc.h): validation with the
+ * reference's error behaviour, device memory, and the count -> scan -> emit
+ * launch sequence of one extraction.
+ */
+#include <hip/hip_runtime.h>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <unordered_map>
+#include <vector>
+
+#include "kmc.h"
+#include "kmc_job.h"
+#include "kmc_internal.h"
+
+namespace kmc {
+
+hipError_t launch_march(int dtype, const JobArgs& a, bool count, uint32_t lds_bytes, hipStream_t st);
+hipError_t launch_scan(const uint32_t* counts, uint64_t n, uint32_t narr, uint64_t* partials,
+                       uint64_t* prefix, uint64_t* totals, hipStream_t st);
+uint32_t scan_partials_needed(uint64_t n, uint32_t narr);
+hipError_t launch_minmax(int dtype, const void* vol, uint64_t n, void* scratch, double* out, hipStream_t st);
+size_t minmax_scratch_bytes();
+
+/* ------------------------------------------------------------------------- */
+/* device memory pool                                                          */
+/* ------------------------------------------------------------------------- */
+
+namespace {
+
+std::mutex g_pool_mutex;
+std::multimap<size_t, void*> g_free;           // size -> block
+std::unordered_map<void*, size_t> g_size;      // block -> size
+size_t g_cached_bytes = 0;
+const size_t kPoolKeepBytes = size_t(64) << 30;  // cached free bytes kept before trimming
+
+size_t round_size(size_t n)
+{
+    if (n <= (size_t(1) << 20)) return (n + 4095) & ~size_t(4095);
+    return (n + (size_t(2) << 20) - 1) & ~((size_t(2) << 20) - 1);
+}
+
+void pool_trim_locked()
+{
+    for (auto& kv : g_free) {
+        (void)hipFree(kv.second);
+        g_size.erase(kv.second);
+    }
+    g_free.clear();
+    g_cached_bytes = 0;
+}
+
+}  // namespace
+
+void* device_alloc(size_t n)
+{
+    if (n == 0) n = 1;
+    const size_t sz = round_size(n);
+    std::lock_guard<std::mutex> lk(g_pool_mutex);
+    auto it = g_free.lower_bound(sz);
+    if (it != g_free.end() && it->first <= sz * 2) {
+        void* p = it->second;
+        g_cached_bytes -= it->first;
+        g_free.erase(it);
+        return p;
+    }
+    void* p = nullptr;
+    if (hipMalloc(&p, sz) != hipSuccess) {
+        (void)hipGetLastError();
+        pool_trim_locked();
+        if (hipMalloc(&p, sz) != hipSuccess) {
+            (void)hipGetLastError();
+            return nullptr;
+        }
+    }
+    g_size[p] = sz;
+    return p;
+}
+
+void device_free(void* p)
+{
+    if (!p) return;
+    std::lock_guard<std::mutex> lk(g_pool_mutex);
+    auto it = g_size.find(p);
+    if (it == g_size.end()) return;
+    g_free.emplace(it->second, p);
+    g_cached_bytes += it->second;
+    if (g_cached_bytes > kPoolKeepBytes) pool_trim_locked();
+}
+
+void device_pool_trim()
+{
+    std::lock_guard<std::mutex> lk(g_pool_mutex);
+    pool_trim_locked();
+}
+
+/* ------------------------------------------------------------------------- */
+/* surface ownership                                                           */
+/* ------------------------------------------------------------------------- */
+
+struct SurfaceStorage {
+    int memory;
+    void* coords;
+    void* connections;
+    void* normals;
+};
+
+static void free_storage(SurfaceStorage* s)
+{
+    if (!s) return;
+    if (s->memory == KMC_MEMORY_DEVICE) {
+        device_free(s->coords);
+        device_free(s->connections);
+        device_free(s->normals);
+    } else {
+        if (s->coords) (void)hipHostFree(s->coords);
+        if (s->connections) (void)hipHostFree(s->connections);
+        if (s->normals) (void)hipHostFree(s->normals);
+    }
+    delete s;
+}
+
+/* ------------------------------------------------------------------------- */
+/* helpers                                                                     */
+/* ------------------------------------------------------------------------- */
+
+size_t type_size(int t)
+{
+    switch (t) {
+    case KMC_INT8: case KMC_UINT8: return 1;
+    case KMC_INT16: case KMC_UINT16: return 2;
+    case KMC_INT32: case KMC_UINT32: case KMC_REAL32: return 4;
+    case KMC_INT64: case KMC_UINT64: case KMC_REAL64: return 8;
+    default: return 0;
+    }
+}
+
+/* The largest float t with double(t) <= iso, so that for float v:
+ * double(v) > iso  <=>  v > t.  NaN stays NaN (every comparison false). */
+float float_threshold(double iso)
+{
+    if (std::isnan(iso)) return NAN;
+    float t = (float)iso;
+    if ((double)t > iso) t = std::nextafter(t, -INFINITY);
+    return t;
+}
+
+kmc_status validate_volume(const kmc_volume* v)
+{
+    if (!v) return KMC_ERR_NULL_INPUT;
+    if (v->kind != KMC_STRUCTURED) return KMC_ERR_NOT_STRUCTURED;
+    if (v->veclen != 1) return KMC_ERR_NOT_SCALAR;
+    if (type_size(v->type) == 0) return KMC_ERR_UNSUPPORTED_TYPE;
+    if (!v->values) return KMC_ERR_INVALID_ARGUMENT;
+    if (v->resolution[0] == 0 || v->resolution[1] == 0 || v->resolution[2] == 0) return KMC_ERR_INVALID_ARGUMENT;
+    if (v->resolution[0] > (1u << 20)) return KMC_ERR_INVALID_ARGUMENT;   // x packed in 20 bits
+    if (v->memory != KMC_MEMORY_HOST && v->memory != KMC_MEMORY_DEVICE) return KMC_ERR_INVALID_ARGUMENT;
+    return KMC_OK;
+}
+
+void report(kmc_status s)
+{
+    if (s == KMC_OK) return;
+    if (std::getenv("KMC_QUIET")) return;
+    std::fprintf(stderr, "KVS ERROR: %s\n", kmc_status_string(s));
+}
+
+namespace {
+thread_local int t_device = 0;
+thread_local bool t_device_set = false;
+}
+
+kmc_status ensure_device()
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) {
+        (void)hipGetLastError();
+        return KMC_ERR_NO_DEVICE;
+    }
+    if (!t_device_set) { t_device = 0; t_device_set = true; }
+    if (t_device >= n) return KMC_ERR_NO_DEVICE;
+    if (hipSetDevice(t_device) != hipSuccess) { (void)hipGetLastError(); return KMC_ERR_DEVICE; }
+    return KMC_OK;
+}
+
+/* Device copy of the volume values (or the caller's device pointer). */
+kmc_status device_values(const kmc_volume* v, hipStream_t st, const void** dvals, void** owned)
+{
+    *owned = nullptr;
+    if (v->memory == KMC_MEMORY_DEVICE) { *dvals = v->values; return KMC_OK; }
+    const size_t bytes = (size_t)v->resolution[0] * v->resolution[1] * v->resolution[2] * type_size(v->type);
+    void* p = device_alloc(bytes);
+    if (!p) return KMC_ERR_OUT_OF_MEMORY;
+    if (hipMemcpyAsync(p, v->values, bytes, hipMemcpyHostToDevice, st) != hipSuccess) {
+        device_free(p);
+        return KMC_ERR_DEVICE;
+    }
+    *owned = p;
+    *dvals = p;
+    return KMC_OK;
+}
+
+kmc_status min_max_on_device(const kmc_volume* v, const void* dvals, hipStream_t st, double* mn, double* mx)
+{
+    const uint64_t n = (uint64_t)v->resolution[0] * v->resolution[1] * v->resolution[2];
+    void* scratch = device_alloc(minmax_scratch_bytes() + 16);
+    if (!scratch) return KMC_ERR_OUT_OF_MEMORY;
+    double* dout = (double*)((char*)scratch + minmax_scratch_bytes());
+    kmc_status s = KMC_OK;
+    double host[2] = {0, 0};
+    if (launch_minmax(v->type, dvals, n, scratch, dout, st) != hipSuccess ||
+        hipMemcpyAsync(host, dout, sizeof(host), hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess) {
+        s = KMC_ERR_DEVICE;
+    }
+    device_free(scratch);
+    *mn = host[0];
+    *mx = host[1];
+    return s;
+}
+
+/* Geometry and LDS budget of a pass; R rows per band. */
+bool plan_geometry(JobArgs& a, uint32_t* lds_bytes)
+{
+    const uint32_t candidates[] = { 4, 2, 1 };
+    for (uint32_t R : candidates) {
+        a.R = R;
+        a.nb = (a.ny + R - 1) / R;
+        a.NC = (a.nx + 63) / 64;
+        a.queue_cap = 2048;
+        const LdsLayout L = lds_layout(a.R, a.NC, a.nlev, a.queue_cap);
+        if (L.bytes <= 60 * 1024) {
+            *lds_bytes = L.bytes;
+            a.slab = 32;
+            a.nslabs = (a.nz + a.slab - 1) / a.slab;
+            a.ntiles = (uint64_t)a.nz * a.nb;
+            return true;
+        }
+    }
+    return false;
+}
+
+void init_surface(kmc_surface* s, double iso, int normal_type, int dup, double mn, double mx,
+                  const float minc[3], const float maxc[3])
+{
+    std::memset(s, 0, sizeof(*s));
+    s->isolevel = iso;
+    s->memory = KMC_MEMORY_DEVICE;
+    s->polygon_type = 3;
+    s->normal_type = dup ? KMC_POLYGON_NORMAL : normal_type;
+    s->color_type = 1;
+    s->min_value = mn;
+    s->max_value = mx;
+    for (int k = 0; k < 3; ++k) { s->min_coord[k] = minc[k]; s->max_coord[k] = maxc[k]; }
+}
+
+/* Object coordinates the mapper uses (MapperBase::setMinMaxCoords -> updateMinMaxCoords). */
+void object_coords(const kmc_volume* v, float minc[3], float maxc[3])
+{
+    if (v->has_min_max_coords) {
+        for (int k = 0; k < 3; ++k) { minc[k] = v->min_coord[k]; maxc[k] = v->max_coord[k]; }
+    } else {
+        for (int k = 0; k < 3; ++k) { minc[k] = 0.0f; maxc[k] = (float)v->resolution[k] - 1.0f; }
+    }
+}
+
+/* One pass of up to kMaxLevels levels over a device-resident volume.
+ * On success outs[i] hold device arrays. */
+kmc_status run_pass(const kmc_volume* v, const void* dvals, const double* iso, uint32_t nlev,
+                    int normal_type, int dup, double mn, double mx, const float minc[3], const float maxc[3],
+                    hipStream_t st, kmc_surface* outs)
+{
+    JobArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.values = dvals;
+    a.nx = v->resolution[0];
+    a.ny = v->resolution[1];
+    a.nz = v->resolution[2];
+    a.nlev = nlev;
+    a.dup = dup ? 1u : 0u;
+    a.vnormal = (!dup && normal_type == KMC_VERTEX_NORMAL) ? 1u : 0u;
+    a.minv = mn;
+    a.maxv = mx;
+    for (int k = 0; k < 3; ++k) {
+        a.minc[k] = minc[k];
+        const float span = maxc[k] - minc[k];          // (max_coord - min_coord), float
+        a.sc[k] = span / (float)(v->resolution[k] - 1); // / Vec3(ncells), float (MarchingCubes.cpp:166-168)
+    }
+    for (uint32_t l = 0; l < nlev; ++l) {
+        a.iso[l] = iso[l];
+        a.thr[l] = float_threshold(iso[l]);
+    }
+    uint32_t lds = 0;
+    if (!plan_geometry(a, &lds)) return KMC_ERR_INVALID_ARGUMENT;
+
+    const uint32_t narr = 2 * nlev;
+    const uint64_t n = a.ntiles;
+    const uint32_t nparts = scan_partials_needed(n, narr);
+    const size_t counts_bytes = (size_t)narr * n * sizeof(uint32_t);
+    const size_t prefix_bytes = (size_t)narr * n * sizeof(uint64_t);
+    uint32_t* counts = (uint32_t*)device_alloc(counts_bytes);
+    uint64_t* prefix = (uint64_t*)device_alloc(prefix_bytes);
+    uint64_t* aux = (uint64_t*)device_alloc((size_t)(nparts + narr) * sizeof(uint64_t));
+    std::vector<void*> allocated;
+    auto cleanup = [&]() {
+        device_free(counts); device_free(prefix); device_free(aux);
+    };
+    if (!counts || !prefix || !aux) { cleanup(); return KMC_ERR_OUT_OF_MEMORY; }
+    uint64_t* partials = aux;
+    uint64_t* totals = aux + nparts;
+    a.counts = counts;
+    a.prefix = prefix;
+
+    if (launch_march(v->type, a, true, lds, st) != hipSuccess ||
+        launch_scan(counts, n, narr, partials, prefix, totals, st) != hipSuccess) {
+        cleanup();
+        return KMC_ERR_DEVICE;
+    }
+    std::vector<uint64_t> tot(narr, 0);
+    if (hipMemcpyAsync(tot.data(), totals, narr * sizeof(uint64_t), hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess) {
+        cleanup();
+        return KMC_ERR_DEVICE;
+    }
+
+    kmc_status status = KMC_OK;
+    for (uint32_t l = 0; l < nlev; ++l) {
+        kmc_surface* s = &outs[l];
+        init_surface(s, iso[l], normal_type, dup, mn, mx, minc, maxc);
+        const uint64_t nv = tot[l * 2 + 0], nt = tot[l * 2 + 1];
+        SurfaceStorage* store = new SurfaceStorage();
+        store->memory = KMC_MEMORY_DEVICE;
+        store->coords = store->connections = store->normals = nullptr;
+        s->internal = store;
+        uint64_t verts = dup ? 3 * nt : nv;
+        if (verts == 0) {
+            /* coords.size() == 0: the reference sets no arrays (:236-242, :282-289) */
+            s->n_triangles = 0;
+            continue;
+        }
+        s->n_vertices = verts;
+        s->n_triangles = nt;
+        s->n_connections = dup ? 0 : 3 * nt;
+        s->n_normals = (!dup && normal_type == KMC_VERTEX_NORMAL) ? nv : nt;
+        store->coords = device_alloc(verts * 3 * sizeof(float));
+        store->normals = s->n_normals ? device_alloc(s->n_normals * 3 * sizeof(float)) : nullptr;
+        store->connections = s->n_connections ? device_alloc(s->n_connections * sizeof(uint32_t)) : nullptr;
+        if (!store->coords || (s->n_normals && !store->normals) || (s->n_connections && !store->connections)) {
+            status = KMC_ERR_OUT_OF_MEMORY;
+            break;
+        }
+        s->coords = (float*)store->coords;
+        s->normals = (float*)store->normals;
+        s->connections = (uint32_t*)store->connections;
+        a.coords[l] = s->coords;
+        a.normals[l] = s->normals;
+        a.conn[l] = s->connections;
+        a.cap_vertices[l] = verts;
+        a.cap_triangles[l] = nt;
+    }
+    if (status == KMC_OK) {
+        bool any = false;
+        for (uint32_t l = 0; l < nlev; ++l) any = any || outs[l].n_vertices > 0;
+        if (any && (launch_march(v->type, a, false, lds, st) != hipSuccess || hipStreamSynchronize(st) != hipSuccess))
+            status = KMC_ERR_DEVICE;
+    }
+    cleanup();
+    if (status != KMC_OK) {
+        for (uint32_t l = 0; l < nlev; ++l) kmc_surface_release(&outs[l]);
+    }
+    return status;
+}
+
+kmc_status to_host(kmc_surface* s, hipStream_t st)
+{
+    SurfaceStorage* old = (SurfaceStorage*)s->internal;
+    SurfaceStorage* hs = new SurfaceStorage();
+    hs->memory = KMC_MEMORY_HOST;
+    hs->coords = hs->connections = hs->normals = nullptr;
+    const size_t bc = s->n_vertices * 3 * sizeof(float);
+    const size_t bn = s->n_normals * 3 * sizeof(float);
+    const size_t bk = s->n_connections * sizeof(uint32_t);
+    bool ok = true;
+    if (bc) ok = ok && hipHostMalloc(&hs->coords, bc, hipHostMallocDefault) == hipSuccess;
+    if (bn) ok = ok && hipHostMalloc(&hs->normals, bn, hipHostMallocDefault) == hipSuccess;
+    if (bk) ok = ok && hipHostMalloc(&hs->connections, bk, hipHostMallocDefault) == hipSuccess;
+    if (ok && bc) ok = hipMemcpyAsync(hs->coords, s->coords, bc, hipMemcpyDeviceToHost, st) == hipSuccess;
+    if (ok && bn) ok = hipMemcpyAsync(hs->normals, s->normals, bn, hipMemcpyDeviceToHost, st) == hipSuccess;
+    if (ok && bk) ok = hipMemcpyAsync(hs->connections, s->connections, bk, hipMemcpyDeviceToHost, st) == hipSuccess;
+    if (ok) ok = hipStreamSynchronize(st) == hipSuccess;
+    if (!ok) {
+        (void)hipGetLastError();
+        free_storage(hs);
+        return KMC_ERR_OUT_OF_MEMORY;
+    }
+    free_storage(old);
+    s->internal = hs;
+    s->memory = KMC_MEMORY_HOST;
+    s->coords = (float*)hs->coords;
+    s->normals = (float*)hs->normals;
+    s->connections = (uint32_t*)hs->connections;
+    return KMC_OK;
+}
+
+kmc_status extract_levels_impl(const kmc_volume* v, const double* levels, uint32_t n_levels, int32_t normal_type,
+                               int32_t duplication, int32_t out_memory, void* stream, kmc_surface* outs)
+{
+    kmc_status s = validate_volume(v);
+    if (s != KMC_OK) return s;
+    if (!outs || (n_levels > 0 && !levels)) return KMC_ERR_INVALID_ARGUMENT;
+    if (normal_type != KMC_VERTEX_NORMAL && normal_type != KMC_POLYGON_NORMAL) return KMC_ERR_INVALID_ARGUMENT;
+    if (out_memory != KMC_MEMORY_HOST && out_memory != KMC_MEMORY_DEVICE) return KMC_ERR_INVALID_ARGUMENT;
+    const int dup = (normal_type == KMC_VERTEX_NORMAL) ? 0 : (duplication ? 1 : 0);   // :69-73
+    s = ensure_device();
+    if (s != KMC_OK) return s;
+    hipStream_t st = (hipStream_t)stream;
+
+    const void* dvals = nullptr;
+    void* owned = nullptr;
+    s = device_values(v, st, &dvals, &owned);
+    if (s != KMC_OK) return s;
+
+    double mn = v->min_value, mx = v->max_value;
+    if (!v->has_min_max_values) {
+        s = min_max_on_device(v, dvals, st, &mn, &mx);
+        if (s != KMC_OK) { device_free(owned); return s; }
+    }
+    float minc[3], maxc[3];
+    object_coords(v, minc, maxc);
+
+    const double d = mn - mx;
+    if ((d < 0 ? -d : d) < 1.e-6) {
+        /* Math::Equal(min, max): no surface (MarchingCubes.cpp:108-110) */
+        for (uint32_t l = 0; l < n_levels; ++l) {
+            init_surface(&outs[l], levels[l], normal_type, dup, mn, mx, minc, maxc);
+            outs[l].empty_range = 1;
+            outs[l].memory = out_memory;
+        }
+        if (owned) { (void)hipStreamSynchronize(st); device_free(owned); }
+        return KMC_OK;
+    }
+
+    for (uint32_t l0 = 0; l0 < n_levels && s == KMC_OK; l0 += kMaxLevels) {
+        const uint32_t nl = (n_levels - l0) < (uint32_t)kMaxLevels ? (n_levels - l0) : (uint32_t)kMaxLevels;
+        s = run_pass(v, dvals, levels + l0, nl, normal_type, dup, mn, mx, minc, maxc, st, outs + l0);
+        if (s != KMC_OK) {
+            for (uint32_t l = 0; l < l0; ++l) kmc_surface_release(&outs[l]);
+        }
+    }
+    if (s == KMC_OK && out_memory == KMC_MEMORY_HOST) {
+        for (uint32_t l = 0; l < n_levels && s == KMC_OK; ++l) s = to_host(&outs[l], st);
+        if (s != KMC_OK)
+            for (uint32_t l = 0; l < n_levels; ++l) kmc_surface_release(&outs[l]);
+    }
+    if (owned) { (void)hipStreamSynchronize(st); device_free(owned); }
+    return s;
+}
+
+}  // namespace kmc
+
+using namespace kmc;
+
+extern "C" {
+
+kmc_status kmc_init(int device)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) { (void)hipGetLastError(); return KMC_ERR_NO_DEVICE; }
+    if (device < 0 || device >= n) return KMC_ERR_INVALID_ARGUMENT;
+    t_device = device;
+    t_device_set = true;
+    return hipSetDevice(device) == hipSuccess ? KMC_OK : KMC_ERR_DEVICE;
+}
+
+const char* kmc_status_string(kmc_status s)
+{
+    switch (s) {
+    case KMC_OK: return "OK";
+    case KMC_ERR_NULL_INPUT: return "Input object is NULL.";
+    case KMC_ERR_NOT_STRUCTURED: return "Input object is not structured volume object.";
+    case KMC_ERR_NOT_SCALAR: return "Input volume is not sclar field data.";
+    case KMC_ERR_UNSUPPORTED_TYPE: return "Unsupported data type.";
+    case KMC_ERR_INVALID_ARGUMENT: return "Invalid argument.";
+    case KMC_ERR_OUT_OF_MEMORY: return "Cannot allocate memory.";
+    case KMC_ERR_DEVICE: return "HIP runtime error.";
+    case KMC_ERR_BUDGET: return "Request exceeds the memory budget.";
+    case KMC_ERR_NOT_READY: return "Result not ready.";
+    case KMC_ERR_NO_DEVICE: return "No HIP device available.";
+    }
+    return "Unknown status.";
+}
+
+kmc_status kmc_extract(const kmc_volume* volume, double isolevel, int32_t normal_type, int32_t duplication,
+                       int32_t out_memory, void* stream, kmc_surface* out)
+{
+    const kmc_status s = extract_levels_impl(volume, &isolevel, 1, normal_type, duplication, out_memory, stream, out);
+    report(s);
+    return s;
+}
+
+kmc_status kmc_extract_levels(const kmc_volume* volume, const double* isolevels, uint32_t n_levels,
+                              int32_t normal_type, int32_t duplication, int32_t out_memory, void* stream,
+                              kmc_surface* outs)
+{
+    const kmc_status s = extract_levels_impl(volume, isolevels, n_levels, normal_type, duplication, out_memory,
+                                             stream, outs);
+    report(s);
+    return s;
+}
+
+void kmc_surface_release(kmc_surface* s)
+{
+    if (!s) return;
+    free_storage((SurfaceStorage*)s->internal);
+    s->internal = nullptr;
+    s->coords = nullptr;
+    s->connections = nullptr;
+    s->normals = nullptr;
+    s->n_vertices = s->n_triangles = s->n_connections = s->n_normals = 0;
+}
+
+kmc_status kmc_volume_min_max(const kmc_volume* v, void* stream, double* mn, double* mx)
+{
+    kmc_status s = validate_volume(v);
+    if (s == KMC_OK && (!mn || !mx)) s = KMC_ERR_INVALID_ARGUMENT;
+    if (s == KMC_OK) s = ensure_device();
+    if (s != KMC_OK) { report(s); return s; }
+    hipStream_t st = (hipStream_t)stream;
+    const void* dvals = nullptr;
+    void* owned = nullptr;
+    s = device_values(v, st, &dvals, &owned);
+    if (s == KMC_OK) s = min_max_on_device(v, dvals, st, mn, mx);
+    if (owned) { (void)hipStreamSynchronize(st); device_free(owned); }
+    report(s);
+    return s;
+}
+
+}  // extern "C"

@@ -1,0 +1,254 @@
+"""THE ORACLE -- test infrastructure only.
+
+Python bindings of
+  * liboracle.so   -- mc_oracle.c, the plain-C restatement of kvs::MarchingCubes;
+  * _ref/libkvsref.so -- the reference's own marching-cubes path compiled from
+    its sources by oracle/Makefile (present when it was built).
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import
+this package; the product never does.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_SO = os.path.join(HERE, "liboracle.so")
+REF_SO = os.path.join(HERE, "_ref", "libkvsref.so")
+
+NUMPY_TYPES = {
+    np.dtype(np.int8): 0, np.dtype(np.int16): 1, np.dtype(np.int32): 2, np.dtype(np.int64): 3,
+    np.dtype(np.uint8): 4, np.dtype(np.uint16): 5, np.dtype(np.uint32): 6, np.dtype(np.uint64): 7,
+    np.dtype(np.float32): 8, np.dtype(np.float64): 9,
+}
+VERTEX_NORMAL, POLYGON_NORMAL = 0, 1
+
+
+class _OracleMesh(ctypes.Structure):
+    _fields_ = [
+        ("coords", ctypes.POINTER(ctypes.c_float)), ("n_coords", ctypes.c_uint64),
+        ("connections", ctypes.POINTER(ctypes.c_uint32)), ("n_connections", ctypes.c_uint64),
+        ("normals", ctypes.POINTER(ctypes.c_float)), ("n_normals", ctypes.c_uint64),
+        ("coords_f64", ctypes.POINTER(ctypes.c_double)),
+        ("n_triangles", ctypes.c_uint64),
+        ("polygon_type", ctypes.c_int32), ("normal_type", ctypes.c_int32),
+        ("color_type", ctypes.c_int32), ("empty_range", ctypes.c_int32),
+    ]
+
+
+class _RefVolume(ctypes.Structure):
+    _fields_ = [
+        ("values", ctypes.c_void_p), ("dtype", ctypes.c_int32), ("dims", ctypes.c_uint32 * 3),
+        ("has_min_max_values", ctypes.c_int32), ("min_value", ctypes.c_double), ("max_value", ctypes.c_double),
+        ("has_min_max_coords", ctypes.c_int32), ("min_coord", ctypes.c_float * 3), ("max_coord", ctypes.c_float * 3),
+    ]
+
+
+class _RefMesh(ctypes.Structure):
+    _fields_ = [
+        ("coords", ctypes.POINTER(ctypes.c_float)), ("n_coords", ctypes.c_uint64),
+        ("connections", ctypes.POINTER(ctypes.c_uint32)), ("n_connections", ctypes.c_uint64),
+        ("normals", ctypes.POINTER(ctypes.c_float)), ("n_normals", ctypes.c_uint64),
+        ("colors", ctypes.POINTER(ctypes.c_uint8)), ("n_colors", ctypes.c_uint64),
+        ("opacities", ctypes.POINTER(ctypes.c_uint8)), ("n_opacities", ctypes.c_uint64),
+        ("polygon_type", ctypes.c_int32), ("normal_type", ctypes.c_int32), ("color_type", ctypes.c_int32),
+        ("success", ctypes.c_int32),
+        ("volume_min_value", ctypes.c_double), ("volume_max_value", ctypes.c_double),
+        ("min_obj", ctypes.c_float * 3), ("max_obj", ctypes.c_float * 3),
+    ]
+
+
+_oracle = None
+_ref = None
+
+
+def oracle_lib():
+    global _oracle
+    if _oracle is None:
+        if not os.path.exists(ORACLE_SO):
+            raise RuntimeError(f"{ORACLE_SO} missing: run `make -C oracle oracle`")
+        lib = ctypes.CDLL(ORACLE_SO)
+        lib.oracle_extract.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_uint32),
+                                       ctypes.c_double, ctypes.c_double, ctypes.POINTER(ctypes.c_float),
+                                       ctypes.POINTER(ctypes.c_float), ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                                       ctypes.c_int, ctypes.POINTER(_OracleMesh)]
+        lib.oracle_extract.restype = ctypes.c_int
+        lib.oracle_free.argtypes = [ctypes.POINTER(_OracleMesh)]
+        lib.oracle_min_max.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_uint64,
+                                       ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
+        lib.oracle_min_max.restype = ctypes.c_int
+        lib.oracle_case_edges.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
+        lib.oracle_case_edges.restype = ctypes.c_int
+        _oracle = lib
+    return _oracle
+
+
+def ref_available():
+    return os.path.exists(REF_SO)
+
+
+def ref_lib():
+    global _ref
+    if _ref is None:
+        if not ref_available():
+            raise RuntimeError(f"{REF_SO} missing: run `make -C oracle ref` where the reference sources exist")
+        lib = ctypes.CDLL(REF_SO)
+        lib.ref_marching_cubes.argtypes = [ctypes.POINTER(_RefVolume), ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                                           ctypes.c_int, ctypes.c_int, ctypes.POINTER(_RefMesh)]
+        lib.ref_isosurface.argtypes = [ctypes.POINTER(_RefVolume), ctypes.c_int, ctypes.c_double, ctypes.c_int,
+                                       ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_RefMesh)]
+        lib.ref_mesh_free.argtypes = [ctypes.POINTER(_RefMesh)]
+        lib.ref_min_max.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_uint32),
+                                    ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
+        lib.ref_tf_color.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                                     ctypes.POINTER(ctypes.c_uint8)]
+        lib.ref_tf_table.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_uint8)]
+        lib.ref_tf_table.restype = ctypes.c_int
+        lib.ref_mc_tables.argtypes = [ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)]
+        _ref = lib
+    return _ref
+
+
+def _arr(ptr, n, dtype):
+    if n == 0:
+        return np.zeros(0, dtype=dtype)
+    return np.ctypeslib.as_array(ptr, shape=(int(n),)).copy()
+
+
+def _default_coords(dims):
+    return [0.0, 0.0, 0.0], [float(np.float32(d) - np.float32(1)) for d in dims]
+
+
+def min_max(values):
+    """GetMinMaxValues<T> as restated by the oracle."""
+    a = np.ascontiguousarray(values)
+    mn, mx = ctypes.c_double(), ctypes.c_double()
+    rc = oracle_lib().oracle_min_max(a.ctypes.data, NUMPY_TYPES[a.dtype], a.size, ctypes.byref(mn), ctypes.byref(mx))
+    assert rc == 0
+    return mn.value, mx.value
+
+
+def extract(values, isolevel, normal_type=POLYGON_NORMAL, duplication=True, min_max_values=None, coords=None,
+            want_f64=False):
+    """kvs::MarchingCubes as restated by the oracle.  values: numpy [z][y][x]."""
+    a = np.ascontiguousarray(values)
+    nz, ny, nx = a.shape
+    dims = (ctypes.c_uint32 * 3)(nx, ny, nz)
+    if min_max_values is None:
+        min_max_values = min_max(a)
+    if coords is None:
+        coords = _default_coords((nx, ny, nz))
+    minc = (ctypes.c_float * 3)(*coords[0])
+    maxc = (ctypes.c_float * 3)(*coords[1])
+    m = _OracleMesh()
+    rc = oracle_lib().oracle_extract(a.ctypes.data, NUMPY_TYPES[a.dtype], dims, float(min_max_values[0]),
+                                     float(min_max_values[1]), minc, maxc, float(isolevel), int(normal_type),
+                                     int(bool(duplication)), int(bool(want_f64)), ctypes.byref(m))
+    if rc != 0:
+        raise RuntimeError(f"oracle_extract failed: {rc}")
+    try:
+        out = {
+            "coords": _arr(m.coords, m.n_coords, np.float32),
+            "connections": _arr(m.connections, m.n_connections, np.uint32),
+            "normals": _arr(m.normals, m.n_normals, np.float32),
+            "n_triangles": int(m.n_triangles),
+            "polygon_type": m.polygon_type, "normal_type": m.normal_type, "color_type": m.color_type,
+            "empty_range": m.empty_range,
+        }
+        if want_f64:
+            out["coords_f64"] = _arr(m.coords_f64, m.n_coords if m.coords_f64 else 0, np.float64)
+        return out
+    finally:
+        oracle_lib().oracle_free(ctypes.byref(m))
+
+
+def case_edges(ci):
+    e = (ctypes.c_int * 16)()
+    n = oracle_lib().oracle_case_edges(ci, e)
+    return n, list(e)
+
+
+def _ref_volume(a, min_max_values, coords):
+    v = _RefVolume()
+    nz, ny, nx = a.shape
+    v.values = a.ctypes.data
+    v.dtype = NUMPY_TYPES[a.dtype]
+    v.dims[:] = [nx, ny, nz]
+    if min_max_values is not None:
+        v.has_min_max_values = 1
+        v.min_value, v.max_value = float(min_max_values[0]), float(min_max_values[1])
+    if coords is not None:
+        v.has_min_max_coords = 1
+        v.min_coord[:] = list(coords[0])
+        v.max_coord[:] = list(coords[1])
+    return v
+
+
+def _ref_mesh_dict(m):
+    return {
+        "coords": _arr(m.coords, m.n_coords, np.float32),
+        "connections": _arr(m.connections, m.n_connections, np.uint32),
+        "normals": _arr(m.normals, m.n_normals, np.float32),
+        "colors": _arr(m.colors, m.n_colors, np.uint8),
+        "opacities": _arr(m.opacities, m.n_opacities, np.uint8),
+        "polygon_type": m.polygon_type, "normal_type": m.normal_type, "color_type": m.color_type,
+        "success": m.success, "volume_min_value": m.volume_min_value, "volume_max_value": m.volume_max_value,
+        "min_obj": list(m.min_obj), "max_obj": list(m.max_obj),
+    }
+
+
+def ref_marching_cubes(values, isolevel, normal_type=POLYGON_NORMAL, duplication=True, min_max_values=None,
+                       coords=None, tf_kind=0, tf_resolution=256):
+    """The reference's own kvs::MarchingCubes (oracle/_ref)."""
+    a = np.ascontiguousarray(values)
+    v = _ref_volume(a, min_max_values, coords)
+    m = _RefMesh()
+    ref_lib().ref_marching_cubes(ctypes.byref(v), float(isolevel), int(normal_type), int(bool(duplication)),
+                                 int(tf_kind), int(tf_resolution), ctypes.byref(m))
+    try:
+        return _ref_mesh_dict(m)
+    finally:
+        ref_lib().ref_mesh_free(ctypes.byref(m))
+
+
+def ref_isosurface(values, isolevel=None, normal_type=POLYGON_NORMAL, duplication=True, tf_kind=0,
+                   tf_resolution=256, min_max_values=None, coords=None):
+    """The reference's own kvs::Isosurface (oracle/_ref); isolevel None uses the
+    (min+max)/2 constructor."""
+    a = np.ascontiguousarray(values)
+    v = _ref_volume(a, min_max_values, coords)
+    m = _RefMesh()
+    ref_lib().ref_isosurface(ctypes.byref(v), int(isolevel is None), float(isolevel or 0.0), int(normal_type),
+                             int(bool(duplication)), int(tf_kind), int(tf_resolution), ctypes.byref(m))
+    try:
+        return _ref_mesh_dict(m)
+    finally:
+        ref_lib().ref_mesh_free(ctypes.byref(m))
+
+
+def ref_min_max(values):
+    a = np.ascontiguousarray(values)
+    nz, ny, nx = a.shape
+    dims = (ctypes.c_uint32 * 3)(nx, ny, nz)
+    mn, mx = ctypes.c_double(), ctypes.c_double()
+    ref_lib().ref_min_max(a.ctypes.data, NUMPY_TYPES[a.dtype], dims, ctypes.byref(mn), ctypes.byref(mx))
+    return mn.value, mx.value
+
+
+def ref_tf_color(tf_kind, resolution, range_min, range_max, value):
+    rgb = (ctypes.c_uint8 * 3)()
+    ref_lib().ref_tf_color(int(tf_kind), int(resolution), float(range_min), float(range_max), float(value), rgb)
+    return tuple(rgb)
+
+
+def ref_tf_table(tf_kind, resolution=256):
+    out = (ctypes.c_uint8 * (3 * resolution))()
+    n = ref_lib().ref_tf_table(int(tf_kind), int(resolution), out)
+    return np.frombuffer(bytes(out), dtype=np.uint8)[:n].copy()
+
+
+def ref_tables():
+    tri = (ctypes.c_int32 * (256 * 16))()
+    vert = (ctypes.c_int32 * (12 * 2 * 3))()
+    ref_lib().ref_mc_tables(tri, vert)
+    return np.array(tri, dtype=np.int32).reshape(256, 16), np.array(vert, dtype=np.int32).reshape(12, 2, 3)

@@ -1,0 +1,217 @@
+/*
+ * ref_shim.cpp -- test infrastructure only.
+ *
+ * A C ABI over the reference's own kvs::MarchingCubes / kvs::Isosurface,
+ * compiled together with the reference's sources (ref_sources.txt) into
+ * oracle/_ref/libkvsref.so by oracle/Makefile.  It lets the tests check the
+ * oracle restatement (mc_oracle.c) and the colour restatement against the
+ * real reference, and lets bench.py time the real reference as the CPU
+ * baseline.  Nothing here is part of the product.
+ */
+#include <kvs/MarchingCubes>
+#include <kvs/MarchingCubesTable>
+#include <kvs/Isosurface>
+#include <kvs/StructuredVolumeObject>
+#include <kvs/TransferFunction>
+#include <kvs/ColorMap>
+#include <kvs/ValueArray>
+#include <kvs/AnyValueArray>
+#include <kvs/Type>
+#include <cstdint>
+#include <cstdlib>
+#include <cstring>
+
+namespace
+{
+
+template <typename T>
+kvs::AnyValueArray MakeValues( const void* data, size_t n )
+{
+    kvs::ValueArray<T> a( n );
+    std::memcpy( a.data(), data, n * sizeof( T ) );
+    return kvs::AnyValueArray( a );
+}
+
+kvs::AnyValueArray MakeAny( const void* data, int dtype, size_t n )
+{
+    switch ( dtype )
+    {
+    case 0: return MakeValues<kvs::Int8>( data, n );
+    case 1: return MakeValues<kvs::Int16>( data, n );
+    case 2: return MakeValues<kvs::Int32>( data, n );
+    case 3: return MakeValues<kvs::Int64>( data, n );
+    case 4: return MakeValues<kvs::UInt8>( data, n );
+    case 5: return MakeValues<kvs::UInt16>( data, n );
+    case 6: return MakeValues<kvs::UInt32>( data, n );
+    case 7: return MakeValues<kvs::UInt64>( data, n );
+    case 8: return MakeValues<kvs::Real32>( data, n );
+    default: return MakeValues<kvs::Real64>( data, n );
+    }
+}
+
+kvs::TransferFunction MakeTF( int tf_kind, int resolution )
+{
+    // 0: TransferFunction(resolution) (the reference default, BrewerRdBu);
+    // 1: BrewerSingleHueGreys(resolution); 2: Rainbow(resolution).
+    if ( tf_kind == 1 ) return kvs::TransferFunction( kvs::ColorMap::BrewerSingleHueGreys( resolution ) );
+    if ( tf_kind == 2 ) return kvs::TransferFunction( kvs::ColorMap::Rainbow( resolution ) );
+    return kvs::TransferFunction( resolution );
+}
+
+template <typename A>
+void* CopyOut( const A& a, uint64_t* n )
+{
+    *n = a.size();
+    if ( a.size() == 0 ) return nullptr;
+    void* p = std::malloc( a.byteSize() );
+    std::memcpy( p, a.data(), a.byteSize() );
+    return p;
+}
+
+} // namespace
+
+extern "C" {
+
+struct ref_volume
+{
+    const void* values;
+    int32_t dtype;
+    uint32_t dims[3];
+    int32_t has_min_max_values;
+    double min_value, max_value;
+    int32_t has_min_max_coords;
+    float min_coord[3], max_coord[3];
+};
+
+struct ref_mesh
+{
+    float* coords; uint64_t n_coords;
+    uint32_t* connections; uint64_t n_connections;
+    float* normals; uint64_t n_normals;
+    uint8_t* colors; uint64_t n_colors;
+    uint8_t* opacities; uint64_t n_opacities;
+    int32_t polygon_type, normal_type, color_type;
+    int32_t success;
+    double volume_min_value, volume_max_value;   // after the call (cached by the mapper)
+    float min_obj[3], max_obj[3];                // polygon's min/max object coords
+};
+
+static kvs::StructuredVolumeObject* MakeVolume( const ref_volume* v )
+{
+    auto* vol = new kvs::StructuredVolumeObject();
+    vol->setGridTypeToUniform();
+    vol->setVeclen( 1 );
+    vol->setResolution( kvs::Vec3u( v->dims[0], v->dims[1], v->dims[2] ) );
+    const size_t n = size_t( v->dims[0] ) * v->dims[1] * v->dims[2];
+    vol->setValues( MakeAny( v->values, v->dtype, n ) );
+    if ( v->has_min_max_values ) vol->setMinMaxValues( v->min_value, v->max_value );
+    if ( v->has_min_max_coords )
+    {
+        vol->setMinMaxObjectCoords(
+            kvs::Vec3( v->min_coord[0], v->min_coord[1], v->min_coord[2] ),
+            kvs::Vec3( v->max_coord[0], v->max_coord[1], v->max_coord[2] ) );
+    }
+    return vol;
+}
+
+static void Harvest( const kvs::PolygonObject* p, const kvs::StructuredVolumeObject* vol, int success, ref_mesh* out )
+{
+    std::memset( out, 0, sizeof( *out ) );
+    out->coords = static_cast<float*>( CopyOut( p->coords(), &out->n_coords ) );
+    out->connections = static_cast<uint32_t*>( CopyOut( p->connections(), &out->n_connections ) );
+    out->normals = static_cast<float*>( CopyOut( p->normals(), &out->n_normals ) );
+    out->colors = static_cast<uint8_t*>( CopyOut( p->colors(), &out->n_colors ) );
+    out->opacities = static_cast<uint8_t*>( CopyOut( p->opacities(), &out->n_opacities ) );
+    out->polygon_type = p->polygonType();
+    out->normal_type = p->normalType();
+    out->color_type = p->colorType();
+    out->success = success;
+    out->volume_min_value = vol->minValue();
+    out->volume_max_value = vol->maxValue();
+    for ( int k = 0; k < 3; ++k )
+    {
+        out->min_obj[k] = p->minObjectCoord()[k];
+        out->max_obj[k] = p->maxObjectCoord()[k];
+    }
+}
+
+// kvs::MarchingCubes( volume, isolevel, normal_type, duplication, tf ).
+int ref_marching_cubes( const ref_volume* v, double isolevel, int normal_type, int duplication,
+                        int tf_kind, int tf_resolution, ref_mesh* out )
+{
+    auto* vol = MakeVolume( v );
+    auto* mc = new kvs::MarchingCubes(
+        vol, isolevel, kvs::PolygonObject::NormalType( normal_type ), duplication != 0,
+        MakeTF( tf_kind, tf_resolution ) );
+    Harvest( mc, vol, mc->isSuccess() ? 1 : 0, out );
+    delete mc;
+    delete vol;
+    return 0;
+}
+
+// kvs::Isosurface( volume, isolevel, normal_type, duplication, tf ) when
+// use_default_level == 0, else kvs::Isosurface( volume, normal_type ).
+int ref_isosurface( const ref_volume* v, int use_default_level, double isolevel, int normal_type,
+                    int duplication, int tf_kind, int tf_resolution, ref_mesh* out )
+{
+    auto* vol = MakeVolume( v );
+    kvs::Isosurface* s = nullptr;
+    if ( use_default_level )
+        s = new kvs::Isosurface( vol, kvs::PolygonObject::NormalType( normal_type ) );
+    else
+        s = new kvs::Isosurface( vol, isolevel, kvs::PolygonObject::NormalType( normal_type ),
+                                 duplication != 0, MakeTF( tf_kind, tf_resolution ) );
+    Harvest( s, vol, s->isSuccess() ? 1 : 0, out );
+    delete s;
+    delete vol;
+    return 0;
+}
+
+void ref_mesh_free( ref_mesh* m )
+{
+    std::free( m->coords ); std::free( m->connections ); std::free( m->normals );
+    std::free( m->colors ); std::free( m->opacities );
+    std::memset( m, 0, sizeof( *m ) );
+}
+
+// GetMinMaxValues<T> through StructuredVolumeObject::updateMinMaxValues.
+int ref_min_max( const void* values, int dtype, const uint32_t dims[3], double* mn, double* mx )
+{
+    ref_volume v{};
+    v.values = values; v.dtype = dtype;
+    v.dims[0] = dims[0]; v.dims[1] = dims[1]; v.dims[2] = dims[2];
+    auto* vol = MakeVolume( &v );
+    vol->updateMinMaxValues();
+    *mn = vol->minValue(); *mx = vol->maxValue();
+    delete vol;
+    return 0;
+}
+
+// ColorMap::at( value ) of the given transfer function after
+// TransferFunction::setRange( range_min, range_max ).
+int ref_tf_color( int tf_kind, int tf_resolution, float range_min, float range_max, float value, uint8_t rgb[3] )
+{
+    auto tf = MakeTF( tf_kind, tf_resolution );
+    tf.setRange( range_min, range_max );
+    const kvs::RGBColor c = tf.colorMap().at( value );
+    rgb[0] = c.r(); rgb[1] = c.g(); rgb[2] = c.b();
+    return 0;
+}
+
+// The created colour table of a transfer function (3*resolution bytes).
+int ref_tf_table( int tf_kind, int tf_resolution, uint8_t* out )
+{
+    auto tf = MakeTF( tf_kind, tf_resolution );
+    const auto& t = tf.colorMap().table();
+    std::memcpy( out, t.data(), t.size() );
+    return int( t.size() );
+}
+
+// MarchingCubesTable::TriangleID and VertexID, for pinning transcriptions.
+void ref_mc_tables( int32_t tri[256 * 16], int32_t vert[12 * 2 * 3] )
+{
+    std::memcpy( tri, kvs::MarchingCubesTable::TriangleID, sizeof( int32_t ) * 256 * 16 );
+    std::memcpy( vert, kvs::MarchingCubesTable::VertexID, sizeof( int32_t ) * 12 * 2 * 3 );
+}
+
+} // extern "C"
