@@ -1,0 +1,197 @@
+"""Benchmark of the marching-cubes isosurface path on one MI355X.
+
+Workload: a 1024^3 float32 turbulence-like field (sum of 64 random-phase
+Fourier modes, |k| in [1, 64], amplitude |k|^(-5/3), normalised to [-1, 1]),
+isolevels {-0.3, 0.0, 0.3} extracted together, VertexNormal (shared vertices,
+per-vertex normals).  One step = one kmc_extract_levels call on the
+HBM-resident volume with device-resident outputs.  The volume min/max is
+computed once beforehand, as the reference caches it in the volume object.
+
+Printed: one JSON line with Gvoxels/s (voxel classifications per second,
+N * levels / t), Mtriangles/s, the HBM roofline of the job and the
+reference's own CPU path timed on a z-slab sample of the same volume.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--size 1024] [--no-cpu-baseline]
+"""
+import argparse
+import ctypes
+import json
+import os
+import platform
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+LEVELS = (-0.3, 0.0, 0.3)
+METRIC = "Gvoxels/sec + Mtriangles/sec, % of HBM roofline; batched vs plain speedup; p50/p99"
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor()
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--size", type=int, default=1024)
+    ap.add_argument("--cpu-slices", type=int, default=192)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: every MI355X box here has a single GPU and the "
+              "path shards by z-slab (see DESIGN.md); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from kvs_amd import datasets, kmc
+
+    lib = kmc.load()
+    dev = torch.device("cuda:0")
+    n = args.size
+    print(f"[bench] generating {n}^3 float32 turbulence field on the GPU", file=sys.stderr, flush=True)
+    t0 = time.time()
+    vol = datasets.turbulence_torch(n, seed=2, device=dev)
+    torch.cuda.synchronize()
+    print(f"[bench] generated in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
+
+    stream = torch.cuda.current_stream(dev)
+    sptr = ctypes.c_void_p(stream.cuda_stream)
+    v = kmc.make_volume(vol.data_ptr(), dims=(n, n, n), memory=kmc.MEMORY_DEVICE, dtype=np.float32)
+    mn, mx = ctypes.c_double(), ctypes.c_double()
+    st = lib.kmc_volume_min_max(ctypes.byref(v), sptr, ctypes.byref(mn), ctypes.byref(mx))
+    assert st == 0, st
+    v.has_min_max_values = 1
+    v.min_value, v.max_value = mn.value, mx.value
+
+    nlev = len(LEVELS)
+    levels = (ctypes.c_double * nlev)(*LEVELS)
+    outs = (kmc.Surface * nlev)()
+
+    def step():
+        s = lib.kmc_extract_levels(ctypes.byref(v), levels, nlev, kmc.VERTEX_NORMAL, 0, kmc.MEMORY_DEVICE,
+                                   sptr, outs)
+        if s != 0:
+            raise kmc.KmcError(s)
+        res = [(int(o.n_vertices), int(o.n_triangles)) for o in outs]
+        for i in range(nlev):
+            lib.kmc_surface_release(ctypes.byref(outs[i]))
+        return res
+
+    print("[bench] warmup", file=sys.stderr, flush=True)
+    for _ in range(args.warmup):
+        counts = step()
+    torch.cuda.synchronize()
+    start = torch.cuda.Event(enable_timing=True)
+    end = torch.cuda.Event(enable_timing=True)
+    print(f"[bench] timing {args.steps} steps", file=sys.stderr, flush=True)
+    wall0 = time.perf_counter()
+    start.record(stream)
+    for _ in range(args.steps):
+        counts = step()
+    end.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - wall0
+    ms = start.elapsed_time(end) / args.steps
+
+    nodes = n ** 3
+    ntri = sum(t for _, t in counts)
+    nvert = sum(vv for vv, _ in counts)
+    sec = ms / 1e3
+    gvox = nodes * nlev / sec / 1e9
+    mtri = ntri / sec / 1e6
+    # algorithmic bytes of the job: volume read once + returned arrays
+    # (VertexNormal: coords 12 B and normals 12 B per vertex, connections 12 B per triangle)
+    b_alg = nodes * 4 + sum(24 * vv + 12 * t for vv, t in counts)
+    achieved = b_alg / sec / 1e9
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(vol, args.cpu_slices, mn.value, mx.value)
+
+    out = {
+        "metric": METRIC,
+        "value": round(gvox, 3),
+        "unit": "Gvoxels/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"{n}^3 float32 turbulence field (64 Fourier modes, |k|^-5/3), isolevels "
+                        f"{list(LEVELS)} fused, VertexNormal, device-resident volume and outputs",
+            "resolution": [n, n, n], "levels": list(LEVELS), "normal_type": "VertexNormal",
+            "duplication": False,
+        },
+        "mtriangles_per_s": round(mtri, 2),
+        "triangles_per_step": ntri,
+        "vertices_per_step": nvert,
+        "wall_ms_per_step": round(wall / args.steps * 1e3, 3),
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None,
+            "algorithmic_bytes": b_alg,
+            "scope": "whole job (count + scan + emit kernels) on the stream, host sync included",
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def cpu_baseline(vol, slices, vmin, vmax):
+    """The reference's own kvs::MarchingCubes (oracle/_ref, 1 thread) on a
+    z-slab sample of the same volume bytes, min/max pre-cached."""
+    import numpy as np
+
+    import oracle
+
+    if not oracle.ref_available():
+        return None
+    n = vol.shape[2]
+    z0 = (vol.shape[0] - slices) // 2
+    sample = vol[z0:z0 + slices].cpu().numpy()
+    nodes = sample.size
+    total = 0.0
+    tris = 0
+    for lv in LEVELS:
+        print(f"[bench] cpu baseline level {lv}", file=sys.stderr, flush=True)
+        r = oracle.ref_marching_cubes(sample, lv, oracle.VERTEX_NORMAL, False, min_max_values=(vmin, vmax))
+        total += r["seconds"]
+        tris += r["connections"].size // 3
+    return {
+        "value": round(nodes * len(LEVELS) / total / 1e9, 4),
+        "unit": "Gvoxels/s",
+        "cores": 1,
+        "kind": "reference",
+        "sample": f"z-slab {n}x{n}x{slices} of the same volume ({nodes} nodes), {len(LEVELS)} levels, "
+                  f"VertexNormal, kvs::MarchingCubes constructor wall time {total:.2f}s, {tris} triangles",
+        "mtriangles_per_s": round(tris / total / 1e6, 3),
+        "cpu_model": cpu_model(),
+        "nproc": os.cpu_count(),
+    }
+
+
+if __name__ == "__main__":
+    main()

@@ -55,6 +55,7 @@ class _RefMesh(ctypes.Structure):
         ("success", ctypes.c_int32),
         ("volume_min_value", ctypes.c_double), ("volume_max_value", ctypes.c_double),
         ("min_obj", ctypes.c_float * 3), ("max_obj", ctypes.c_float * 3),
+        ("seconds", ctypes.c_double),
     ]
 
 
@@ -193,7 +194,7 @@ def _ref_mesh_dict(m):
         "opacities": _arr(m.opacities, m.n_opacities, np.uint8),
         "polygon_type": m.polygon_type, "normal_type": m.normal_type, "color_type": m.color_type,
         "success": m.success, "volume_min_value": m.volume_min_value, "volume_max_value": m.volume_max_value,
-        "min_obj": list(m.min_obj), "max_obj": list(m.max_obj),
+        "min_obj": list(m.min_obj), "max_obj": list(m.max_obj), "seconds": m.seconds,
     }
 
 

@@ -20,6 +20,7 @@
 #include <cstdint>
 #include <cstdlib>
 #include <cstring>
+#include <chrono>
 
 namespace
 {
@@ -94,6 +95,7 @@ struct ref_mesh
     int32_t success;
     double volume_min_value, volume_max_value;   // after the call (cached by the mapper)
     float min_obj[3], max_obj[3];                // polygon's min/max object coords
+    double seconds;                              // wall time of the mapper constructor alone
 };
 
 static kvs::StructuredVolumeObject* MakeVolume( const ref_volume* v )
@@ -140,10 +142,13 @@ int ref_marching_cubes( const ref_volume* v, double isolevel, int normal_type, i
                         int tf_kind, int tf_resolution, ref_mesh* out )
 {
     auto* vol = MakeVolume( v );
+    const auto tf = MakeTF( tf_kind, tf_resolution );
+    const auto t0 = std::chrono::steady_clock::now();
     auto* mc = new kvs::MarchingCubes(
-        vol, isolevel, kvs::PolygonObject::NormalType( normal_type ), duplication != 0,
-        MakeTF( tf_kind, tf_resolution ) );
+        vol, isolevel, kvs::PolygonObject::NormalType( normal_type ), duplication != 0, tf );
+    const auto t1 = std::chrono::steady_clock::now();
     Harvest( mc, vol, mc->isSuccess() ? 1 : 0, out );
+    out->seconds = std::chrono::duration<double>( t1 - t0 ).count();
     delete mc;
     delete vol;
     return 0;
