@@ -19,7 +19,9 @@
 
 namespace kmc {
 
-hipError_t launch_march(int dtype, const JobArgs& a, bool count, uint32_t lds_bytes, hipStream_t st);
+hipError_t launch_emit(int dtype, const JobArgs& a, uint32_t lds_bytes, hipStream_t st);
+hipError_t launch_classify(int dtype, const JobArgs& a, hipStream_t st);
+hipError_t launch_count(const JobArgs& a, hipStream_t st);
 hipError_t launch_scan(const uint32_t* counts, uint64_t n, uint32_t narr, uint64_t* partials,
                        uint64_t* prefix, uint64_t* totals, hipStream_t st);
 uint32_t scan_partials_needed(uint64_t n, uint32_t narr);
@@ -226,18 +228,23 @@ kmc_status min_max_on_device(const kmc_volume* v, const void* dvals, hipStream_t
 /* Geometry and LDS budget of a pass; R rows per band. */
 bool plan_geometry(JobArgs& a, uint32_t* lds_bytes)
 {
-    const uint32_t candidates[] = { 4, 2, 1 };
+    const uint32_t candidates[] = { 8, 4, 2, 1 };
     for (uint32_t R : candidates) {
         a.R = R;
         a.nb = (a.ny + R - 1) / R;
         a.NC = (a.nx + 63) / 64;
-        a.queue_cap = 2048;
-        const LdsLayout L = lds_layout(a.R, a.NC, a.nlev, a.queue_cap);
+        a.cshift = 0;
+        while ((1u << a.cshift) < a.NC && a.cshift < 6) ++a.cshift;
+        a.queue_cap = 1024;
+        a.cache_cap = 1024;      // fill_cache stages its work in the vertex queue: needs queue_cap >= cache_cap
+        a.use_cache = std::getenv("KMC_WINDOW_CACHE") ? 1u : 0u;
+        const LdsLayout L = lds_layout(a.R, a.NC, a.nlev, a.queue_cap, a.cache_cap);
         if (L.bytes <= 60 * 1024) {
             *lds_bytes = L.bytes;
             a.slab = 32;
             a.nslabs = (a.nz + a.slab - 1) / a.slab;
             a.ntiles = (uint64_t)a.nz * a.nb;
+            a.nrows = (uint64_t)a.nz * a.ny;
             return true;
         }
     }
@@ -298,24 +305,27 @@ kmc_status run_pass(const kmc_volume* v, const void* dvals, const double* iso, u
     if (!plan_geometry(a, &lds)) return KMC_ERR_INVALID_ARGUMENT;
 
     const uint32_t narr = 2 * nlev;
-    const uint64_t n = a.ntiles;
+    const uint64_t n = a.nrows;
     const uint32_t nparts = scan_partials_needed(n, narr);
     const size_t counts_bytes = (size_t)narr * n * sizeof(uint32_t);
     const size_t prefix_bytes = (size_t)narr * n * sizeof(uint64_t);
     uint32_t* counts = (uint32_t*)device_alloc(counts_bytes);
     uint64_t* prefix = (uint64_t*)device_alloc(prefix_bytes);
     uint64_t* aux = (uint64_t*)device_alloc((size_t)(nparts + narr) * sizeof(uint64_t));
-    std::vector<void*> allocated;
+    const size_t bitvol_bytes = (size_t)nlev * a.nz * a.ny * a.NC * sizeof(uint64_t);
+    uint64_t* bitvol = (uint64_t*)device_alloc(bitvol_bytes);
     auto cleanup = [&]() {
-        device_free(counts); device_free(prefix); device_free(aux);
+        device_free(counts); device_free(prefix); device_free(aux); device_free(bitvol);
     };
-    if (!counts || !prefix || !aux) { cleanup(); return KMC_ERR_OUT_OF_MEMORY; }
+    if (!counts || !prefix || !aux || !bitvol) { cleanup(); return KMC_ERR_OUT_OF_MEMORY; }
+    a.bitvol = bitvol;
     uint64_t* partials = aux;
     uint64_t* totals = aux + nparts;
     a.counts = counts;
     a.prefix = prefix;
 
-    if (launch_march(v->type, a, true, lds, st) != hipSuccess ||
+    if (launch_classify(v->type, a, st) != hipSuccess ||
+        launch_count(a, st) != hipSuccess ||
         launch_scan(counts, n, narr, partials, prefix, totals, st) != hipSuccess) {
         cleanup();
         return KMC_ERR_DEVICE;
@@ -365,7 +375,7 @@ kmc_status run_pass(const kmc_volume* v, const void* dvals, const double* iso, u
     if (status == KMC_OK) {
         bool any = false;
         for (uint32_t l = 0; l < nlev; ++l) any = any || outs[l].n_vertices > 0;
-        if (any && (launch_march(v->type, a, false, lds, st) != hipSuccess || hipStreamSynchronize(st) != hipSuccess))
+        if (any && (launch_emit(v->type, a, lds, st) != hipSuccess || hipStreamSynchronize(st) != hipSuccess))
             status = KMC_ERR_DEVICE;
     }
     cleanup();

@@ -17,6 +17,11 @@
  *     Source/Core/Visualization/Mapper/MarchingCubes.cpp:155-243,365-521;
  *   - a workgroup marches z through a *slab* of consecutive slices of one band.
  *
+ * Passes of one job: classify (stream the values once, write the bit volume),
+ * count (per-tile vertex / triangle counts from the bit volume), scan (tile
+ * prefixes), emit (outputs at the prefixes; raw values are read again only at
+ * the endpoints of crossing edges).
+ *
  * Within a slice, the classification bits of 64 consecutive nodes of a row
  * are held as one 64-bit mask per level (a *chunk*), built with one wave
  * ballot; edge crossings and cell activity are then bitwise operations on
@@ -34,25 +39,34 @@ constexpr int kWaves = kBlock / 64;
 struct JobArgs {
     const void* values;
     uint32_t nx, ny, nz;
-    uint32_t R;        // node rows per band
+    uint32_t R;        // node rows per band (emit tiles)
     uint32_t nb;       // number of bands, ceil(ny / R)
     uint32_t NC;       // 64-node chunks per row, ceil(nx / 64)
+    uint32_t cshift;   // lanes are split into 64 >> cshift rows of (1 << cshift) chunk lanes
+                       // (1 << cshift = the power of two >= min(NC, 64))
     uint32_t nlev;     // isolevels in this pass
     uint32_t slab;     // slices marched by one workgroup
     uint32_t nslabs;
     uint32_t dup;      // 1: duplicated vertices (per-triangle output)
     uint32_t vnormal;  // 1: VertexNormal (only without duplication)
     uint32_t queue_cap;
+    uint32_t cache_cap;   // window vertex coordinates held in LDS by the emit pass
+    uint32_t use_cache;   // 1: interpolate each window vertex once into LDS; 0: interpolate per use
     uint64_t ntiles;   // nz * nb
+    uint64_t nrows;    // nz * ny: counts and prefixes are kept per node row
     double minv, maxv; // clamp range of the interpolation (volume min/max)
     float minc[3];     // min object coordinate
     float sc[3];       // (max - min) / ncells, per axis (float ops, host side)
     double iso[kMaxLevels];
     float thr[kMaxLevels];    // float32 data: v > thr  <=>  double(v) > iso
-    // counts[(l*2 + k) * ntiles + tile], k = 0 vertices, 1 triangles
+    // counts[(l*2 + k) * nrows + z*ny + y], k = 0: vertices owned by node row
+    // (y, z); k = 1: triangles of cell row (y, z-1)
     uint32_t* counts;
     // exclusive prefixes, same layout
     const uint64_t* prefix;
+    // classification bit volume: bitvol[((l * nz + z) * ny + y) * NC + c] is the
+    // 64-bit mask of nodes x in [64c, 64c+64) of row y, slice z, level l
+    uint64_t* bitvol;
     // per-level outputs
     float* coords[kMaxLevels];
     float* normals[kMaxLevels];
@@ -61,18 +75,49 @@ struct JobArgs {
     uint64_t cap_triangles[kMaxLevels];
 };
 
-/* LDS layout of one workgroup; sizes in bytes, offsets 16-byte aligned. */
+/* Per-level parameters staged in LDS so that emission can index them by a
+ * runtime level without touching private memory. */
+struct LevelInfo {
+    float* coords;
+    float* normals;
+    uint32_t* conn;
+    uint64_t cap_v, cap_t;
+    double iso;
+    uint64_t vbase[4];  // vertex prefix of tiles (z-1,b), (z-1,b+1), (z,b), (z,b+1)
+    uint64_t tbase;     // triangle prefix of tile (z,b)
+};
+
+/* LDS layout of the emit workgroup; sizes in bytes, offsets 16-byte aligned.
+ * The window of tile (z, b) is node slices z-1, z, z+1 (window slices 0..2)
+ * and node rows y0-1 .. y0+R (window rows 0..R+1).  Arrays are indexed
+ * [slice][level][row][chunk] so that consecutive lanes (chunks) touch
+ * consecutive words. */
 struct LdsLayout {
-    uint32_t nrb;       // rows held per slice: y0-1 .. y0+R+1
-    uint32_t ncp;       // chunks per row incl. one zero pad chunk
-    uint32_t off_bits;  // uint64 [3][nrb][ncp][nlev]
-    uint32_t off_pv;    // uint32 [2][R+1][NC][nlev]    vertex chunk prefixes
-    uint32_t off_tm;    // uint64 [R][NC][nlev][3]       triangle-count bit planes
-    uint32_t off_pt;    // uint32 [R][NC][nlev]          triangle chunk prefixes
-    uint32_t off_scan;  // uint32 [kBlock + 64]
-    uint32_t off_tot;   // uint32 [2][kMaxLevels]
-    uint32_t off_queue; // uint32 [queue_cap]
+    uint32_t nrb;       // bit rows held per slice: y0-1 .. y0+R+1
+    uint32_t ncp;       // chunks per stored row (= NC)
+    uint32_t off_tri;   // uint64 [256]                     case table
+    uint32_t off_lev;   // LevelInfo [kMaxLevels]
+    uint32_t off_bits;  // uint64 [3][nlev][nrb][ncp]        classification masks
+    uint32_t off_wpv;   // uint32 [3][nlev][R+2][NC]         window vertex chunk prefixes
+    uint32_t off_act;   // uint64 [nlev][R][NC]              active cells, cell slice z-1, band rows
+    uint32_t off_pt;    // uint32 [nlev][R][NC]              triangle chunk prefixes
+    uint32_t off_misc;  // uint32 [64]                       totals, offsets, flags
+    uint32_t off_cache; // float  [cache_cap][3]             window vertex coordinates
+    uint32_t off_vq;    // uint32 [queue_cap]                vertex work queue
+    uint32_t off_tq;    // uint32 [queue_cap]                triangle work queue
     uint32_t bytes;
+};
+
+/* misc words */
+enum : uint32_t {
+    kMiscVTot = 0,        // [kMaxLevels] tile vertex totals
+    kMiscTTot = 8,        // [kMaxLevels] tile triangle totals
+    kMiscQV = 16,         // [kMaxLevels + 1] vertex queue offsets per level
+    kMiscQT = 25,         // [kMaxLevels + 1] triangle queue offsets per level
+    kMiscWOff = 34,       // [3 * kMaxLevels] cache offsets of (window slice, level) -- needs 24
+    kMiscOverflow = 58,   // cache overflow flag
+    kMiscWTotal = 59,     // window vertices in all slices and levels
+    kMiscWords = 64
 };
 
 #if defined(__HIPCC__)
@@ -83,19 +128,22 @@ inline uint32_t align16(uint32_t x) { return (x + 15u) & ~15u; }
 #if defined(__HIPCC__)
 __host__ __device__
 #endif
-inline LdsLayout lds_layout(uint32_t R, uint32_t NC, uint32_t nlev, uint32_t queue_cap)
+inline LdsLayout lds_layout(uint32_t R, uint32_t NC, uint32_t nlev, uint32_t queue_cap, uint32_t cache_cap)
 {
     LdsLayout L;
     L.nrb = R + 3;
-    L.ncp = NC + 1;
+    L.ncp = NC;
     uint32_t o = 0;
-    L.off_bits = o;  o = align16(o + 3u * L.nrb * L.ncp * nlev * 8u);
-    L.off_pv = o;    o = align16(o + 2u * (R + 1) * NC * nlev * 4u);
-    L.off_tm = o;    o = align16(o + R * NC * nlev * 3u * 8u);
-    L.off_pt = o;    o = align16(o + R * NC * nlev * 4u);
-    L.off_scan = o;  o = align16(o + (kBlock + 64) * 4u);
-    L.off_tot = o;   o = align16(o + 2u * kMaxLevels * 4u);
-    L.off_queue = o; o = align16(o + queue_cap * 4u);
+    L.off_tri = o;   o = align16(o + 256u * 8u);
+    L.off_lev = o;   o = align16(o + kMaxLevels * (uint32_t)sizeof(LevelInfo));
+    L.off_bits = o;  o = align16(o + 3u * nlev * L.nrb * L.ncp * 8u);
+    L.off_wpv = o;   o = align16(o + 3u * nlev * (R + 2) * NC * 4u);
+    L.off_act = o;   o = align16(o + nlev * R * NC * 8u);
+    L.off_pt = o;    o = align16(o + nlev * R * NC * 4u);
+    L.off_misc = o;  o = align16(o + kMiscWords * 4u);
+    L.off_cache = o; o = align16(o + cache_cap * 12u);
+    L.off_vq = o;    o = align16(o + queue_cap * 4u);
+    L.off_tq = o;    o = align16(o + queue_cap * 4u);
     L.bytes = o;
     return L;
 }

@@ -29,18 +29,6 @@ namespace kmc {
 
 __constant__ uint64_t d_tri[256] = { KMC_TABLE_WORDS };
 
-/* Owner node offset and axis of each cell edge (see kEdgeNode). */
-__constant__ uint8_t d_edge_node[12][4] = {
-    {0, 0, 0, 0}, {1, 0, 0, 1}, {0, 1, 0, 0}, {0, 0, 0, 1},
-    {0, 0, 1, 0}, {1, 0, 1, 1}, {0, 1, 1, 0}, {0, 0, 1, 1},
-    {0, 0, 0, 2}, {1, 0, 0, 2}, {1, 1, 0, 2}, {0, 1, 0, 2}
-};
-/* Table-orientation endpoints of each edge as corner offsets (x,y,z) packed
- * in 3 bits each: bit0 x, bit1 y, bit2 z. */
-__constant__ uint8_t d_edge_ends[12][2] = {
-    {0, 1}, {1, 3}, {3, 2}, {2, 0}, {4, 5}, {5, 7}, {7, 6}, {6, 4}, {0, 4}, {1, 5}, {3, 7}, {2, 6}
-};
-
 /* ------------------------------------------------------------------------- */
 /* classification                                                              */
 /* ------------------------------------------------------------------------- */
@@ -132,40 +120,54 @@ __device__ __forceinline__ void face_normal(const float v0[3], const float v1[3]
 /* ------------------------------------------------------------------------- */
 
 struct Views {
-    uint64_t* bits;     // [3][nrb][ncp][nlev]
-    uint32_t* pv;       // [2][R+1][NC][nlev]
-    uint64_t* tm;       // [R][NC][nlev][3]
-    uint32_t* pt;       // [R][NC][nlev]
-    uint32_t* scan;     // [kBlock + 64]
-    uint32_t* tot;      // [2][kMaxLevels]
-    uint32_t* queue;    // [queue_cap]
-    uint32_t nrb, ncp;
+    uint64_t* tri;      // [256] case table
+    LevelInfo* lev;     // [kMaxLevels]
+    uint64_t* bits;     // [3][nlev][nrb][ncp]
+    uint32_t* wpv;      // [3][nlev][R+2][NC]
+    uint64_t* act;      // [nlev][R][NC]
+    uint32_t* pt;       // [nlev][R][NC]
+    uint32_t* misc;     // [kMiscWords]
+    float* cache;       // [cache_cap][3]
+    uint32_t* vq;       // [queue_cap]
+    uint32_t* tq;       // [queue_cap]
+    uint32_t nrb, ncp, R, NC, nlev;
 };
 
 __device__ __forceinline__ Views make_views(const JobArgs& a, uint8_t* smem)
 {
-    const LdsLayout L = lds_layout(a.R, a.NC, a.nlev, a.queue_cap);
+    const LdsLayout L = lds_layout(a.R, a.NC, a.nlev, a.queue_cap, a.cache_cap);
     Views v;
+    v.tri = (uint64_t*)(smem + L.off_tri);
+    v.lev = (LevelInfo*)(smem + L.off_lev);
     v.bits = (uint64_t*)(smem + L.off_bits);
-    v.pv = (uint32_t*)(smem + L.off_pv);
-    v.tm = (uint64_t*)(smem + L.off_tm);
+    v.wpv = (uint32_t*)(smem + L.off_wpv);
+    v.act = (uint64_t*)(smem + L.off_act);
     v.pt = (uint32_t*)(smem + L.off_pt);
-    v.scan = (uint32_t*)(smem + L.off_scan);
-    v.tot = (uint32_t*)(smem + L.off_tot);
-    v.queue = (uint32_t*)(smem + L.off_queue);
+    v.misc = (uint32_t*)(smem + L.off_misc);
+    v.cache = (float*)(smem + L.off_cache);
+    v.vq = (uint32_t*)(smem + L.off_vq);
+    v.tq = (uint32_t*)(smem + L.off_tq);
     v.nrb = L.nrb;
     v.ncp = L.ncp;
+    v.R = a.R;
+    v.NC = a.NC;
+    v.nlev = a.nlev;
     return v;
 }
 
-/* Ring slot of node slice s (s >= -1). */
+/* Ring slot of node slice s (s >= -1): any three consecutive slices get
+ * distinct slots. */
 __device__ __forceinline__ uint32_t bslot(int s) { return (uint32_t)(s + 1) % 3u; }
-__device__ __forceinline__ uint32_t pslot(int s) { return (uint32_t)(s + 1) & 1u; }
 
-/* Chunk mask of slice-slot sl, LDS row r (row y = y0 - 1 + r), chunk c. */
-__device__ __forceinline__ uint64_t B(const JobArgs& a, const Views& v, uint32_t sl, uint32_t r, uint32_t c, uint32_t l)
+/* Row of chunk masks: slice slot sl, level l, LDS row r (row y = y0 - 1 + r). */
+__device__ __forceinline__ uint64_t* brow(const Views& v, uint32_t sl, uint32_t l, uint32_t r)
 {
-    return v.bits[((sl * v.nrb + r) * v.ncp + c) * a.nlev + l];
+    return v.bits + ((sl * v.nlev + l) * v.nrb + r) * v.ncp;
+}
+/* Window vertex chunk prefixes of window slice ws (0..2), level l, window row w. */
+__device__ __forceinline__ uint32_t* wrow(const Views& v, uint32_t ws, uint32_t l, uint32_t w)
+{
+    return v.wpv + ((ws * v.nlev + l) * (v.R + 2) + w) * v.NC;
 }
 
 /* Mask of the nodes x < limit inside chunk c. */
@@ -177,74 +179,56 @@ __device__ __forceinline__ uint64_t valid_mask(uint32_t c, uint32_t limit)
     return n >= 64u ? ~0ull : ((1ull << n) - 1ull);
 }
 
-/* Bits of chunk c shifted so that bit i holds node x+1. */
-__device__ __forceinline__ uint64_t shift_next(const JobArgs& a, const Views& v, uint32_t sl, uint32_t r, uint32_t c, uint32_t l)
+/* Bits of chunk c of a row of NC chunks shifted so that bit i holds node x+1. */
+__device__ __forceinline__ uint64_t shift_next(const uint64_t* row, uint32_t c, uint32_t NC)
 {
-    return (B(a, v, sl, r, c, l) >> 1) | (B(a, v, sl, r, c + 1, l) << 63);
+    return (row[c] >> 1) | ((c + 1 < NC ? row[c + 1] : 0ull) << 63);
 }
 
-struct Crossings { uint64_t x, y, z; };
+struct Cross { uint64_t x, y, z; };
 
-/* Crossing edges owned by the nodes of row y (LDS row r) of slice z in chunk c. */
-__device__ __forceinline__ Crossings crossings(const JobArgs& a, const Views& v, int z, int y,
-                                               uint32_t r, uint32_t c, uint32_t l)
+/* Crossing edges owned by the nodes of chunk c of window row w (node row
+ * y0-1+w) of window slice ws (node slice z-1+ws).  Slice z+1's z-edges are
+ * outside the window (their far end is slice z+2) and count as none. */
+__device__ __forceinline__ Cross window_cross(const JobArgs& a, const Views& v, int z, int y0,
+                                              uint32_t ws, uint32_t w, uint32_t c, uint32_t l)
 {
-    Crossings k;
-    const uint32_t s0 = bslot(z);
-    const uint64_t b = B(a, v, s0, r, c, l);
-    k.x = (b ^ shift_next(a, v, s0, r, c, l)) & valid_mask(c, a.nx - 1);
-    k.y = (y + 1 < (int)a.ny) ? ((b ^ B(a, v, s0, r + 1, c, l)) & valid_mask(c, a.nx)) : 0ull;
-    k.z = (z + 1 < (int)a.nz) ? ((b ^ B(a, v, bslot(z + 1), r, c, l)) & valid_mask(c, a.nx)) : 0ull;
+    Cross k;
+    const int s = z - 1 + (int)ws;
+    const int y = y0 - 1 + (int)w;
+    if (s < 0 || s >= (int)a.nz || y < 0 || y >= (int)a.ny) { k.x = k.y = k.z = 0; return k; }
+    const uint64_t* row = brow(v, bslot(s), l, w);
+    const uint64_t b = row[c];
+    k.x = (b ^ shift_next(row, c, a.NC)) & valid_mask(c, a.nx - 1);
+    k.y = (y + 1 < (int)a.ny) ? ((b ^ brow(v, bslot(s), l, w + 1)[c]) & valid_mask(c, a.nx)) : 0ull;
+    k.z = (ws < 2 && s + 1 < (int)a.nz) ? ((b ^ brow(v, bslot(s + 1), l, w)[c]) & valid_mask(c, a.nx)) : 0ull;
     return k;
 }
 
-/* Active-cell mask of the cells of row y (LDS row r) at cell slice zc, chunk c. */
-__device__ __forceinline__ uint64_t active_cells(const JobArgs& a, const Views& v, int zc,
-                                                 uint32_t r, uint32_t c, uint32_t l, uint64_t corner[8])
+/* Cube index of cell (cx, cy, cz) from the bit rows (y0: first band row). */
+__device__ __forceinline__ uint32_t cube_index_at(const Views& v, int y0, uint32_t cx, int cy, int cz, uint32_t l)
 {
-    const uint32_t s0 = bslot(zc), s1 = bslot(zc + 1);
-    corner[0] = B(a, v, s0, r, c, l);
-    corner[1] = shift_next(a, v, s0, r, c, l);
-    corner[3] = B(a, v, s0, r + 1, c, l);
-    corner[2] = shift_next(a, v, s0, r + 1, c, l);
-    corner[4] = B(a, v, s1, r, c, l);
-    corner[5] = shift_next(a, v, s1, r, c, l);
-    corner[7] = B(a, v, s1, r + 1, c, l);
-    corner[6] = shift_next(a, v, s1, r + 1, c, l);
-    uint64_t o = 0, n = ~0ull;
-#pragma unroll
-    for (int k = 0; k < 8; ++k) { o |= corner[k]; n &= corner[k]; }
-    return (o & ~n) & valid_mask(c, a.nx - 1);
-}
-
-__device__ __forceinline__ uint32_t cube_bits(const uint64_t corner[8], uint32_t i)
-{
+    const uint32_t r = (uint32_t)(cy - y0 + 1);
+    const uint64_t* b00 = brow(v, bslot(cz), l, r);
+    const uint64_t* b01 = brow(v, bslot(cz), l, r + 1);
+    const uint64_t* b10 = brow(v, bslot(cz + 1), l, r);
+    const uint64_t* b11 = brow(v, bslot(cz + 1), l, r + 1);
+    const uint32_t c0 = cx >> 6, i0 = cx & 63u;
+    const uint32_t c1 = (cx + 1) >> 6, i1 = (cx + 1) & 63u;
     uint32_t ci = 0;
-#pragma unroll
-    for (int k = 0; k < 8; ++k) ci |= (uint32_t)((corner[k] >> i) & 1ull) << k;
-    return ci;
-}
-
-/* Cube index of cell (cx, cy, cz) from the bit ring (y0: first band row). */
-__device__ __forceinline__ uint32_t cube_index_at(const JobArgs& a, const Views& v, int y0,
-                                                  uint32_t cx, int cy, int cz, uint32_t l)
-{
-    uint32_t ci = 0;
-#pragma unroll
-    for (int k = 0; k < 8; ++k) {
-        const uint32_t dx = (k == 1 || k == 2 || k == 5 || k == 6) ? 1u : 0u;
-        const uint32_t dy = (k == 2 || k == 3 || k == 6 || k == 7) ? 1u : 0u;
-        const uint32_t dz = (k >= 4) ? 1u : 0u;
-        const uint32_t x = cx + dx;
-        const uint32_t r = (uint32_t)(cy + (int)dy - y0 + 1);
-        const uint64_t m = B(a, v, bslot(cz + (int)dz), r, x >> 6, l);
-        ci |= (uint32_t)((m >> (x & 63u)) & 1ull) << k;
-    }
+    ci |= (uint32_t)((b00[c0] >> i0) & 1ull) << 0;
+    ci |= (uint32_t)((b00[c1] >> i1) & 1ull) << 1;
+    ci |= (uint32_t)((b01[c1] >> i1) & 1ull) << 2;
+    ci |= (uint32_t)((b01[c0] >> i0) & 1ull) << 3;
+    ci |= (uint32_t)((b10[c0] >> i0) & 1ull) << 4;
+    ci |= (uint32_t)((b10[c1] >> i1) & 1ull) << 5;
+    ci |= (uint32_t)((b11[c1] >> i1) & 1ull) << 6;
+    ci |= (uint32_t)((b11[c0] >> i0) & 1ull) << 7;
     return ci;
 }
 
 /* ------------------------------------------------------------------------- */
-/* block scan of up to kMaxLevels u32 lanes                                    */
+/* wave scan                                                                   */
 /* ------------------------------------------------------------------------- */
 
 __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x)
@@ -258,220 +242,181 @@ __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x)
     return x;
 }
 
-/* Exclusive scan across the workgroup of val[l] (l < nlev); returns the
- * exclusive prefix in val and the totals in total.  Uses sh[kMaxLevels*kWaves]. */
-__device__ __forceinline__ void block_scan(uint32_t nlev, uint32_t val[kMaxLevels], uint32_t total[kMaxLevels], uint32_t* sh)
+/* In-place exclusive scan of n u32 values by one wave; returns the total. */
+__device__ __forceinline__ uint32_t wave_exclusive_scan(uint32_t* p, uint32_t n)
 {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    uint32_t incl[kMaxLevels];
-#pragma unroll
-    for (int l = 0; l < kMaxLevels; ++l) {
-        incl[l] = 0;
-        if (l < (int)nlev) {
-            incl[l] = wave_incl_scan(val[l]);
-            if (lane == 63) sh[l * kWaves + wave] = incl[l];
-        }
+    const uint32_t lane = threadIdx.x & 63u;
+    uint32_t carry = 0;
+    for (uint32_t g = 0; g < n; g += 64) {
+        const uint32_t i = g + lane;
+        const uint32_t x = i < n ? p[i] : 0u;
+        const uint32_t inc = wave_incl_scan(x);
+        if (i < n) p[i] = carry + inc - x;
+        carry += __shfl(inc, 63);
     }
-    __syncthreads();
-#pragma unroll
-    for (int l = 0; l < kMaxLevels; ++l) {
-        if (l < (int)nlev) {
-            uint32_t before = 0, all = 0;
-#pragma unroll
-            for (int w = 0; w < kWaves; ++w) {
-                const uint32_t t = sh[l * kWaves + w];
-                if (w < wave) before += t;
-                all += t;
-            }
-            val[l] = before + incl[l] - val[l];
-            total[l] = all;
-        }
-    }
-    __syncthreads();
+    return carry;
 }
 
 /* ------------------------------------------------------------------------- */
 /* slice loading                                                               */
 /* ------------------------------------------------------------------------- */
 
-/* Classify node rows y0-1 .. y0+R+1 of slice s into chunk masks of its ring
- * slot.  Nodes outside the volume classify as all-zero. */
-template <typename T>
-__device__ void load_slice(const JobArgs& a, const Views& v, const T* vol, int s, int y0)
+/* Bring the chunk masks of node rows y0-1 .. y0+R+1 of slices z-1, z, z+1
+ * from the bit volume into LDS (slot bslot(s) of each slice); rows and
+ * slices outside the volume are zero.  For one (slice, level) the rows are
+ * contiguous both in the bit volume and in LDS, so the copy is a set of flat
+ * ranges; loads are issued in batches before their stores. */
+__device__ void load_window(const JobArgs& a, const Views& v, int z, int y0)
 {
-    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
-    const uint32_t sl = bslot(s);
-    const uint32_t total = v.nrb * a.NC;
-    const bool slice_ok = s >= 0 && s < (int)a.nz;
-    for (uint32_t j = wave; j < total; j += kWaves) {
-        const uint32_t r = j / a.NC, c = j - r * a.NC;
-        const int y = y0 - 1 + (int)r;
-        const uint32_t x = c * 64u + lane;
-        uint32_t bits = 0;
-        if (slice_ok && y >= 0 && y < (int)a.ny && x < a.nx) {
-            const T val = vol[((uint64_t)s * a.ny + (uint64_t)y) * a.nx + x];
-            bits = Classify<T>::bits(val, a);
-        }
-        uint64_t* dst = v.bits + ((sl * v.nrb + r) * v.ncp + c) * a.nlev;
+    const uint32_t nseg = v.nrb * a.NC;              // elements of one (slice, level)
+    const uint32_t ng = 3u * a.nlev;
+    /* valid element range of every segment: rows y0-1 .. y0+R+1 clipped to the volume */
+    const uint32_t e_lo = (y0 - 1 < 0) ? a.NC : 0u;
+    const int rows_left = (int)a.ny - (y0 - 1);
+    const uint32_t e_hi = (rows_left >= (int)v.nrb ? v.nrb : (uint32_t)(rows_left > 0 ? rows_left : 0)) * a.NC;
+    uint32_t g = 0, e = threadIdx.x;
+    while (g < ng && e >= nseg) { ++g; e = threadIdx.x; }
+    while (g < ng) {
+        constexpr int kB = 8;
+        uint64_t val[kB];
+        uint32_t dst[kB];
+        int cnt = 0;
 #pragma unroll
-        for (int l = 0; l < kMaxLevels; ++l) {
-            if (l < (int)a.nlev) {
-                const uint64_t m = __ballot((bits >> l) & 1u);
-                if (lane == 0) dst[l] = m;
+        for (int u = 0; u < kB; ++u) {
+            if (g < ng) {
+                const uint32_t ws = g / a.nlev, l = g - ws * a.nlev;
+                const int s = z - 1 + (int)ws;
+                const bool ok = s >= 0 && s < (int)a.nz && e >= e_lo && e < e_hi;
+                /* element e of the segment is node row y0-1 + e / NC of slice s */
+                const uint64_t idx = ok ? (((uint64_t)l * a.nz + (uint64_t)s) * a.ny + (uint64_t)(y0 - 1)) * a.NC + e
+                                        : 0ull;
+                val[u] = ok ? a.bitvol[idx] : 0ull;
+                dst[u] = ((bslot(s) * a.nlev + l) * v.nrb) * a.NC + e;
+                cnt = u + 1;
+                e += kBlock;
+                while (g < ng && e >= nseg) { ++g; e = threadIdx.x; }
             }
         }
-    }
-    for (uint32_t j = threadIdx.x; j < v.nrb * a.nlev; j += kBlock) {
-        const uint32_t r = j / a.nlev, l = j - r * a.nlev;
-        v.bits[((sl * v.nrb + r) * v.ncp + a.NC) * a.nlev + l] = 0ull;
+#pragma unroll
+        for (int u = 0; u < kB; ++u)
+            if (u < cnt) v.bits[dst[u]] = val[u];
     }
 }
 
 /* ------------------------------------------------------------------------- */
-/* per-tile prefixes                                                           */
+/* per-tile counts and prefixes (one wave per row, one lane per chunk)         */
 /* ------------------------------------------------------------------------- */
 
-/* Vertex chunk prefixes of node slice z for band rows 0..R-1 (relative to
- * tile (z, b)) and for row R (relative to tile (z, b+1)); band totals go to
- * tot[0][l].  Writes pv slot of z. */
-__device__ void vertex_prefixes(const JobArgs& a, const Views& v, int z, int y0)
+/* Window vertex counts per chunk for window slices 0..2 and rows 0..R+1 (not
+ * needed with duplication), and active cells / triangle counts of the band's
+ * cells at cell slice z-1; then exclusive prefixes: window prefixes per
+ * (slice, level), triangle prefixes per level.  Fills misc with the tile
+ * totals and the cache offsets.  Lanes cover 64 >> cshift rows of
+ * 1 << cshift chunks each. */
+__device__ void tile_prefixes(const JobArgs& a, const Views& v, int z, int y0)
 {
-    uint32_t* pv = v.pv + pslot(z) * (a.R + 1) * a.NC * a.nlev;
-    const uint32_t nall = (a.R + 1) * a.NC;
-    for (uint32_t j = threadIdx.x; j < nall; j += kBlock) {
-        const uint32_t k = j / a.NC, c = j - k * a.NC;
-        const int y = y0 + (int)k;
-        for (uint32_t l = 0; l < a.nlev; ++l) {
-            uint32_t cnt = 0;
-            if (y < (int)a.ny) {
-                const Crossings e = crossings(a, v, z, y, k + 1, c, l);
-                cnt = __popcll(e.x) + __popcll(e.y) + __popcll(e.z);
-            }
-            pv[j * a.nlev + l] = cnt;
-        }
-    }
-    __syncthreads();
-    /* two segments: band rows [0, R*NC) and row R [R*NC, (R+1)*NC) */
-    for (int seg = 0; seg < 2; ++seg) {
-        const uint32_t j0 = seg == 0 ? 0u : a.R * a.NC;
-        const uint32_t nj = seg == 0 ? a.R * a.NC : a.NC;
-        const uint32_t q = (nj + kBlock - 1) / kBlock;
-        const uint32_t jb = j0 + threadIdx.x * q;
-        uint32_t val[kMaxLevels], total[kMaxLevels];
-#pragma unroll
-        for (int l = 0; l < kMaxLevels; ++l) { val[l] = 0; total[l] = 0; }
-        for (uint32_t t = 0; t < q; ++t) {
-            const uint32_t j = jb + t;
-            if (j < j0 + nj) {
-#pragma unroll
-                for (int l = 0; l < kMaxLevels; ++l)
-                    if (l < (int)a.nlev) val[l] += pv[j * a.nlev + l];
-            }
-        }
-        block_scan(a.nlev, val, total, v.scan);
-        for (uint32_t t = 0; t < q; ++t) {
-            const uint32_t j = jb + t;
-            if (j < j0 + nj) {
-#pragma unroll
-                for (int l = 0; l < kMaxLevels; ++l) {
-                    if (l < (int)a.nlev) {
-                        const uint32_t c = pv[j * a.nlev + l];
-                        pv[j * a.nlev + l] = val[l];
-                        val[l] += c;
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    const uint32_t sub = lane >> a.cshift, cl = lane & ((1u << a.cshift) - 1u);
+    const uint32_t G = 64u >> a.cshift, CS = 1u << a.cshift;
+    const uint32_t nwr = a.R + 2;
+    /* Without the cache only the tile ranks are needed: window slices 0, 1
+     * (node slices z-1, z) and window rows 1..R+1. */
+    const uint32_t nws = a.use_cache ? 3u : 2u;
+    const uint32_t wfirst = a.use_cache ? 0u : 1u;
+    if (!a.dup) {
+        for (uint32_t ws = 0; ws < nws; ++ws) {
+            for (uint32_t l = 0; l < a.nlev; ++l) {
+                for (uint32_t w0 = wfirst + wave * G; w0 < nwr; w0 += kWaves * G) {
+                    const uint32_t w = w0 + sub;
+                    if (w >= nwr) continue;
+                    uint32_t* dst = wrow(v, ws, l, w);
+                    for (uint32_t c = cl; c < a.NC; c += CS) {
+                        const Cross k = window_cross(a, v, z, y0, ws, w, c, l);
+                        dst[c] = (uint32_t)(__popcll(k.x) + __popcll(k.y) + __popcll(k.z));
                     }
                 }
             }
         }
-        if (seg == 0 && threadIdx.x == 0) {
-#pragma unroll
-            for (int l = 0; l < kMaxLevels; ++l)
-                if (l < (int)a.nlev) v.tot[l] = total[l];
-        }
-        __syncthreads();
     }
-}
-
-/* Triangle counts of the cells of band rows at cell slice zc = z - 1: bit
- * planes of the per-cell counts into tm, chunk prefixes into pt, totals into
- * tot[1][l]. */
-__device__ void triangle_prefixes(const JobArgs& a, const Views& v, int z, int y0)
-{
-    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
     const int zc = z - 1;
-    const uint32_t nj = a.R * a.NC;
-    for (uint32_t j = wave; j < nj; j += kWaves) {
-        const uint32_t k = j / a.NC, c = j - k * a.NC;
-        const int y = y0 + (int)k;
-        const bool row_ok = zc >= 0 && y + 1 < (int)a.ny;
-        for (uint32_t l = 0; l < a.nlev; ++l) {
-            uint32_t nt = 0;
-            if (row_ok) {
-                uint64_t corner[8];
-                const uint64_t act = active_cells(a, v, zc, k + 1, c, l, corner);
-                if ((act >> lane) & 1ull) nt = (uint32_t)(d_tri[cube_bits(corner, lane)] >> 60);
-            }
-            const uint64_t m0 = __ballot(nt & 1u), m1 = __ballot((nt >> 1) & 1u), m2 = __ballot((nt >> 2) & 1u);
-            if (lane == 0) {
-                uint64_t* t = v.tm + (j * a.nlev + l) * 3;
-                t[0] = m0; t[1] = m1; t[2] = m2;
-                v.pt[j * a.nlev + l] = __popcll(m0) + 2u * __popcll(m1) + 4u * __popcll(m2);
-            }
-        }
-    }
-    __syncthreads();
-    const uint32_t q = (nj + kBlock - 1) / kBlock;
-    const uint32_t jb = threadIdx.x * q;
-    uint32_t val[kMaxLevels], total[kMaxLevels];
-#pragma unroll
-    for (int l = 0; l < kMaxLevels; ++l) { val[l] = 0; total[l] = 0; }
-    for (uint32_t t = 0; t < q; ++t) {
-        const uint32_t j = jb + t;
-        if (j < nj) {
-#pragma unroll
-            for (int l = 0; l < kMaxLevels; ++l)
-                if (l < (int)a.nlev) val[l] += v.pt[j * a.nlev + l];
-        }
-    }
-    block_scan(a.nlev, val, total, v.scan);
-    for (uint32_t t = 0; t < q; ++t) {
-        const uint32_t j = jb + t;
-        if (j < nj) {
-#pragma unroll
-            for (int l = 0; l < kMaxLevels; ++l) {
-                if (l < (int)a.nlev) {
-                    const uint32_t c = v.pt[j * a.nlev + l];
-                    v.pt[j * a.nlev + l] = val[l];
-                    val[l] += c;
+    for (uint32_t l = 0; l < a.nlev; ++l) {
+        for (uint32_t k0 = wave * G; k0 < a.R; k0 += kWaves * G) {
+            const uint32_t k = k0 + sub;
+            if (k >= a.R) continue;
+            const int y = y0 + (int)k;
+            for (uint32_t c = cl; c < a.NC; c += CS) {
+                uint64_t act = 0;
+                uint32_t cnt = 0;
+                if (zc >= 0 && y + 1 < (int)a.ny) {
+                    const uint64_t* r0 = brow(v, bslot(zc), l, k + 1);
+                    const uint64_t* r1 = brow(v, bslot(zc), l, k + 2);
+                    const uint64_t* r2 = brow(v, bslot(z), l, k + 1);
+                    const uint64_t* r3 = brow(v, bslot(z), l, k + 2);
+                    const uint64_t c0 = r0[c], c1 = shift_next(r0, c, a.NC), c3 = r1[c], c2 = shift_next(r1, c, a.NC);
+                    const uint64_t c4 = r2[c], c5 = shift_next(r2, c, a.NC), c7 = r3[c], c6 = shift_next(r3, c, a.NC);
+                    const uint64_t o = c0 | c1 | c2 | c3 | c4 | c5 | c6 | c7;
+                    const uint64_t an = c0 & c1 & c2 & c3 & c4 & c5 & c6 & c7;
+                    act = (o & ~an) & valid_mask(c, a.nx - 1);
+                    uint64_t m = act;
+                    while (m) {
+                        const uint32_t i = (uint32_t)__ffsll((unsigned long long)m) - 1u;
+                        m &= m - 1ull;
+                        const uint32_t ci = (uint32_t)((c0 >> i) & 1ull) | (uint32_t)((c1 >> i) & 1ull) << 1 |
+                                            (uint32_t)((c2 >> i) & 1ull) << 2 | (uint32_t)((c3 >> i) & 1ull) << 3 |
+                                            (uint32_t)((c4 >> i) & 1ull) << 4 | (uint32_t)((c5 >> i) & 1ull) << 5 |
+                                            (uint32_t)((c6 >> i) & 1ull) << 6 | (uint32_t)((c7 >> i) & 1ull) << 7;
+                        cnt += (uint32_t)(v.tri[ci] >> 60);
+                    }
                 }
+                v.act[(l * a.R + k) * a.NC + c] = act;
+                v.pt[(l * a.R + k) * a.NC + c] = cnt;
             }
         }
     }
+    __syncthreads();
+    /* one wave per sequence */
+    const uint32_t nwseq = a.dup ? 0u : nws * a.nlev;
+    const uint32_t nseq = nwseq + a.nlev;
+    for (uint32_t q = wave; q < nseq; q += kWaves) {
+        if (q < nwseq) {
+            uint32_t* base = wrow(v, q / a.nlev, q % a.nlev, wfirst);
+            const uint32_t tot = wave_exclusive_scan(base, (nwr - wfirst) * a.NC);
+            if (lane == 0) v.misc[kMiscWOff + q] = tot;   // total for now; offsets below
+        } else {
+            const uint32_t l = q - nwseq;
+            const uint32_t tot = wave_exclusive_scan(v.pt + l * a.R * a.NC, a.R * a.NC);
+            if (lane == 0) v.misc[kMiscTTot + l] = tot;
+        }
+    }
+    __syncthreads();
     if (threadIdx.x == 0) {
-#pragma unroll
-        for (int l = 0; l < kMaxLevels; ++l)
-            if (l < (int)a.nlev) v.tot[kMaxLevels + l] = total[l];
+        uint32_t off = 0;
+        for (uint32_t q = 0; q < nwseq; ++q) {
+            const uint32_t t = v.misc[kMiscWOff + q];
+            v.misc[kMiscWOff + q] = off;
+            off += t;
+        }
+        v.misc[kMiscWTotal] = off;
+        v.misc[kMiscOverflow] = (a.dup || !a.use_cache || off > a.cache_cap) ? 1u : 0u;
+        uint32_t ov = 0, ot = 0;
+        for (uint32_t l = 0; l < a.nlev; ++l) {
+            /* tile vertices: window slice 1, window rows 1..R */
+            const uint32_t vt = a.dup ? 0u : (wrow(v, 1, l, a.R + 1)[0] - wrow(v, 1, l, 1)[0]);
+            v.misc[kMiscVTot + l] = vt;
+            v.misc[kMiscQV + l] = ov;
+            v.misc[kMiscQT + l] = ot;
+            ov += vt;
+            ot += v.misc[kMiscTTot + l];
+        }
+        v.misc[kMiscQV + a.nlev] = ov;
+        v.misc[kMiscQT + a.nlev] = ot;
     }
     __syncthreads();
 }
 
 /* ------------------------------------------------------------------------- */
-/* emission                                                                    */
+/* vertex coordinates                                                          */
 /* ------------------------------------------------------------------------- */
-
-/* Rank of the vertex of (node x in band row k (0..R) of slice s, axis)
- * within its tile. */
-__device__ __forceinline__ uint32_t vertex_rank(const JobArgs& a, const Views& v, int s, int y0,
-                                                uint32_t x, uint32_t k, uint32_t axis, uint32_t l)
-{
-    const uint32_t c = x >> 6, i = x & 63u;
-    const Crossings e = crossings(a, v, s, y0 + (int)k, k + 1, c, l);
-    const uint64_t lo = (1ull << i) - 1ull;
-    uint32_t r = v.pv[((pslot(s) * (a.R + 1) + k) * a.NC + c) * a.nlev + l];
-    r += __popcll(e.x & lo) + __popcll(e.y & lo) + __popcll(e.z & lo);
-    if (axis >= 1) r += (uint32_t)((e.x >> i) & 1ull);
-    if (axis >= 2) r += (uint32_t)((e.y >> i) & 1ull);
-    return r;
-}
 
 /* Coordinates of the shared vertex on the edge of node (x,y,z) along axis,
  * interpolated low -> high as calculate_isopoints does (:406,420,434). */
@@ -482,13 +427,112 @@ __device__ __forceinline__ void node_edge_coords(const JobArgs& a, const T* vol,
     interpolate<T>(a, vol, iso, x, y, z, x + (axis == 0), y + (axis == 1), z + (axis == 2), out);
 }
 
-/* Coordinates of local edge e of cell (cx,cy,cz) as a shared vertex. */
-template <typename T>
-__device__ __forceinline__ void cell_edge_coords(const JobArgs& a, const T* vol, double iso,
-                                                 uint32_t cx, uint32_t cy, uint32_t cz, uint32_t e, float out[3])
+/* Owner node offset (dx, dy, dz) and axis of local edge e, packed as
+ * dx | dy << 1 | dz << 2 | axis << 3 (connect_isopoints, :493-504). */
+__device__ __forceinline__ uint32_t edge_owner(uint32_t e)
 {
-    node_edge_coords<T>(a, vol, iso, cx + d_edge_node[e][0], cy + d_edge_node[e][1], cz + d_edge_node[e][2],
-                        d_edge_node[e][3], out);
+    switch (e) {
+    case 0: return 0x00u;                 // (0,0,0) x
+    case 1: return 0x01u | (1u << 3);     // (1,0,0) y
+    case 2: return 0x02u;                 // (0,1,0) x
+    case 3: return 0x00u | (1u << 3);     // (0,0,0) y
+    case 4: return 0x04u;                 // (0,0,1) x
+    case 5: return 0x05u | (1u << 3);     // (1,0,1) y
+    case 6: return 0x06u;                 // (0,1,1) x
+    case 7: return 0x04u | (1u << 3);     // (0,0,1) y
+    case 8: return 0x00u | (2u << 3);     // (0,0,0) z
+    case 9: return 0x01u | (2u << 3);     // (1,0,0) z
+    case 10: return 0x03u | (2u << 3);    // (1,1,0) z
+    default: return 0x02u | (2u << 3);    // (0,1,0) z
+    }
+}
+
+/* Table-orientation endpoints of local edge e as two corner offsets packed
+ * bit0 x, bit1 y, bit2 z, first endpoint in the low nibble
+ * (MarchingCubesTable.cpp:276-290). */
+__device__ __forceinline__ uint32_t edge_ends(uint32_t e)
+{
+    switch (e) {
+    case 0: return 0x10u; case 1: return 0x31u; case 2: return 0x23u; case 3: return 0x02u;
+    case 4: return 0x54u; case 5: return 0x75u; case 6: return 0x67u; case 7: return 0x46u;
+    case 8: return 0x40u; case 9: return 0x51u; case 10: return 0x73u; default: return 0x62u;
+    }
+}
+
+/* Index in the window cache of the vertex on (node x of node row yy of node
+ * slice zz, axis); the node must lie in the window. */
+__device__ __forceinline__ uint32_t cache_index(const JobArgs& a, const Views& v, int z, int y0,
+                                                uint32_t x, int yy, int zz, uint32_t axis, uint32_t l)
+{
+    const uint32_t ws = (uint32_t)(zz - z + 1), w = (uint32_t)(yy - y0 + 1);
+    const uint32_t c = x >> 6, i = x & 63u;
+    const Cross k = window_cross(a, v, z, y0, ws, w, c, l);
+    const uint64_t lo = (1ull << i) - 1ull;
+    uint32_t r = v.misc[kMiscWOff + ws * a.nlev + l] + wrow(v, ws, l, w)[c];
+    r += __popcll(k.x & lo) + __popcll(k.y & lo) + __popcll(k.z & lo);
+    if (axis >= 1) r += (uint32_t)((k.x >> i) & 1ull);
+    if (axis >= 2) r += (uint32_t)((k.y >> i) & 1ull);
+    return r;
+}
+
+/* Coordinates of the shared vertex on (x, yy, zz, axis): from the window
+ * cache, or interpolated when the window overflowed the cache. */
+template <typename T>
+__device__ __forceinline__ void vertex_coords(const JobArgs& a, const Views& v, const T* vol, double iso, int z, int y0,
+                                              uint32_t x, int yy, int zz, uint32_t axis, uint32_t l, float out[3])
+{
+    if (!v.misc[kMiscOverflow]) {
+        const float* p = v.cache + 3u * cache_index(a, v, z, y0, x, yy, zz, axis, l);
+        out[0] = p[0]; out[1] = p[1]; out[2] = p[2];
+    } else {
+        node_edge_coords<T>(a, vol, iso, x, (uint32_t)yy, (uint32_t)zz, axis, out);
+    }
+}
+
+/* Fill the window cache: every crossing edge of the window is interpolated
+ * once, one thread per vertex. */
+template <typename T>
+__device__ void fill_cache(const JobArgs& a, const Views& v, const T* vol, int z, int y0)
+{
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    const uint32_t sub = lane >> a.cshift, cl = lane & ((1u << a.cshift) - 1u);
+    const uint32_t G = 64u >> a.cshift, CS = 1u << a.cshift;
+    const uint32_t nwr = a.R + 2;
+    for (uint32_t ws = 0; ws < 3; ++ws) {
+        for (uint32_t l = 0; l < a.nlev; ++l) {
+            const uint32_t off = v.misc[kMiscWOff + ws * a.nlev + l];
+            for (uint32_t w0 = wave * G; w0 < nwr; w0 += kWaves * G) {
+                const uint32_t w = w0 + sub;
+                if (w >= nwr) continue;
+                const uint32_t* pre = wrow(v, ws, l, w);
+                for (uint32_t c = cl; c < a.NC; c += CS) {
+                    const Cross k = window_cross(a, v, z, y0, ws, w, c, l);
+                    uint64_t m = k.x | k.y | k.z;
+                    uint32_t r = off + pre[c];
+                    while (m) {
+                        const uint32_t i = (uint32_t)__ffsll((unsigned long long)m) - 1u;
+                        m &= m - 1ull;
+                        const uint32_t base = (c * 64u + i) | (w << 20) | (ws << 25) | (l << 27);
+                        if ((k.x >> i) & 1ull) v.vq[r++] = base;
+                        if ((k.y >> i) & 1ull) v.vq[r++] = base | (1u << 30);
+                        if ((k.z >> i) & 1ull) v.vq[r++] = base | (2u << 30);
+                    }
+                }
+            }
+        }
+    }
+    __syncthreads();
+    const uint32_t n = v.misc[kMiscWTotal];
+    for (uint32_t q = threadIdx.x; q < n; q += kBlock) {
+        const uint32_t e = v.vq[q];
+        const uint32_t x = e & 0xFFFFFu, w = (e >> 20) & 31u, ws = (e >> 25) & 3u, l = (e >> 27) & 7u, axis = e >> 30;
+        float p[3];
+        node_edge_coords<T>(a, vol, v.lev[l].iso, x, (uint32_t)(y0 - 1 + (int)w), (uint32_t)(z - 1 + (int)ws), axis, p);
+        v.cache[3 * q + 0] = p[0];
+        v.cache[3 * q + 1] = p[1];
+        v.cache[3 * q + 2] = p[2];
+    }
+    __syncthreads();
 }
 
 /* VertexNormal of the vertex on (x,y,z,axis): the float sum, in triangle order,
@@ -496,40 +540,39 @@ __device__ __forceinline__ void cell_edge_coords(const JobArgs& a, const T* vol,
  * calculate_normals_on_vertex (:569-605).  Incident cells are visited in cell
  * scan order and triangles in table order. */
 template <typename T>
-__device__ void vertex_normal(const JobArgs& a, const Views& v, const T* vol, double iso, int y0,
-                              uint32_t x, uint32_t y, uint32_t z, uint32_t axis, uint32_t l, float n[3])
+__device__ void vertex_normal(const JobArgs& a, const Views& v, const T* vol, double iso, int z, int y0,
+                              uint32_t x, uint32_t y, uint32_t axis, uint32_t l, float n[3])
 {
     n[0] = 0.0f; n[1] = 0.0f; n[2] = 0.0f;
-    /* (dx, dy, dz, local edge) of the four cells around the edge, scan order */
-    int cd[4][4];
-    if (axis == 0) {
-        const int t[4][4] = { {0, -1, -1, 6}, {0, 0, -1, 4}, {0, -1, 0, 2}, {0, 0, 0, 0} };
-        for (int i = 0; i < 4; ++i) for (int j = 0; j < 4; ++j) cd[i][j] = t[i][j];
-    } else if (axis == 1) {
-        const int t[4][4] = { {-1, 0, -1, 5}, {0, 0, -1, 7}, {-1, 0, 0, 1}, {0, 0, 0, 3} };
-        for (int i = 0; i < 4; ++i) for (int j = 0; j < 4; ++j) cd[i][j] = t[i][j];
-    } else {
-        const int t[4][4] = { {-1, -1, 0, 10}, {0, -1, 0, 11}, {-1, 0, 0, 9}, {0, 0, 0, 8} };
-        for (int i = 0; i < 4; ++i) for (int j = 0; j < 4; ++j) cd[i][j] = t[i][j];
-    }
+    /* The four cells around the edge in scan order: the two offsets across the
+     * edge are -1/0 along the other two axes (low axis first), and the edge's
+     * local id in each cell is the q-th nibble of `around`. */
+    const uint32_t around = axis == 0 ? 0x0246u : (axis == 1 ? 0x3175u : 0x89BAu);
     for (int q = 0; q < 4; ++q) {
-        const int cx = (int)x + cd[q][0], cy = (int)y + cd[q][1], cz = (int)z + cd[q][2];
+        const int lo = (q & 1) ? 0 : -1, hi = (q & 2) ? 0 : -1;
+        const int cx = (int)x + (axis == 0 ? 0 : lo);
+        const int cy = (int)y + (axis == 0 ? lo : (axis == 1 ? 0 : hi));
+        const int cz = z + (axis == 2 ? 0 : hi);
         if (cx < 0 || cy < 0 || cz < 0) continue;
         if (cx + 1 >= (int)a.nx || cy + 1 >= (int)a.ny || cz + 1 >= (int)a.nz) continue;
-        const uint32_t ci = cube_index_at(a, v, y0, (uint32_t)cx, cy, cz, l);
-        const uint64_t w = d_tri[ci];
+        const uint32_t ci = cube_index_at(v, y0, (uint32_t)cx, cy, cz, l);
+        const uint64_t w = v.tri[ci];
         const int nt = (int)(w >> 60);
-        const uint32_t me = (uint32_t)cd[q][3];
+        const uint32_t me = (around >> (4 * q)) & 0xFu;
         for (int t = 0; t < nt; ++t) {
             const uint32_t e0 = (uint32_t)((w >> (12 * t)) & 0xF);
             const uint32_t e1 = (uint32_t)((w >> (12 * t + 8)) & 0xF);
             const uint32_t e2 = (uint32_t)((w >> (12 * t + 4)) & 0xF);
             if (e0 != me && e1 != me && e2 != me) continue;
-            float p0[3], p1[3], p2[3], fn[3];
-            cell_edge_coords<T>(a, vol, iso, cx, cy, cz, e0, p0);
-            cell_edge_coords<T>(a, vol, iso, cx, cy, cz, e1, p1);
-            cell_edge_coords<T>(a, vol, iso, cx, cy, cz, e2, p2);
-            face_normal(p0, p1, p2, fn);
+            float p[3][3], fn[3];
+            const uint32_t es[3] = { e0, e1, e2 };
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                const uint32_t o = edge_owner(es[j]);
+                vertex_coords<T>(a, v, vol, iso, z, y0, (uint32_t)cx + (o & 1u), cy + (int)((o >> 1) & 1u),
+                                 cz + (int)((o >> 2) & 1u), o >> 3, l, p[j]);
+            }
+            face_normal(p[0], p[1], p[2], fn);
             n[0] = n[0] + fn[0];
             n[1] = n[1] + fn[1];
             n[2] = n[2] + fn[2];
@@ -537,158 +580,169 @@ __device__ void vertex_normal(const JobArgs& a, const Views& v, const T* vol, do
     }
 }
 
-/* Emit the vertices of tile (z, b) for level l. */
-template <typename T>
-__device__ void emit_vertices(const JobArgs& a, const Views& v, const T* vol, int z, int y0,
-                              uint32_t l, uint64_t base)
+/* Rank within its tile of the vertex on (node x of band row k (0..R) of node
+ * slice s in {z-1, z}, axis): rows 0..R-1 rank from the tile's first row,
+ * row R (first row of the next band) from its own start. */
+__device__ __forceinline__ uint32_t tile_rank(const JobArgs& a, const Views& v, int z, int y0,
+                                              uint32_t x, uint32_t k, int s, uint32_t axis, uint32_t l)
 {
-    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
-    const uint32_t nv = v.tot[l];
-    const double iso = iso_of(a, (int)l);
-    const uint32_t nj = a.R * a.NC;
-    for (uint32_t w0 = 0; w0 < nv; w0 += a.queue_cap) {
-        /* fill: queue[rank - w0] = x | k << 20 | axis << 28 */
-        for (uint32_t j = wave; j < nj; j += kWaves) {
-            const uint32_t k = j / a.NC, c = j - k * a.NC;
-            const int y = y0 + (int)k;
-            if (y >= (int)a.ny) continue;
-            const Crossings e = crossings(a, v, z, y, k + 1, c, l);
-            const uint64_t any = e.x | e.y | e.z;
-            if (!((any >> lane) & 1ull)) continue;
-            const uint64_t lo = (1ull << lane) - 1ull;
-            uint32_t r = v.pv[((pslot(z) * (a.R + 1) + k) * a.NC + c) * a.nlev + l]
-                         + __popcll(e.x & lo) + __popcll(e.y & lo) + __popcll(e.z & lo);
-            const uint32_t x = c * 64u + lane;
-            const uint64_t fx = (e.x >> lane) & 1ull, fy = (e.y >> lane) & 1ull, fz = (e.z >> lane) & 1ull;
-            if (fx) { if (r >= w0 && r - w0 < a.queue_cap) v.queue[r - w0] = x | (k << 20) | (0u << 28); ++r; }
-            if (fy) { if (r >= w0 && r - w0 < a.queue_cap) v.queue[r - w0] = x | (k << 20) | (1u << 28); ++r; }
-            if (fz) { if (r >= w0 && r - w0 < a.queue_cap) v.queue[r - w0] = x | (k << 20) | (2u << 28); ++r; }
-        }
-        __syncthreads();
-        const uint32_t nq = (nv - w0) < a.queue_cap ? (nv - w0) : a.queue_cap;
-        for (uint32_t q = threadIdx.x; q < nq; q += kBlock) {
-            const uint32_t e = v.queue[q];
-            const uint32_t x = e & 0xFFFFFu, k = (e >> 20) & 0xFFu, axis = e >> 28;
-            const uint32_t y = (uint32_t)y0 + k;
-            const uint64_t vid = base + w0 + q;
-            float p[3];
-            node_edge_coords<T>(a, vol, iso, x, y, (uint32_t)z, axis, p);
-            float* co = a.coords[0];
-            float* no = a.normals[0];
-#pragma unroll
-            for (int m = 1; m < kMaxLevels; ++m)
-                if (m == (int)l) { co = a.coords[m]; no = a.normals[m]; }
-            uint64_t capv = a.cap_vertices[0];
-#pragma unroll
-            for (int m = 1; m < kMaxLevels; ++m)
-                if (m == (int)l) capv = a.cap_vertices[m];
-            if (vid < capv) {
-                co[3 * vid + 0] = p[0];
-                co[3 * vid + 1] = p[1];
-                co[3 * vid + 2] = p[2];
-                if (a.vnormal) {
-                    float n[3];
-                    vertex_normal<T>(a, v, vol, iso, y0, x, y, (uint32_t)z, axis, l, n);
-                    no[3 * vid + 0] = n[0];
-                    no[3 * vid + 1] = n[1];
-                    no[3 * vid + 2] = n[2];
-                }
-            }
-        }
-        __syncthreads();
-    }
+    const uint32_t ws = (uint32_t)(s - z + 1), w = k + 1;
+    const uint32_t c = x >> 6, i = x & 63u;
+    const Cross e = window_cross(a, v, z, y0, ws, w, c, l);
+    const uint64_t lo = (1ull << i) - 1ull;
+    const uint32_t start = wrow(v, ws, l, k == a.R ? a.R + 1 : 1)[0];
+    uint32_t r = wrow(v, ws, l, w)[c] - start;
+    r += __popcll(e.x & lo) + __popcll(e.y & lo) + __popcll(e.z & lo);
+    if (axis >= 1) r += (uint32_t)((e.x >> i) & 1ull);
+    if (axis >= 2) r += (uint32_t)((e.y >> i) & 1ull);
+    return r;
 }
 
-/* Emit the triangles of the cells of band rows at cell slice z-1, level l. */
+/* Emit tile (z, b): the vertices of node slice z (shared-vertex modes) and the
+ * triangles of cell slice z-1, all levels, through two LDS work queues whose
+ * entry index is the output rank. */
 template <typename T>
-__device__ void emit_triangles(const JobArgs& a, const Views& v, const T* vol, int z, int y0,
-                               uint32_t l, uint64_t tbase, const uint64_t vbase[4])
+__device__ void emit_tile(const JobArgs& a, const Views& v, const T* vol, int z, int y0)
 {
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
-    const uint32_t nt = v.tot[kMaxLevels + l];
-    const double iso = iso_of(a, (int)l);
-    const uint32_t nj = a.R * a.NC;
+    const uint32_t sub = lane >> a.cshift, cl = lane & ((1u << a.cshift) - 1u);
+    const uint32_t G = 64u >> a.cshift, CS = 1u << a.cshift;
+    const uint32_t nv_all = v.misc[kMiscQV + a.nlev];
+    const uint32_t nt_all = v.misc[kMiscQT + a.nlev];
+    const uint32_t Q = a.queue_cap;
     const int zc = z - 1;
-    float* co = a.coords[0];
-    float* no = a.normals[0];
-    uint32_t* cn = a.conn[0];
-    uint64_t capv = a.cap_vertices[0], capt = a.cap_triangles[0];
-#pragma unroll
-    for (int m = 1; m < kMaxLevels; ++m)
-        if (m == (int)l) {
-            co = a.coords[m]; no = a.normals[m]; cn = a.conn[m];
-            capv = a.cap_vertices[m]; capt = a.cap_triangles[m];
-        }
-    for (uint32_t w0 = 0; w0 < nt; w0 += a.queue_cap) {
-        /* fill: queue[rank - w0] = x | k << 20 | t << 26 */
-        for (uint32_t j = wave; j < nj; j += kWaves) {
-            const uint32_t k = j / a.NC, c = j - k * a.NC;
-            const uint64_t* tmj = v.tm + (j * a.nlev + l) * 3;
-            const uint64_t m0 = tmj[0], m1 = tmj[1], m2 = tmj[2];
-            const uint32_t cnt = (uint32_t)((m0 >> lane) & 1ull) + 2u * (uint32_t)((m1 >> lane) & 1ull)
-                               + 4u * (uint32_t)((m2 >> lane) & 1ull);
-            if (cnt == 0) continue;
-            const uint64_t lo = (1ull << lane) - 1ull;
-            const uint32_t r0 = v.pt[j * a.nlev + l] + __popcll(m0 & lo) + 2u * __popcll(m1 & lo) + 4u * __popcll(m2 & lo);
-            const uint32_t x = c * 64u + lane;
-            for (uint32_t t = 0; t < cnt; ++t) {
-                const uint32_t r = r0 + t;
-                if (r >= w0 && r - w0 < a.queue_cap) v.queue[r - w0] = x | (k << 20) | (t << 26);
+    for (uint32_t w0 = 0; w0 < nv_all || w0 < nt_all; w0 += Q) {
+        /* fill: lanes on (band row, chunk), set bits in scan order */
+        for (uint32_t l = 0; l < a.nlev; ++l) {
+            for (uint32_t k0 = wave * G; k0 < a.R; k0 += kWaves * G) {
+                const uint32_t k = k0 + sub;
+                const int y = y0 + (int)k;
+                if (k >= a.R || y >= (int)a.ny) continue;
+                for (uint32_t c = cl; c < a.NC; c += CS) {
+                    if (!a.dup) {
+                        const Cross e = window_cross(a, v, z, y0, 1, k + 1, c, l);
+                        uint64_t m = e.x | e.y | e.z;
+                        uint32_t r = wrow(v, 1, l, k + 1)[c] - wrow(v, 1, l, 1)[0] + v.misc[kMiscQV + l];
+                        while (m) {
+                            const uint32_t i = (uint32_t)__ffsll((unsigned long long)m) - 1u;
+                            m &= m - 1ull;
+                            const uint32_t base = (c * 64u + i) | (k << 20) | (l << 28);
+                            if ((e.x >> i) & 1ull) { if (r >= w0 && r - w0 < Q) v.vq[r - w0] = base; ++r; }
+                            if ((e.y >> i) & 1ull) { if (r >= w0 && r - w0 < Q) v.vq[r - w0] = base | (1u << 26); ++r; }
+                            if ((e.z >> i) & 1ull) { if (r >= w0 && r - w0 < Q) v.vq[r - w0] = base | (2u << 26); ++r; }
+                        }
+                    }
+                    uint64_t m = v.act[(l * a.R + k) * a.NC + c];
+                    if (m) {
+                        const uint64_t* r0 = brow(v, bslot(zc), l, k + 1);
+                        const uint64_t* r1 = brow(v, bslot(zc), l, k + 2);
+                        const uint64_t* r2 = brow(v, bslot(z), l, k + 1);
+                        const uint64_t* r3 = brow(v, bslot(z), l, k + 2);
+                        const uint64_t c0 = r0[c], c1 = shift_next(r0, c, a.NC), c3 = r1[c], c2 = shift_next(r1, c, a.NC);
+                        const uint64_t c4 = r2[c], c5 = shift_next(r2, c, a.NC), c7 = r3[c], c6 = shift_next(r3, c, a.NC);
+                        uint32_t r = v.pt[(l * a.R + k) * a.NC + c] + v.misc[kMiscQT + l];
+                        while (m) {
+                            const uint32_t i = (uint32_t)__ffsll((unsigned long long)m) - 1u;
+                            m &= m - 1ull;
+                            const uint32_t ci = (uint32_t)((c0 >> i) & 1ull) | (uint32_t)((c1 >> i) & 1ull) << 1 |
+                                                (uint32_t)((c2 >> i) & 1ull) << 2 | (uint32_t)((c3 >> i) & 1ull) << 3 |
+                                                (uint32_t)((c4 >> i) & 1ull) << 4 | (uint32_t)((c5 >> i) & 1ull) << 5 |
+                                                (uint32_t)((c6 >> i) & 1ull) << 6 | (uint32_t)((c7 >> i) & 1ull) << 7;
+                            const uint32_t nt = (uint32_t)(v.tri[ci] >> 60);
+                            const uint32_t x = c * 64u + i;
+                            for (uint32_t tt = 0; tt < nt; ++tt, ++r)
+                                if (r >= w0 && r - w0 < Q) v.tq[r - w0] = x | (k << 20) | (tt << 26) | (l << 29);
+                        }
+                    }
+                }
             }
         }
         __syncthreads();
-        const uint32_t nq = (nt - w0) < a.queue_cap ? (nt - w0) : a.queue_cap;
-        for (uint32_t q = threadIdx.x; q < nq; q += kBlock) {
-            const uint32_t e = v.queue[q];
-            const uint32_t x = e & 0xFFFFFu, k = (e >> 20) & 0x3Fu, t = e >> 26;
+        /* process: vertices */
+        const uint32_t nvq = nv_all > w0 ? ((nv_all - w0) < Q ? (nv_all - w0) : Q) : 0u;
+        for (uint32_t q = threadIdx.x; q < nvq; q += kBlock) {
+            const uint32_t e = v.vq[q];
+            const uint32_t x = e & 0xFFFFFu, k = (e >> 20) & 0x3Fu, axis = (e >> 26) & 3u, l = e >> 28;
+            const LevelInfo& L = v.lev[l];
+            const uint64_t vid = L.vbase[2] + (w0 + q - v.misc[kMiscQV + l]);
+            if (vid >= L.cap_v) continue;
+            const uint32_t y = (uint32_t)y0 + k;
+            float p[3];
+            vertex_coords<T>(a, v, vol, L.iso, z, y0, x, (int)y, z, axis, l, p);
+            L.coords[3 * vid + 0] = p[0];
+            L.coords[3 * vid + 1] = p[1];
+            L.coords[3 * vid + 2] = p[2];
+            if (a.vnormal) {
+                float nn[3];
+                vertex_normal<T>(a, v, vol, L.iso, z, y0, x, y, axis, l, nn);
+                L.normals[3 * vid + 0] = nn[0];
+                L.normals[3 * vid + 1] = nn[1];
+                L.normals[3 * vid + 2] = nn[2];
+            }
+        }
+        /* process: triangles */
+        const uint32_t ntq = nt_all > w0 ? ((nt_all - w0) < Q ? (nt_all - w0) : Q) : 0u;
+        for (uint32_t q = threadIdx.x; q < ntq; q += kBlock) {
+            const uint32_t e = v.tq[q];
+            const uint32_t x = e & 0xFFFFFu, k = (e >> 20) & 0x3Fu, t = (e >> 26) & 7u, l = e >> 29;
+            const LevelInfo& L = v.lev[l];
+            const uint64_t tri = L.tbase + (w0 + q - v.misc[kMiscQT + l]);
+            if (tri >= L.cap_t) continue;
             const int y = y0 + (int)k;
-            const uint64_t tri = tbase + w0 + q;
-            if (tri >= capt) continue;
-            const uint32_t ci = cube_index_at(a, v, y0, x, y, zc, l);
-            const uint64_t w = d_tri[ci];
-            const uint32_t ed[3] = { (uint32_t)((w >> (12 * t)) & 0xF), (uint32_t)((w >> (12 * t + 8)) & 0xF),
-                                     (uint32_t)((w >> (12 * t + 4)) & 0xF) };
+            const uint32_t ci = cube_index_at(v, y0, x, y, zc, l);
+            const uint64_t w = v.tri[ci];
+            const uint32_t eds[3] = { (uint32_t)((w >> (12 * t)) & 0xF), (uint32_t)((w >> (12 * t + 8)) & 0xF),
+                                      (uint32_t)((w >> (12 * t + 4)) & 0xF) };
+            float p[3][3], nn[3];
             if (a.dup) {
                 /* extract_surfaces_with_duplication: table orientation (:206-228) */
-                float p[3][3];
+#pragma unroll
                 for (int j = 0; j < 3; ++j) {
-                    const uint32_t e0 = d_edge_ends[ed[j]][0], e1 = d_edge_ends[ed[j]][1];
-                    interpolate<T>(a, vol, iso, x + (e0 & 1u), (uint32_t)y + ((e0 >> 1) & 1u), (uint32_t)zc + (e0 >> 2),
-                                   x + (e1 & 1u), (uint32_t)y + ((e1 >> 1) & 1u), (uint32_t)zc + (e1 >> 2), p[j]);
+                    const uint32_t qq = edge_ends(eds[j]);
+                    interpolate<T>(a, vol, L.iso,
+                                   x + (qq & 1u), (uint32_t)y + ((qq >> 1) & 1u), (uint32_t)zc + ((qq >> 2) & 1u),
+                                   x + ((qq >> 4) & 1u), (uint32_t)y + ((qq >> 5) & 1u), (uint32_t)zc + ((qq >> 6) & 1u),
+                                   p[j]);
                 }
-                float n[3];
-                face_normal(p[0], p[1], p[2], n);
-                if (3 * tri + 2 < capv) {
-                    for (int j = 0; j < 3; ++j)
-                        for (int m = 0; m < 3; ++m) co[9 * tri + 3 * j + m] = p[j][m];
+                face_normal(p[0], p[1], p[2], nn);
+                float* co = L.coords + 9 * tri;
+#pragma unroll
+                for (int j = 0; j < 3; ++j) {
+                    co[3 * j + 0] = p[j][0];
+                    co[3 * j + 1] = p[j][1];
+                    co[3 * j + 2] = p[j][2];
                 }
-                no[3 * tri + 0] = n[0];
-                no[3 * tri + 1] = n[1];
-                no[3 * tri + 2] = n[2];
+                L.normals[3 * tri + 0] = nn[0];
+                L.normals[3 * tri + 1] = nn[1];
+                L.normals[3 * tri + 2] = nn[2];
             } else {
                 /* connect_isopoints: vertex ids of the three edges (:493-514) */
                 uint32_t id[3];
+#pragma unroll
                 for (int j = 0; j < 3; ++j) {
-                    const uint32_t dx = d_edge_node[ed[j]][0], dy = d_edge_node[ed[j]][1];
-                    const uint32_t dz = d_edge_node[ed[j]][2], ax = d_edge_node[ed[j]][3];
+                    const uint32_t o = edge_owner(eds[j]);
+                    const uint32_t dx = o & 1u, dy = (o >> 1) & 1u, dz = (o >> 2) & 1u, ax = o >> 3;
                     const uint32_t kk = k + dy;
-                    const int s = zc + (int)dz;
-                    const uint32_t rank = vertex_rank(a, v, s, y0, x + dx, kk, ax, l);
-                    const uint64_t b = vbase[(dz << 1) | (kk == a.R ? 1u : 0u)];
+                    const uint32_t rank = tile_rank(a, v, z, y0, x + dx, kk, zc + (int)dz, ax, l);
+                    const uint32_t sel = (dz << 1) | (kk == a.R ? 1u : 0u);
+                    const uint64_t b = sel == 0 ? L.vbase[0] : sel == 1 ? L.vbase[1] : sel == 2 ? L.vbase[2] : L.vbase[3];
                     id[j] = (uint32_t)(b + rank);
                 }
-                cn[3 * tri + 0] = id[0];
-                cn[3 * tri + 1] = id[1];
-                cn[3 * tri + 2] = id[2];
+                L.conn[3 * tri + 0] = id[0];
+                L.conn[3 * tri + 1] = id[1];
+                L.conn[3 * tri + 2] = id[2];
                 if (!a.vnormal) {
                     /* calculate_normals_on_polygon (:531-559) */
-                    float p[3][3], n[3];
-                    for (int j = 0; j < 3; ++j)
-                        cell_edge_coords<T>(a, vol, iso, x, (uint32_t)y, (uint32_t)zc, ed[j], p[j]);
-                    face_normal(p[0], p[1], p[2], n);
-                    no[3 * tri + 0] = n[0];
-                    no[3 * tri + 1] = n[1];
-                    no[3 * tri + 2] = n[2];
+#pragma unroll
+                    for (int j = 0; j < 3; ++j) {
+                        const uint32_t o = edge_owner(eds[j]);
+                        vertex_coords<T>(a, v, vol, L.iso, z, y0, x + (o & 1u), y + (int)((o >> 1) & 1u),
+                                         zc + (int)((o >> 2) & 1u), o >> 3, l, p[j]);
+                    }
+                    face_normal(p[0], p[1], p[2], nn);
+                    L.normals[3 * tri + 0] = nn[0];
+                    L.normals[3 * tri + 1] = nn[1];
+                    L.normals[3 * tri + 2] = nn[2];
                 }
             }
         }
@@ -696,60 +750,210 @@ __device__ void emit_triangles(const JobArgs& a, const Views& v, const T* vol, i
     }
 }
 
+/* Stage the case table and the per-level parameters in LDS. */
+__device__ void stage_tables(const JobArgs& a, const Views& v)
+{
+    for (uint32_t i = threadIdx.x; i < 256; i += kBlock) v.tri[i] = d_tri[i];
+    if (threadIdx.x < kMaxLevels) {
+        const uint32_t l = threadIdx.x;
+        LevelInfo L;
+        L.coords = nullptr; L.normals = nullptr; L.conn = nullptr;
+        L.cap_v = 0; L.cap_t = 0; L.iso = 0.0; L.tbase = 0;
+        L.vbase[0] = L.vbase[1] = L.vbase[2] = L.vbase[3] = 0;
+#pragma unroll
+        for (int m = 0; m < kMaxLevels; ++m) {
+            if (m == (int)l) {
+                L.coords = a.coords[m]; L.normals = a.normals[m]; L.conn = a.conn[m];
+                L.cap_v = a.cap_vertices[m]; L.cap_t = a.cap_triangles[m]; L.iso = a.iso[m];
+            }
+        }
+        v.lev[l] = L;
+    }
+}
+
 /* ------------------------------------------------------------------------- */
-/* the z-marching kernel                                                       */
+/* classify pass: values -> bit volume                                         */
 /* ------------------------------------------------------------------------- */
 
-/* One workgroup per (slab, band).  kCount: write per-tile vertex / triangle
- * counts; otherwise emit the outputs at the prefixes of a.prefix. */
-template <typename T, bool kCount>
-__global__ __launch_bounds__(kBlock) void k_march(JobArgs a)
+/* Streams the volume once.  Each wave owns whole rows: its lanes load 64
+ * consecutive nodes per chunk (one coalesced 64-lane access), a ballot per
+ * level turns the comparisons into the chunk mask, and the masks of up to 64
+ * chunks of a row are gathered into lanes and stored with one coalesced
+ * 64-bit store per level. */
+/* Predicate of level l for a value (double(v) > iso[l]). */
+template <typename T>
+__device__ __forceinline__ bool above(T v, const JobArgs& a, int l) { return (double)v > a.iso[l]; }
+template <>
+__device__ __forceinline__ bool above<float>(float v, const JobArgs& a, int l) { return v > a.thr[l]; }
+
+/* Streams the volume once.  Each wave owns whole rows: its lanes load 64
+ * consecutive nodes per chunk (one coalesced access), the comparison against
+ * level l is a ballot (v_cmp into a lane mask), and the mask is parked in
+ * lane c of a register by a lane-select; the masks of up to 64 chunks of a
+ * row leave with one coalesced 64-bit store per level.  NLEV is the number of
+ * levels, a compile-time constant so that nothing is guarded per level. */
+template <typename T, int NLEV>
+__global__ __launch_bounds__(kBlock) void k_classify(JobArgs a)
+{
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint64_t nrows = (uint64_t)a.ny * a.nz;
+    const uint64_t wave_id = ((uint64_t)blockIdx.x * kBlock + threadIdx.x) >> 6;
+    const uint64_t nwaves = ((uint64_t)gridDim.x * kBlock) >> 6;
+    const T* vol = (const T*)a.values;
+    constexpr uint32_t kBatch = 16;
+    for (uint64_t row = wave_id; row < nrows; row += nwaves) {
+        const T* src = vol + row * a.nx;
+        const uint64_t z = row / a.ny, y = row - z * a.ny;
+        for (uint32_t c0 = 0; c0 < a.NC; c0 += 64) {
+            uint32_t lo[NLEV], hi[NLEV];
+#pragma unroll
+            for (int l = 0; l < NLEV; ++l) { lo[l] = 0; hi[l] = 0; }
+            const uint32_t cend = (a.NC - c0) < 64u ? (a.NC - c0) : 64u;
+            for (uint32_t cb = 0; cb < cend; cb += kBatch) {
+                T val[kBatch];
+#pragma unroll
+                for (uint32_t i = 0; i < kBatch; ++i) {
+                    const uint32_t x = (c0 + cb + i) * 64u + lane;
+                    val[i] = (cb + i < cend && x < a.nx) ? src[x] : T(0);
+                }
+#pragma unroll
+                for (uint32_t i = 0; i < kBatch; ++i) {
+                    if (cb + i < cend) {
+                        const bool in = (c0 + cb + i) * 64u + lane < a.nx;
+                        const bool mine = lane == cb + i;
+#pragma unroll
+                        for (int l = 0; l < NLEV; ++l) {
+                            const uint64_t m = __ballot(in && above<T>(val[i], a, l));
+                            lo[l] = mine ? (uint32_t)m : lo[l];
+                            hi[l] = mine ? (uint32_t)(m >> 32) : hi[l];
+                        }
+                    }
+                }
+            }
+            if (lane < cend) {
+#pragma unroll
+                for (int l = 0; l < NLEV; ++l)
+                    a.bitvol[(((uint64_t)l * a.nz + z) * a.ny + y) * a.NC + c0 + lane] =
+                        ((uint64_t)hi[l] << 32) | lo[l];
+            }
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------- */
+/* count pass: per-row counts from the bit volume                              */
+/* ------------------------------------------------------------------------- */
+
+/* Per-row counts of tile (z, b): for node row y of slice z the crossing edges
+ * it owns, and for cell row (y, z-1) its triangles.  LDS holds only the chunk
+ * masks of slices z-1, z, z+1 (rows y0-1 .. y0+R+1) and the case table; a
+ * wave takes one (level, row) at a time with its lanes on the row's chunks. */
+__global__ __launch_bounds__(kBlock) void k_count(JobArgs a)
+{
+    extern __shared__ __align__(16) uint8_t smem[];
+    const LdsLayout L = lds_layout(a.R, a.NC, a.nlev, 0, 0);
+    Views v;
+    v.tri = (uint64_t*)(smem + L.off_tri);
+    v.bits = (uint64_t*)(smem + L.off_bits);
+    v.nrb = L.nrb; v.ncp = L.ncp; v.R = a.R; v.NC = a.NC; v.nlev = a.nlev;
+    const uint64_t tile = blockIdx.x;
+    const int z = (int)(tile / a.nb);
+    const uint32_t band = (uint32_t)(tile - (uint64_t)z * a.nb);
+    const int y0 = (int)(band * a.R);
+    for (uint32_t i = threadIdx.x; i < 256; i += kBlock) v.tri[i] = d_tri[i];
+    load_window(a, v, z, y0);
+    __syncthreads();
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    const uint32_t sub = lane >> a.cshift, cl = lane & ((1u << a.cshift) - 1u);
+    const uint32_t G = 64u >> a.cshift, CS = 1u << a.cshift;
+    const uint32_t s0 = bslot(z), s1 = bslot(z + 1), sp = bslot(z - 1);
+    for (uint32_t l = 0; l < a.nlev; ++l) {
+        for (uint32_t k0 = wave * G; k0 < a.R; k0 += kWaves * G) {
+            const uint32_t k = k0 + sub;
+            const int y = y0 + (int)k;
+            const bool ok = k < a.R && y < (int)a.ny;
+            uint32_t vc = 0, tc = 0;
+            if (ok) {
+                const uint64_t* row = brow(v, s0, l, k + 1);
+                const uint64_t* up = brow(v, s0, l, k + 2);
+                const uint64_t* nz = brow(v, s1, l, k + 1);
+                const uint64_t* pr = brow(v, sp, l, k + 1);
+                const uint64_t* pu = brow(v, sp, l, k + 2);
+                for (uint32_t c = cl; c < a.NC; c += CS) {
+                    const uint64_t b = row[c];
+                    const uint64_t X = (b ^ shift_next(row, c, a.NC)) & valid_mask(c, a.nx - 1);
+                    const uint64_t Y = (y + 1 < (int)a.ny) ? ((b ^ up[c]) & valid_mask(c, a.nx)) : 0ull;
+                    const uint64_t Zm = (z + 1 < (int)a.nz) ? ((b ^ nz[c]) & valid_mask(c, a.nx)) : 0ull;
+                    vc += (uint32_t)(__popcll(X) + __popcll(Y) + __popcll(Zm));
+                    if (z >= 1 && y + 1 < (int)a.ny) {
+                        const uint64_t c0 = pr[c], c1 = shift_next(pr, c, a.NC), c3 = pu[c], c2 = shift_next(pu, c, a.NC);
+                        const uint64_t c4 = b, c5 = shift_next(row, c, a.NC), c7 = up[c], c6 = shift_next(up, c, a.NC);
+                        const uint64_t o = c0 | c1 | c2 | c3 | c4 | c5 | c6 | c7;
+                        const uint64_t an = c0 & c1 & c2 & c3 & c4 & c5 & c6 & c7;
+                        uint64_t m = (o & ~an) & valid_mask(c, a.nx - 1);
+                        while (m) {
+                            const uint32_t i = (uint32_t)__ffsll((unsigned long long)m) - 1u;
+                            m &= m - 1ull;
+                            const uint32_t ci = (uint32_t)((c0 >> i) & 1ull) | (uint32_t)((c1 >> i) & 1ull) << 1 |
+                                                (uint32_t)((c2 >> i) & 1ull) << 2 | (uint32_t)((c3 >> i) & 1ull) << 3 |
+                                                (uint32_t)((c4 >> i) & 1ull) << 4 | (uint32_t)((c5 >> i) & 1ull) << 5 |
+                                                (uint32_t)((c6 >> i) & 1ull) << 6 | (uint32_t)((c7 >> i) & 1ull) << 7;
+                            tc += (uint32_t)(v.tri[ci] >> 60);
+                        }
+                    }
+                }
+            }
+            /* sum over the CS lanes of each row group */
+            for (uint32_t d = CS >> 1; d >= 1; d >>= 1) {
+                vc += __shfl_xor(vc, (int)d);
+                tc += __shfl_xor(tc, (int)d);
+            }
+            if (ok && cl == 0) {
+                const uint64_t r = (uint64_t)z * a.ny + (uint64_t)y;
+                a.counts[(uint64_t)(l * 2 + 0) * a.nrows + r] = vc;
+                a.counts[(uint64_t)(l * 2 + 1) * a.nrows + r] = tc;
+            }
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------- */
+/* emit pass                                                                   */
+/* ------------------------------------------------------------------------- */
+
+/* One workgroup per tile (z, b): the bit rows of slices z-1, z, z+1 are
+ * loaded once, so tiles are independent and the grid needs no ordering; the
+ * outputs land at the per-row prefixes of a.prefix. */
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_emit(JobArgs a)
 {
     extern __shared__ __align__(16) uint8_t smem[];
     const Views v = make_views(a, smem);
     const T* vol = (const T*)a.values;
-    const uint32_t band = blockIdx.x % a.nb;
-    const uint32_t slab = blockIdx.x / a.nb;
-    const int zb = (int)(slab * a.slab);
-    const int ze = (int)((slab + 1) * a.slab < a.nz ? (slab + 1) * a.slab : a.nz);
+    const uint64_t tile = blockIdx.x;
+    const int z = (int)(tile / a.nb);
+    const uint32_t band = (uint32_t)(tile - (uint64_t)z * a.nb);
     const int y0 = (int)(band * a.R);
 
-    load_slice<T>(a, v, vol, zb - 1, y0);
-    load_slice<T>(a, v, vol, zb, y0);
+    stage_tables(a, v);
+    load_window(a, v, z, y0);
+    if (threadIdx.x < a.nlev) {
+        const uint32_t l = threadIdx.x;
+        const uint64_t* pre_v = a.prefix + (uint64_t)(l * 2 + 0) * a.nrows;
+        const uint64_t* pre_t = a.prefix + (uint64_t)(l * 2 + 1) * a.nrows;
+        const uint64_t row = (uint64_t)z * a.ny + (uint64_t)y0;
+        const bool next = (uint32_t)y0 + a.R < a.ny;
+        LevelInfo& L = v.lev[l];
+        L.tbase = pre_t[row];
+        L.vbase[2] = pre_v[row];
+        L.vbase[3] = next ? pre_v[row + a.R] : 0;
+        L.vbase[0] = z > 0 ? pre_v[row - a.ny] : 0;
+        L.vbase[1] = (z > 0 && next) ? pre_v[row - a.ny + a.R] : 0;
+    }
     __syncthreads();
-    if (!kCount && zb > 0) {
-        vertex_prefixes(a, v, zb - 1, y0);
-    }
-    for (int z = zb; z < ze; ++z) {
-        load_slice<T>(a, v, vol, z + 1, y0);
-        __syncthreads();
-        vertex_prefixes(a, v, z, y0);
-        triangle_prefixes(a, v, z, y0);
-        const uint64_t tile = (uint64_t)z * a.nb + band;
-        if (kCount) {
-            if (threadIdx.x < a.nlev) {
-                const uint32_t l = threadIdx.x;
-                a.counts[(uint64_t)(l * 2 + 0) * a.ntiles + tile] = v.tot[l];
-                a.counts[(uint64_t)(l * 2 + 1) * a.ntiles + tile] = v.tot[kMaxLevels + l];
-            }
-        } else {
-            for (uint32_t l = 0; l < a.nlev; ++l) {
-                const uint64_t* pre_v = a.prefix + (uint64_t)(l * 2 + 0) * a.ntiles;
-                const uint64_t* pre_t = a.prefix + (uint64_t)(l * 2 + 1) * a.ntiles;
-                if (!a.dup) emit_vertices<T>(a, v, vol, z, y0, l, pre_v[tile]);
-                uint64_t vb[4] = {0, 0, 0, 0};
-                if (!a.dup && z > 0) {
-                    const uint64_t tprev = (uint64_t)(z - 1) * a.nb + band;
-                    vb[0] = pre_v[tprev];
-                    vb[1] = (band + 1 < a.nb) ? pre_v[tprev + 1] : 0;
-                    vb[2] = pre_v[tile];
-                    vb[3] = (band + 1 < a.nb) ? pre_v[tile + 1] : 0;
-                }
-                if (z > 0) emit_triangles<T>(a, v, vol, z, y0, l, pre_t[tile], vb);
-            }
-        }
-        __syncthreads();
-    }
+    tile_prefixes(a, v, z, y0);
+    if (!v.misc[kMiscOverflow]) fill_cache<T>(a, v, vol, z, y0);
+    emit_tile<T>(a, v, vol, z, y0);
 }
 
 /* ------------------------------------------------------------------------- */
@@ -904,27 +1108,66 @@ __global__ void k_minmax_final(const T* vol, const Extreme<T>* parts, uint32_t n
 /* ------------------------------------------------------------------------- */
 
 template <typename T>
-hipError_t launch_march_t(const JobArgs& a, bool count, uint32_t lds_bytes, hipStream_t st)
+hipError_t launch_emit_t(const JobArgs& a, uint32_t lds_bytes, hipStream_t st)
 {
-    const dim3 grid(a.nb * a.nslabs), block(kBlock);
-    if (count) hipLaunchKernelGGL((k_march<T, true>), grid, block, lds_bytes, st, a);
-    else hipLaunchKernelGGL((k_march<T, false>), grid, block, lds_bytes, st, a);
+    hipLaunchKernelGGL((k_emit<T>), dim3((uint32_t)a.ntiles), dim3(kBlock), lds_bytes, st, a);
     return hipGetLastError();
 }
 
-hipError_t launch_march(int dtype, const JobArgs& a, bool count, uint32_t lds_bytes, hipStream_t st)
+template <typename T>
+hipError_t launch_classify_t(const JobArgs& a, hipStream_t st)
+{
+    const uint64_t rows = (uint64_t)a.ny * a.nz;
+    uint64_t blocks = (rows + kWaves - 1) / kWaves;
+    if (blocks > 65536) blocks = 65536;
+    const dim3 g((uint32_t)blocks), b(kBlock);
+    switch (a.nlev) {
+    case 1: hipLaunchKernelGGL((k_classify<T, 1>), g, b, 0, st, a); break;
+    case 2: hipLaunchKernelGGL((k_classify<T, 2>), g, b, 0, st, a); break;
+    case 3: hipLaunchKernelGGL((k_classify<T, 3>), g, b, 0, st, a); break;
+    case 4: hipLaunchKernelGGL((k_classify<T, 4>), g, b, 0, st, a); break;
+    default: hipLaunchKernelGGL((k_classify<T, 8>), g, b, 0, st, a); break;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_classify(int dtype, const JobArgs& a, hipStream_t st)
 {
     switch (dtype) {
-    case 0: return launch_march_t<int8_t>(a, count, lds_bytes, st);
-    case 1: return launch_march_t<int16_t>(a, count, lds_bytes, st);
-    case 2: return launch_march_t<int32_t>(a, count, lds_bytes, st);
-    case 3: return launch_march_t<int64_t>(a, count, lds_bytes, st);
-    case 4: return launch_march_t<uint8_t>(a, count, lds_bytes, st);
-    case 5: return launch_march_t<uint16_t>(a, count, lds_bytes, st);
-    case 6: return launch_march_t<uint32_t>(a, count, lds_bytes, st);
-    case 7: return launch_march_t<uint64_t>(a, count, lds_bytes, st);
-    case 8: return launch_march_t<float>(a, count, lds_bytes, st);
-    default: return launch_march_t<double>(a, count, lds_bytes, st);
+    case 0: return launch_classify_t<int8_t>(a, st);
+    case 1: return launch_classify_t<int16_t>(a, st);
+    case 2: return launch_classify_t<int32_t>(a, st);
+    case 3: return launch_classify_t<int64_t>(a, st);
+    case 4: return launch_classify_t<uint8_t>(a, st);
+    case 5: return launch_classify_t<uint16_t>(a, st);
+    case 6: return launch_classify_t<uint32_t>(a, st);
+    case 7: return launch_classify_t<uint64_t>(a, st);
+    case 8: return launch_classify_t<float>(a, st);
+    default: return launch_classify_t<double>(a, st);
+    }
+}
+
+hipError_t launch_count(const JobArgs& a, hipStream_t st)
+{
+    const LdsLayout L = lds_layout(a.R, a.NC, a.nlev, 0, 0);
+    const uint32_t lds = L.off_bits + 3u * a.nlev * L.nrb * L.ncp * 8u;
+    hipLaunchKernelGGL(k_count, dim3((uint32_t)a.ntiles), dim3(kBlock), lds, st, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_emit(int dtype, const JobArgs& a, uint32_t lds_bytes, hipStream_t st)
+{
+    switch (dtype) {
+    case 0: return launch_emit_t<int8_t>(a, lds_bytes, st);
+    case 1: return launch_emit_t<int16_t>(a, lds_bytes, st);
+    case 2: return launch_emit_t<int32_t>(a, lds_bytes, st);
+    case 3: return launch_emit_t<int64_t>(a, lds_bytes, st);
+    case 4: return launch_emit_t<uint8_t>(a, lds_bytes, st);
+    case 5: return launch_emit_t<uint16_t>(a, lds_bytes, st);
+    case 6: return launch_emit_t<uint32_t>(a, lds_bytes, st);
+    case 7: return launch_emit_t<uint64_t>(a, lds_bytes, st);
+    case 8: return launch_emit_t<float>(a, lds_bytes, st);
+    default: return launch_emit_t<double>(a, lds_bytes, st);
     }
 }
 

@@ -135,3 +135,18 @@ def test_two_dimensional_volume_vertices_without_cells():
     v = rng.standard_normal((1, 17, 23)).astype(np.float32)
     for normal, dup in MODES:
         check(v, 0.0, normal, dup)
+
+
+def test_rows_wider_than_fused_pass():
+    # rows of more than 64 chunks (4096 nodes) take the separate classify + count path
+    rng = np.random.default_rng(12)
+    v = rng.standard_normal((3, 4, 4100)).astype(np.float32)
+    for normal, dup in MODES:
+        check(v, 0.2, normal, dup)
+
+
+def test_many_rows_across_groups():
+    # more rows than one fused-pass workgroup holds, and a partial last group
+    v = datasets.blobs((40, 67, 9), seed=13, nblobs=5)
+    for normal, dup in MODES:
+        check(v, 0.3, normal, dup)
