@@ -106,6 +106,8 @@ def ref_lib():
         lib.ref_tf_table.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_uint8)]
         lib.ref_tf_table.restype = ctypes.c_int
         lib.ref_mc_tables.argtypes = [ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)]
+        lib.ref_hydrogen.argtypes = [ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint8)]
+        lib.ref_hydrogen.restype = ctypes.c_int
         _ref = lib
     return _ref
 
@@ -253,3 +255,12 @@ def ref_tables():
     vert = (ctypes.c_int32 * (12 * 2 * 3))()
     ref_lib().ref_mc_tables(tri, vert)
     return np.array(tri, dtype=np.int32).reshape(256, 16), np.array(vert, dtype=np.int32).reshape(12, 2, 3)
+
+
+def ref_hydrogen(dims):
+    """kvs::HydrogenVolumeData(dims) values, [z][y][x] uint8 (oracle/_ref)."""
+    nx, ny, nz = dims
+    out = (ctypes.c_uint8 * (nx * ny * nz))()
+    d = (ctypes.c_uint32 * 3)(nx, ny, nz)
+    ref_lib().ref_hydrogen(d, out)
+    return np.frombuffer(bytes(out), dtype=np.uint8).reshape(nz, ny, nx).copy()

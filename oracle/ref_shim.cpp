@@ -17,6 +17,7 @@
 #include <kvs/ValueArray>
 #include <kvs/AnyValueArray>
 #include <kvs/Type>
+#include <kvs/HydrogenVolumeData>
 #include <cstdint>
 #include <cstdlib>
 #include <cstring>
@@ -210,6 +211,16 @@ int ref_tf_table( int tf_kind, int tf_resolution, uint8_t* out )
     const auto& t = tf.colorMap().table();
     std::memcpy( out, t.data(), t.size() );
     return int( t.size() );
+}
+
+// kvs::HydrogenVolumeData( dims ): the reference's own uint8 test volume
+// (Visualization/Data/HydrogenVolumeData.cpp:51-98).
+int ref_hydrogen( const uint32_t dims[3], uint8_t* out )
+{
+    kvs::HydrogenVolumeData h( kvs::Vec3u( dims[0], dims[1], dims[2] ) );
+    const auto& v = h.values();
+    std::memcpy( out, v.data(), v.byteSize() );
+    return int( v.byteSize() );
 }
 
 // MarchingCubesTable::TriangleID and VertexID, for pinning transcriptions.

@@ -1,0 +1,203 @@
+"""Write the golden fixtures of tests/golden/ from the reference itself.
+
+Run in the build container, where oracle/_ref/libkvsref.so -- the
+reference's own kvs::MarchingCubes / kvs::Isosurface / ColorMap code compiled
+from its sources by oracle/Makefile -- exists:
+
+    make -C oracle && python tests/golden/make_golden.py
+
+Every expected output below comes from that library; inputs are either the
+reference's own HydrogenVolumeData or seeded synthetic volumes.  The files are
+data only (inputs and expected outputs) and are what the GPU box checks
+against, since the reference does not exist there.
+"""
+import hashlib
+import json
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+import oracle  # noqa: E402
+from kvs_amd import datasets  # noqa: E402
+
+MODES = [(1, True), (1, False), (0, False)]   # (normal_type, duplication)
+DTYPES = [np.int8, np.int16, np.int32, np.int64, np.uint8, np.uint16, np.uint32, np.uint64, np.float32, np.float64]
+
+
+def sha(a):
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def digest(r):
+    return {
+        "n_coords": int(r["coords"].size), "n_connections": int(r["connections"].size),
+        "n_normals": int(r["normals"].size),
+        "coords": sha(r["coords"]), "connections": sha(r["connections"]), "normals": sha(r["normals"]),
+        "colors": [int(c) for c in r["colors"]], "opacities": [int(c) for c in r["opacities"]],
+        "polygon_type": r["polygon_type"], "normal_type": r["normal_type"], "color_type": r["color_type"],
+        "success": r["success"],
+    }
+
+
+class Packer:
+    """Concatenate many small arrays per field; offsets[i]..offsets[i+1] is case i."""
+
+    def __init__(self):
+        self.data = {}
+
+    def add(self, field, arr):
+        self.data.setdefault(field, []).append(np.ravel(arr))
+
+    def save(self, path):
+        out = {}
+        for field, parts in self.data.items():
+            out[field] = np.concatenate(parts) if parts else np.zeros(0)
+            out[field + "_off"] = np.cumsum([0] + [p.size for p in parts]).astype(np.int64)
+        np.savez_compressed(path, **out)
+
+
+def random_case(rng, dtype, shape):
+    if np.dtype(dtype).kind == "f":
+        v = rng.standard_normal(shape).astype(dtype)
+        iso = float(np.float32(rng.standard_normal() * 0.5))
+    else:
+        info = np.iinfo(dtype)
+        lo, hi = max(int(info.min), -60), min(int(info.max), 60)
+        v = rng.integers(lo, hi + 1, shape).astype(dtype)
+        iso = float(rng.integers(lo, hi)) + (0.5 if rng.random() < 0.5 else 0.0)
+    return v, iso
+
+
+def main():
+    if not oracle.ref_available():
+        sys.exit("oracle/_ref/libkvsref.so is missing: run `make -C oracle` first")
+    meta = {"generator": "tests/golden/make_golden.py", "reference": "oracle/_ref/libkvsref.so"}
+
+    # 1. the reference's case and edge tables, pinned by hash
+    tri, vert = oracle.ref_tables()
+    meta["tables"] = {"TriangleID_sha256": sha(tri.astype("<i4")), "VertexID_sha256": sha(vert.astype("<i4"))}
+    np.savez_compressed(os.path.join(HERE, "tables.npz"), triangle_counts=(tri >= 0).sum(axis=1) // 3)
+
+    # 2. brute force: every 2x2x2 sign pattern, four dtypes, three modes
+    rng = np.random.default_rng(2024)
+    pk = Packer()
+    cases = []
+    for dtype in (np.float32, np.float64, np.uint8, np.int16):
+        for ci in range(256):
+            if np.dtype(dtype).kind == "f":
+                lo_v = rng.uniform(-1.0, -0.01, 8)
+                hi_v = rng.uniform(0.01, 1.0, 8)
+                iso = 0.0
+            else:
+                lo_v = rng.integers(-20 if np.dtype(dtype).kind == "i" else 0, 10, 8)
+                hi_v = rng.integers(11, 40, 8)
+                iso = 10.5
+            # corner k of the single cell: (x, y, z) offsets of MarchingCubes.cpp:187-194
+            corners = [(0, 0, 0), (1, 0, 0), (1, 1, 0), (0, 1, 0), (0, 0, 1), (1, 0, 1), (1, 1, 1), (0, 1, 1)]
+            v = np.zeros((2, 2, 2), dtype=dtype)
+            for k, (x, y, z) in enumerate(corners):
+                v[z, y, x] = hi_v[k] if (ci >> k) & 1 else lo_v[k]
+            pk.add("in_" + np.dtype(dtype).name, v)
+            for normal, dup in MODES:
+                r = oracle.ref_marching_cubes(v, iso, normal, dup)
+                mk = f"n{normal}_d{int(dup)}"
+                pk.add(mk + "_coords", r["coords"])
+                pk.add(mk + "_conn", r["connections"])
+                pk.add(mk + "_normals", r["normals"])
+            cases.append({"iso": iso, "dtype": np.dtype(dtype).name, "cube_index": ci})
+    pk.save(os.path.join(HERE, "cubes.npz"))
+    meta["cubes"] = cases
+
+    # 3. random small volumes of every dtype, with and without user min/max and coords
+    pk = Packer()
+    cases = []
+    for t in range(40):
+        dtype = DTYPES[t % len(DTYPES)]
+        shape = tuple(int(s) for s in rng.integers(2, 14, 3))
+        v, iso = random_case(rng, dtype, shape)
+        mm = None
+        coords = None
+        if t % 4 == 1:
+            lo, hi = float(v.min()), float(v.max())
+            mm = (lo + 0.2 * (hi - lo), hi - 0.25 * (hi - lo))
+        if t % 4 == 2:
+            coords = ([float(rng.uniform(-3, 3)) for _ in range(3)], [float(rng.uniform(4, 9)) for _ in range(3)])
+        pk.add("in_bytes", np.frombuffer(v.tobytes(), dtype=np.uint8))
+        for normal, dup in MODES:
+            r = oracle.ref_marching_cubes(v, iso, normal, dup, min_max_values=mm, coords=coords)
+            mk = f"n{normal}_d{int(dup)}"
+            pk.add(mk + "_coords", r["coords"])
+            pk.add(mk + "_conn", r["connections"])
+            pk.add(mk + "_normals", r["normals"])
+            pk.add(mk + "_colors", r["colors"])
+        cases.append({"iso": iso, "dtype": np.dtype(dtype).name, "shape": list(shape), "min_max": mm,
+                      "coords": coords})
+    pk.save(os.path.join(HERE, "random.npz"))
+    meta["random"] = cases
+
+    # 4. HydrogenVolumeData 64^3 (the reference's own fixture), isolevels 37.5 and 100
+    h = oracle.ref_hydrogen((64, 64, 64))
+    np.savez_compressed(os.path.join(HERE, "hydrogen64.npz"), values=h)
+    meta["hydrogen64"] = {"sha256": sha(h), "levels": {}}
+    for iso in (37.5, 100.0):
+        for normal, dup in MODES:
+            r = oracle.ref_marching_cubes(h, iso, normal, dup)
+            meta["hydrogen64"]["levels"][f"{iso}_n{normal}_d{int(dup)}"] = digest(r)
+    r = oracle.ref_isosurface(h, None, 1, True)
+    meta["hydrogen64"]["isosurface_default_level"] = digest(r)
+
+    # 5. the north star's CPU configuration: 64^3 float32, 3 Gaussian blobs + noise,
+    #    isolevel 0.5, PolygonNormal, duplication, default transfer function
+    c0 = datasets.blobs(64, seed=1000)
+    meta["c0"] = {"sha256": sha(c0), "levels": {}}
+    for normal, dup in MODES:
+        r = oracle.ref_marching_cubes(c0, 0.5, normal, dup)
+        meta["c0"]["levels"][f"0.5_n{normal}_d{int(dup)}"] = digest(r)
+        if normal == 1 and dup:
+            np.savez_compressed(os.path.join(HERE, "c0_dup.npz"), coords=r["coords"], normals=r["normals"])
+
+    # 6. colours: created tables and ColorMap::at after setRange
+    meta["colormap_tables"] = {}
+    for kind, name in ((0, "BrewerRdBu"), (1, "BrewerSingleHueGreys")):
+        for res in (256, 17):
+            meta["colormap_tables"][f"{name}_{res}"] = [int(x) for x in oracle.ref_tf_table(kind, res)]
+    colors = []
+    for kind in (0, 1):
+        for (lo, hi) in ((0.0, 255.0), (-1.0, 1.0), (0.0, 1.0), (-128.0, 127.0), (3.5, 3.5000002)):
+            for value in np.linspace(lo - 0.2 * abs(hi - lo + 1), hi + 0.2 * abs(hi - lo + 1), 23):
+                rgb = oracle.ref_tf_color(kind, 256, lo, hi, float(np.float32(value)))
+                colors.append({"kind": kind, "lo": lo, "hi": hi, "value": float(np.float32(value)), "rgb": list(rgb)})
+    meta["colors"] = colors
+    # the mapper's own colour choice: MapperBase::setRange from the dtype / volume range
+    surf = []
+    for dtype in (np.uint8, np.int8, np.float32, np.int16):
+        v, iso = random_case(rng, dtype, (6, 7, 8))
+        r = oracle.ref_marching_cubes(v, iso, 1, True)
+        surf.append({"dtype": np.dtype(dtype).name, "min": float(v.min()), "max": float(v.max()), "iso": iso,
+                     "colors": [int(c) for c in r["colors"]]})
+    meta["surface_colors"] = surf
+
+    # 7. GetMinMaxValues<T> corner cases: NaN first / inside, +-0
+    mm = []
+    for name, arr in (("nan_first", np.array([np.nan, 1.0, -2.0, 3.0], np.float32)),
+                      ("nan_inside", np.array([0.5, np.nan, -2.0, 3.0], np.float32)),
+                      ("zeros", np.array([0.0, -0.0, 0.0, -0.0], np.float32)),
+                      ("neg_zero_first", np.array([-0.0, 0.0, 1.0, -1.0], np.float64))):
+        a = arr.reshape(1, 1, 4)
+        lo, hi = oracle.ref_min_max(a)
+        mm.append({"name": name, "values": [repr(float(x)) for x in arr], "dtype": arr.dtype.name,
+                   "min": repr(lo), "max": repr(hi)})
+    meta["minmax"] = mm
+
+    with open(os.path.join(HERE, "golden.json"), "w") as f:
+        json.dump(meta, f, indent=1, sort_keys=True)
+    print("wrote", sorted(os.listdir(HERE)))
+
+
+if __name__ == "__main__":
+    main()
