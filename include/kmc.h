@@ -151,6 +151,71 @@ kmc_status kmc_surface_color(int32_t colormap, uint32_t resolution, int32_t type
 /* The created colour table (3*resolution bytes) of a colour map (ColorMap::create, ColorMap.cpp:913-1019). */
 kmc_status kmc_colormap_table(int32_t colormap, uint32_t resolution, uint8_t* rgb_out);
 
+/* ------------------------------------------------------------------------- */
+/* Batched form: many volumes of uneven sizes, each with its own isolevels,   */
+/* extracted by one hipGraph launch.  No reference counterpart beyond calling */
+/* kvs::MarchingCubes once per (volume, isolevel); each surface of a batch is */
+/* identical to that call's output.                                           */
+/* ------------------------------------------------------------------------- */
+
+/* One job: a volume and the isolevels extracted from it.  Values in device
+ * memory are borrowed and must stay valid (and may change) between runs;
+ * values in host memory are copied to the device once, at creation. */
+typedef struct kmc_job {
+    kmc_volume volume;
+    const double* isolevels;     /* n_levels values, copied at creation */
+    uint32_t n_levels;
+    int32_t normal_type;         /* kmc_normal_type */
+    int32_t duplication;         /* forced off for KMC_VERTEX_NORMAL, as by the reference */
+} kmc_job;
+
+typedef struct kmc_batch kmc_batch;
+
+/* Flags of kmc_batch_create. */
+#define KMC_BATCH_AUTOTUNE 1u    /* time the candidate tile shapes on this batch and keep the fastest */
+
+/* What a batch was planned with and what its last run produced. */
+typedef struct kmc_batch_info {
+    uint32_t n_jobs;             /* jobs as given */
+    uint32_t n_surfaces;         /* sum of n_levels */
+    uint32_t n_passes;           /* internal passes (a job with more than KMC_MAX_LEVELS_PER_PASS levels takes several) */
+    uint32_t n_type_groups;      /* value types present: one kernel launch per pass kind and type */
+    uint32_t tile_rows;          /* upper bound of node rows per emit tile (autotuned or default) */
+    uint32_t queue_capacity;     /* work-queue entries per emit window (autotuned or default) */
+    uint32_t graph_nodes;        /* nodes of the captured graph */
+    uint32_t recaptures;         /* graph captures so far (output arena growth recaptures) */
+    uint64_t nodes_classified;   /* sum over passes of nodes x levels */
+    uint64_t n_triangles;        /* sum over surfaces, last run */
+    uint64_t arena_bytes;        /* device bytes of the output arena */
+    uint64_t work_bytes;         /* device bytes of counts, prefixes and classification masks */
+    double last_run_ms;          /* device time of the last run's graph (HIP events) */
+} kmc_batch_info;
+
+/* Plan a batch: validate every job (first failing job decides the status),
+ * compute missing min/max values, size the output arena and capture the
+ * graph.  stream is used for the planning work; the call returns when done. */
+kmc_status kmc_batch_create(const kmc_job* jobs, uint32_t n_jobs, uint32_t flags, void* stream, kmc_batch** out);
+
+/* Run the batch: one graph launch on stream, then wait for it.  The output
+ * arena grows (and the graph is captured again) if a run's surfaces do not fit,
+ * so results are always complete when the call returns. */
+kmc_status kmc_batch_run(kmc_batch* batch, void* stream);
+
+/* Views of the last run's surfaces in device memory, in job order and, within
+ * a job, in isolevel order (n_surfaces entries).  The views own nothing
+ * (kmc_surface_release only clears them) and stay valid until the next run or
+ * kmc_batch_destroy. */
+kmc_status kmc_batch_surfaces(const kmc_batch* batch, kmc_surface* outs, uint32_t n_outs);
+
+kmc_status kmc_batch_info_get(const kmc_batch* batch, kmc_batch_info* info);
+
+void kmc_batch_destroy(kmc_batch* batch);
+
+/* One-shot batch: create, run once, copy every surface to storage of its own
+ * in out_memory (released with kmc_surface_release), destroy. */
+kmc_status kmc_batch_extract(const kmc_job* jobs, uint32_t n_jobs, int32_t out_memory, void* stream,
+                             kmc_surface* outs);
+
 #ifdef __cplusplus
 }
 #endif

@@ -19,15 +19,6 @@
 
 namespace kmc {
 
-hipError_t launch_emit(int dtype, const JobArgs& a, uint32_t lds_bytes, hipStream_t st);
-hipError_t launch_classify(int dtype, const JobArgs& a, hipStream_t st);
-hipError_t launch_count(const JobArgs& a, hipStream_t st);
-hipError_t launch_scan(const uint32_t* counts, uint64_t n, uint32_t narr, uint64_t* partials,
-                       uint64_t* prefix, uint64_t* totals, hipStream_t st);
-uint32_t scan_partials_needed(uint64_t n, uint32_t narr);
-hipError_t launch_minmax(int dtype, const void* vol, uint64_t n, void* scratch, double* out, hipStream_t st);
-size_t minmax_scratch_bytes();
-
 /* ------------------------------------------------------------------------- */
 /* device memory pool                                                          */
 /* ------------------------------------------------------------------------- */
@@ -104,14 +95,7 @@ void device_pool_trim()
 /* surface ownership                                                           */
 /* ------------------------------------------------------------------------- */
 
-struct SurfaceStorage {
-    int memory;
-    void* coords;
-    void* connections;
-    void* normals;
-};
-
-static void free_storage(SurfaceStorage* s)
+void free_storage(SurfaceStorage* s)
 {
     if (!s) return;
     if (s->memory == KMC_MEMORY_DEVICE) {
@@ -225,18 +209,18 @@ kmc_status min_max_on_device(const kmc_volume* v, const void* dvals, hipStream_t
     return s;
 }
 
-/* Geometry and LDS budget of a pass; R rows per band. */
-bool plan_geometry(JobArgs& a, uint32_t* lds_bytes)
+bool plan_geometry(JobArgs& a, uint32_t* lds_bytes, uint32_t r_max, uint32_t queue_cap)
 {
     const uint32_t candidates[] = { 8, 4, 2, 1 };
     for (uint32_t R : candidates) {
+        if (R > r_max) continue;
         a.R = R;
         a.nb = (a.ny + R - 1) / R;
         a.NC = (a.nx + 63) / 64;
         a.cshift = 0;
         while ((1u << a.cshift) < a.NC && a.cshift < 6) ++a.cshift;
-        a.queue_cap = 1024;
-        a.cache_cap = 1024;      // fill_cache stages its work in the vertex queue: needs queue_cap >= cache_cap
+        a.queue_cap = queue_cap;
+        a.cache_cap = queue_cap;  // fill_cache stages its work in the vertex queue: needs queue_cap >= cache_cap
         a.use_cache = std::getenv("KMC_WINDOW_CACHE") ? 1u : 0u;
         const LdsLayout L = lds_layout(a.R, a.NC, a.nlev, a.queue_cap, a.cache_cap);
         if (L.bytes <= 60 * 1024) {
@@ -249,6 +233,21 @@ bool plan_geometry(JobArgs& a, uint32_t* lds_bytes)
         }
     }
     return false;
+}
+
+uint32_t levels_per_pass(const kmc_volume* v)
+{
+    for (uint32_t nl = kMaxLevels; nl >= 1; --nl) {
+        JobArgs a;
+        std::memset(&a, 0, sizeof(a));
+        a.nx = v->resolution[0];
+        a.ny = v->resolution[1];
+        a.nz = v->resolution[2];
+        a.nlev = nl;
+        uint32_t lds = 0;
+        if (plan_geometry(a, &lds)) return nl;
+    }
+    return 0;
 }
 
 void init_surface(kmc_surface* s, double iso, int normal_type, int dup, double mn, double mx,
@@ -275,13 +274,10 @@ void object_coords(const kmc_volume* v, float minc[3], float maxc[3])
     }
 }
 
-/* One pass of up to kMaxLevels levels over a device-resident volume.
- * On success outs[i] hold device arrays. */
-kmc_status run_pass(const kmc_volume* v, const void* dvals, const double* iso, uint32_t nlev,
-                    int normal_type, int dup, double mn, double mx, const float minc[3], const float maxc[3],
-                    hipStream_t st, kmc_surface* outs)
+/* Per-job fields of a pass of up to kMaxLevels levels. */
+void init_job(JobArgs& a, const kmc_volume* v, const void* dvals, const double* iso, uint32_t nlev, int normal_type,
+              int dup, double mn, double mx, const float minc[3], const float maxc[3])
 {
-    JobArgs a;
     std::memset(&a, 0, sizeof(a));
     a.values = dvals;
     a.nx = v->resolution[0];
@@ -301,6 +297,16 @@ kmc_status run_pass(const kmc_volume* v, const void* dvals, const double* iso, u
         a.iso[l] = iso[l];
         a.thr[l] = float_threshold(iso[l]);
     }
+}
+
+/* One pass of up to kMaxLevels levels over a device-resident volume.
+ * On success outs[i] hold device arrays. */
+kmc_status run_pass(const kmc_volume* v, const void* dvals, const double* iso, uint32_t nlev,
+                    int normal_type, int dup, double mn, double mx, const float minc[3], const float maxc[3],
+                    hipStream_t st, kmc_surface* outs)
+{
+    JobArgs a;
+    init_job(a, v, dvals, iso, nlev, normal_type, dup, mn, mx, minc, maxc);
     uint32_t lds = 0;
     if (!plan_geometry(a, &lds)) return KMC_ERR_INVALID_ARGUMENT;
 
@@ -454,8 +460,10 @@ kmc_status extract_levels_impl(const kmc_volume* v, const double* levels, uint32
         return KMC_OK;
     }
 
-    for (uint32_t l0 = 0; l0 < n_levels && s == KMC_OK; l0 += kMaxLevels) {
-        const uint32_t nl = (n_levels - l0) < (uint32_t)kMaxLevels ? (n_levels - l0) : (uint32_t)kMaxLevels;
+    const uint32_t per = levels_per_pass(v);
+    if (per == 0 && n_levels > 0) s = KMC_ERR_INVALID_ARGUMENT;   // rows too long for the emit tile
+    for (uint32_t l0 = 0; l0 < n_levels && s == KMC_OK; l0 += per) {
+        const uint32_t nl = (n_levels - l0) < per ? (n_levels - l0) : per;
         s = run_pass(v, dvals, levels + l0, nl, normal_type, dup, mn, mx, minc, maxc, st, outs + l0);
         if (s != KMC_OK) {
             for (uint32_t l = 0; l < l0; ++l) kmc_surface_release(&outs[l]);

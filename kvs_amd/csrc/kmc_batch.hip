@@ -1,0 +1,689 @@
+/*
+ * kmc_batch.hip -- the batched form: many volumes of uneven sizes, each with
+ * its own isolevels, extracted by one captured hipGraph.
+ *
+ * A batch is cut into passes (one volume, up to kMaxLevels isolevels each),
+ * ordered by value type.  Per type the graph holds one classify, one count and
+ * one emit launch over the rows / tiles of all its passes (a workgroup finds
+ * its pass by binary search of the pass start table); the per-row counts of
+ * all passes sit in one array and take one scan.  A layout kernel then turns
+ * each surface's totals into offsets in a shared output arena, so that sizes
+ * never go through the host inside the graph; when a run's surfaces do not fit
+ * the arena, the emit writes nothing, the host grows the arena, captures the
+ * graph again and reruns.  Each surface equals the plain call's output.
+ */
+#include <hip/hip_runtime.h>
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "kmc.h"
+#include "kmc_internal.h"
+#include "kmc_job.h"
+
+namespace kmc {
+
+/* Where one surface of a run landed (element offsets into the arenas). */
+struct SegOut {
+    uint64_t nv, nt;          // vertices (node-shared) and triangles counted
+    uint64_t oc, on, ok;      // offsets of coords (floats), normals (floats), connections (uint32)
+};
+
+constexpr uint32_t kLayoutBlock = 256;
+constexpr uint32_t kLayoutPer = 8;
+
+__device__ __forceinline__ uint64_t level_total(const JobArgs& a, uint32_t l, uint32_t k)
+{
+    const uint64_t i0 = (uint64_t)(2 * l + k) * a.nrows, last = i0 + a.nrows - 1;
+    return a.prefix[last] + a.counts[last] - a.prefix[i0];
+}
+
+/* Block-wide exclusive scan of three 64-bit values; returns the totals. */
+__device__ void block_scan3(uint64_t v[3], uint64_t tot[3])
+{
+    __shared__ uint64_t sh[3][kLayoutBlock / 64];
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    for (int q = 0; q < 3; ++q) {
+        uint64_t incl = v[q];
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const uint64_t t = (uint64_t)__shfl_up((unsigned long long)incl, d);
+            if ((int)lane >= d) incl += t;
+        }
+        if (lane == 63) sh[q][wave] = incl;
+        v[q] = incl - v[q];
+    }
+    __syncthreads();
+    for (int q = 0; q < 3; ++q) {
+        uint64_t run = 0, all = 0;
+        for (uint32_t w = 0; w < kLayoutBlock / 64; ++w) {
+            if (w < wave) run += sh[q][w];
+            all += sh[q][w];
+        }
+        v[q] += run;
+        tot[q] = all;
+    }
+}
+
+/* Sizes and arena offsets of every surface of the batch; points each pass's
+ * level outputs into the arena, or at nothing (capacity 0) when the batch
+ * does not fit.  out[nseg] receives the totals needed and the overflow flag. */
+__global__ __launch_bounds__(kLayoutBlock) void k_batch_layout(JobArgs* jobs, const uint32_t* seg_map, uint32_t nseg,
+                                                               float* arena_c, float* arena_n, uint32_t* arena_k,
+                                                               uint64_t cap_c, uint64_t cap_n, uint64_t cap_k,
+                                                               SegOut* out)
+{
+    __shared__ uint32_t overflow;
+    /* one sweep per kLayoutBlock * kLayoutPer surfaces, carrying the offsets */
+    uint64_t carry[3] = { 0, 0, 0 };
+    for (uint32_t b0 = 0; b0 < nseg; b0 += kLayoutBlock * kLayoutPer) {
+        const uint32_t s0 = b0 + threadIdx.x * kLayoutPer;
+        uint64_t sz[kLayoutPer][3];
+        uint64_t nv[kLayoutPer], nt[kLayoutPer];
+        uint64_t v[3] = { 0, 0, 0 };
+        for (uint32_t i = 0; i < kLayoutPer; ++i) {
+            nv[i] = nt[i] = 0;
+            sz[i][0] = sz[i][1] = sz[i][2] = 0;
+            if (s0 + i < nseg) {
+                const uint32_t m = seg_map[s0 + i];
+                const JobArgs& a = jobs[m >> 3];
+                const uint32_t l = m & 7u;
+                nv[i] = level_total(a, l, 0);
+                nt[i] = level_total(a, l, 1);
+                const uint64_t verts = a.dup ? 3 * nt[i] : nv[i];
+                if (verts > 0) {
+                    sz[i][0] = 3 * verts;
+                    sz[i][1] = 3 * (a.vnormal ? nv[i] : nt[i]);
+                    sz[i][2] = a.dup ? 0 : 3 * nt[i];
+                }
+            }
+            v[0] += sz[i][0]; v[1] += sz[i][1]; v[2] += sz[i][2];
+        }
+        uint64_t tot[3];
+        block_scan3(v, tot);
+        for (uint32_t i = 0; i < kLayoutPer; ++i) {
+            if (s0 + i < nseg) {
+                SegOut o;
+                o.nv = nv[i];
+                o.nt = nt[i];
+                o.oc = carry[0] + v[0];
+                o.on = carry[1] + v[1];
+                o.ok = carry[2] + v[2];
+                out[s0 + i] = o;
+            }
+            v[0] += sz[i][0]; v[1] += sz[i][1]; v[2] += sz[i][2];
+        }
+        carry[0] += tot[0]; carry[1] += tot[1]; carry[2] += tot[2];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) overflow = (carry[0] > cap_c || carry[1] > cap_n || carry[2] > cap_k) ? 1u : 0u;
+    __syncthreads();
+    const bool fits = overflow == 0;
+    for (uint32_t s = threadIdx.x; s < nseg; s += kLayoutBlock) {
+        const uint32_t m = seg_map[s];
+        JobArgs& a = jobs[m >> 3];
+        const uint32_t l = m & 7u;
+        const SegOut o = out[s];
+        const uint64_t verts = a.dup ? 3 * o.nt : o.nv;
+        a.coords[l] = arena_c + o.oc;
+        a.normals[l] = arena_n + o.on;
+        a.conn[l] = arena_k + o.ok;
+        a.cap_vertices[l] = (fits && verts > 0) ? verts : 0;
+        a.cap_triangles[l] = (fits && verts > 0) ? o.nt : 0;
+    }
+    if (threadIdx.x == 0) {
+        SegOut t;
+        t.nv = overflow;
+        t.nt = 0;
+        t.oc = carry[0];
+        t.on = carry[1];
+        t.ok = carry[2];
+        out[nseg] = t;
+    }
+}
+
+}  // namespace kmc
+
+using namespace kmc;
+
+/* ------------------------------------------------------------------------- */
+/* host side                                                                   */
+/* ------------------------------------------------------------------------- */
+
+namespace {
+
+struct UserJob {
+    kmc_volume volume;           // values point at the device copy
+    std::vector<double> levels;
+    int normal_type;
+    int dup;
+    double mn, mx;
+    float minc[3], maxc[3];
+    bool empty;                  // Math::Equal(min, max)
+    uint32_t first_surface;
+};
+
+struct Pass {
+    uint32_t user;
+    uint32_t level0;
+    uint32_t nlev;
+    int dtype;
+    JobArgs a;
+    uint32_t lds;
+    uint64_t counts_off;         // first element of its counts in the shared arrays
+    uint64_t bitvol_off;
+    uint32_t seg0;               // first surface segment
+};
+
+struct Group {
+    int dtype;
+    uint32_t p0, p1;             // passes [p0, p1)
+    uint64_t rows, tiles;
+    uint32_t emit_lds, count_lds;
+    uint64_t* row_start_d;       // p1 - p0 + 1 entries
+    uint64_t* tile_start_d;
+};
+
+}  // namespace
+
+struct kmc_batch {
+    int device;
+    uint32_t flags;
+    std::vector<UserJob> users;
+    std::vector<void*> owned_values;
+    uint32_t n_surfaces;
+    // plan
+    uint32_t r_max, queue_cap;
+    std::vector<Pass> passes;
+    std::vector<Group> groups;
+    std::vector<uint32_t> seg_pass_level;   // (pass << 3) | level, per segment
+    std::vector<uint32_t> surface_seg;      // surface -> segment (or ~0u when the job is empty)
+    std::vector<uint32_t> surface_user;     // surface -> job
+    JobArgs* jobs_d;
+    uint32_t* seg_map_d;
+    uint32_t* counts_d;
+    uint64_t* prefix_d;
+    uint64_t* scan_aux_d;
+    uint64_t n_counts;
+    uint32_t scan_parts;
+    uint64_t* bitvol_d;
+    uint64_t bitvol_words;
+    SegOut* segout_d;
+    SegOut* segout_h;                       // pinned
+    float* arena_c;
+    float* arena_n;
+    uint32_t* arena_k;
+    uint64_t cap_c, cap_n, cap_k;
+    hipStream_t cap_stream;
+    hipGraph_t graph;
+    hipGraphExec_t exec;
+    hipEvent_t ev0, ev1;
+    uint32_t graph_nodes;
+    uint32_t recaptures;
+    double last_ms;
+    bool have_results;
+};
+
+namespace {
+
+void free_plan_buffers(kmc_batch* b)
+{
+    for (Group& g : b->groups) {
+        device_free(g.row_start_d);
+        device_free(g.tile_start_d);
+        g.row_start_d = g.tile_start_d = nullptr;
+    }
+}
+
+void destroy_graph(kmc_batch* b)
+{
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+}
+
+void destroy(kmc_batch* b)
+{
+    if (!b) return;
+    (void)hipSetDevice(b->device);
+    if (b->cap_stream) (void)hipStreamSynchronize(b->cap_stream);
+    destroy_graph(b);
+    free_plan_buffers(b);
+    device_free(b->jobs_d);
+    device_free(b->seg_map_d);
+    device_free(b->counts_d);
+    device_free(b->prefix_d);
+    device_free(b->scan_aux_d);
+    device_free(b->bitvol_d);
+    device_free(b->segout_d);
+    device_free(b->arena_c);
+    device_free(b->arena_n);
+    device_free(b->arena_k);
+    if (b->segout_h) (void)hipHostFree(b->segout_h);
+    for (void* p : b->owned_values) device_free(p);
+    if (b->ev0) (void)hipEventDestroy(b->ev0);
+    if (b->ev1) (void)hipEventDestroy(b->ev1);
+    if (b->cap_stream) (void)hipStreamDestroy(b->cap_stream);
+    delete b;
+}
+
+/* Tile geometry of every pass for (r_max, queue_cap); tile tables per group. */
+kmc_status plan_tiles(kmc_batch* b, uint32_t r_max, uint32_t queue_cap)
+{
+    free_plan_buffers(b);
+    b->r_max = r_max;
+    b->queue_cap = queue_cap;
+    for (Pass& p : b->passes)
+        if (!plan_geometry(p.a, &p.lds, r_max, queue_cap)) return KMC_ERR_INVALID_ARGUMENT;
+    for (Group& g : b->groups) {
+        std::vector<uint64_t> rs, ts;
+        uint64_t r = 0, t = 0;
+        g.emit_lds = g.count_lds = 0;
+        for (uint32_t i = g.p0; i < g.p1; ++i) {
+            rs.push_back(r);
+            ts.push_back(t);
+            r += b->passes[i].a.nrows;
+            t += b->passes[i].a.ntiles;
+            g.emit_lds = std::max(g.emit_lds, b->passes[i].lds);
+            g.count_lds = std::max(g.count_lds, count_lds_bytes(b->passes[i].a));
+        }
+        rs.push_back(r);
+        ts.push_back(t);
+        g.rows = r;
+        g.tiles = t;
+        if (t > 0xFFFFFFFFull) return KMC_ERR_INVALID_ARGUMENT;
+        g.row_start_d = (uint64_t*)device_alloc(rs.size() * 8);
+        g.tile_start_d = (uint64_t*)device_alloc(ts.size() * 8);
+        if (!g.row_start_d || !g.tile_start_d) return KMC_ERR_OUT_OF_MEMORY;
+        if (hipMemcpy(g.row_start_d, rs.data(), rs.size() * 8, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(g.tile_start_d, ts.data(), ts.size() * 8, hipMemcpyHostToDevice) != hipSuccess)
+            return KMC_ERR_DEVICE;
+    }
+    std::vector<JobArgs> host(b->passes.size());
+    for (size_t i = 0; i < b->passes.size(); ++i) host[i] = b->passes[i].a;
+    if (!host.empty() &&
+        hipMemcpy(b->jobs_d, host.data(), host.size() * sizeof(JobArgs), hipMemcpyHostToDevice) != hipSuccess)
+        return KMC_ERR_DEVICE;
+    return KMC_OK;
+}
+
+/* Record the batch's launches on st. */
+hipError_t record(kmc_batch* b, hipStream_t st)
+{
+    const uint32_t nseg = (uint32_t)b->seg_pass_level.size();
+    if (nseg == 0) return hipSuccess;
+    hipError_t e = hipSuccess;
+    for (const Group& g : b->groups) {
+        e = launch_classify_batch(g.dtype, b->jobs_d + g.p0, g.row_start_d, g.p1 - g.p0, g.rows, st);
+        if (e != hipSuccess) return e;
+    }
+    for (const Group& g : b->groups) {
+        e = launch_count_batch(b->jobs_d + g.p0, g.tile_start_d, g.p1 - g.p0, g.tiles, g.count_lds, st);
+        if (e != hipSuccess) return e;
+    }
+    uint64_t* partials = b->scan_aux_d;
+    uint64_t* totals = b->scan_aux_d + b->scan_parts;
+    e = launch_scan(b->counts_d, b->n_counts, 1, partials, b->prefix_d, totals, st);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_batch_layout, dim3(1), dim3(kLayoutBlock), 0, st, b->jobs_d, b->seg_map_d, nseg, b->arena_c,
+                       b->arena_n, b->arena_k, b->cap_c, b->cap_n, b->cap_k, b->segout_d);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    for (const Group& g : b->groups) {
+        e = launch_emit_batch(g.dtype, b->jobs_d + g.p0, g.tile_start_d, g.p1 - g.p0, g.tiles, g.emit_lds, st);
+        if (e != hipSuccess) return e;
+    }
+    return hipMemcpyAsync(b->segout_h, b->segout_d, (nseg + 1) * sizeof(SegOut), hipMemcpyDeviceToHost, st);
+}
+
+kmc_status capture(kmc_batch* b)
+{
+    destroy_graph(b);
+    if (hipStreamBeginCapture(b->cap_stream, hipStreamCaptureModeThreadLocal) != hipSuccess) return KMC_ERR_DEVICE;
+    const hipError_t rec = record(b, b->cap_stream);
+    hipGraph_t g = nullptr;
+    const hipError_t end = hipStreamEndCapture(b->cap_stream, &g);
+    if (rec != hipSuccess || end != hipSuccess || !g) {
+        if (g) (void)hipGraphDestroy(g);
+        (void)hipGetLastError();
+        return KMC_ERR_DEVICE;
+    }
+    b->graph = g;
+    size_t n = 0;
+    (void)hipGraphGetNodes(g, nullptr, &n);
+    b->graph_nodes = (uint32_t)n;
+    if (hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0) != hipSuccess) {
+        (void)hipGetLastError();
+        return KMC_ERR_DEVICE;
+    }
+    ++b->recaptures;
+    return KMC_OK;
+}
+
+kmc_status grow_arena(kmc_batch* b, uint64_t need_c, uint64_t need_n, uint64_t need_k)
+{
+    auto grow = [](void*& p, uint64_t& cap, uint64_t need, size_t esz) -> bool {
+        if (need <= cap) return true;
+        device_free(p);
+        const uint64_t n = need + need / 8 + 1024;   // headroom for runs over changed values
+        p = device_alloc(n * esz);
+        cap = p ? n : 0;
+        return p != nullptr;
+    };
+    void* c = b->arena_c; void* n = b->arena_n; void* k = b->arena_k;
+    const bool ok = grow(c, b->cap_c, need_c, 4) && grow(n, b->cap_n, need_n, 4) && grow(k, b->cap_k, need_k, 4);
+    b->arena_c = (float*)c; b->arena_n = (float*)n; b->arena_k = (uint32_t*)k;
+    return ok ? KMC_OK : KMC_ERR_OUT_OF_MEMORY;
+}
+
+/* Launch the graph on st and wait; grows the arena and reruns on overflow. */
+kmc_status run(kmc_batch* b, hipStream_t st)
+{
+    const uint32_t nseg = (uint32_t)b->seg_pass_level.size();
+    b->have_results = false;
+    if (nseg == 0) { b->have_results = true; b->last_ms = 0; return KMC_OK; }
+    for (int attempt = 0; attempt < 3; ++attempt) {
+        if (!b->exec) {
+            const kmc_status s = capture(b);
+            if (s != KMC_OK) return s;
+        }
+        if (hipEventRecord(b->ev0, st) != hipSuccess || hipGraphLaunch(b->exec, st) != hipSuccess ||
+            hipEventRecord(b->ev1, st) != hipSuccess || hipStreamSynchronize(st) != hipSuccess) {
+            (void)hipGetLastError();
+            return KMC_ERR_DEVICE;
+        }
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, b->ev0, b->ev1);
+        b->last_ms = ms;
+        const SegOut& t = b->segout_h[nseg];
+        if (t.nv == 0) { b->have_results = true; return KMC_OK; }
+        const kmc_status s = grow_arena(b, t.oc, t.on, t.ok);
+        if (s != KMC_OK) return s;
+        destroy_graph(b);
+    }
+    return KMC_ERR_DEVICE;
+}
+
+/* Time the candidate tile shapes on this batch; keep the fastest. */
+kmc_status autotune(kmc_batch* b, hipStream_t st)
+{
+    struct Cand { uint32_t r, q; };
+    const Cand cands[] = { { 8, 1024 }, { 4, 1024 }, { 2, 1024 }, { 8, 512 }, { 4, 512 }, { 8, 2048 } };
+    double best = 1e30;
+    Cand best_c = { b->r_max, b->queue_cap };
+    for (const Cand& c : cands) {
+        kmc_status s = plan_tiles(b, c.r, c.q);
+        if (s == KMC_OK) { destroy_graph(b); s = run(b, st); }
+        if (s != KMC_OK) continue;          // a shape whose LDS does not fit is skipped
+        double t = 1e30;
+        for (int i = 0; i < 3 && s == KMC_OK; ++i) {
+            s = run(b, st);
+            t = std::min(t, b->last_ms);
+        }
+        if (s == KMC_OK && t < best) { best = t; best_c = c; }
+    }
+    kmc_status s = plan_tiles(b, best_c.r, best_c.q);
+    if (s == KMC_OK) { destroy_graph(b); s = capture(b); }
+    return s;
+}
+
+kmc_status create_impl(const kmc_job* jobs, uint32_t n_jobs, uint32_t flags, hipStream_t st, kmc_batch** out)
+{
+    if (!out || (n_jobs > 0 && !jobs)) return KMC_ERR_INVALID_ARGUMENT;
+    *out = nullptr;
+    for (uint32_t j = 0; j < n_jobs; ++j) {
+        const kmc_status s = validate_volume(&jobs[j].volume);
+        if (s != KMC_OK) return s;
+        if (jobs[j].n_levels > 0 && !jobs[j].isolevels) return KMC_ERR_INVALID_ARGUMENT;
+        if (jobs[j].normal_type != KMC_VERTEX_NORMAL && jobs[j].normal_type != KMC_POLYGON_NORMAL)
+            return KMC_ERR_INVALID_ARGUMENT;
+    }
+    kmc_status s = ensure_device();
+    if (s != KMC_OK) return s;
+
+    kmc_batch* b = new kmc_batch();
+    (void)hipGetDevice(&b->device);
+    b->flags = flags;
+    b->r_max = 8;
+    b->queue_cap = 1024;
+    if (hipStreamCreateWithFlags(&b->cap_stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&b->ev0) != hipSuccess || hipEventCreate(&b->ev1) != hipSuccess) {
+        (void)hipGetLastError();
+        destroy(b);
+        return KMC_ERR_DEVICE;
+    }
+
+    /* jobs: device values, min/max, object coordinates */
+    b->users.resize(n_jobs);
+    uint32_t nsurf = 0;
+    for (uint32_t j = 0; j < n_jobs && s == KMC_OK; ++j) {
+        UserJob& u = b->users[j];
+        u.volume = jobs[j].volume;
+        u.levels.assign(jobs[j].isolevels, jobs[j].isolevels + jobs[j].n_levels);
+        u.normal_type = jobs[j].normal_type;
+        u.dup = (u.normal_type == KMC_VERTEX_NORMAL) ? 0 : (jobs[j].duplication ? 1 : 0);   // MarchingCubes.cpp:69-73
+        u.first_surface = nsurf;
+        nsurf += jobs[j].n_levels;
+        const void* dvals = nullptr;
+        void* owned = nullptr;
+        s = device_values(&u.volume, st, &dvals, &owned);
+        if (s != KMC_OK) break;
+        if (owned) b->owned_values.push_back(owned);
+        u.volume.values = dvals;
+        u.volume.memory = KMC_MEMORY_DEVICE;
+        u.mn = u.volume.min_value;
+        u.mx = u.volume.max_value;
+        if (!u.volume.has_min_max_values) s = min_max_on_device(&u.volume, dvals, st, &u.mn, &u.mx);
+        object_coords(&u.volume, u.minc, u.maxc);
+        const double d = u.mn - u.mx;
+        u.empty = (d < 0 ? -d : d) < 1.e-6;                    // MarchingCubes.cpp:108-110
+    }
+    if (s == KMC_OK && hipStreamSynchronize(st) != hipSuccess) s = KMC_ERR_DEVICE;
+    if (s != KMC_OK) { destroy(b); return s; }
+    b->n_surfaces = nsurf;
+    b->surface_seg.assign(nsurf, ~0u);
+    b->surface_user.resize(nsurf);
+    for (uint32_t j = 0; j < n_jobs; ++j)
+        for (uint32_t l = 0; l < b->users[j].levels.size(); ++l) b->surface_user[b->users[j].first_surface + l] = j;
+
+    /* passes, grouped by value type */
+    for (int t = 0; t <= KMC_REAL64; ++t) {
+        Group g{};
+        g.dtype = t;
+        g.p0 = (uint32_t)b->passes.size();
+        for (uint32_t j = 0; j < n_jobs; ++j) {
+            const UserJob& u = b->users[j];
+            if (u.volume.type != t || u.empty) continue;
+            const uint32_t per = levels_per_pass(&u.volume);
+            if (per == 0) { destroy(b); return KMC_ERR_INVALID_ARGUMENT; }
+            for (uint32_t l0 = 0; l0 < u.levels.size(); l0 += per) {
+                Pass p{};
+                p.user = j;
+                p.level0 = l0;
+                p.nlev = std::min<uint32_t>((uint32_t)u.levels.size() - l0, per);
+                p.dtype = t;
+                init_job(p.a, &u.volume, u.volume.values, u.levels.data() + l0, p.nlev, u.normal_type, u.dup, u.mn,
+                         u.mx, u.minc, u.maxc);
+                b->passes.push_back(p);
+            }
+        }
+        g.p1 = (uint32_t)b->passes.size();
+        if (g.p1 > g.p0) b->groups.push_back(g);
+    }
+
+    /* shared counts / prefixes / masks, surface segments */
+    uint64_t coff = 0, boff = 0;
+    for (uint32_t i = 0; i < b->passes.size(); ++i) {
+        Pass& p = b->passes[i];
+        JobArgs& a = p.a;
+        a.nrows = (uint64_t)a.nz * a.ny;
+        a.NC = (a.nx + 63) / 64;
+        p.counts_off = coff;
+        p.bitvol_off = boff;
+        coff += 2ull * p.nlev * a.nrows;
+        boff += (uint64_t)p.nlev * a.nz * a.ny * a.NC;
+        p.seg0 = (uint32_t)b->seg_pass_level.size();
+        for (uint32_t l = 0; l < p.nlev; ++l) {
+            b->surface_seg[b->users[p.user].first_surface + p.level0 + l] = (uint32_t)b->seg_pass_level.size();
+            b->seg_pass_level.push_back((i << 3) | l);
+        }
+    }
+    b->n_counts = coff;
+    b->bitvol_words = boff;
+    const uint32_t nseg = (uint32_t)b->seg_pass_level.size();
+    if (!b->passes.empty()) {
+        b->scan_parts = scan_partials_needed(coff, 1);
+        b->counts_d = (uint32_t*)device_alloc(coff * 4);
+        b->prefix_d = (uint64_t*)device_alloc(coff * 8);
+        b->scan_aux_d = (uint64_t*)device_alloc((b->scan_parts + 1) * 8);
+        b->bitvol_d = (uint64_t*)device_alloc(boff * 8);
+        b->jobs_d = (JobArgs*)device_alloc(b->passes.size() * sizeof(JobArgs));
+        b->seg_map_d = (uint32_t*)device_alloc(nseg * 4);
+        b->segout_d = (SegOut*)device_alloc((nseg + 1) * sizeof(SegOut));
+        if (hipHostMalloc((void**)&b->segout_h, (nseg + 1) * sizeof(SegOut), hipHostMallocDefault) != hipSuccess)
+            b->segout_h = nullptr;
+        if (!b->counts_d || !b->prefix_d || !b->scan_aux_d || !b->bitvol_d || !b->jobs_d || !b->seg_map_d ||
+            !b->segout_d || !b->segout_h) {
+            (void)hipGetLastError();
+            destroy(b);
+            return KMC_ERR_OUT_OF_MEMORY;
+        }
+        for (Pass& p : b->passes) {
+            p.a.counts = b->counts_d + p.counts_off;
+            p.a.prefix = b->prefix_d + p.counts_off;
+            p.a.bitvol = b->bitvol_d + p.bitvol_off;
+        }
+        if (hipMemcpy(b->seg_map_d, b->seg_pass_level.data(), nseg * 4, hipMemcpyHostToDevice) != hipSuccess) {
+            destroy(b);
+            return KMC_ERR_DEVICE;
+        }
+    }
+    s = plan_tiles(b, 8, 1024);
+    if (s == KMC_OK) s = run(b, st);            // sizes the arena (first capture writes nothing)
+    if (s == KMC_OK && (flags & KMC_BATCH_AUTOTUNE)) s = autotune(b, st);
+    if (s != KMC_OK) { destroy(b); return s; }
+    *out = b;
+    return KMC_OK;
+}
+
+void surface_view(const kmc_batch* b, uint32_t i, kmc_surface* o)
+{
+    const UserJob& u = b->users[b->surface_user[i]];
+    const uint32_t l = i - u.first_surface;
+    init_surface(o, u.levels[l], u.normal_type, u.dup, u.mn, u.mx, u.minc, u.maxc);
+    o->memory = KMC_MEMORY_DEVICE;
+    if (u.empty) { o->empty_range = 1; return; }
+    const uint32_t sg = b->surface_seg[i];
+    const SegOut& so = b->segout_h[sg];
+    const uint64_t verts = u.dup ? 3 * so.nt : so.nv;
+    if (verts == 0) return;
+    o->n_vertices = verts;
+    o->n_triangles = so.nt;
+    o->n_connections = u.dup ? 0 : 3 * so.nt;
+    o->n_normals = (!u.dup && u.normal_type == KMC_VERTEX_NORMAL) ? so.nv : so.nt;
+    o->coords = b->arena_c + so.oc;
+    o->normals = b->arena_n + so.on;
+    o->connections = u.dup ? nullptr : b->arena_k + so.ok;
+}
+
+}  // namespace
+
+extern "C" {
+
+kmc_status kmc_batch_create(const kmc_job* jobs, uint32_t n_jobs, uint32_t flags, void* stream, kmc_batch** out)
+{
+    const kmc_status s = create_impl(jobs, n_jobs, flags, (hipStream_t)stream, out);
+    report(s);
+    return s;
+}
+
+kmc_status kmc_batch_run(kmc_batch* b, void* stream)
+{
+    if (!b) { report(KMC_ERR_INVALID_ARGUMENT); return KMC_ERR_INVALID_ARGUMENT; }
+    if (hipSetDevice(b->device) != hipSuccess) { report(KMC_ERR_DEVICE); return KMC_ERR_DEVICE; }
+    const kmc_status s = run(b, (hipStream_t)stream);
+    report(s);
+    return s;
+}
+
+kmc_status kmc_batch_surfaces(const kmc_batch* b, kmc_surface* outs, uint32_t n_outs)
+{
+    if (!b || !outs || n_outs < b->n_surfaces) return KMC_ERR_INVALID_ARGUMENT;
+    if (!b->have_results) return KMC_ERR_NOT_READY;
+    for (uint32_t i = 0; i < b->n_surfaces; ++i) surface_view(b, i, &outs[i]);
+    return KMC_OK;
+}
+
+kmc_status kmc_batch_info_get(const kmc_batch* b, kmc_batch_info* info)
+{
+    if (!b || !info) return KMC_ERR_INVALID_ARGUMENT;
+    std::memset(info, 0, sizeof(*info));
+    info->n_jobs = (uint32_t)b->users.size();
+    info->n_surfaces = b->n_surfaces;
+    info->n_passes = (uint32_t)b->passes.size();
+    info->n_type_groups = (uint32_t)b->groups.size();
+    info->tile_rows = b->r_max;
+    info->queue_capacity = b->queue_cap;
+    info->graph_nodes = b->graph_nodes;
+    info->recaptures = b->recaptures;
+    for (const Pass& p : b->passes) info->nodes_classified += (uint64_t)p.a.nx * p.a.ny * p.a.nz * p.nlev;
+    if (b->have_results)
+        for (uint32_t s = 0; s < b->seg_pass_level.size(); ++s) info->n_triangles += b->segout_h[s].nt;
+    info->arena_bytes = 4 * (b->cap_c + b->cap_n + b->cap_k);
+    info->work_bytes = b->n_counts * 12 + b->bitvol_words * 8;
+    info->last_run_ms = b->last_ms;
+    return KMC_OK;
+}
+
+void kmc_batch_destroy(kmc_batch* b) { destroy(b); }
+
+kmc_status kmc_batch_extract(const kmc_job* jobs, uint32_t n_jobs, int32_t out_memory, void* stream,
+                             kmc_surface* outs)
+{
+    if (out_memory != KMC_MEMORY_HOST && out_memory != KMC_MEMORY_DEVICE) {
+        report(KMC_ERR_INVALID_ARGUMENT);
+        return KMC_ERR_INVALID_ARGUMENT;
+    }
+    kmc_batch* b = nullptr;
+    kmc_status s = create_impl(jobs, n_jobs, 0, (hipStream_t)stream, &b);
+    if (s != KMC_OK) { report(s); return s; }
+    if (!outs && b->n_surfaces > 0) { destroy(b); report(KMC_ERR_INVALID_ARGUMENT); return KMC_ERR_INVALID_ARGUMENT; }
+    hipStream_t st = (hipStream_t)stream;
+    const hipMemcpyKind kind = out_memory == KMC_MEMORY_HOST ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice;
+    uint32_t done = 0;
+    for (uint32_t i = 0; i < b->n_surfaces && s == KMC_OK; ++i, ++done) {
+        kmc_surface v;
+        surface_view(b, i, &v);
+        SurfaceStorage* store = new SurfaceStorage();
+        store->memory = out_memory;
+        store->coords = store->connections = store->normals = nullptr;
+        const size_t bc = v.n_vertices * 12, bn = v.n_normals * 12, bk = v.n_connections * 4;
+        auto alloc = [&](void** p, size_t n) -> bool {
+            if (n == 0) return true;
+            if (out_memory == KMC_MEMORY_HOST) return hipHostMalloc(p, n, hipHostMallocDefault) == hipSuccess;
+            *p = device_alloc(n);
+            return *p != nullptr;
+        };
+        bool ok = alloc(&store->coords, bc) && alloc(&store->normals, bn) && alloc(&store->connections, bk);
+        if (ok && bc) ok = hipMemcpyAsync(store->coords, v.coords, bc, kind, st) == hipSuccess;
+        if (ok && bn) ok = hipMemcpyAsync(store->normals, v.normals, bn, kind, st) == hipSuccess;
+        if (ok && bk) ok = hipMemcpyAsync(store->connections, v.connections, bk, kind, st) == hipSuccess;
+        v.memory = out_memory;
+        v.coords = (float*)store->coords;
+        v.normals = (float*)store->normals;
+        v.connections = (uint32_t*)store->connections;
+        v.internal = store;
+        outs[i] = v;
+        if (!ok) { (void)hipGetLastError(); s = KMC_ERR_OUT_OF_MEMORY; }
+    }
+    if (s == KMC_OK && hipStreamSynchronize(st) != hipSuccess) s = KMC_ERR_DEVICE;
+    if (s != KMC_OK)
+        for (uint32_t i = 0; i < done; ++i) kmc_surface_release(&outs[i]);
+    destroy(b);
+    report(s);
+    return s;
+}
+
+}  // extern "C"

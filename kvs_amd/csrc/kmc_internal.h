@@ -2,7 +2,12 @@
  * kmc_internal.h -- helpers shared by the host-side modules of libkmc.
  */
 #pragma once
+#include <hip/hip_runtime.h>
 #include <stddef.h>
+#include <stdint.h>
+
+#include "kmc.h"
+#include "kmc_job.h"
 
 namespace kmc {
 
@@ -10,5 +15,54 @@ namespace kmc {
 void* device_alloc(size_t bytes);
 void device_free(void* p);
 void device_pool_trim();
+
+/* Arrays owned by a kmc_surface (its `internal` handle). */
+struct SurfaceStorage {
+    int memory;
+    void* coords;
+    void* connections;
+    void* normals;
+};
+void free_storage(SurfaceStorage* s);
+
+size_t type_size(int t);
+float float_threshold(double iso);
+kmc_status validate_volume(const kmc_volume* v);
+void report(kmc_status s);
+kmc_status ensure_device();
+kmc_status device_values(const kmc_volume* v, hipStream_t st, const void** dvals, void** owned);
+kmc_status min_max_on_device(const kmc_volume* v, const void* dvals, hipStream_t st, double* mn, double* mx);
+void object_coords(const kmc_volume* v, float minc[3], float maxc[3]);
+void init_surface(kmc_surface* s, double iso, int normal_type, int dup, double mn, double mx,
+                  const float minc[3], const float maxc[3]);
+
+/* Fill the per-job fields of a (geometry, levels, clamp range, coordinates). */
+void init_job(JobArgs& a, const kmc_volume* v, const void* dvals, const double* iso, uint32_t nlev, int normal_type,
+              int dup, double mn, double mx, const float minc[3], const float maxc[3]);
+
+/* Tile geometry and emit LDS size: the largest band height R <= r_max whose
+ * LDS state fits.  False if none does. */
+bool plan_geometry(JobArgs& a, uint32_t* lds_bytes, uint32_t r_max = 8, uint32_t queue_cap = 1024);
+
+/* The most isolevels (<= kMaxLevels) one pass over this volume can hold in
+ * the emit LDS at the default queue size; 0 if not even one fits. */
+uint32_t levels_per_pass(const kmc_volume* v);
+
+/* Kernel launchers (kmc_kernels.hip). */
+hipError_t launch_emit(int dtype, const JobArgs& a, uint32_t lds_bytes, hipStream_t st);
+hipError_t launch_classify(int dtype, const JobArgs& a, hipStream_t st);
+hipError_t launch_count(const JobArgs& a, hipStream_t st);
+hipError_t launch_scan(const uint32_t* counts, uint64_t n, uint32_t narr, uint64_t* partials,
+                       uint64_t* prefix, uint64_t* totals, hipStream_t st);
+uint32_t scan_partials_needed(uint64_t n, uint32_t narr);
+hipError_t launch_minmax(int dtype, const void* vol, uint64_t n, void* scratch, double* out, hipStream_t st);
+size_t minmax_scratch_bytes();
+uint32_t count_lds_bytes(const JobArgs& a);
+hipError_t launch_classify_batch(int dtype, const JobArgs* jobs, const uint64_t* row_start, uint32_t njobs,
+                                 uint64_t rows, hipStream_t st);
+hipError_t launch_count_batch(const JobArgs* jobs, const uint64_t* tile_start, uint32_t njobs, uint64_t tiles,
+                              uint32_t lds_bytes, hipStream_t st);
+hipError_t launch_emit_batch(int dtype, const JobArgs* jobs, const uint64_t* tile_start, uint32_t njobs,
+                             uint64_t tiles, uint32_t lds_bytes, hipStream_t st);
 
 }  // namespace kmc

@@ -775,11 +775,6 @@ __device__ void stage_tables(const JobArgs& a, const Views& v)
 /* classify pass: values -> bit volume                                         */
 /* ------------------------------------------------------------------------- */
 
-/* Streams the volume once.  Each wave owns whole rows: its lanes load 64
- * consecutive nodes per chunk (one coalesced 64-lane access), a ballot per
- * level turns the comparisons into the chunk mask, and the masks of up to 64
- * chunks of a row are gathered into lanes and stored with one coalesced
- * 64-bit store per level. */
 /* Predicate of level l for a value (double(v) > iso[l]). */
 template <typename T>
 __device__ __forceinline__ bool above(T v, const JobArgs& a, int l) { return (double)v > a.iso[l]; }
@@ -793,49 +788,99 @@ __device__ __forceinline__ bool above<float>(float v, const JobArgs& a, int l) {
  * row leave with one coalesced 64-bit store per level.  NLEV is the number of
  * levels, a compile-time constant so that nothing is guarded per level. */
 template <typename T, int NLEV>
-__global__ __launch_bounds__(kBlock) void k_classify(JobArgs a)
+__device__ __forceinline__ void classify_row(const JobArgs& a, uint64_t row)
 {
     const uint32_t lane = threadIdx.x & 63u;
-    const uint64_t nrows = (uint64_t)a.ny * a.nz;
-    const uint64_t wave_id = ((uint64_t)blockIdx.x * kBlock + threadIdx.x) >> 6;
-    const uint64_t nwaves = ((uint64_t)gridDim.x * kBlock) >> 6;
-    const T* vol = (const T*)a.values;
+    const T* src = (const T*)a.values + row * a.nx;
+    const uint64_t z = row / a.ny, y = row - z * a.ny;
     constexpr uint32_t kBatch = 16;
-    for (uint64_t row = wave_id; row < nrows; row += nwaves) {
-        const T* src = vol + row * a.nx;
-        const uint64_t z = row / a.ny, y = row - z * a.ny;
-        for (uint32_t c0 = 0; c0 < a.NC; c0 += 64) {
-            uint32_t lo[NLEV], hi[NLEV];
+    for (uint32_t c0 = 0; c0 < a.NC; c0 += 64) {
+        uint32_t lo[NLEV], hi[NLEV];
 #pragma unroll
-            for (int l = 0; l < NLEV; ++l) { lo[l] = 0; hi[l] = 0; }
-            const uint32_t cend = (a.NC - c0) < 64u ? (a.NC - c0) : 64u;
-            for (uint32_t cb = 0; cb < cend; cb += kBatch) {
-                T val[kBatch];
+        for (int l = 0; l < NLEV; ++l) { lo[l] = 0; hi[l] = 0; }
+        const uint32_t cend = (a.NC - c0) < 64u ? (a.NC - c0) : 64u;
+        for (uint32_t cb = 0; cb < cend; cb += kBatch) {
+            T val[kBatch];
 #pragma unroll
-                for (uint32_t i = 0; i < kBatch; ++i) {
-                    const uint32_t x = (c0 + cb + i) * 64u + lane;
-                    val[i] = (cb + i < cend && x < a.nx) ? src[x] : T(0);
-                }
+            for (uint32_t i = 0; i < kBatch; ++i) {
+                const uint32_t x = (c0 + cb + i) * 64u + lane;
+                val[i] = (cb + i < cend && x < a.nx) ? src[x] : T(0);
+            }
 #pragma unroll
-                for (uint32_t i = 0; i < kBatch; ++i) {
-                    if (cb + i < cend) {
-                        const bool in = (c0 + cb + i) * 64u + lane < a.nx;
-                        const bool mine = lane == cb + i;
+            for (uint32_t i = 0; i < kBatch; ++i) {
+                if (cb + i < cend) {
+                    const bool in = (c0 + cb + i) * 64u + lane < a.nx;
+                    const bool mine = lane == cb + i;
 #pragma unroll
-                        for (int l = 0; l < NLEV; ++l) {
-                            const uint64_t m = __ballot(in && above<T>(val[i], a, l));
-                            lo[l] = mine ? (uint32_t)m : lo[l];
-                            hi[l] = mine ? (uint32_t)(m >> 32) : hi[l];
-                        }
+                    for (int l = 0; l < NLEV; ++l) {
+                        const uint64_t m = __ballot(in && above<T>(val[i], a, l));
+                        lo[l] = mine ? (uint32_t)m : lo[l];
+                        hi[l] = mine ? (uint32_t)(m >> 32) : hi[l];
                     }
                 }
             }
-            if (lane < cend) {
+        }
+        /* the 8-level instance also serves passes of 5..7 levels: store only
+         * the pass's own levels */
+        if (lane < cend) {
 #pragma unroll
-                for (int l = 0; l < NLEV; ++l)
-                    a.bitvol[(((uint64_t)l * a.nz + z) * a.ny + y) * a.NC + c0 + lane] =
-                        ((uint64_t)hi[l] << 32) | lo[l];
-            }
+            for (int l = 0; l < NLEV; ++l)
+                if (NLEV < 8 || l < (int)a.nlev)
+                    a.bitvol[(((uint64_t)l * a.nz + z) * a.ny + y) * a.NC + c0 + lane] = ((uint64_t)hi[l] << 32) | lo[l];
+        }
+    }
+}
+
+/* Streams the volume once.  Each wave owns whole rows: its lanes load 64
+ * consecutive nodes per chunk (one coalesced access), the comparison against
+ * level l is a ballot (v_cmp into a lane mask), and the mask is parked in
+ * lane c of a register by a lane-select; the masks of up to 64 chunks of a
+ * row leave with one coalesced 64-bit store per level.  NLEV is the number of
+ * levels, a compile-time constant so that nothing is guarded per level. */
+template <typename T, int NLEV>
+__global__ __launch_bounds__(kBlock) void k_classify(JobArgs a)
+{
+    const uint64_t nrows = (uint64_t)a.ny * a.nz;
+    const uint64_t wave_id = ((uint64_t)blockIdx.x * kBlock + threadIdx.x) >> 6;
+    const uint64_t nwaves = ((uint64_t)gridDim.x * kBlock) >> 6;
+    for (uint64_t row = wave_id; row < nrows; row += nwaves) classify_row<T, NLEV>(a, row);
+}
+
+/* Index j with start[j] <= g < start[j+1] (start has n+1 ascending entries). */
+__device__ __forceinline__ uint32_t find_job(const uint64_t* start, uint32_t n, uint64_t g)
+{
+    uint32_t lo = 0, hi = n;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (start[mid] <= g) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+/* Classify pass of a batch of jobs sharing the value type T: the node rows of
+ * all jobs form one sequence (row_start[j] = first row of job j), cut into one
+ * contiguous range per wave so that the job changes rarely along a range. */
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_classify_batch(const JobArgs* jobs, const uint64_t* row_start, uint32_t njobs)
+{
+    const uint64_t total = row_start[njobs];
+    const uint64_t wave_id = ((uint64_t)blockIdx.x * kBlock + threadIdx.x) >> 6;
+    const uint64_t nwaves = ((uint64_t)gridDim.x * kBlock) >> 6;
+    const uint64_t per = (total + nwaves - 1) / nwaves;
+    uint64_t g = wave_id * per;
+    const uint64_t end = g + per < total ? g + per : total;
+    if (g >= end) return;
+    uint32_t j = __builtin_amdgcn_readfirstlane(find_job(row_start, njobs, g));
+    for (; g < end; ++g) {
+        while (g >= row_start[j + 1]) ++j;
+        const JobArgs& a = jobs[j];
+        const uint64_t row = g - row_start[j];
+        switch (a.nlev) {
+        case 1: classify_row<T, 1>(a, row); break;
+        case 2: classify_row<T, 2>(a, row); break;
+        case 3: classify_row<T, 3>(a, row); break;
+        case 4: classify_row<T, 4>(a, row); break;
+        default: classify_row<T, 8>(a, row); break;
         }
     }
 }
@@ -848,7 +893,7 @@ __global__ __launch_bounds__(kBlock) void k_classify(JobArgs a)
  * it owns, and for cell row (y, z-1) its triangles.  LDS holds only the chunk
  * masks of slices z-1, z, z+1 (rows y0-1 .. y0+R+1) and the case table; a
  * wave takes one (level, row) at a time with its lanes on the row's chunks. */
-__global__ __launch_bounds__(kBlock) void k_count(JobArgs a)
+__device__ void count_tile(const JobArgs& a, uint64_t tile)
 {
     extern __shared__ __align__(16) uint8_t smem[];
     const LdsLayout L = lds_layout(a.R, a.NC, a.nlev, 0, 0);
@@ -856,7 +901,6 @@ __global__ __launch_bounds__(kBlock) void k_count(JobArgs a)
     v.tri = (uint64_t*)(smem + L.off_tri);
     v.bits = (uint64_t*)(smem + L.off_bits);
     v.nrb = L.nrb; v.ncp = L.ncp; v.R = a.R; v.NC = a.NC; v.nlev = a.nlev;
-    const uint64_t tile = blockIdx.x;
     const int z = (int)(tile / a.nb);
     const uint32_t band = (uint32_t)(tile - (uint64_t)z * a.nb);
     const int y0 = (int)(band * a.R);
@@ -917,6 +961,15 @@ __global__ __launch_bounds__(kBlock) void k_count(JobArgs a)
     }
 }
 
+__global__ __launch_bounds__(kBlock) void k_count(JobArgs a) { count_tile(a, blockIdx.x); }
+
+/* Count pass of a batch: the tiles of all jobs form one sequence. */
+__global__ __launch_bounds__(kBlock) void k_count_batch(const JobArgs* jobs, const uint64_t* tile_start, uint32_t njobs)
+{
+    const uint32_t j = find_job(tile_start, njobs, blockIdx.x);
+    count_tile(jobs[j], blockIdx.x - tile_start[j]);
+}
+
 /* ------------------------------------------------------------------------- */
 /* emit pass                                                                   */
 /* ------------------------------------------------------------------------- */
@@ -925,12 +978,11 @@ __global__ __launch_bounds__(kBlock) void k_count(JobArgs a)
  * loaded once, so tiles are independent and the grid needs no ordering; the
  * outputs land at the per-row prefixes of a.prefix. */
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_emit(JobArgs a)
+__device__ void emit_tile_job(const JobArgs& a, uint64_t tile)
 {
     extern __shared__ __align__(16) uint8_t smem[];
     const Views v = make_views(a, smem);
     const T* vol = (const T*)a.values;
-    const uint64_t tile = blockIdx.x;
     const int z = (int)(tile / a.nb);
     const uint32_t band = (uint32_t)(tile - (uint64_t)z * a.nb);
     const int y0 = (int)(band * a.R);
@@ -943,17 +995,30 @@ __global__ __launch_bounds__(kBlock) void k_emit(JobArgs a)
         const uint64_t* pre_t = a.prefix + (uint64_t)(l * 2 + 1) * a.nrows;
         const uint64_t row = (uint64_t)z * a.ny + (uint64_t)y0;
         const bool next = (uint32_t)y0 + a.R < a.ny;
+        /* prefixes are relative to the first row of the (level, kind) segment */
+        const uint64_t bv = pre_v[0], bt = pre_t[0];
         LevelInfo& L = v.lev[l];
-        L.tbase = pre_t[row];
-        L.vbase[2] = pre_v[row];
-        L.vbase[3] = next ? pre_v[row + a.R] : 0;
-        L.vbase[0] = z > 0 ? pre_v[row - a.ny] : 0;
-        L.vbase[1] = (z > 0 && next) ? pre_v[row - a.ny + a.R] : 0;
+        L.tbase = pre_t[row] - bt;
+        L.vbase[2] = pre_v[row] - bv;
+        L.vbase[3] = next ? pre_v[row + a.R] - bv : 0;
+        L.vbase[0] = z > 0 ? pre_v[row - a.ny] - bv : 0;
+        L.vbase[1] = (z > 0 && next) ? pre_v[row - a.ny + a.R] - bv : 0;
     }
     __syncthreads();
     tile_prefixes(a, v, z, y0);
     if (!v.misc[kMiscOverflow]) fill_cache<T>(a, v, vol, z, y0);
     emit_tile<T>(a, v, vol, z, y0);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_emit(JobArgs a) { emit_tile_job<T>(a, blockIdx.x); }
+
+/* Emit pass of a batch of jobs sharing the value type T. */
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_emit_batch(const JobArgs* jobs, const uint64_t* tile_start, uint32_t njobs)
+{
+    const uint32_t j = find_job(tile_start, njobs, blockIdx.x);
+    emit_tile_job<T>(jobs[j], blockIdx.x - tile_start[j]);
 }
 
 /* ------------------------------------------------------------------------- */
@@ -986,22 +1051,60 @@ __global__ __launch_bounds__(kBlock) void k_scan_partials(const uint32_t* counts
     }
 }
 
-/* Exclusive prefix of each count array; the grand total of array arr goes
- * to totals[arr]. */
-__global__ __launch_bounds__(kBlock) void k_scan_apply(const uint32_t* counts, uint64_t n, uint32_t nseg,
-                                                       const uint64_t* partials, uint64_t* prefix, uint64_t* totals)
+/* In place: partials of each array become their exclusive prefix, and the
+ * grand total of array arr goes to totals[arr].  One workgroup. */
+__global__ __launch_bounds__(kBlock) void k_scan_mid(uint64_t* partials, uint32_t nseg, uint32_t narr, uint64_t* totals)
 {
     __shared__ uint64_t sh[kWaves];
-    __shared__ uint64_t base_sh;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (uint32_t arr = 0; arr < narr; ++arr) {
+        uint64_t* p = partials + (uint64_t)arr * nseg;
+        uint64_t carry = 0;
+        for (uint32_t b0 = 0; b0 < nseg; b0 += kScanTile) {
+            const uint32_t b = b0 + threadIdx.x * kScanPer;
+            uint64_t vals[kScanPer];
+            uint64_t s = 0;
+#pragma unroll
+            for (uint32_t i = 0; i < kScanPer; ++i) {
+                vals[i] = b + i < nseg ? p[b + i] : 0ull;
+                s += vals[i];
+            }
+            uint64_t incl = s;
+#pragma unroll
+            for (int d = 1; d < 64; d <<= 1) {
+                const uint64_t t = (uint64_t)__shfl_up((unsigned long long)incl, d);
+                if (lane >= d) incl += t;
+            }
+            if (lane == 63) sh[wave] = incl;
+            __syncthreads();
+            uint64_t run = carry;
+            uint64_t all = 0;
+            for (int w = 0; w < kWaves; ++w) {
+                if (w < wave) run += sh[w];
+                all += sh[w];
+            }
+            run += incl - s;
+#pragma unroll
+            for (uint32_t i = 0; i < kScanPer; ++i) {
+                if (b + i < nseg) p[b + i] = run;
+                run += vals[i];
+            }
+            carry += all;
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) totals[arr] = carry;
+    }
+}
+
+/* Exclusive prefix of each count array, from the scanned partials. */
+__global__ __launch_bounds__(kBlock) void k_scan_apply(const uint32_t* counts, uint64_t n, uint32_t nseg,
+                                                       const uint64_t* partials, uint64_t* prefix)
+{
+    __shared__ uint64_t sh[kWaves];
     const uint32_t arr = blockIdx.x / nseg, seg = blockIdx.x - arr * nseg;
     const uint32_t* src = counts + (uint64_t)arr * n;
     uint64_t* dst = prefix + (uint64_t)arr * n;
-    if (threadIdx.x == 0) {
-        uint64_t t = 0;
-        for (uint32_t s = 0; s < seg; ++s) t += partials[(uint64_t)arr * nseg + s];
-        base_sh = t;
-        if (seg + 1 == nseg) totals[arr] = t + partials[(uint64_t)arr * nseg + seg];
-    }
+    const uint64_t base = partials[blockIdx.x];
     const uint64_t b = (uint64_t)seg * kScanTile + (uint64_t)threadIdx.x * kScanPer;
     uint32_t vals[kScanPer];
     uint64_t s = 0;
@@ -1021,7 +1124,7 @@ __global__ __launch_bounds__(kBlock) void k_scan_apply(const uint32_t* counts, u
     }
     if (lane == 63) sh[wave] = incl;
     __syncthreads();
-    uint64_t run = base_sh;
+    uint64_t run = base;
     for (int w = 0; w < wave; ++w) run += sh[w];
     run += incl - s;
 #pragma unroll
@@ -1147,11 +1250,11 @@ hipError_t launch_classify(int dtype, const JobArgs& a, hipStream_t st)
     }
 }
 
+uint32_t count_lds_bytes(const JobArgs& a);
+
 hipError_t launch_count(const JobArgs& a, hipStream_t st)
 {
-    const LdsLayout L = lds_layout(a.R, a.NC, a.nlev, 0, 0);
-    const uint32_t lds = L.off_bits + 3u * a.nlev * L.nrb * L.ncp * 8u;
-    hipLaunchKernelGGL(k_count, dim3((uint32_t)a.ntiles), dim3(kBlock), lds, st, a);
+    hipLaunchKernelGGL(k_count, dim3((uint32_t)a.ntiles), dim3(kBlock), count_lds_bytes(a), st, a);
     return hipGetLastError();
 }
 
@@ -1171,12 +1274,79 @@ hipError_t launch_emit(int dtype, const JobArgs& a, uint32_t lds_bytes, hipStrea
     }
 }
 
+template <typename T>
+hipError_t launch_classify_batch_t(const JobArgs* jobs, const uint64_t* row_start, uint32_t njobs, uint64_t rows,
+                                   hipStream_t st)
+{
+    uint64_t blocks = (rows + kWaves - 1) / kWaves;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL((k_classify_batch<T>), dim3((uint32_t)blocks), dim3(kBlock), 0, st, jobs, row_start, njobs);
+    return hipGetLastError();
+}
+
+hipError_t launch_classify_batch(int dtype, const JobArgs* jobs, const uint64_t* row_start, uint32_t njobs,
+                                 uint64_t rows, hipStream_t st)
+{
+    switch (dtype) {
+    case 0: return launch_classify_batch_t<int8_t>(jobs, row_start, njobs, rows, st);
+    case 1: return launch_classify_batch_t<int16_t>(jobs, row_start, njobs, rows, st);
+    case 2: return launch_classify_batch_t<int32_t>(jobs, row_start, njobs, rows, st);
+    case 3: return launch_classify_batch_t<int64_t>(jobs, row_start, njobs, rows, st);
+    case 4: return launch_classify_batch_t<uint8_t>(jobs, row_start, njobs, rows, st);
+    case 5: return launch_classify_batch_t<uint16_t>(jobs, row_start, njobs, rows, st);
+    case 6: return launch_classify_batch_t<uint32_t>(jobs, row_start, njobs, rows, st);
+    case 7: return launch_classify_batch_t<uint64_t>(jobs, row_start, njobs, rows, st);
+    case 8: return launch_classify_batch_t<float>(jobs, row_start, njobs, rows, st);
+    default: return launch_classify_batch_t<double>(jobs, row_start, njobs, rows, st);
+    }
+}
+
+hipError_t launch_count_batch(const JobArgs* jobs, const uint64_t* tile_start, uint32_t njobs, uint64_t tiles,
+                              uint32_t lds_bytes, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_count_batch, dim3((uint32_t)tiles), dim3(kBlock), lds_bytes, st, jobs, tile_start, njobs);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_emit_batch_t(const JobArgs* jobs, const uint64_t* tile_start, uint32_t njobs, uint64_t tiles,
+                               uint32_t lds_bytes, hipStream_t st)
+{
+    hipLaunchKernelGGL((k_emit_batch<T>), dim3((uint32_t)tiles), dim3(kBlock), lds_bytes, st, jobs, tile_start, njobs);
+    return hipGetLastError();
+}
+
+hipError_t launch_emit_batch(int dtype, const JobArgs* jobs, const uint64_t* tile_start, uint32_t njobs,
+                             uint64_t tiles, uint32_t lds_bytes, hipStream_t st)
+{
+    switch (dtype) {
+    case 0: return launch_emit_batch_t<int8_t>(jobs, tile_start, njobs, tiles, lds_bytes, st);
+    case 1: return launch_emit_batch_t<int16_t>(jobs, tile_start, njobs, tiles, lds_bytes, st);
+    case 2: return launch_emit_batch_t<int32_t>(jobs, tile_start, njobs, tiles, lds_bytes, st);
+    case 3: return launch_emit_batch_t<int64_t>(jobs, tile_start, njobs, tiles, lds_bytes, st);
+    case 4: return launch_emit_batch_t<uint8_t>(jobs, tile_start, njobs, tiles, lds_bytes, st);
+    case 5: return launch_emit_batch_t<uint16_t>(jobs, tile_start, njobs, tiles, lds_bytes, st);
+    case 6: return launch_emit_batch_t<uint32_t>(jobs, tile_start, njobs, tiles, lds_bytes, st);
+    case 7: return launch_emit_batch_t<uint64_t>(jobs, tile_start, njobs, tiles, lds_bytes, st);
+    case 8: return launch_emit_batch_t<float>(jobs, tile_start, njobs, tiles, lds_bytes, st);
+    default: return launch_emit_batch_t<double>(jobs, tile_start, njobs, tiles, lds_bytes, st);
+    }
+}
+
+
+uint32_t count_lds_bytes(const JobArgs& a)
+{
+    const LdsLayout L = lds_layout(a.R, a.NC, a.nlev, 0, 0);
+    return L.off_bits + 3u * a.nlev * L.nrb * L.ncp * 8u;
+}
+
 hipError_t launch_scan(const uint32_t* counts, uint64_t n, uint32_t narr, uint64_t* partials,
                        uint64_t* prefix, uint64_t* totals, hipStream_t st)
 {
     const uint32_t nseg = (uint32_t)((n + kScanTile - 1) / kScanTile);
     hipLaunchKernelGGL(k_scan_partials, dim3(nseg * narr), dim3(kBlock), 0, st, counts, n, nseg, partials);
-    hipLaunchKernelGGL(k_scan_apply, dim3(nseg * narr), dim3(kBlock), 0, st, counts, n, nseg, partials, prefix, totals);
+    hipLaunchKernelGGL(k_scan_mid, dim3(1), dim3(kBlock), 0, st, partials, nseg, narr, totals);
+    hipLaunchKernelGGL(k_scan_apply, dim3(nseg * narr), dim3(kBlock), 0, st, counts, n, nseg, partials, prefix);
     return hipGetLastError();
 }
 

@@ -71,10 +71,42 @@ class Surface(ctypes.Structure):
     ]
 
 
+class Job(ctypes.Structure):
+    _fields_ = [
+        ("volume", Volume),
+        ("isolevels", ctypes.POINTER(ctypes.c_double)),
+        ("n_levels", ctypes.c_uint32),
+        ("normal_type", ctypes.c_int32),
+        ("duplication", ctypes.c_int32),
+    ]
+
+
+class BatchInfo(ctypes.Structure):
+    _fields_ = [
+        ("n_jobs", ctypes.c_uint32),
+        ("n_surfaces", ctypes.c_uint32),
+        ("n_passes", ctypes.c_uint32),
+        ("n_type_groups", ctypes.c_uint32),
+        ("tile_rows", ctypes.c_uint32),
+        ("queue_capacity", ctypes.c_uint32),
+        ("graph_nodes", ctypes.c_uint32),
+        ("recaptures", ctypes.c_uint32),
+        ("nodes_classified", ctypes.c_uint64),
+        ("n_triangles", ctypes.c_uint64),
+        ("arena_bytes", ctypes.c_uint64),
+        ("work_bytes", ctypes.c_uint64),
+        ("last_run_ms", ctypes.c_double),
+    ]
+
+
+BATCH_AUTOTUNE = 1
+
 # Every symbol include/kmc.h declares (checked by the CPU tests).
 EXPORTED = [
     "kmc_init", "kmc_status_string", "kmc_extract", "kmc_extract_levels", "kmc_surface_release",
     "kmc_volume_min_max", "kmc_surface_color", "kmc_colormap_table",
+    "kmc_batch_create", "kmc_batch_run", "kmc_batch_surfaces", "kmc_batch_info_get", "kmc_batch_destroy",
+    "kmc_batch_extract",
 ]
 
 _lib = None
@@ -109,6 +141,20 @@ def load():
     lib.kmc_surface_color.restype = ctypes.c_int
     lib.kmc_colormap_table.argtypes = [ctypes.c_int32, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint8)]
     lib.kmc_colormap_table.restype = ctypes.c_int
+    lib.kmc_batch_create.argtypes = [ctypes.POINTER(Job), ctypes.c_uint32, ctypes.c_uint32, ctypes.c_void_p,
+                                     ctypes.POINTER(ctypes.c_void_p)]
+    lib.kmc_batch_create.restype = ctypes.c_int
+    lib.kmc_batch_run.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+    lib.kmc_batch_run.restype = ctypes.c_int
+    lib.kmc_batch_surfaces.argtypes = [ctypes.c_void_p, ctypes.POINTER(Surface), ctypes.c_uint32]
+    lib.kmc_batch_surfaces.restype = ctypes.c_int
+    lib.kmc_batch_info_get.argtypes = [ctypes.c_void_p, ctypes.POINTER(BatchInfo)]
+    lib.kmc_batch_info_get.restype = ctypes.c_int
+    lib.kmc_batch_destroy.argtypes = [ctypes.c_void_p]
+    lib.kmc_batch_destroy.restype = None
+    lib.kmc_batch_extract.argtypes = [ctypes.POINTER(Job), ctypes.c_uint32, ctypes.c_int32, ctypes.c_void_p,
+                                      ctypes.POINTER(Surface)]
+    lib.kmc_batch_extract.restype = ctypes.c_int
     _lib = lib
     return lib
 
@@ -226,3 +272,174 @@ def colormap_table(colormap, resolution=256):
     if st != OK:
         raise KmcError(st)
     return np.frombuffer(bytes(out), dtype=np.uint8).copy()
+
+
+def make_jobs(specs):
+    """specs: list of (kmc_volume, isolevels, normal_type, duplication).
+    Returns a ctypes Job array (keeping the level arrays alive)."""
+    jobs = (Job * len(specs))()
+    keep = []
+    for i, (vol, levels, normal_type, dup) in enumerate(specs):
+        lv = (ctypes.c_double * max(1, len(levels)))(*[float(x) for x in levels])
+        keep.append((vol, lv))
+        jobs[i].volume = vol
+        jobs[i].isolevels = ctypes.cast(lv, ctypes.POINTER(ctypes.c_double))
+        jobs[i].n_levels = len(levels)
+        jobs[i].normal_type = int(normal_type)
+        jobs[i].duplication = int(bool(dup))
+    jobs._keep = keep
+    return jobs
+
+
+def batch_extract(specs):
+    """Host-out one-shot batch (kmc_batch_extract); specs as for make_jobs with
+    host numpy volumes or kmc_volume structs.  Returns one list of surfaces per job."""
+    lib = load()
+    vols = [sp[0] if isinstance(sp[0], Volume) else make_volume(sp[0]) for sp in specs]
+    jobs = make_jobs([(v,) + tuple(sp[1:]) for v, sp in zip(vols, specs)])
+    n = sum(len(sp[1]) for sp in specs)
+    outs = (Surface * max(1, n))()
+    st = lib.kmc_batch_extract(jobs, len(specs), MEMORY_HOST, None, outs)
+    if st != OK:
+        raise KmcError(st)
+    try:
+        res, k = [], 0
+        for sp in specs:
+            res.append([surface_to_numpy(outs[k + i]) for i in range(len(sp[1]))])
+            k += len(sp[1])
+        return res
+    finally:
+        for i in range(n):
+            lib.kmc_surface_release(ctypes.byref(outs[i]))
+
+
+class Batch:
+    """A planned batch (kmc_batch_create) that can be run repeatedly."""
+
+    def __init__(self, specs, autotune=False, stream=None):
+        self.lib = load()
+        self.jobs = make_jobs(specs)
+        self.n_surfaces = sum(len(sp[1]) for sp in specs)
+        h = ctypes.c_void_p()
+        st = self.lib.kmc_batch_create(self.jobs, len(specs), BATCH_AUTOTUNE if autotune else 0, stream,
+                                       ctypes.byref(h))
+        if st != OK:
+            raise KmcError(st)
+        self.handle = h
+
+    def run(self, stream=None):
+        st = self.lib.kmc_batch_run(self.handle, stream)
+        if st != OK:
+            raise KmcError(st)
+
+    def surfaces(self):
+        outs = (Surface * max(1, self.n_surfaces))()
+        st = self.lib.kmc_batch_surfaces(self.handle, outs, self.n_surfaces)
+        if st != OK:
+            raise KmcError(st)
+        return [outs[i] for i in range(self.n_surfaces)]
+
+    def info(self):
+        i = BatchInfo()
+        st = self.lib.kmc_batch_info_get(self.handle, ctypes.byref(i))
+        if st != OK:
+            raise KmcError(st)
+        return {f: getattr(i, f) for f, _ in BatchInfo._fields_}
+
+    def close(self):
+        if self.handle:
+            self.lib.kmc_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_hip_rt = None
+
+
+def _hip():
+    """The HIP runtime (libamdhip64), for copies of device-memory surfaces."""
+    global _hip_rt
+    if _hip_rt is None:
+        for name in ("libamdhip64.so", "/opt/rocm/lib/libamdhip64.so"):
+            try:
+                _hip_rt = ctypes.CDLL(name)
+                break
+            except OSError:
+                continue
+        if _hip_rt is None:
+            raise RuntimeError("libamdhip64.so not found")
+        _hip_rt.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+        _hip_rt.hipMemcpy.restype = ctypes.c_int
+        _hip_rt.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+        _hip_rt.hipMalloc.restype = ctypes.c_int
+        _hip_rt.hipFree.argtypes = [ctypes.c_void_p]
+        _hip_rt.hipFree.restype = ctypes.c_int
+        _hip_rt.hipDeviceSynchronize.argtypes = []
+        _hip_rt.hipDeviceSynchronize.restype = ctypes.c_int
+    return _hip_rt
+
+
+class DeviceArray:
+    """A numpy array copied into device memory (hipMalloc), for device-resident
+    volumes without a framework.  Only the HIP runtime libkmc itself uses is
+    involved (a process must not mix it with another HIP runtime)."""
+
+    def __init__(self, arr):
+        load()
+        self.hip = _hip()
+        arr = np.ascontiguousarray(arr)
+        self.shape, self.dtype, self.nbytes = arr.shape, arr.dtype, arr.nbytes
+        self.ptr = ctypes.c_void_p()
+        if self.hip.hipMalloc(ctypes.byref(self.ptr), max(1, arr.nbytes)) != 0:
+            raise MemoryError("hipMalloc failed")
+        self.upload(arr)
+
+    def upload(self, arr):
+        arr = np.ascontiguousarray(arr, dtype=self.dtype)
+        assert arr.shape == self.shape
+        if self.hip.hipMemcpy(self.ptr, arr.ctypes.data, arr.nbytes, 1) != 0:
+            raise RuntimeError("hipMemcpy to device failed")
+
+    def volume(self, **kw):
+        nz, ny, nx = self.shape
+        return make_volume(self.ptr.value, dims=(nx, ny, nz), memory=MEMORY_DEVICE, dtype=self.dtype, **kw)
+
+    def free(self):
+        if self.ptr:
+            self.hip.hipFree(self.ptr)
+            self.ptr = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def device_surface_to_numpy(s):
+    """Copy a device-memory surface (or view) into numpy arrays, like surface_to_numpy."""
+    assert s.memory == MEMORY_DEVICE
+    hip = _hip()
+
+    def fetch(ptr, n, dtype):
+        out = np.zeros(n, dtype)
+        if n:
+            rc = hip.hipMemcpy(out.ctypes.data, ctypes.cast(ptr, ctypes.c_void_p), out.nbytes, 2)
+            if rc != 0:
+                raise RuntimeError(f"hipMemcpy failed: {rc}")
+        return out
+
+    return {
+        "coords": fetch(s.coords, int(s.n_vertices) * 3, np.float32),
+        "connections": fetch(s.connections, int(s.n_connections), np.uint32),
+        "normals": fetch(s.normals, int(s.n_normals) * 3, np.float32),
+        "n_triangles": int(s.n_triangles), "polygon_type": s.polygon_type, "normal_type": s.normal_type,
+        "color_type": s.color_type, "empty_range": s.empty_range,
+        "min_value": s.min_value, "max_value": s.max_value,
+        "min_coord": list(s.min_coord), "max_coord": list(s.max_coord),
+    }

@@ -86,6 +86,16 @@ def test_more_levels_than_one_pass():
         assert_same_surface(got, oracle.extract(v, lv, kmc.VERTEX_NORMAL, False), f"level {lv}")
 
 
+@pytest.mark.parametrize("nlev", [5, 6, 7])
+def test_pass_of_five_to_seven_levels(nlev):
+    # passes of 5..7 levels run the 8-level classify instance
+    v = datasets.blobs((70, 21, 13), seed=40 + nlev, nblobs=4)
+    levels = list(np.linspace(0.15, 0.85, nlev))
+    for normal, dup in MODES:
+        for lv, got in zip(levels, kmc.extract_levels(v, levels, normal, dup)):
+            assert_same_surface(got, oracle.extract(v, lv, normal, dup), f"{nlev} levels, {lv} mode {normal},{dup}")
+
+
 def test_constant_volume_is_empty():
     v = np.full((8, 8, 8), 3.0, dtype=np.float32)
     got = kmc.extract(v, 3.0, kmc.POLYGON_NORMAL, True)
