@@ -48,6 +48,8 @@ def main():
     ap.add_argument("--size", type=int, default=1024)
     ap.add_argument("--cpu-slices", type=int, default=192)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--batch-jobs", type=int, default=256, help="volumes of the batched leg (0: skip it)")
+    ap.add_argument("--batch-steps", type=int, default=5)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: every MI355X box here has a single GPU and the "
@@ -121,6 +123,12 @@ def main():
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(vol, args.cpu_slices, mn.value, mx.value)
+    del vol
+    torch.cuda.empty_cache()
+
+    batched = None
+    if args.batch_jobs > 0:
+        batched = batched_leg(lib, kmc, datasets, torch, args)
 
     out = {
         "metric": METRIC,
@@ -156,8 +164,88 @@ def main():
             "scope": "whole job (count + scan + emit kernels) on the stream, host sync included",
         },
         "cpu_baseline": cpu,
+        "batched": batched,
     }
     print(json.dumps(out), flush=True)
+
+
+def batched_leg(lib, kmc, datasets, torch, args):
+    """Batched vs plain on the same GPU, both on HBM-resident volumes with
+    device-resident outputs: `plain` is one kmc_extract per (volume, isolevel),
+    in sequence (each call returns when its surface is complete); `batched` is
+    one run of a planned kmc_batch (one graph launch) over all of them."""
+    import numpy as np
+
+    jobs = datasets.batch_workload(args.batch_jobs)
+    print(f"[bench] batched leg: generating {len(jobs)} blob volumes on the GPU", file=sys.stderr, flush=True)
+    vols, specs = [], []
+    for dims, seed, nb, levels in jobs:
+        t = datasets.blobs_torch(dims, seed=seed, nblobs=nb, device="cuda")
+        v = kmc.make_volume(t.data_ptr(), dims=dims, memory=kmc.MEMORY_DEVICE, dtype=np.float32)
+        mn, mx = ctypes.c_double(), ctypes.c_double()
+        torch.cuda.synchronize()
+        assert lib.kmc_volume_min_max(ctypes.byref(v), None, ctypes.byref(mn), ctypes.byref(mx)) == 0
+        v.has_min_max_values = 1
+        v.min_value, v.max_value = mn.value, mx.value
+        vols.append(t)
+        specs.append((v, levels, kmc.POLYGON_NORMAL, True))
+    torch.cuda.synchronize()
+    nodes = sum(d[0] * d[1] * d[2] for d, _, _, _ in jobs)
+    classified = sum(d[0] * d[1] * d[2] * len(lv) for d, _, _, lv in jobs)
+    nsurf = sum(len(lv) for _, _, _, lv in jobs)
+
+    out = kmc.Surface()
+
+    def plain_all():
+        tris = 0
+        for v, levels, normal, dup in specs:
+            for lv in levels:
+                s = lib.kmc_extract(ctypes.byref(v), float(lv), normal, int(dup), kmc.MEMORY_DEVICE, None,
+                                    ctypes.byref(out))
+                if s != 0:
+                    raise kmc.KmcError(s)
+                tris += int(out.n_triangles)
+                lib.kmc_surface_release(ctypes.byref(out))
+        return tris
+
+    print("[bench] batched leg: plain calls", file=sys.stderr, flush=True)
+    plain_all()
+    reps = max(1, args.batch_steps // 2)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        tris_plain = plain_all()
+    plain_ms = (time.perf_counter() - t0) / reps * 1e3
+
+    print("[bench] batched leg: planning + autotuning the batch", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    b = kmc.Batch(specs, autotune=True)
+    plan_s = time.perf_counter() - t0
+    b.run()
+    print(f"[bench] batched leg: timing {args.batch_steps} runs", file=sys.stderr, flush=True)
+    dev_ms = []
+    t0 = time.perf_counter()
+    for _ in range(args.batch_steps):
+        b.run()
+        dev_ms.append(b.info()["last_run_ms"])
+    batch_ms = (time.perf_counter() - t0) / args.batch_steps * 1e3
+    info = b.info()
+    b.close()
+    assert info["n_triangles"] == tris_plain, (info["n_triangles"], tris_plain)
+    return {
+        "workload": f"{len(jobs)} float32 blob volumes, dims uniform 32..256 per axis, 1..16 blobs, "
+                    f"1..4 isolevels in [0.2, 0.8] ({nsurf} surfaces), PolygonNormal + duplication",
+        "volumes": len(jobs), "surfaces": nsurf, "nodes": nodes, "triangles": tris_plain,
+        "plain_ms": round(plain_ms, 3),
+        "batched_ms": round(batch_ms, 3),
+        "batched_device_ms": round(float(np.median(dev_ms)), 3),
+        "speedup": round(plain_ms / batch_ms, 2),
+        "batched_gvoxels_per_s": round(classified / (batch_ms / 1e3) / 1e9, 2),
+        "batched_mtriangles_per_s": round(tris_plain / (batch_ms / 1e3) / 1e6, 2),
+        "plain_gvoxels_per_s": round(classified / (plain_ms / 1e3) / 1e9, 2),
+        "plan_s": round(plan_s, 2),
+        "autotuned": {"tile_rows": info["tile_rows"], "queue_capacity": info["queue_capacity"]},
+        "graph_nodes": info["graph_nodes"], "passes": info["n_passes"],
+    }
 
 
 def cpu_baseline(vol, slices, vmin, vmax):

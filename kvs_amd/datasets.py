@@ -223,3 +223,60 @@ def turbulence_torch(dims, seed=2, nmodes=64, period=None, device="cuda", slab=3
         t = out[z0:z1].to(torch.float64)
         out[z0:z1] = ((t - lo) * scale - 1.0).to(torch.float32)
     return out
+
+
+def _splitmix_uniform_torch(state, n, device):
+    """The next n outputs of a SplitMix64 stream at `state`, as uniforms in
+    [0,1) (float64 tensor), bit-identical to SplitMix64.bulk_uniform.  int64
+    arithmetic wraps like uint64; shifts are made logical by masking."""
+    import torch
+
+    def s64(u):
+        return u - (1 << 64) if u >= (1 << 63) else u
+
+    golden, m1, m2 = s64(0x9E3779B97F4A7C15), s64(0xBF58476D1CE4E5B9), s64(0x94D049BB133111EB)
+    k = torch.arange(1, n + 1, dtype=torch.int64, device=device)
+    z = k * golden + s64(state)
+
+    def lshr(v, s):
+        return (v >> s) & ((1 << (64 - s)) - 1)
+
+    z = (z ^ lshr(z, 30)) * m1
+    z = (z ^ lshr(z, 27)) * m2
+    z = z ^ lshr(z, 31)
+    return lshr(z, 11).to(torch.float64) * (1.0 / 9007199254740992.0)
+
+
+def blobs_torch(dims, seed=0, nblobs=3, noise=0.01, device="cuda"):
+    """blobs() evaluated with torch (on the GPU for large batches): the same
+    formula and noise stream; on the CPU device the bytes equal blobs()'s, on
+    the GPU the float64 exp may differ from the host's in the last bit."""
+    import torch
+
+    if isinstance(dims, int):
+        dims = (dims, dims, dims)
+    nx, ny, nz = dims
+    rng, params = blob_params(seed, nblobs)
+    ax = [torch.arange(n, dtype=torch.float64, device=device) / max(n - 1, 1) for n in (nx, ny, nz)]
+    f = torch.zeros((nz, ny, nx), dtype=torch.float64, device=device)
+    for (c, s) in params:
+        g = [torch.exp(-((ax[i] - c[i]) ** 2) / (2 * s * s)) for i in range(3)]
+        f += g[2][:, None, None] * g[1][None, :, None] * g[0][None, None, :]
+    if noise:
+        u = _splitmix_uniform_torch(rng.state, nx * ny * nz, device).reshape(nz, ny, nx)
+        f += (u * 2.0 - 1.0) * noise
+    return f.to(torch.float32)
+
+
+def batch_workload(n=256, seed=3000, dmin=32, dmax=256, max_blobs=16, max_levels=4):
+    """Jobs of the batched workload: per volume, dims uniform in [dmin, dmax]
+    per axis, a blob field with 1..max_blobs blobs and 1..max_levels isolevels
+    uniform in [0.2, 0.8].  Returns [(dims (nx,ny,nz), blob seed, nblobs, levels)]."""
+    rng = SplitMix64(seed)
+    jobs = []
+    for i in range(n):
+        dims = tuple(rng.integer(dmin, dmax) for _ in range(3))
+        nb = rng.integer(1, max_blobs)
+        levels = [rng.uniform(0.2, 0.8) for _ in range(rng.integer(1, max_levels))]
+        jobs.append((dims, seed + 1 + i, nb, levels))
+    return jobs
