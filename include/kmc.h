@@ -216,6 +216,66 @@ void kmc_batch_destroy(kmc_batch* batch);
 kmc_status kmc_batch_extract(const kmc_job* jobs, uint32_t n_jobs, int32_t out_memory, void* stream,
                              kmc_surface* outs);
 
+/* ------------------------------------------------------------------------- */
+/* Serving form: volumes stay resident in HBM; isosurface requests arrive at  */
+/* any time and are served by continuous batching.  Worker threads, each with */
+/* its own stream, repeatedly take every queued request that fits the free    */
+/* slots and the HBM budget and run them as one batch, so requests arriving   */
+/* while others are on the GPU join the next batch and batches overlap on the */
+/* device.  Each result equals kmc_extract on the same volume and isolevel.   */
+/* No reference counterpart (the reference serves one mapper call at a time). */
+/* ------------------------------------------------------------------------- */
+
+typedef struct kmc_server kmc_server;
+
+typedef struct kmc_server_config {
+    int32_t device;              /* HIP device */
+    uint32_t workers;            /* worker threads / streams (batches in flight), default 3 */
+    uint32_t max_inflight;       /* requests on the device at once (slots), default 256 */
+    uint32_t max_batch;          /* requests per batch, default 64 */
+    uint64_t budget_bytes;       /* device bytes for work in flight (masks, counts, outputs), default 16 GiB */
+    int32_t out_memory;          /* kmc_memory of returned surfaces */
+} kmc_server_config;
+
+typedef struct kmc_server_stats {
+    uint64_t submitted, completed, failed;
+    uint64_t batches;
+    uint64_t triangles;          /* over completed requests */
+    uint32_t peak_inflight;      /* most requests admitted at once */
+    uint32_t deferred;           /* requests put back because their outputs did not fit the budget */
+    uint64_t peak_budget_bytes;  /* most budget bytes held at once */
+    double p50_ms, p99_ms, mean_ms, max_ms;  /* latency: submission to result ready */
+    double span_s;               /* first submission to last completion */
+    double triangles_per_s;      /* triangles / span_s */
+} kmc_server_stats;
+
+kmc_status kmc_server_create(const kmc_server_config* config, kmc_server** out);
+
+/* Make a volume resident: host values are copied to the device once; missing
+ * min/max values are computed once (they are what every request clamps to and
+ * colours by, as the reference caches them in the volume). */
+kmc_status kmc_server_register_volume(kmc_server* server, const kmc_volume* volume, uint32_t* volume_id);
+
+/* Queue one request; returns at once with its id.  KMC_ERR_BUDGET if the
+ * request alone can never fit the budget. */
+kmc_status kmc_server_submit(kmc_server* server, uint32_t volume_id, double isolevel, int32_t normal_type,
+                             int32_t duplication, uint64_t* request_id);
+
+/* KMC_ERR_NOT_READY while the request runs; otherwise its status, and on
+ * KMC_OK the surface moves to *out (release with kmc_surface_release).  A
+ * finished request is forgotten once polled. */
+kmc_status kmc_server_poll(kmc_server* server, uint64_t request_id, kmc_surface* out);
+
+/* Like poll, but waits up to timeout_ms (negative: no limit). */
+kmc_status kmc_server_wait(kmc_server* server, uint64_t request_id, int64_t timeout_ms, kmc_surface* out);
+
+kmc_status kmc_server_get_stats(kmc_server* server, kmc_server_stats* stats);
+kmc_status kmc_server_reset_stats(kmc_server* server);
+
+/* Waits for every queued request, then stops the workers and frees the
+ * resident volumes.  Unpolled results are released. */
+void kmc_server_destroy(kmc_server* server);
+
 #ifdef __cplusplus
 }
 #endif

@@ -220,8 +220,10 @@ bool plan_geometry(JobArgs& a, uint32_t* lds_bytes, uint32_t r_max, uint32_t que
         a.cshift = 0;
         while ((1u << a.cshift) < a.NC && a.cshift < 6) ++a.cshift;
         a.queue_cap = queue_cap;
-        a.cache_cap = queue_cap;  // fill_cache stages its work in the vertex queue: needs queue_cap >= cache_cap
         a.use_cache = std::getenv("KMC_WINDOW_CACHE") ? 1u : 0u;
+        // fill_cache stages its work in the vertex queue: needs queue_cap >= cache_cap.
+        // Without the cache no LDS is reserved for it (more workgroups per CU).
+        a.cache_cap = a.use_cache ? queue_cap : 0u;
         const LdsLayout L = lds_layout(a.R, a.NC, a.nlev, a.queue_cap, a.cache_cap);
         if (L.bytes <= 60 * 1024) {
             *lds_bytes = L.bytes;

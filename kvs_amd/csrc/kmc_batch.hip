@@ -201,6 +201,7 @@ struct kmc_batch {
     std::vector<uint32_t> surface_seg;      // surface -> segment (or ~0u when the job is empty)
     std::vector<uint32_t> surface_user;     // surface -> job
     JobArgs* jobs_d;
+    std::vector<JobArgs> jobs_h;            // host staging of the job table (two-phase use)
     uint32_t* seg_map_d;
     uint32_t* counts_d;
     uint64_t* prefix_d;
@@ -309,8 +310,9 @@ kmc_status plan_tiles(kmc_batch* b, uint32_t r_max, uint32_t queue_cap)
     return KMC_OK;
 }
 
-/* Record the batch's launches on st. */
-hipError_t record(kmc_batch* b, hipStream_t st)
+/* Launches up to the output layout: classify, count, scan, layout (and, with
+ * copy_sizes, the surface sizes to segout_h). */
+hipError_t record_front(kmc_batch* b, hipStream_t st, bool copy_sizes)
 {
     const uint32_t nseg = (uint32_t)b->seg_pass_level.size();
     if (nseg == 0) return hipSuccess;
@@ -330,11 +332,28 @@ hipError_t record(kmc_batch* b, hipStream_t st)
     hipLaunchKernelGGL(k_batch_layout, dim3(1), dim3(kLayoutBlock), 0, st, b->jobs_d, b->seg_map_d, nseg, b->arena_c,
                        b->arena_n, b->arena_k, b->cap_c, b->cap_n, b->cap_k, b->segout_d);
     e = hipGetLastError();
-    if (e != hipSuccess) return e;
+    if (e != hipSuccess || !copy_sizes) return e;
+    return hipMemcpyAsync(b->segout_h, b->segout_d, (nseg + 1) * sizeof(SegOut), hipMemcpyDeviceToHost, st);
+}
+
+hipError_t record_emit(kmc_batch* b, hipStream_t st)
+{
+    hipError_t e = hipSuccess;
     for (const Group& g : b->groups) {
         e = launch_emit_batch(g.dtype, b->jobs_d + g.p0, g.tile_start_d, g.p1 - g.p0, g.tiles, g.emit_lds, st);
         if (e != hipSuccess) return e;
     }
+    return e;
+}
+
+/* Record the batch's launches on st (the captured graph). */
+hipError_t record(kmc_batch* b, hipStream_t st)
+{
+    const uint32_t nseg = (uint32_t)b->seg_pass_level.size();
+    if (nseg == 0) return hipSuccess;
+    hipError_t e = record_front(b, st, false);
+    if (e == hipSuccess) e = record_emit(b, st);
+    if (e != hipSuccess) return e;
     return hipMemcpyAsync(b->segout_h, b->segout_d, (nseg + 1) * sizeof(SegOut), hipMemcpyDeviceToHost, st);
 }
 
@@ -429,7 +448,8 @@ kmc_status autotune(kmc_batch* b, hipStream_t st)
     return s;
 }
 
-kmc_status create_impl(const kmc_job* jobs, uint32_t n_jobs, uint32_t flags, hipStream_t st, kmc_batch** out)
+kmc_status create_impl(const kmc_job* jobs, uint32_t n_jobs, uint32_t flags, hipStream_t st, kmc_batch** out,
+                       bool plan_only = false)
 {
     if (!out || (n_jobs > 0 && !jobs)) return KMC_ERR_INVALID_ARGUMENT;
     *out = nullptr;
@@ -561,6 +581,7 @@ kmc_status create_impl(const kmc_job* jobs, uint32_t n_jobs, uint32_t flags, hip
         }
     }
     s = plan_tiles(b, 8, 1024);
+    if (s == KMC_OK && plan_only) { *out = b; return KMC_OK; }
     if (s == KMC_OK) s = run(b, st);            // sizes the arena (first capture writes nothing)
     if (s == KMC_OK && (flags & KMC_BATCH_AUTOTUNE)) s = autotune(b, st);
     if (s != KMC_OK) { destroy(b); return s; }
@@ -589,6 +610,71 @@ void surface_view(const kmc_batch* b, uint32_t i, kmc_surface* o)
 }
 
 }  // namespace
+
+/* ------------------------------------------------------------------------- */
+/* two-phase use without a graph (the serving loop)                            */
+/* ------------------------------------------------------------------------- */
+
+namespace kmc {
+
+kmc_status batch_plan(const kmc_job* jobs, uint32_t n_jobs, hipStream_t st, kmc_batch** out)
+{
+    return create_impl(jobs, n_jobs, 0, st, out, true);
+}
+
+hipError_t batch_launch_front(kmc_batch* b, hipStream_t st) { return record_front(b, st, true); }
+
+uint32_t batch_surfaces(const kmc_batch* b) { return b->n_surfaces; }
+
+/* Sizes of surface i after the front phase has completed (arrays unset). */
+void batch_surface_sizes(const kmc_batch* b, uint32_t i, kmc_surface* o)
+{
+    surface_view(b, i, o);
+    o->coords = nullptr;
+    o->normals = nullptr;
+    o->connections = nullptr;
+}
+
+/* Point surface i's outputs at caller memory sized as batch_surface_sizes
+ * says (nullptr: write nothing for it). */
+void batch_set_outputs(kmc_batch* b, uint32_t i, float* coords, float* normals, uint32_t* conn)
+{
+    const uint32_t sg = b->surface_seg[i];
+    if (sg == ~0u) return;
+    const uint32_t m = b->seg_pass_level[sg];
+    Pass& p = b->passes[m >> 3];
+    const uint32_t l = m & 7u;
+    const SegOut& so = b->segout_h[sg];
+    const uint64_t verts = p.a.dup ? 3 * so.nt : so.nv;
+    const bool on = coords != nullptr && verts > 0;
+    p.a.coords[l] = coords;
+    p.a.normals[l] = normals;
+    p.a.conn[l] = conn;
+    p.a.cap_vertices[l] = on ? verts : 0;
+    p.a.cap_triangles[l] = on ? so.nt : 0;
+}
+
+hipError_t batch_launch_emit(kmc_batch* b, hipStream_t st)
+{
+    if (b->passes.empty()) return hipSuccess;
+    b->jobs_h.resize(b->passes.size());
+    for (size_t i = 0; i < b->passes.size(); ++i) b->jobs_h[i] = b->passes[i].a;
+    const hipError_t e = hipMemcpyAsync(b->jobs_d, b->jobs_h.data(), b->jobs_h.size() * sizeof(JobArgs),
+                                        hipMemcpyHostToDevice, st);
+    if (e != hipSuccess) return e;
+    return record_emit(b, st);
+}
+
+/* Device bytes the plan holds besides the volumes (counts, prefixes, masks, tables). */
+uint64_t batch_work_bytes(const kmc_batch* b)
+{
+    return b->n_counts * 12 + b->bitvol_words * 8 + b->passes.size() * sizeof(JobArgs) +
+           b->seg_pass_level.size() * (4 + sizeof(SegOut)) + (uint64_t)(b->scan_parts + 1) * 8;
+}
+
+void batch_destroy(kmc_batch* b) { destroy(b); }
+
+}  // namespace kmc
 
 extern "C" {
 

@@ -65,4 +65,16 @@ hipError_t launch_count_batch(const JobArgs* jobs, const uint64_t* tile_start, u
 hipError_t launch_emit_batch(int dtype, const JobArgs* jobs, const uint64_t* tile_start, uint32_t njobs,
                              uint64_t tiles, uint32_t lds_bytes, hipStream_t st);
 
+/* Two-phase use of a batch plan without a graph (kmc_batch.hip): plan, launch
+ * the front (classify, count, scan, sizes), wait, read the sizes, point the
+ * outputs, launch the emit. */
+kmc_status batch_plan(const kmc_job* jobs, uint32_t n_jobs, hipStream_t st, kmc_batch** out);
+hipError_t batch_launch_front(kmc_batch* b, hipStream_t st);
+uint32_t batch_surfaces(const kmc_batch* b);
+void batch_surface_sizes(const kmc_batch* b, uint32_t i, kmc_surface* o);
+void batch_set_outputs(kmc_batch* b, uint32_t i, float* coords, float* normals, uint32_t* conn);
+hipError_t batch_launch_emit(kmc_batch* b, hipStream_t st);
+uint64_t batch_work_bytes(const kmc_batch* b);
+void batch_destroy(kmc_batch* b);
+
 }  // namespace kmc

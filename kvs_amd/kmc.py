@@ -101,12 +101,44 @@ class BatchInfo(ctypes.Structure):
 
 BATCH_AUTOTUNE = 1
 
+
+class ServerConfig(ctypes.Structure):
+    _fields_ = [
+        ("device", ctypes.c_int32),
+        ("workers", ctypes.c_uint32),
+        ("max_inflight", ctypes.c_uint32),
+        ("max_batch", ctypes.c_uint32),
+        ("budget_bytes", ctypes.c_uint64),
+        ("out_memory", ctypes.c_int32),
+    ]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [
+        ("submitted", ctypes.c_uint64),
+        ("completed", ctypes.c_uint64),
+        ("failed", ctypes.c_uint64),
+        ("batches", ctypes.c_uint64),
+        ("triangles", ctypes.c_uint64),
+        ("peak_inflight", ctypes.c_uint32),
+        ("deferred", ctypes.c_uint32),
+        ("peak_budget_bytes", ctypes.c_uint64),
+        ("p50_ms", ctypes.c_double),
+        ("p99_ms", ctypes.c_double),
+        ("mean_ms", ctypes.c_double),
+        ("max_ms", ctypes.c_double),
+        ("span_s", ctypes.c_double),
+        ("triangles_per_s", ctypes.c_double),
+    ]
+
 # Every symbol include/kmc.h declares (checked by the CPU tests).
 EXPORTED = [
     "kmc_init", "kmc_status_string", "kmc_extract", "kmc_extract_levels", "kmc_surface_release",
     "kmc_volume_min_max", "kmc_surface_color", "kmc_colormap_table",
     "kmc_batch_create", "kmc_batch_run", "kmc_batch_surfaces", "kmc_batch_info_get", "kmc_batch_destroy",
     "kmc_batch_extract",
+    "kmc_server_create", "kmc_server_register_volume", "kmc_server_submit", "kmc_server_poll", "kmc_server_wait",
+    "kmc_server_get_stats", "kmc_server_reset_stats", "kmc_server_destroy",
 ]
 
 _lib = None
@@ -155,6 +187,23 @@ def load():
     lib.kmc_batch_extract.argtypes = [ctypes.POINTER(Job), ctypes.c_uint32, ctypes.c_int32, ctypes.c_void_p,
                                       ctypes.POINTER(Surface)]
     lib.kmc_batch_extract.restype = ctypes.c_int
+    lib.kmc_server_create.argtypes = [ctypes.POINTER(ServerConfig), ctypes.POINTER(ctypes.c_void_p)]
+    lib.kmc_server_create.restype = ctypes.c_int
+    lib.kmc_server_register_volume.argtypes = [ctypes.c_void_p, ctypes.POINTER(Volume), ctypes.POINTER(ctypes.c_uint32)]
+    lib.kmc_server_register_volume.restype = ctypes.c_int
+    lib.kmc_server_submit.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_double, ctypes.c_int32,
+                                      ctypes.c_int32, ctypes.POINTER(ctypes.c_uint64)]
+    lib.kmc_server_submit.restype = ctypes.c_int
+    lib.kmc_server_poll.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.POINTER(Surface)]
+    lib.kmc_server_poll.restype = ctypes.c_int
+    lib.kmc_server_wait.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int64, ctypes.POINTER(Surface)]
+    lib.kmc_server_wait.restype = ctypes.c_int
+    lib.kmc_server_get_stats.argtypes = [ctypes.c_void_p, ctypes.POINTER(ServerStats)]
+    lib.kmc_server_get_stats.restype = ctypes.c_int
+    lib.kmc_server_reset_stats.argtypes = [ctypes.c_void_p]
+    lib.kmc_server_reset_stats.restype = ctypes.c_int
+    lib.kmc_server_destroy.argtypes = [ctypes.c_void_p]
+    lib.kmc_server_destroy.restype = None
     _lib = lib
     return lib
 
@@ -443,3 +492,89 @@ def device_surface_to_numpy(s):
         "min_value": s.min_value, "max_value": s.max_value,
         "min_coord": list(s.min_coord), "max_coord": list(s.max_coord),
     }
+
+
+class Server:
+    """A kmc_server: resident volumes and a continuously batched request loop."""
+
+    def __init__(self, budget_bytes=0, workers=0, max_inflight=0, max_batch=0, out_memory=MEMORY_HOST, device=0):
+        self.lib = load()
+        cfg = ServerConfig(device, workers, max_inflight, max_batch, budget_bytes, out_memory)
+        h = ctypes.c_void_p()
+        st = self.lib.kmc_server_create(ctypes.byref(cfg), ctypes.byref(h))
+        if st != OK:
+            raise KmcError(st)
+        self.handle = h
+        self.out_memory = out_memory
+        self._vols = []
+
+    def register(self, vol):
+        """vol: a kmc Volume (host or device values).  Returns the volume id."""
+        vid = ctypes.c_uint32()
+        st = self.lib.kmc_server_register_volume(self.handle, ctypes.byref(vol), ctypes.byref(vid))
+        if st != OK:
+            raise KmcError(st)
+        self._vols.append(vol)
+        return vid.value
+
+    def submit(self, vid, isolevel, normal_type=POLYGON_NORMAL, duplication=True):
+        rid = ctypes.c_uint64()
+        st = self.lib.kmc_server_submit(self.handle, vid, float(isolevel), int(normal_type), int(bool(duplication)),
+                                        ctypes.byref(rid))
+        if st != OK:
+            raise KmcError(st)
+        return rid.value
+
+    def wait(self, rid, timeout_ms=-1):
+        """Wait for a request; returns its surface as numpy arrays (raises on failure)."""
+        s = Surface()
+        st = self.lib.kmc_server_wait(self.handle, rid, int(timeout_ms), ctypes.byref(s))
+        if st != OK:
+            raise KmcError(st)
+        try:
+            return surface_to_numpy(s) if s.memory == MEMORY_HOST else device_surface_to_numpy(s)
+        finally:
+            self.lib.kmc_surface_release(ctypes.byref(s))
+
+    def wait_count(self, rid, timeout_ms=-1):
+        """Wait for a request and release its surface; returns its triangle count."""
+        s = Surface()
+        st = self.lib.kmc_server_wait(self.handle, rid, int(timeout_ms), ctypes.byref(s))
+        if st != OK:
+            raise KmcError(st)
+        n = int(s.n_triangles)
+        self.lib.kmc_surface_release(ctypes.byref(s))
+        return n
+
+    def poll(self, rid):
+        s = Surface()
+        st = self.lib.kmc_server_poll(self.handle, rid, ctypes.byref(s))
+        if st == ERR_NOT_READY:
+            return None
+        if st != OK:
+            raise KmcError(st)
+        try:
+            return surface_to_numpy(s) if s.memory == MEMORY_HOST else device_surface_to_numpy(s)
+        finally:
+            self.lib.kmc_surface_release(ctypes.byref(s))
+
+    def stats(self):
+        st = ServerStats()
+        rc = self.lib.kmc_server_get_stats(self.handle, ctypes.byref(st))
+        if rc != OK:
+            raise KmcError(rc)
+        return {f: getattr(st, f) for f, _ in ServerStats._fields_}
+
+    def reset_stats(self):
+        self.lib.kmc_server_reset_stats(self.handle)
+
+    def close(self):
+        if self.handle:
+            self.lib.kmc_server_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,141 @@
+"""Serving form (kmc_server_*): each result is identical alone (a plain
+kmc_extract call) and in the mix (many requests on resident volumes admitted
+by continuous batching under an HBM budget), bit for bit."""
+import threading
+import time
+
+import numpy as np
+import pytest
+
+from kvs_amd import datasets, kmc
+from tests.conftest import assert_same_surface
+
+pytestmark = pytest.mark.gpu
+
+MODES = [(kmc.POLYGON_NORMAL, True), (kmc.POLYGON_NORMAL, False), (kmc.VERTEX_NORMAL, False)]
+
+
+def volumes():
+    rng = np.random.default_rng(77)
+    vols = [
+        datasets.blobs((40, 33, 21), seed=1, nblobs=5),
+        datasets.turbulence((48, 40, 36), seed=5),
+        (datasets.ct_like((36, 30, 28), seed=3, nellipsoids=6, dtype=np.uint16)),
+        datasets.ct_like((33, 41, 20), seed=4, nellipsoids=5, dtype=np.uint8),
+        rng.standard_normal((17, 19, 70)).astype(np.float64),
+        np.full((6, 6, 6), 3, np.int16),                      # constant: empty range
+    ]
+    return vols
+
+
+def requests(vols, n, seed=9):
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        j = int(rng.integers(len(vols)))
+        v = vols[j]
+        lo, hi = np.percentile(v.astype(np.float64), [10, 90])
+        lv = float(rng.uniform(lo, hi)) if hi > lo else float(lo)
+        if v.dtype.kind in "ui":
+            lv = float(np.floor(lv)) + 0.5
+        normal, dup = MODES[int(rng.integers(3))]
+        out.append((j, lv, normal, dup))
+    return out
+
+
+def check_mix(server, vols, reqs, concurrent_submitters=1):
+    ids = [None] * len(reqs)
+    vids = [server.register(kmc.make_volume(v)) for v in vols]
+
+    def submit(lo, hi):
+        for k in range(lo, hi):
+            j, lv, normal, dup = reqs[k]
+            ids[k] = server.submit(vids[j], lv, normal, dup)
+
+    step = (len(reqs) + concurrent_submitters - 1) // concurrent_submitters
+    ths = [threading.Thread(target=submit, args=(i, min(len(reqs), i + step)))
+           for i in range(0, len(reqs), step)]
+    for t in ths:
+        t.start()
+    for t in ths:
+        t.join()
+    for k, (j, lv, normal, dup) in enumerate(reqs):
+        got = server.wait(ids[k])
+        want = kmc.extract(vols[j], lv, normal, dup)
+        label = f"request {k}: volume {j} {vols[j].dtype} {vols[j].shape} level {lv} mode {normal},{dup}"
+        assert_same_surface(got, want, label)
+        assert got["empty_range"] == want["empty_range"], label
+        assert got["min_value"] == want["min_value"] and got["max_value"] == want["max_value"], label
+
+
+def test_mix_equals_alone():
+    vols = volumes()
+    reqs = requests(vols, 60)
+    s = kmc.Server(budget_bytes=1 << 30, workers=3)
+    try:
+        check_mix(s, vols, reqs, concurrent_submitters=3)
+        st = s.stats()
+        print("stats", st, flush=True)
+        assert st["completed"] == len(reqs) and st["failed"] == 0
+        assert st["batches"] >= 1 and st["peak_inflight"] >= 1
+        assert 0 < st["p50_ms"] <= st["p99_ms"] <= st["max_ms"]
+    finally:
+        s.close()
+
+
+def test_tight_budget_defers_but_stays_exact():
+    """A budget barely above one request's needs: requests wait for memory
+    (admission) or go back to the queue (outputs), and still come out exact."""
+    vols = volumes()[:4]
+    reqs = requests(vols, 24, seed=10)
+    s = kmc.Server(budget_bytes=6 << 20, workers=3, max_batch=8)
+    try:
+        check_mix(s, vols, reqs)
+        st = s.stats()
+        print("stats", st, flush=True)
+        assert st["completed"] == len(reqs)
+        assert st["peak_budget_bytes"] <= 2 * (6 << 20)   # the only batch may overrun by one request
+    finally:
+        s.close()
+
+
+def test_device_outputs_and_poll():
+    vols = volumes()[:2]
+    s = kmc.Server(out_memory=kmc.MEMORY_DEVICE)
+    try:
+        vid = s.register(kmc.make_volume(vols[0]))
+        rid = s.submit(vid, 0.5, kmc.VERTEX_NORMAL, False)
+        got = None
+        t0 = time.time()
+        while got is None and time.time() - t0 < 60:
+            got = s.poll(rid)
+            time.sleep(0.001)
+        assert_same_surface(got, kmc.extract(vols[0], 0.5, kmc.VERTEX_NORMAL, False), "polled device output")
+    finally:
+        s.close()
+
+
+def test_request_larger_than_budget_is_refused():
+    v = datasets.blobs(32, seed=2)
+    s = kmc.Server(budget_bytes=4096)
+    try:
+        vid = s.register(kmc.make_volume(v))
+        with pytest.raises(kmc.KmcError) as e:
+            s.submit(vid, 0.5)
+        assert e.value.status == kmc.ERR_BUDGET
+    finally:
+        s.close()
+
+
+def test_outputs_larger_than_budget_fail_with_budget():
+    v = datasets.turbulence(48, seed=8)
+    # scratch of a 48^3 request is ~0.1 MB; its outputs are far above 0.3 MB
+    s = kmc.Server(budget_bytes=300_000)
+    try:
+        vid = s.register(kmc.make_volume(v))
+        rid = s.submit(vid, 0.0, kmc.POLYGON_NORMAL, True)
+        with pytest.raises(kmc.KmcError) as e:
+            s.wait(rid)
+        assert e.value.status == kmc.ERR_BUDGET
+    finally:
+        s.close()
