@@ -16,9 +16,11 @@ reference's own CPU path timed on a z-slab sample of the same volume.
 import argparse
 import ctypes
 import json
+import math
 import os
 import platform
 import sys
+import threading
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
@@ -50,6 +52,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--batch-jobs", type=int, default=256, help="volumes of the batched leg (0: skip it)")
     ap.add_argument("--batch-steps", type=int, default=5)
+    ap.add_argument("--serve-volumes", type=int, default=32, help="resident volumes of the serving leg (0: skip it)")
+    ap.add_argument("--serve-requests", type=int, default=300)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: every MI355X box here has a single GPU and the "
@@ -72,6 +76,7 @@ def main():
 
     stream = torch.cuda.current_stream(dev)
     sptr = ctypes.c_void_p(stream.cuda_stream)
+    copy = copy_ceiling(torch, dev)
     v = kmc.make_volume(vol.data_ptr(), dims=(n, n, n), memory=kmc.MEMORY_DEVICE, dtype=np.float32)
     mn, mx = ctypes.c_double(), ctypes.c_double()
     st = lib.kmc_volume_min_max(ctypes.byref(v), sptr, ctypes.byref(mn), ctypes.byref(mx))
@@ -129,6 +134,11 @@ def main():
     batched = None
     if args.batch_jobs > 0:
         batched = batched_leg(lib, kmc, datasets, torch, args)
+        torch.cuda.empty_cache()
+
+    serving = None
+    if args.serve_volumes > 0:
+        serving = serving_leg(lib, kmc, datasets, torch, args)
 
     out = {
         "metric": METRIC,
@@ -164,9 +174,30 @@ def main():
             "scope": "whole job (count + scan + emit kernels) on the stream, host sync included",
         },
         "cpu_baseline": cpu,
+        "copy_ceiling": copy,
         "batched": batched,
+        "serving": serving,
     }
     print(json.dumps(out), flush=True)
+
+
+def copy_ceiling(torch, dev, nbytes=1 << 30, reps=5):
+    """Achievable HBM ceiling: a plain device-to-device copy of 1 GiB (torch's
+    copy kernel), read + write bytes over its event-timed duration.  Its
+    FETCH_SIZE / WRITE_SIZE counters also calibrate the traffic counters."""
+    src = torch.empty(nbytes // 4, dtype=torch.float32, device=dev).fill_(1.0)
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(reps):
+        dst.copy_(src)
+    b.record()
+    torch.cuda.synchronize()
+    ms = a.elapsed_time(b) / reps
+    del src, dst
+    return {"bytes_per_copy": nbytes, "ms": round(ms, 4), "gbs": round(2 * nbytes / (ms / 1e3) / 1e9, 1)}
 
 
 def batched_leg(lib, kmc, datasets, torch, args):
@@ -245,6 +276,118 @@ def batched_leg(lib, kmc, datasets, torch, args):
         "plan_s": round(plan_s, 2),
         "autotuned": {"tile_rows": info["tile_rows"], "queue_capacity": info["queue_capacity"]},
         "graph_nodes": info["graph_nodes"], "passes": info["n_passes"],
+    }
+
+
+def serving_leg(lib, kmc, datasets, torch, args):
+    """Serving on resident volumes: requests (volume, isolevel in the volume's
+    10th..90th value percentile, PolygonNormal+duplication or VertexNormal)
+    through kmc_server (continuous batching, 64 GiB budget) -- at saturation
+    (all requests queued at once) and under Poisson arrivals at fractions of
+    the saturation rate -- against the same requests as sequential plain
+    kmc_extract calls.  Outputs stay on the device in both."""
+    import numpy as np
+
+    from kvs_amd.datasets import SplitMix64
+
+    specs = datasets.serving_volumes(args.serve_volumes)
+    print(f"[bench] serving leg: generating {len(specs)} volumes on the GPU", file=sys.stderr, flush=True)
+    tensors, vols, pct = [], [], []
+    nbytes = 0
+    for edge, kind, seed in specs:
+        if kind == "float32":
+            t = datasets.turbulence_torch(edge, seed=seed, device="cuda")
+        else:
+            t = datasets.ct_like_torch(edge, seed=seed, dtype=np.dtype(kind), device="cuda")
+        flat = t.reshape(-1)
+        step = max(1, flat.numel() // 4_000_000)
+        q = torch.quantile(flat[::step].to(torch.float64), torch.tensor([0.1, 0.9], dtype=torch.float64, device="cuda"))
+        pct.append((float(q[0]), float(q[1])))
+        tensors.append(t)
+        vols.append(kmc.make_volume(t.data_ptr(), dims=(edge, edge, edge), memory=kmc.MEMORY_DEVICE, dtype=np.dtype(kind)))
+        nbytes += t.numel() * t.element_size()
+    torch.cuda.synchronize()
+    server = kmc.Server(budget_bytes=64 << 30, out_memory=kmc.MEMORY_DEVICE)
+    vids = [server.register(v) for v in vols]
+    # min/max as registered (the plain calls below use them too)
+    for v in vols:
+        mn, mx = ctypes.c_double(), ctypes.c_double()
+        assert lib.kmc_volume_min_max(ctypes.byref(v), None, ctypes.byref(mn), ctypes.byref(mx)) == 0
+        v.has_min_max_values, v.min_value, v.max_value = 1, mn.value, mx.value
+
+    rng = SplitMix64(4100)
+    reqs = []
+    for _ in range(args.serve_requests):
+        j = rng.integer(0, len(vols) - 1)
+        lo, hi = pct[j]
+        normal = kmc.POLYGON_NORMAL if rng.uniform() < 0.5 else kmc.VERTEX_NORMAL
+        reqs.append((j, rng.uniform(lo, hi), normal, normal == kmc.POLYGON_NORMAL))
+
+    # plain: one kmc_extract per request, in sequence
+    print("[bench] serving leg: plain calls", file=sys.stderr, flush=True)
+    out = kmc.Surface()
+    t0 = time.perf_counter()
+    tris_plain = 0
+    for j, lv, normal, dup in reqs:
+        st = lib.kmc_extract(ctypes.byref(vols[j]), float(lv), normal, int(dup), kmc.MEMORY_DEVICE, None,
+                             ctypes.byref(out))
+        if st != 0:
+            raise kmc.KmcError(st)
+        tris_plain += int(out.n_triangles)
+        lib.kmc_surface_release(ctypes.byref(out))
+    plain_s = time.perf_counter() - t0
+
+    def serve(rate):
+        """Submit every request (Poisson arrivals at `rate` per second, or all
+        at once when rate is None), collect them; returns the server stats."""
+        server.reset_stats()
+        ids = [None] * len(reqs)
+        arr = SplitMix64(4200)
+
+        def producer():
+            t_next = time.perf_counter()
+            for k, (j, lv, normal, dup) in enumerate(reqs):
+                if rate:
+                    t_next += -math.log(1.0 - arr.uniform()) / rate
+                    d = t_next - time.perf_counter()
+                    if d > 0:
+                        time.sleep(d)
+                ids[k] = server.submit(vids[j], lv, normal, dup)
+
+        th = threading.Thread(target=producer)
+        th.start()
+        th.join()
+        tris = sum(server.wait_count(i) for i in ids)
+        st = server.stats()
+        assert tris == tris_plain and st["completed"] == len(reqs), (tris, tris_plain, st)
+        return st
+
+    print("[bench] serving leg: saturation", file=sys.stderr, flush=True)
+    serve(None)   # warm
+    sat = serve(None)
+    sat_rate = len(reqs) / sat["span_s"]
+    sweep = []
+    for frac in (0.25, 0.5, 0.75, 0.9):
+        print(f"[bench] serving leg: Poisson at {frac:.2f} of saturation", file=sys.stderr, flush=True)
+        st = serve(frac * sat_rate)
+        sweep.append({"offered_qps": round(frac * sat_rate, 1), "achieved_qps": round(len(reqs) / st["span_s"], 1),
+                      "p50_ms": round(st["p50_ms"], 3), "p99_ms": round(st["p99_ms"], 3),
+                      "mtriangles_per_s": round(st["triangles_per_s"] / 1e6, 1),
+                      "peak_inflight": st["peak_inflight"], "batches": st["batches"]})
+    server.close()
+    return {
+        "workload": f"{len(specs)} resident volumes (cubic, edge 64..512; float32 turbulence, uint16/uint8 "
+                    f"CT-like; {nbytes / 2**30:.2f} GiB), {len(reqs)} requests: isolevel uniform in the "
+                    f"volume's 10th..90th value percentile, PolygonNormal+duplication or VertexNormal "
+                    f"(1:1); HBM budget 64 GiB; device-resident outputs",
+        "requests": len(reqs), "triangles": tris_plain,
+        "plain_ms_per_request": round(plain_s / len(reqs) * 1e3, 3),
+        "plain_mtriangles_per_s": round(tris_plain / plain_s / 1e6, 1),
+        "saturation": {"qps": round(sat_rate, 1), "mtriangles_per_s": round(sat["triangles_per_s"] / 1e6, 1),
+                       "p50_ms": round(sat["p50_ms"], 3), "p99_ms": round(sat["p99_ms"], 3),
+                       "batches": sat["batches"], "peak_inflight": sat["peak_inflight"]},
+        "speedup_vs_plain": round(plain_s / sat["span_s"], 2),
+        "poisson_sweep": sweep,
     }
 
 

@@ -29,22 +29,36 @@ std::mutex g_pool_mutex;
 std::multimap<size_t, void*> g_free;           // size -> block
 std::unordered_map<void*, size_t> g_size;      // block -> size
 size_t g_cached_bytes = 0;
-const size_t kPoolKeepBytes = size_t(64) << 30;  // cached free bytes kept before trimming
+const size_t kPoolKeepBytes = size_t(64) << 30;  // cached free bytes that trigger a partial release
+const bool g_pool_trace = std::getenv("KMC_POOL_TRACE") != nullptr;
 
+/* Size classes: 4 KiB steps up to 1 MiB, then four classes per power of two
+ * (at most 25% slack), so that blocks are reused across similar sizes. */
 size_t round_size(size_t n)
 {
     if (n <= (size_t(1) << 20)) return (n + 4095) & ~size_t(4095);
-    return (n + (size_t(2) << 20) - 1) & ~((size_t(2) << 20) - 1);
+    size_t p = size_t(1) << 20;
+    while (p * 2 < n) p <<= 1;
+    const size_t q = p / 4;
+    return (n + q - 1) / q * q;
 }
 
-void pool_trim_locked()
+/* Take every cached block out of the pool (caller frees them unlocked). */
+std::vector<void*> pool_take_all_locked()
 {
+    std::vector<void*> out;
     for (auto& kv : g_free) {
-        (void)hipFree(kv.second);
+        out.push_back(kv.second);
         g_size.erase(kv.second);
     }
     g_free.clear();
     g_cached_bytes = 0;
+    return out;
+}
+
+void free_blocks(const std::vector<void*>& blocks)
+{
+    for (void* p : blocks) (void)hipFree(p);
 }
 
 }  // namespace
@@ -53,23 +67,34 @@ void* device_alloc(size_t n)
 {
     if (n == 0) n = 1;
     const size_t sz = round_size(n);
-    std::lock_guard<std::mutex> lk(g_pool_mutex);
-    auto it = g_free.lower_bound(sz);
-    if (it != g_free.end() && it->first <= sz * 2) {
-        void* p = it->second;
-        g_cached_bytes -= it->first;
-        g_free.erase(it);
-        return p;
+    {
+        std::lock_guard<std::mutex> lk(g_pool_mutex);
+        auto it = g_free.lower_bound(sz);
+        if (it != g_free.end() && it->first <= sz * 2) {
+            void* p = it->second;
+            g_cached_bytes -= it->first;
+            g_free.erase(it);
+            return p;
+        }
     }
+    /* a miss: allocate without holding the pool lock */
+    if (g_pool_trace) std::fprintf(stderr, "[kmc_pool] hipMalloc %zu bytes\n", sz);
     void* p = nullptr;
     if (hipMalloc(&p, sz) != hipSuccess) {
         (void)hipGetLastError();
-        pool_trim_locked();
+        std::vector<void*> all;
+        {
+            std::lock_guard<std::mutex> lk(g_pool_mutex);
+            all = pool_take_all_locked();
+        }
+        if (g_pool_trace) std::fprintf(stderr, "[kmc_pool] out of memory: releasing %zu cached blocks\n", all.size());
+        free_blocks(all);
         if (hipMalloc(&p, sz) != hipSuccess) {
             (void)hipGetLastError();
             return nullptr;
         }
     }
+    std::lock_guard<std::mutex> lk(g_pool_mutex);
     g_size[p] = sz;
     return p;
 }
@@ -77,18 +102,75 @@ void* device_alloc(size_t n)
 void device_free(void* p)
 {
     if (!p) return;
-    std::lock_guard<std::mutex> lk(g_pool_mutex);
-    auto it = g_size.find(p);
-    if (it == g_size.end()) return;
-    g_free.emplace(it->second, p);
-    g_cached_bytes += it->second;
-    if (g_cached_bytes > kPoolKeepBytes) pool_trim_locked();
+    std::vector<void*> release;
+    {
+        std::lock_guard<std::mutex> lk(g_pool_mutex);
+        auto it = g_size.find(p);
+        if (it == g_size.end()) return;
+        g_free.emplace(it->second, p);
+        g_cached_bytes += it->second;
+        if (g_cached_bytes > kPoolKeepBytes) {
+            /* release the largest cached blocks down to half the limit */
+            while (g_cached_bytes > kPoolKeepBytes / 2 && !g_free.empty()) {
+                auto last = std::prev(g_free.end());
+                release.push_back(last->second);
+                g_cached_bytes -= last->first;
+                g_size.erase(last->second);
+                g_free.erase(last);
+            }
+        }
+    }
+    if (!release.empty()) {
+        if (g_pool_trace) std::fprintf(stderr, "[kmc_pool] releasing %zu cached blocks\n", release.size());
+        free_blocks(release);
+    }
 }
 
 void device_pool_trim()
 {
-    std::lock_guard<std::mutex> lk(g_pool_mutex);
-    pool_trim_locked();
+    std::vector<void*> all;
+    {
+        std::lock_guard<std::mutex> lk(g_pool_mutex);
+        all = pool_take_all_locked();
+    }
+    free_blocks(all);
+}
+
+/* Pinned host blocks for small staging buffers, by power-of-two size class;
+ * reused for the life of the process (hipHostMalloc / hipHostFree are slow
+ * and the latter may wait for the device). */
+namespace {
+std::mutex g_pin_mutex;
+std::multimap<size_t, void*> g_pin_free;
+std::unordered_map<void*, size_t> g_pin_size;
+}
+
+void* host_pinned_alloc(size_t n)
+{
+    size_t sz = 4096;
+    while (sz < n) sz <<= 1;
+    std::lock_guard<std::mutex> lk(g_pin_mutex);
+    auto it = g_pin_free.find(sz);
+    if (it != g_pin_free.end()) {
+        void* p = it->second;
+        g_pin_free.erase(it);
+        return p;
+    }
+    void* p = nullptr;
+    if (hipHostMalloc(&p, sz, hipHostMallocDefault) != hipSuccess) {
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    g_pin_size[p] = sz;
+    return p;
+}
+
+void host_pinned_free(void* p)
+{
+    if (!p) return;
+    std::lock_guard<std::mutex> lk(g_pin_mutex);
+    auto it = g_pin_size.find(p);
+    if (it != g_pin_size.end()) g_pin_free.emplace(it->second, p);
 }
 
 /* ------------------------------------------------------------------------- */

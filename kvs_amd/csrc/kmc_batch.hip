@@ -183,6 +183,7 @@ struct Group {
     uint32_t emit_lds, count_lds;
     uint64_t* row_start_d;       // p1 - p0 + 1 entries
     uint64_t* tile_start_d;
+    std::vector<uint64_t> row_start_h, tile_start_h;   // upload sources, kept while the copies may run
 };
 
 }  // namespace
@@ -217,6 +218,7 @@ struct kmc_batch {
     uint32_t* arena_k;
     uint64_t cap_c, cap_n, cap_k;
     hipStream_t cap_stream;
+    hipStream_t upload_stream;              // where table uploads are queued (the creating call's stream)
     hipGraph_t graph;
     hipGraphExec_t exec;
     hipEvent_t ev0, ev1;
@@ -249,6 +251,8 @@ void destroy(kmc_batch* b)
 {
     if (!b) return;
     (void)hipSetDevice(b->device);
+    /* blocks go back to the pool: nothing queued may still use them */
+    (void)hipStreamSynchronize(b->upload_stream);
     if (b->cap_stream) (void)hipStreamSynchronize(b->cap_stream);
     destroy_graph(b);
     free_plan_buffers(b);
@@ -262,7 +266,7 @@ void destroy(kmc_batch* b)
     device_free(b->arena_c);
     device_free(b->arena_n);
     device_free(b->arena_k);
-    if (b->segout_h) (void)hipHostFree(b->segout_h);
+    host_pinned_free(b->segout_h);
     for (void* p : b->owned_values) device_free(p);
     if (b->ev0) (void)hipEventDestroy(b->ev0);
     if (b->ev1) (void)hipEventDestroy(b->ev1);
@@ -298,14 +302,18 @@ kmc_status plan_tiles(kmc_batch* b, uint32_t r_max, uint32_t queue_cap)
         g.row_start_d = (uint64_t*)device_alloc(rs.size() * 8);
         g.tile_start_d = (uint64_t*)device_alloc(ts.size() * 8);
         if (!g.row_start_d || !g.tile_start_d) return KMC_ERR_OUT_OF_MEMORY;
-        if (hipMemcpy(g.row_start_d, rs.data(), rs.size() * 8, hipMemcpyHostToDevice) != hipSuccess ||
-            hipMemcpy(g.tile_start_d, ts.data(), ts.size() * 8, hipMemcpyHostToDevice) != hipSuccess)
+        g.row_start_h = std::move(rs);
+        g.tile_start_h = std::move(ts);
+        if (hipMemcpyAsync(g.row_start_d, g.row_start_h.data(), g.row_start_h.size() * 8, hipMemcpyHostToDevice,
+                           b->upload_stream) != hipSuccess ||
+            hipMemcpyAsync(g.tile_start_d, g.tile_start_h.data(), g.tile_start_h.size() * 8, hipMemcpyHostToDevice,
+                           b->upload_stream) != hipSuccess)
             return KMC_ERR_DEVICE;
     }
-    std::vector<JobArgs> host(b->passes.size());
-    for (size_t i = 0; i < b->passes.size(); ++i) host[i] = b->passes[i].a;
-    if (!host.empty() &&
-        hipMemcpy(b->jobs_d, host.data(), host.size() * sizeof(JobArgs), hipMemcpyHostToDevice) != hipSuccess)
+    b->jobs_h.resize(b->passes.size());
+    for (size_t i = 0; i < b->passes.size(); ++i) b->jobs_h[i] = b->passes[i].a;
+    if (!b->jobs_h.empty() && hipMemcpyAsync(b->jobs_d, b->jobs_h.data(), b->jobs_h.size() * sizeof(JobArgs),
+                                             hipMemcpyHostToDevice, b->upload_stream) != hipSuccess)
         return KMC_ERR_DEVICE;
     return KMC_OK;
 }
@@ -357,9 +365,22 @@ hipError_t record(kmc_batch* b, hipStream_t st)
     return hipMemcpyAsync(b->segout_h, b->segout_d, (nseg + 1) * sizeof(SegOut), hipMemcpyDeviceToHost, st);
 }
 
+/* The capture stream and timing events, made on first use (graph form only). */
+kmc_status graph_resources(kmc_batch* b)
+{
+    if (b->cap_stream) return KMC_OK;
+    if (hipStreamCreateWithFlags(&b->cap_stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&b->ev0) != hipSuccess || hipEventCreate(&b->ev1) != hipSuccess) {
+        (void)hipGetLastError();
+        return KMC_ERR_DEVICE;
+    }
+    return KMC_OK;
+}
+
 kmc_status capture(kmc_batch* b)
 {
     destroy_graph(b);
+    if (graph_resources(b) != KMC_OK) return KMC_ERR_DEVICE;
     if (hipStreamBeginCapture(b->cap_stream, hipStreamCaptureModeThreadLocal) != hipSuccess) return KMC_ERR_DEVICE;
     const hipError_t rec = record(b, b->cap_stream);
     hipGraph_t g = nullptr;
@@ -468,12 +489,7 @@ kmc_status create_impl(const kmc_job* jobs, uint32_t n_jobs, uint32_t flags, hip
     b->flags = flags;
     b->r_max = 8;
     b->queue_cap = 1024;
-    if (hipStreamCreateWithFlags(&b->cap_stream, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreate(&b->ev0) != hipSuccess || hipEventCreate(&b->ev1) != hipSuccess) {
-        (void)hipGetLastError();
-        destroy(b);
-        return KMC_ERR_DEVICE;
-    }
+    b->upload_stream = st;
 
     /* jobs: device values, min/max, object coordinates */
     b->users.resize(n_jobs);
@@ -562,8 +578,7 @@ kmc_status create_impl(const kmc_job* jobs, uint32_t n_jobs, uint32_t flags, hip
         b->jobs_d = (JobArgs*)device_alloc(b->passes.size() * sizeof(JobArgs));
         b->seg_map_d = (uint32_t*)device_alloc(nseg * 4);
         b->segout_d = (SegOut*)device_alloc((nseg + 1) * sizeof(SegOut));
-        if (hipHostMalloc((void**)&b->segout_h, (nseg + 1) * sizeof(SegOut), hipHostMallocDefault) != hipSuccess)
-            b->segout_h = nullptr;
+        b->segout_h = (SegOut*)host_pinned_alloc((nseg + 1) * sizeof(SegOut));
         if (!b->counts_d || !b->prefix_d || !b->scan_aux_d || !b->bitvol_d || !b->jobs_d || !b->seg_map_d ||
             !b->segout_d || !b->segout_h) {
             (void)hipGetLastError();
@@ -575,7 +590,8 @@ kmc_status create_impl(const kmc_job* jobs, uint32_t n_jobs, uint32_t flags, hip
             p.a.prefix = b->prefix_d + p.counts_off;
             p.a.bitvol = b->bitvol_d + p.bitvol_off;
         }
-        if (hipMemcpy(b->seg_map_d, b->seg_pass_level.data(), nseg * 4, hipMemcpyHostToDevice) != hipSuccess) {
+        if (hipMemcpyAsync(b->seg_map_d, b->seg_pass_level.data(), nseg * 4, hipMemcpyHostToDevice, st) !=
+            hipSuccess) {
             destroy(b);
             return KMC_ERR_DEVICE;
         }

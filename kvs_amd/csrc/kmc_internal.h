@@ -16,6 +16,10 @@ void* device_alloc(size_t bytes);
 void device_free(void* p);
 void device_pool_trim();
 
+/* Pooled pinned host memory for small staging buffers. */
+void* host_pinned_alloc(size_t bytes);
+void host_pinned_free(void* p);
+
 /* Arrays owned by a kmc_surface (its `internal` handle). */
 struct SurfaceStorage {
     int memory;

@@ -24,6 +24,8 @@
 #include <chrono>
 #include <cmath>
 #include <condition_variable>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
 #include <mutex>
@@ -84,6 +86,7 @@ struct kmc_server {
     uint32_t inflight = 0;       // requests admitted and not yet delivered
     uint32_t running = 0;        // batches between admission and delivery
     bool stop = false;
+    bool trace = false;          // KMC_SERVER_TRACE: one line per batch on stderr
     std::vector<std::thread> workers;
     // statistics since the last reset
     kmc_server_stats st;
@@ -165,12 +168,16 @@ void run_batch(kmc_server* S, hipStream_t st, const std::vector<uint64_t>& ids)
     }
     for (size_t i = 0; i < n; ++i) jobs[i].isolevels = &levels[i];
 
+    const Clock::time_point t0 = Clock::now();
     kmc_batch* b = nullptr;
     kmc_status s = batch_plan(jobs.data(), (uint32_t)n, st, &b);
+    const Clock::time_point t1 = Clock::now();
     if (s == KMC_OK && (batch_launch_front(b, st) != hipSuccess || hipStreamSynchronize(st) != hipSuccess)) {
         (void)hipGetLastError();
         s = KMC_ERR_DEVICE;
     }
+    const Clock::time_point t2 = Clock::now();
+    Clock::time_point t3 = t2, t4 = t2;
     std::vector<kmc_surface> out(n);
     std::vector<uint64_t> obytes(n, 0);
     std::vector<char> admitted(n, 0);
@@ -234,6 +241,7 @@ void run_batch(kmc_server* S, hipStream_t st, const std::vector<uint64_t>& ids)
             out[i].connections = (uint32_t*)store->connections;
             batch_set_outputs(b, (uint32_t)i, out[i].coords, out[i].normals, out[i].connections);
         }
+        t3 = Clock::now();
         if (s == KMC_OK && (batch_launch_emit(b, st) != hipSuccess || hipStreamSynchronize(st) != hipSuccess)) {
             (void)hipGetLastError();
             s = KMC_ERR_DEVICE;
@@ -242,7 +250,19 @@ void run_batch(kmc_server* S, hipStream_t st, const std::vector<uint64_t>& ids)
             for (size_t i = 0; i < n && s == KMC_OK; ++i)
                 if (admitted[i]) s = surface_to_host(&out[i], st);
     }
+    t4 = Clock::now();
     if (b) batch_destroy(b);
+    if (S->trace) {
+        auto ms = [](Clock::time_point a, Clock::time_point c) {
+            return std::chrono::duration<double, std::milli>(c - a).count();
+        };
+        size_t na = 0;
+        uint64_t ob = 0;
+        for (size_t i = 0; i < n; ++i) { na += admitted[i]; ob += admitted[i] ? obytes[i] : 0; }
+        std::fprintf(stderr, "[kmc_server] batch %zu req %zu admitted %.1f MB out: plan %.3f front %.3f alloc %.3f "
+                     "emit %.3f destroy %.3f ms\n", n, na, ob / 1e6, ms(t0, t1), ms(t1, t2), ms(t2, t3), ms(t3, t4),
+                     ms(t4, Clock::now()));
+    }
 
     std::lock_guard<std::mutex> lk(S->mu);
     for (size_t i = 0; i < n; ++i) {
@@ -333,6 +353,7 @@ kmc_status kmc_server_create(const kmc_server_config* config, kmc_server** out)
     if (s != KMC_OK) { report(s); return s; }
     kmc_server* S = new kmc_server();
     S->cfg = cfg;
+    S->trace = std::getenv("KMC_SERVER_TRACE") != nullptr;
     std::memset(&S->st, 0, sizeof(S->st));
     for (uint32_t i = 0; i < cfg.workers; ++i) S->workers.emplace_back(worker_main, S);
     *out = S;

@@ -280,3 +280,69 @@ def batch_workload(n=256, seed=3000, dmin=32, dmax=256, max_blobs=16, max_levels
         levels = [rng.uniform(0.2, 0.8) for _ in range(rng.integer(1, max_levels))]
         jobs.append((dims, seed + 1 + i, nb, levels))
     return jobs
+
+
+def ct_like_torch(dims, seed=1, nellipsoids=20, dtype=np.uint8, lattice=32, device="cuda"):
+    """ct_like() evaluated with torch (the same parameter stream and formula;
+    large volumes on the GPU).  Returns a contiguous [z][y][x] tensor."""
+    import torch
+
+    if isinstance(dims, int):
+        dims = (dims, dims, dims)
+    nx, ny, nz = dims
+    rng = SplitMix64(seed)
+    lat = torch.from_numpy(rng.bulk_uniform((lattice + 1) ** 3).reshape(lattice + 1, lattice + 1, lattice + 1)).to(device)
+
+    def axis(n):
+        t = torch.arange(n, dtype=torch.float64, device=device) / max(n - 1, 1)
+        u = t * lattice
+        i0 = torch.clamp(torch.floor(u).to(torch.int64), max=lattice - 1)
+        return t, i0, u - i0
+
+    x, ix, wx = axis(nx)
+    y, iy, wy = axis(ny)
+    z, iz, wz = axis(nz)
+    f = torch.zeros((nz, ny, nx), dtype=torch.float64, device=device)
+    for dz in (0, 1):
+        az = (wz if dz else 1 - wz)[:, None, None]
+        for dy in (0, 1):
+            ay = (wy if dy else 1 - wy)[None, :, None]
+            for dx in (0, 1):
+                ax_ = (wx if dx else 1 - wx)[None, None, :]
+                f += az * ay * ax_ * lat[(iz + dz)[:, None, None], (iy + dy)[None, :, None], (ix + dx)[None, None, :]]
+    zz, yy, xx = z[:, None, None], y[None, :, None], x[None, None, :]
+    for _ in range(nellipsoids):
+        c = [rng.uniform(0.1, 0.9) for _ in range(3)]
+        ax = [rng.uniform(0.03, 0.15) for _ in range(3)]
+        u1, u2, u3 = rng.uniform(), rng.uniform(), rng.uniform()
+        q = (math.sqrt(1 - u1) * math.sin(2 * math.pi * u2), math.sqrt(1 - u1) * math.cos(2 * math.pi * u2),
+             math.sqrt(u1) * math.sin(2 * math.pi * u3), math.sqrt(u1) * math.cos(2 * math.pi * u3))
+        qx, qy, qz, qw = q
+        rot = [[1 - 2 * (qy * qy + qz * qz), 2 * (qx * qy - qz * qw), 2 * (qx * qz + qy * qw)],
+               [2 * (qx * qy + qz * qw), 1 - 2 * (qx * qx + qz * qz), 2 * (qy * qz - qx * qw)],
+               [2 * (qx * qz - qy * qw), 2 * (qy * qz + qx * qw), 1 - 2 * (qx * qx + qy * qy)]]
+        value = rng.integer(60, 220) / 255.0
+        dx, dy, dz = xx - c[0], yy - c[1], zz - c[2]
+        acc = torch.zeros((nz, ny, nx), dtype=torch.float64, device=device)
+        for k in range(3):
+            pk = rot[0][k] * dx + rot[1][k] * dy + rot[2][k] * dz
+            acc = acc + (pk / ax[k]) ** 2
+        f = torch.where(acc <= 1.0, torch.full_like(f, value), f)
+        del acc
+    lo, hi = float(f.min()), float(f.max())
+    top = float(np.iinfo(dtype).max) if np.dtype(dtype).kind in "ui" else 1.0
+    g = (f - lo) / (hi - lo) * top if hi > lo else torch.zeros_like(f)
+    tdt = {np.dtype(np.uint8): torch.uint8, np.dtype(np.uint16): torch.uint16, np.dtype(np.int16): torch.int16,
+           np.dtype(np.float32): torch.float32}[np.dtype(dtype)]
+    if np.dtype(dtype).kind in "ui":
+        return torch.round(g).to(tdt).contiguous()
+    return g.to(tdt).contiguous()
+
+
+def serving_volumes(n=32, seed=4000, sizes=tuple(range(64, 513, 32))):
+    """Volumes of the serving workload: cubic edge drawn from `sizes`, value
+    type cycling float32 (turbulence field), uint16 and uint8 (CT-like).
+    Returns [(edge, dtype name, generator seed)]."""
+    rng = SplitMix64(seed)
+    kinds = ("float32", "uint16", "uint8")
+    return [(sizes[rng.integer(0, len(sizes) - 1)], kinds[i % 3], seed + 1 + i) for i in range(n)]
