@@ -50,6 +50,7 @@ def main():
     ap.add_argument("--size", type=int, default=1024)
     ap.add_argument("--cpu-slices", type=int, default=192)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--profile-steps", type=int, default=3, help="extra steps timed kernel by kernel")
     ap.add_argument("--batch-jobs", type=int, default=256, help="volumes of the batched leg (0: skip it)")
     ap.add_argument("--batch-steps", type=int, default=5)
     ap.add_argument("--serve-volumes", type=int, default=32, help="resident volumes of the serving leg (0: skip it)")
@@ -125,6 +126,18 @@ def main():
     b_alg = nodes * 4 + sum(24 * vv + 12 * t for vv, t in counts)
     achieved = b_alg / sec / 1e9
 
+    # per-kernel device time (HIP events between the launches, on the same stream)
+    lib.kmc_set_timing(1)
+    prof = {"classify": 0.0, "count": 0.0, "scan": 0.0, "emit": 0.0}
+    for _ in range(args.profile_steps):
+        step()
+        t = kmc.Timing()
+        lib.kmc_last_timing(ctypes.byref(t))
+        for k in prof:
+            prof[k] += getattr(t, k + "_ms") / args.profile_steps
+    lib.kmc_set_timing(0)
+    kernels = kernel_rooflines(prof, n, nlev, counts)
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(vol, args.cpu_slices, mn.value, mx.value)
@@ -163,15 +176,13 @@ def main():
         "triangles_per_step": ntri,
         "vertices_per_step": nvert,
         "wall_ms_per_step": round(wall / args.steps * 1e3, 3),
-        "roofline": {
-            "bound": "hbm",
-            "achieved": round(achieved, 1),
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
-            "algorithmic_bytes": b_alg,
-            "scope": "whole job (count + scan + emit kernels) on the stream, host sync included",
+        "roofline": dominant_roofline(kernels),
+        "kernels": kernels,
+        "job_roofline": {
+            "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "algorithmic_bytes": b_alg,
+            "scope": "whole kmc_extract_levels call (classify, count, scan, host sync, emit): volume read "
+                     "once + returned arrays, over ms_per_step",
         },
         "cpu_baseline": cpu,
         "copy_ceiling": copy,
@@ -179,6 +190,60 @@ def main():
         "serving": serving,
     }
     print(json.dumps(out), flush=True)
+
+
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "kmc_turbulence1024_f32_vn3_traffic.md")
+
+
+def measured_traffic():
+    """HBM bytes per launch of each kernel of this workload, as measured by the
+    two rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE) recorded in the profile
+    summary (its json block), or {} if there is none."""
+    try:
+        text = open(TRAFFIC_PROFILE).read()
+    except OSError:
+        return {}
+    if "```json" not in text:
+        return {}
+    block = text.split("```json", 1)[1].split("```", 1)[0]
+    return json.loads(block).get("kernels", {})
+
+
+def kernel_rooflines(prof, n, nlev, counts):
+    """Algorithmic bytes and achieved bandwidth of each kernel of one step.
+    Per launch: classify reads the volume (4 B per node) and writes the
+    classification masks (nlev bits per node, 64-node words); count reads the
+    masks and writes 2 x 4 B per node row and level; scan reads those counts
+    and writes 8 B prefixes; emit reads the masks, the two endpoint values of
+    every crossing edge (2 x 4 B per vertex) and writes the surfaces (12 B
+    coordinates + 12 B normals per vertex, 12 B connections per triangle)."""
+    nc = (n + 63) // 64
+    masks = nlev * n * n * nc * 8
+    rows = n * n
+    nv = sum(v for v, _ in counts)
+    nt = sum(t for _, t in counts)
+    algo = {
+        "classify": n ** 3 * 4 + masks,
+        "count": masks + 2 * nlev * rows * 4,
+        "scan": 2 * nlev * rows * (4 + 8),
+        "emit": masks + 8 * nv + 24 * nv + 12 * nt,
+    }
+    names = {"classify": "k_classify<float, 3>", "count": "k_count", "scan": "k_scan_*", "emit": "k_emit<float>"}
+    traffic = measured_traffic()
+    out = {}
+    for k, ms in prof.items():
+        gbs = algo[k] / (ms / 1e3) / 1e9 if ms > 0 else 0.0
+        out[k] = {"kernel": names[k], "ms": round(ms, 4), "algorithmic_bytes": algo[k], "achieved_gbs": round(gbs, 1),
+                  "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": traffic.get(names[k])}
+    return out
+
+
+def dominant_roofline(kernels):
+    k = max(kernels, key=lambda x: kernels[x]["ms"])
+    e = kernels[k]
+    return {"bound": "hbm", "kernel": e["kernel"], "achieved": e["achieved_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": e["frac"], "traffic": e["traffic"], "algorithmic_bytes": e["algorithmic_bytes"],
+            "launch_ms": e["ms"]}
 
 
 def copy_ceiling(torch, dev, nbytes=1 << 30, reps=5):

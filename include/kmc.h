@@ -126,6 +126,23 @@ kmc_status kmc_extract_levels(const kmc_volume* volume, const double* isolevels,
 void kmc_surface_release(kmc_surface* surface);
 
 /*
+ * Instrumentation (no reference counterpart): when enabled for the calling
+ * thread, kmc_extract / kmc_extract_levels record HIP events on their stream
+ * between kernel launches, and kmc_last_timing returns the device time of
+ * each kernel of the thread's last such call, summed over its passes.
+ */
+typedef struct kmc_timing {
+    double classify_ms;          /* values -> classification masks */
+    double count_ms;             /* per-row vertex / triangle counts */
+    double scan_ms;              /* row prefixes */
+    double emit_ms;              /* coordinates, connections, normals */
+    uint32_t passes;
+} kmc_timing;
+
+void kmc_set_timing(int32_t enable);
+kmc_status kmc_last_timing(kmc_timing* out);
+
+/*
  * GetMinMaxValues<T> (StructuredVolumeObject.cpp:66-111) on the device: the
  * serial scan's result including its NaN behaviour (NaNs are skipped unless
  * the first value is NaN, which then sticks).

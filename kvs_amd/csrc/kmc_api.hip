@@ -383,6 +383,40 @@ void init_job(JobArgs& a, const kmc_volume* v, const void* dvals, const double* 
     }
 }
 
+/* Device time of each kernel of a pass, when this thread asked for it
+ * (kmc_set_timing): HIP events on the pass's stream between the launches. */
+namespace {
+thread_local bool t_timing = false;
+thread_local kmc_timing t_last;
+thread_local hipEvent_t t_ev[6] = { nullptr, nullptr, nullptr, nullptr, nullptr, nullptr };
+}
+
+struct PassTimer {
+    hipStream_t st;
+    bool on;
+    explicit PassTimer(hipStream_t s) : st(s), on(t_timing)
+    {
+        for (int i = 0; i < 6 && on; ++i)
+            if (!t_ev[i] && hipEventCreate(&t_ev[i]) != hipSuccess) { (void)hipGetLastError(); on = false; }
+    }
+    bool mark(int i)
+    {
+        if (on && hipEventRecord(t_ev[i], st) != hipSuccess) { (void)hipGetLastError(); on = false; }
+        return true;
+    }
+    void accumulate()
+    {
+        if (!on) return;
+        float ms[5] = { 0, 0, 0, 0, 0 };
+        for (int i = 0; i < 5; ++i) (void)hipEventElapsedTime(&ms[i], t_ev[i], t_ev[i + 1]);
+        t_last.classify_ms += ms[0];
+        t_last.count_ms += ms[1];
+        t_last.scan_ms += ms[2];
+        t_last.emit_ms += ms[4];
+        t_last.passes += 1;
+    }
+};
+
 /* One pass of up to kMaxLevels levels over a device-resident volume.
  * On success outs[i] hold device arrays. */
 kmc_status run_pass(const kmc_volume* v, const void* dvals, const double* iso, uint32_t nlev,
@@ -414,9 +448,12 @@ kmc_status run_pass(const kmc_volume* v, const void* dvals, const double* iso, u
     a.counts = counts;
     a.prefix = prefix;
 
-    if (launch_classify(v->type, a, st) != hipSuccess ||
-        launch_count(a, st) != hipSuccess ||
-        launch_scan(counts, n, narr, partials, prefix, totals, st) != hipSuccess) {
+    PassTimer timer(st);
+    timer.mark(0);
+    const bool front_ok = launch_classify(v->type, a, st) == hipSuccess && timer.mark(1) &&
+                          launch_count(a, st) == hipSuccess && timer.mark(2) &&
+                          launch_scan(counts, n, narr, partials, prefix, totals, st) == hipSuccess && timer.mark(3);
+    if (!front_ok) {
         cleanup();
         return KMC_ERR_DEVICE;
     }
@@ -465,8 +502,11 @@ kmc_status run_pass(const kmc_volume* v, const void* dvals, const double* iso, u
     if (status == KMC_OK) {
         bool any = false;
         for (uint32_t l = 0; l < nlev; ++l) any = any || outs[l].n_vertices > 0;
-        if (any && (launch_emit(v->type, a, lds, st) != hipSuccess || hipStreamSynchronize(st) != hipSuccess))
-            status = KMC_ERR_DEVICE;
+        timer.mark(4);
+        if (any && launch_emit(v->type, a, lds, st) != hipSuccess) status = KMC_ERR_DEVICE;
+        timer.mark(5);
+        if (status == KMC_OK && hipStreamSynchronize(st) != hipSuccess) status = KMC_ERR_DEVICE;
+        if (status == KMC_OK) timer.accumulate();
     }
     cleanup();
     if (status != KMC_OK) {
@@ -509,6 +549,7 @@ kmc_status to_host(kmc_surface* s, hipStream_t st)
 kmc_status extract_levels_impl(const kmc_volume* v, const double* levels, uint32_t n_levels, int32_t normal_type,
                                int32_t duplication, int32_t out_memory, void* stream, kmc_surface* outs)
 {
+    std::memset(&t_last, 0, sizeof(t_last));
     kmc_status s = validate_volume(v);
     if (s != KMC_OK) return s;
     if (!outs || (n_levels > 0 && !levels)) return KMC_ERR_INVALID_ARGUMENT;
@@ -623,6 +664,15 @@ void kmc_surface_release(kmc_surface* s)
     s->connections = nullptr;
     s->normals = nullptr;
     s->n_vertices = s->n_triangles = s->n_connections = s->n_normals = 0;
+}
+
+void kmc_set_timing(int32_t enable) { t_timing = enable != 0; }
+
+kmc_status kmc_last_timing(kmc_timing* out)
+{
+    if (!out) return KMC_ERR_INVALID_ARGUMENT;
+    *out = t_last;
+    return KMC_OK;
 }
 
 kmc_status kmc_volume_min_max(const kmc_volume* v, void* stream, double* mn, double* mx)

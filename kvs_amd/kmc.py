@@ -139,7 +139,18 @@ EXPORTED = [
     "kmc_batch_extract",
     "kmc_server_create", "kmc_server_register_volume", "kmc_server_submit", "kmc_server_poll", "kmc_server_wait",
     "kmc_server_get_stats", "kmc_server_reset_stats", "kmc_server_destroy",
+    "kmc_set_timing", "kmc_last_timing",
 ]
+
+
+class Timing(ctypes.Structure):
+    _fields_ = [
+        ("classify_ms", ctypes.c_double),
+        ("count_ms", ctypes.c_double),
+        ("scan_ms", ctypes.c_double),
+        ("emit_ms", ctypes.c_double),
+        ("passes", ctypes.c_uint32),
+    ]
 
 _lib = None
 
@@ -202,6 +213,10 @@ def load():
     lib.kmc_server_get_stats.restype = ctypes.c_int
     lib.kmc_server_reset_stats.argtypes = [ctypes.c_void_p]
     lib.kmc_server_reset_stats.restype = ctypes.c_int
+    lib.kmc_set_timing.argtypes = [ctypes.c_int32]
+    lib.kmc_set_timing.restype = None
+    lib.kmc_last_timing.argtypes = [ctypes.POINTER(Timing)]
+    lib.kmc_last_timing.restype = ctypes.c_int
     lib.kmc_server_destroy.argtypes = [ctypes.c_void_p]
     lib.kmc_server_destroy.restype = None
     _lib = lib
