@@ -158,8 +158,7 @@ kmc_status kmc_volume_min_max(const kmc_volume* volume, void* stream, double* mi
 typedef enum kmc_colormap {
     KMC_COLORMAP_DEFAULT = 0,              /* ColorMap::DefaultColorMap = BrewerRdBu (ColorMap.cpp:23) */
     KMC_COLORMAP_BREWER_RDBU = 0,
-    KMC_COLORMAP_BREWER_SINGLE_HUE_GREYS = 1,
-    KMC_COLORMAP_RAINBOW = 2
+    KMC_COLORMAP_BREWER_SINGLE_HUE_GREYS = 1     /* other maps: KMC_ERR_INVALID_ARGUMENT */
 } kmc_colormap;
 
 kmc_status kmc_surface_color(int32_t colormap, uint32_t resolution, int32_t type,
