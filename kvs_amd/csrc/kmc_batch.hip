@@ -180,7 +180,7 @@ struct Group {
     int dtype;
     uint32_t p0, p1;             // passes [p0, p1)
     uint64_t rows, tiles;
-    uint32_t emit_lds, count_lds;
+    uint32_t emit_lds;
     uint64_t* row_start_d;       // p1 - p0 + 1 entries
     uint64_t* tile_start_d;
     std::vector<uint64_t> row_start_h, tile_start_h;   // upload sources, kept while the copies may run
@@ -285,14 +285,13 @@ kmc_status plan_tiles(kmc_batch* b, uint32_t r_max, uint32_t queue_cap)
     for (Group& g : b->groups) {
         std::vector<uint64_t> rs, ts;
         uint64_t r = 0, t = 0;
-        g.emit_lds = g.count_lds = 0;
+        g.emit_lds = 0;
         for (uint32_t i = g.p0; i < g.p1; ++i) {
             rs.push_back(r);
             ts.push_back(t);
             r += b->passes[i].a.nrows;
             t += b->passes[i].a.ntiles;
             g.emit_lds = std::max(g.emit_lds, b->passes[i].lds);
-            g.count_lds = std::max(g.count_lds, count_lds_bytes(b->passes[i].a));
         }
         rs.push_back(r);
         ts.push_back(t);
@@ -330,7 +329,7 @@ hipError_t record_front(kmc_batch* b, hipStream_t st, bool copy_sizes)
         if (e != hipSuccess) return e;
     }
     for (const Group& g : b->groups) {
-        e = launch_count_batch(b->jobs_d + g.p0, g.tile_start_d, g.p1 - g.p0, g.tiles, g.count_lds, st);
+        e = launch_count_batch(b->jobs_d + g.p0, g.row_start_d, g.p1 - g.p0, g.rows, st);
         if (e != hipSuccess) return e;
     }
     uint64_t* partials = b->scan_aux_d;

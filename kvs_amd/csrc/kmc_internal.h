@@ -61,11 +61,10 @@ hipError_t launch_scan(const uint32_t* counts, uint64_t n, uint32_t narr, uint64
 uint32_t scan_partials_needed(uint64_t n, uint32_t narr);
 hipError_t launch_minmax(int dtype, const void* vol, uint64_t n, void* scratch, double* out, hipStream_t st);
 size_t minmax_scratch_bytes();
-uint32_t count_lds_bytes(const JobArgs& a);
 hipError_t launch_classify_batch(int dtype, const JobArgs* jobs, const uint64_t* row_start, uint32_t njobs,
                                  uint64_t rows, hipStream_t st);
-hipError_t launch_count_batch(const JobArgs* jobs, const uint64_t* tile_start, uint32_t njobs, uint64_t tiles,
-                              uint32_t lds_bytes, hipStream_t st);
+hipError_t launch_count_batch(const JobArgs* jobs, const uint64_t* row_start, uint32_t njobs, uint64_t rows,
+                              hipStream_t st);
 hipError_t launch_emit_batch(int dtype, const JobArgs* jobs, const uint64_t* tile_start, uint32_t njobs,
                              uint64_t tiles, uint32_t lds_bytes, hipStream_t st);
 

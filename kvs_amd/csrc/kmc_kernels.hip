@@ -893,45 +893,51 @@ __global__ __launch_bounds__(kBlock) void k_classify_batch(const JobArgs* jobs, 
  * it owns, and for cell row (y, z-1) its triangles.  LDS holds only the chunk
  * masks of slices z-1, z, z+1 (rows y0-1 .. y0+R+1) and the case table; a
  * wave takes one (level, row) at a time with its lanes on the row's chunks. */
-__device__ void count_tile(const JobArgs& a, uint64_t tile)
+/* Counts of node rows [r0, r1) of a job: one row group of 64 >> cshift rows
+ * per wave iteration, one lane per 64-node chunk; masks are read straight
+ * from the bit volume (coalesced, neighbours from L1/L2), so there is no LDS
+ * window and no barrier.  ntri: triangle count per cube index (LDS). */
+__device__ __forceinline__ void count_rows(const JobArgs& a, const uint8_t* ntri, uint64_t r0, uint64_t r1,
+                                           uint64_t wave_first, uint64_t wave_step)
 {
-    extern __shared__ __align__(16) uint8_t smem[];
-    const LdsLayout L = lds_layout(a.R, a.NC, a.nlev, 0, 0);
-    Views v;
-    v.tri = (uint64_t*)(smem + L.off_tri);
-    v.bits = (uint64_t*)(smem + L.off_bits);
-    v.nrb = L.nrb; v.ncp = L.ncp; v.R = a.R; v.NC = a.NC; v.nlev = a.nlev;
-    const int z = (int)(tile / a.nb);
-    const uint32_t band = (uint32_t)(tile - (uint64_t)z * a.nb);
-    const int y0 = (int)(band * a.R);
-    for (uint32_t i = threadIdx.x; i < 256; i += kBlock) v.tri[i] = d_tri[i];
-    load_window(a, v, z, y0);
-    __syncthreads();
-    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    const uint32_t lane = threadIdx.x & 63u;
     const uint32_t sub = lane >> a.cshift, cl = lane & ((1u << a.cshift) - 1u);
     const uint32_t G = 64u >> a.cshift, CS = 1u << a.cshift;
-    const uint32_t s0 = bslot(z), s1 = bslot(z + 1), sp = bslot(z - 1);
-    for (uint32_t l = 0; l < a.nlev; ++l) {
-        for (uint32_t k0 = wave * G; k0 < a.R; k0 += kWaves * G) {
-            const uint32_t k = k0 + sub;
-            const int y = y0 + (int)k;
-            const bool ok = k < a.R && y < (int)a.ny;
+    const uint64_t slice_words = (uint64_t)a.ny * a.NC;
+    const uint64_t level_words = (uint64_t)a.nz * slice_words;
+    for (uint64_t g = r0 + wave_first * G; g < r1; g += wave_step * G) {
+        const uint64_t r = g + sub;
+        const bool ok = r < r1;
+        const uint32_t z = ok ? (uint32_t)(r / a.ny) : 0u;
+        const uint32_t y = ok ? (uint32_t)(r - (uint64_t)z * a.ny) : 0u;
+        const bool has_up = y + 1 < a.ny, has_next = z + 1 < a.nz, has_cells = z >= 1 && has_up;
+        for (uint32_t l = 0; l < a.nlev; ++l) {
             uint32_t vc = 0, tc = 0;
             if (ok) {
-                const uint64_t* row = brow(v, s0, l, k + 1);
-                const uint64_t* up = brow(v, s0, l, k + 2);
-                const uint64_t* nz = brow(v, s1, l, k + 1);
-                const uint64_t* pr = brow(v, sp, l, k + 1);
-                const uint64_t* pu = brow(v, sp, l, k + 2);
+                const uint64_t* lev = a.bitvol + (uint64_t)l * level_words;
+                const uint64_t* row = lev + (uint64_t)z * slice_words + (uint64_t)y * a.NC;
+                const uint64_t* up = row + a.NC;
+                const uint64_t* nz = row + slice_words;
+                const uint64_t* pr = row - slice_words;
+                const uint64_t* pu = pr + a.NC;
                 for (uint32_t c = cl; c < a.NC; c += CS) {
+                    const bool nxt = c + 1 < a.NC;
                     const uint64_t b = row[c];
-                    const uint64_t X = (b ^ shift_next(row, c, a.NC)) & valid_mask(c, a.nx - 1);
-                    const uint64_t Y = (y + 1 < (int)a.ny) ? ((b ^ up[c]) & valid_mask(c, a.nx)) : 0ull;
-                    const uint64_t Zm = (z + 1 < (int)a.nz) ? ((b ^ nz[c]) & valid_mask(c, a.nx)) : 0ull;
+                    const uint64_t b1 = nxt ? row[c + 1] : 0ull;
+                    const uint64_t u = has_up ? up[c] : 0ull;
+                    const uint64_t u1 = (has_up && nxt) ? up[c + 1] : 0ull;
+                    const uint64_t n0 = has_next ? nz[c] : 0ull;
+                    const uint64_t c4 = b, c5 = (b >> 1) | (b1 << 63);
+                    const uint64_t c7 = u, c6 = (u >> 1) | (u1 << 63);
+                    const uint64_t X = (b ^ c5) & valid_mask(c, a.nx - 1);
+                    const uint64_t Y = has_up ? ((b ^ u) & valid_mask(c, a.nx)) : 0ull;
+                    const uint64_t Zm = has_next ? ((b ^ n0) & valid_mask(c, a.nx)) : 0ull;
                     vc += (uint32_t)(__popcll(X) + __popcll(Y) + __popcll(Zm));
-                    if (z >= 1 && y + 1 < (int)a.ny) {
-                        const uint64_t c0 = pr[c], c1 = shift_next(pr, c, a.NC), c3 = pu[c], c2 = shift_next(pu, c, a.NC);
-                        const uint64_t c4 = b, c5 = shift_next(row, c, a.NC), c7 = up[c], c6 = shift_next(up, c, a.NC);
+                    if (has_cells) {
+                        const uint64_t p = pr[c], p1 = nxt ? pr[c + 1] : 0ull;
+                        const uint64_t q = pu[c], q1 = nxt ? pu[c + 1] : 0ull;
+                        const uint64_t c0 = p, c1 = (p >> 1) | (p1 << 63);
+                        const uint64_t c3 = q, c2 = (q >> 1) | (q1 << 63);
                         const uint64_t o = c0 | c1 | c2 | c3 | c4 | c5 | c6 | c7;
                         const uint64_t an = c0 & c1 & c2 & c3 & c4 & c5 & c6 & c7;
                         uint64_t m = (o & ~an) & valid_mask(c, a.nx - 1);
@@ -942,7 +948,7 @@ __device__ void count_tile(const JobArgs& a, uint64_t tile)
                                                 (uint32_t)((c2 >> i) & 1ull) << 2 | (uint32_t)((c3 >> i) & 1ull) << 3 |
                                                 (uint32_t)((c4 >> i) & 1ull) << 4 | (uint32_t)((c5 >> i) & 1ull) << 5 |
                                                 (uint32_t)((c6 >> i) & 1ull) << 6 | (uint32_t)((c7 >> i) & 1ull) << 7;
-                            tc += (uint32_t)(v.tri[ci] >> 60);
+                            tc += ntri[ci];
                         }
                     }
                 }
@@ -953,7 +959,6 @@ __device__ void count_tile(const JobArgs& a, uint64_t tile)
                 tc += __shfl_xor(tc, (int)d);
             }
             if (ok && cl == 0) {
-                const uint64_t r = (uint64_t)z * a.ny + (uint64_t)y;
                 a.counts[(uint64_t)(l * 2 + 0) * a.nrows + r] = vc;
                 a.counts[(uint64_t)(l * 2 + 1) * a.nrows + r] = tc;
             }
@@ -961,13 +966,39 @@ __device__ void count_tile(const JobArgs& a, uint64_t tile)
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_count(JobArgs a) { count_tile(a, blockIdx.x); }
-
-/* Count pass of a batch: the tiles of all jobs form one sequence. */
-__global__ __launch_bounds__(kBlock) void k_count_batch(const JobArgs* jobs, const uint64_t* tile_start, uint32_t njobs)
+__device__ __forceinline__ void stage_ntri(uint8_t* ntri)
 {
-    const uint32_t j = find_job(tile_start, njobs, blockIdx.x);
-    count_tile(jobs[j], blockIdx.x - tile_start[j]);
+    for (uint32_t i = threadIdx.x; i < 256; i += kBlock) ntri[i] = (uint8_t)(d_tri[i] >> 60);
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(kBlock) void k_count(JobArgs a)
+{
+    __shared__ uint8_t ntri[256];
+    stage_ntri(ntri);
+    const uint64_t wave_id = ((uint64_t)blockIdx.x * kBlock + threadIdx.x) >> 6;
+    const uint64_t nwaves = ((uint64_t)gridDim.x * kBlock) >> 6;
+    count_rows(a, ntri, 0, a.nrows, wave_id, nwaves);
+}
+
+/* Count pass of a batch: the node rows of all jobs form one sequence, cut
+ * into one contiguous range per wave. */
+__global__ __launch_bounds__(kBlock) void k_count_batch(const JobArgs* jobs, const uint64_t* row_start, uint32_t njobs)
+{
+    __shared__ uint8_t ntri[256];
+    stage_ntri(ntri);
+    const uint64_t total = row_start[njobs];
+    const uint64_t wave_id = ((uint64_t)blockIdx.x * kBlock + threadIdx.x) >> 6;
+    const uint64_t nwaves = ((uint64_t)gridDim.x * kBlock) >> 6;
+    const uint64_t per = (total + nwaves - 1) / nwaves;
+    uint64_t g = wave_id * per;
+    const uint64_t end = g + per < total ? g + per : total;
+    while (g < end) {
+        const uint32_t j = __builtin_amdgcn_readfirstlane(find_job(row_start, njobs, g));
+        const uint64_t je = row_start[j + 1] < end ? row_start[j + 1] : end;
+        count_rows(jobs[j], ntri, g - row_start[j], je - row_start[j], 0, 1);
+        g = je;
+    }
 }
 
 /* ------------------------------------------------------------------------- */
@@ -1250,11 +1281,12 @@ hipError_t launch_classify(int dtype, const JobArgs& a, hipStream_t st)
     }
 }
 
-uint32_t count_lds_bytes(const JobArgs& a);
-
 hipError_t launch_count(const JobArgs& a, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_count, dim3((uint32_t)a.ntiles), dim3(kBlock), count_lds_bytes(a), st, a);
+    const uint64_t G = 64u >> a.cshift;
+    uint64_t blocks = ((a.nrows + G - 1) / G + kWaves - 1) / kWaves;
+    if (blocks > 16384) blocks = 16384;
+    hipLaunchKernelGGL(k_count, dim3((uint32_t)blocks), dim3(kBlock), 0, st, a);
     return hipGetLastError();
 }
 
@@ -1301,10 +1333,12 @@ hipError_t launch_classify_batch(int dtype, const JobArgs* jobs, const uint64_t*
     }
 }
 
-hipError_t launch_count_batch(const JobArgs* jobs, const uint64_t* tile_start, uint32_t njobs, uint64_t tiles,
-                              uint32_t lds_bytes, hipStream_t st)
+hipError_t launch_count_batch(const JobArgs* jobs, const uint64_t* row_start, uint32_t njobs, uint64_t rows,
+                              hipStream_t st)
 {
-    hipLaunchKernelGGL(k_count_batch, dim3((uint32_t)tiles), dim3(kBlock), lds_bytes, st, jobs, tile_start, njobs);
+    uint64_t blocks = (rows / 4 + kWaves - 1) / kWaves + 1;
+    if (blocks > 16384) blocks = 16384;
+    hipLaunchKernelGGL(k_count_batch, dim3((uint32_t)blocks), dim3(kBlock), 0, st, jobs, row_start, njobs);
     return hipGetLastError();
 }
 
@@ -1333,12 +1367,6 @@ hipError_t launch_emit_batch(int dtype, const JobArgs* jobs, const uint64_t* til
     }
 }
 
-
-uint32_t count_lds_bytes(const JobArgs& a)
-{
-    const LdsLayout L = lds_layout(a.R, a.NC, a.nlev, 0, 0);
-    return L.off_bits + 3u * a.nlev * L.nrb * L.ncp * 8u;
-}
 
 hipError_t launch_scan(const uint32_t* counts, uint64_t n, uint32_t narr, uint64_t* partials,
                        uint64_t* prefix, uint64_t* totals, hipStream_t st)
