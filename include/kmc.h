@@ -195,7 +195,7 @@ typedef struct kmc_batch_info {
     uint32_t n_jobs;             /* jobs as given */
     uint32_t n_surfaces;         /* sum of n_levels */
     uint32_t n_passes;           /* internal passes (a job with more than KMC_MAX_LEVELS_PER_PASS levels takes several) */
-    uint32_t n_type_groups;      /* value types present: one kernel launch per pass kind and type */
+    uint32_t n_type_groups;      /* (value type, emit mode) groups: one launch per pass kind and group */
     uint32_t tile_rows;          /* upper bound of node rows per emit tile (autotuned or default) */
     uint32_t queue_capacity;     /* work-queue entries per emit window (autotuned or default) */
     uint32_t graph_nodes;        /* nodes of the captured graph */

@@ -4,6 +4,7 @@
  * launch sequence of one extraction.
  */
 #include <hip/hip_runtime.h>
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -293,33 +294,45 @@ kmc_status min_max_on_device(const kmc_volume* v, const void* dvals, hipStream_t
 
 bool plan_geometry(JobArgs& a, uint32_t* lds_bytes, uint32_t r_max, uint32_t queue_cap)
 {
-    const uint32_t candidates[] = { 8, 4, 2, 1 };
+    /* tuning overrides: KMC_TILE_ROWS caps R, KMC_QUEUE_CAP sets the work-queue size,
+     * KMC_FN_CAP the face normals kept per cell slice (VertexNormal) */
+    static const char* env_r = std::getenv("KMC_TILE_ROWS");
+    static const char* env_q = std::getenv("KMC_QUEUE_CAP");
+    static const char* env_f = std::getenv("KMC_FN_CAP");
+    if (env_r) r_max = std::min<uint32_t>(r_max, (uint32_t)std::max(1, std::atoi(env_r)));
+    if (queue_cap == 0) queue_cap = a.vnormal ? 512u : 1024u;
+    if (env_q) queue_cap = (uint32_t)std::max(64, std::atoi(env_q));
+    a.NC = (a.nx + 63) / 64;
+    a.cshift = 0;
+    while ((1u << a.cshift) < a.NC && a.cshift < 6) ++a.cshift;
+    a.queue_cap = queue_cap;
+    a.fn_cap = a.vnormal ? (env_f ? (uint32_t)std::max(64, std::atoi(env_f)) : 512u) : 0u;
+    a.nrows = (uint64_t)a.nz * a.ny;
+    const uint32_t candidates[] = { 16, 8, 4, 2, 1 };
     for (uint32_t R : candidates) {
         if (R > r_max) continue;
         a.R = R;
         a.nb = (a.ny + R - 1) / R;
-        a.NC = (a.nx + 63) / 64;
-        a.cshift = 0;
-        while ((1u << a.cshift) < a.NC && a.cshift < 6) ++a.cshift;
-        a.queue_cap = queue_cap;
-        a.use_cache = std::getenv("KMC_WINDOW_CACHE") ? 1u : 0u;
-        // fill_cache stages its work in the vertex queue: needs queue_cap >= cache_cap.
-        // Without the cache no LDS is reserved for it (more workgroups per CU).
-        a.cache_cap = a.use_cache ? queue_cap : 0u;
-        const LdsLayout L = lds_layout(a.R, a.NC, a.nlev, a.queue_cap, a.cache_cap);
+        const LdsLayout L = lds_layout(a);
         if (L.bytes <= 60 * 1024) {
             *lds_bytes = L.bytes;
-            a.slab = 32;
+            /* steps per unit: about 8192 units when the volume allows (load balance over
+             * persistent blocks), at most 64 steps; VertexNormal needs 2 (a unit's first
+             * step only prepares the face normals of the slice below) */
+            static const char* env_s = std::getenv("KMC_SLAB");
+            const uint64_t want = ((uint64_t)a.nz * a.nb) / 8192;
+            const uint64_t lo = a.vnormal ? 2 : 1;
+            a.slab = env_s ? (uint32_t)std::max(1, std::atoi(env_s))
+                           : (uint32_t)std::min<uint64_t>(64, std::max<uint64_t>(lo, want));
             a.nslabs = (a.nz + a.slab - 1) / a.slab;
-            a.ntiles = (uint64_t)a.nz * a.nb;
-            a.nrows = (uint64_t)a.nz * a.ny;
+            a.nunits = (uint64_t)a.nb * a.nslabs;
             return true;
         }
     }
     return false;
 }
 
-uint32_t levels_per_pass(const kmc_volume* v)
+uint32_t levels_per_pass(const kmc_volume* v, int vnormal)
 {
     for (uint32_t nl = kMaxLevels; nl >= 1; --nl) {
         JobArgs a;
@@ -328,6 +341,7 @@ uint32_t levels_per_pass(const kmc_volume* v)
         a.ny = v->resolution[1];
         a.nz = v->resolution[2];
         a.nlev = nl;
+        a.vnormal = vnormal ? 1u : 0u;
         uint32_t lds = 0;
         if (plan_geometry(a, &lds)) return nl;
     }
@@ -585,7 +599,7 @@ kmc_status extract_levels_impl(const kmc_volume* v, const double* levels, uint32
         return KMC_OK;
     }
 
-    const uint32_t per = levels_per_pass(v);
+    const uint32_t per = levels_per_pass(v, !dup && normal_type == KMC_VERTEX_NORMAL);
     if (per == 0 && n_levels > 0) s = KMC_ERR_INVALID_ARGUMENT;   // rows too long for the emit tile
     for (uint32_t l0 = 0; l0 < n_levels && s == KMC_OK; l0 += per) {
         const uint32_t nl = (n_levels - l0) < per ? (n_levels - l0) : per;

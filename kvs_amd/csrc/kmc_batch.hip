@@ -3,9 +3,10 @@
  * its own isolevels, extracted by one captured hipGraph.
  *
  * A batch is cut into passes (one volume, up to kMaxLevels isolevels each),
- * ordered by value type.  Per type the graph holds one classify, one count and
- * one emit launch over the rows / tiles of all its passes (a workgroup finds
- * its pass by binary search of the pass start table); the per-row counts of
+ * grouped by value type and emit mode.  Per group the graph holds one
+ * classify, one count and one emit launch over the rows / units of all its
+ * passes (a workgroup finds its pass by binary search of the pass start
+ * table); the per-row counts of
  * all passes sit in one array and take one scan.  A layout kernel then turns
  * each surface's totals into offsets in a shared output arena, so that sizes
  * never go through the host inside the graph; when a run's surfaces do not fit
@@ -178,12 +179,13 @@ struct Pass {
 
 struct Group {
     int dtype;
+    int mode;                    // 0 duplicated, 1 PolygonNormal, 2 VertexNormal
     uint32_t p0, p1;             // passes [p0, p1)
-    uint64_t rows, tiles;
+    uint64_t rows, units;
     uint32_t emit_lds;
     uint64_t* row_start_d;       // p1 - p0 + 1 entries
-    uint64_t* tile_start_d;
-    std::vector<uint64_t> row_start_h, tile_start_h;   // upload sources, kept while the copies may run
+    uint64_t* unit_start_d;
+    std::vector<uint64_t> row_start_h, unit_start_h;   // upload sources, kept while the copies may run
 };
 
 }  // namespace
@@ -234,8 +236,8 @@ void free_plan_buffers(kmc_batch* b)
 {
     for (Group& g : b->groups) {
         device_free(g.row_start_d);
-        device_free(g.tile_start_d);
-        g.row_start_d = g.tile_start_d = nullptr;
+        device_free(g.unit_start_d);
+        g.row_start_d = g.unit_start_d = nullptr;
     }
 }
 
@@ -290,22 +292,22 @@ kmc_status plan_tiles(kmc_batch* b, uint32_t r_max, uint32_t queue_cap)
             rs.push_back(r);
             ts.push_back(t);
             r += b->passes[i].a.nrows;
-            t += b->passes[i].a.ntiles;
+            t += b->passes[i].a.nunits;
             g.emit_lds = std::max(g.emit_lds, b->passes[i].lds);
         }
         rs.push_back(r);
         ts.push_back(t);
         g.rows = r;
-        g.tiles = t;
+        g.units = t;
         if (t > 0xFFFFFFFFull) return KMC_ERR_INVALID_ARGUMENT;
         g.row_start_d = (uint64_t*)device_alloc(rs.size() * 8);
-        g.tile_start_d = (uint64_t*)device_alloc(ts.size() * 8);
-        if (!g.row_start_d || !g.tile_start_d) return KMC_ERR_OUT_OF_MEMORY;
+        g.unit_start_d = (uint64_t*)device_alloc(ts.size() * 8);
+        if (!g.row_start_d || !g.unit_start_d) return KMC_ERR_OUT_OF_MEMORY;
         g.row_start_h = std::move(rs);
-        g.tile_start_h = std::move(ts);
+        g.unit_start_h = std::move(ts);
         if (hipMemcpyAsync(g.row_start_d, g.row_start_h.data(), g.row_start_h.size() * 8, hipMemcpyHostToDevice,
                            b->upload_stream) != hipSuccess ||
-            hipMemcpyAsync(g.tile_start_d, g.tile_start_h.data(), g.tile_start_h.size() * 8, hipMemcpyHostToDevice,
+            hipMemcpyAsync(g.unit_start_d, g.unit_start_h.data(), g.unit_start_h.size() * 8, hipMemcpyHostToDevice,
                            b->upload_stream) != hipSuccess)
             return KMC_ERR_DEVICE;
     }
@@ -347,7 +349,7 @@ hipError_t record_emit(kmc_batch* b, hipStream_t st)
 {
     hipError_t e = hipSuccess;
     for (const Group& g : b->groups) {
-        e = launch_emit_batch(g.dtype, b->jobs_d + g.p0, g.tile_start_d, g.p1 - g.p0, g.tiles, g.emit_lds, st);
+        e = launch_emit_batch(g.dtype, g.mode, b->jobs_d + g.p0, g.unit_start_d, g.p1 - g.p0, g.units, g.emit_lds, st);
         if (e != hipSuccess) return e;
     }
     return e;
@@ -449,7 +451,7 @@ kmc_status run(kmc_batch* b, hipStream_t st)
 kmc_status autotune(kmc_batch* b, hipStream_t st)
 {
     struct Cand { uint32_t r, q; };
-    const Cand cands[] = { { 8, 1024 }, { 4, 1024 }, { 2, 1024 }, { 8, 512 }, { 4, 512 }, { 8, 2048 } };
+    const Cand cands[] = { { 16, 1024 }, { 8, 1024 }, { 4, 1024 }, { 16, 512 }, { 8, 512 }, { 8, 2048 } };
     double best = 1e30;
     Cand best_c = { b->r_max, b->queue_cap };
     for (const Cand& c : cands) {
@@ -486,7 +488,7 @@ kmc_status create_impl(const kmc_job* jobs, uint32_t n_jobs, uint32_t flags, hip
     kmc_batch* b = new kmc_batch();
     (void)hipGetDevice(&b->device);
     b->flags = flags;
-    b->r_max = 8;
+    b->r_max = 16;
     b->queue_cap = 1024;
     b->upload_stream = st;
 
@@ -523,15 +525,18 @@ kmc_status create_impl(const kmc_job* jobs, uint32_t n_jobs, uint32_t flags, hip
     for (uint32_t j = 0; j < n_jobs; ++j)
         for (uint32_t l = 0; l < b->users[j].levels.size(); ++l) b->surface_user[b->users[j].first_surface + l] = j;
 
-    /* passes, grouped by value type */
-    for (int t = 0; t <= KMC_REAL64; ++t) {
+    /* passes, grouped by value type and emit mode (one kernel instance each) */
+    for (int gi = 0; gi < 3 * (KMC_REAL64 + 1); ++gi) {
+        const int t = gi / 3, mode = gi % 3;
         Group g{};
         g.dtype = t;
+        g.mode = mode;
         g.p0 = (uint32_t)b->passes.size();
         for (uint32_t j = 0; j < n_jobs; ++j) {
             const UserJob& u = b->users[j];
-            if (u.volume.type != t || u.empty) continue;
-            const uint32_t per = levels_per_pass(&u.volume);
+            const int um = u.dup ? 0 : (u.normal_type == KMC_VERTEX_NORMAL ? 2 : 1);
+            if (u.volume.type != t || um != mode || u.empty) continue;
+            const uint32_t per = levels_per_pass(&u.volume, !u.dup && u.normal_type == KMC_VERTEX_NORMAL);
             if (per == 0) { destroy(b); return KMC_ERR_INVALID_ARGUMENT; }
             for (uint32_t l0 = 0; l0 < u.levels.size(); l0 += per) {
                 Pass p{};
@@ -595,7 +600,7 @@ kmc_status create_impl(const kmc_job* jobs, uint32_t n_jobs, uint32_t flags, hip
             return KMC_ERR_DEVICE;
         }
     }
-    s = plan_tiles(b, 8, 1024);
+    s = plan_tiles(b, 16, 0);
     if (s == KMC_OK && plan_only) { *out = b; return KMC_OK; }
     if (s == KMC_OK) s = run(b, st);            // sizes the arena (first capture writes nothing)
     if (s == KMC_OK && (flags & KMC_BATCH_AUTOTUNE)) s = autotune(b, st);

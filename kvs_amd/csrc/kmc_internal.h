@@ -45,12 +45,13 @@ void init_job(JobArgs& a, const kmc_volume* v, const void* dvals, const double* 
               int dup, double mn, double mx, const float minc[3], const float maxc[3]);
 
 /* Tile geometry and emit LDS size: the largest band height R <= r_max whose
- * LDS state fits.  False if none does. */
-bool plan_geometry(JobArgs& a, uint32_t* lds_bytes, uint32_t r_max = 8, uint32_t queue_cap = 1024);
+ * LDS state fits (queue_cap 0: the mode's default).  False if none does. */
+bool plan_geometry(JobArgs& a, uint32_t* lds_bytes, uint32_t r_max = 16, uint32_t queue_cap = 0);
 
 /* The most isolevels (<= kMaxLevels) one pass over this volume can hold in
- * the emit LDS at the default queue size; 0 if not even one fits. */
-uint32_t levels_per_pass(const kmc_volume* v);
+ * the emit LDS at the default sizes (for VertexNormal when vnormal); 0 if
+ * not even one fits. */
+uint32_t levels_per_pass(const kmc_volume* v, int vnormal);
 
 /* Kernel launchers (kmc_kernels.hip). */
 hipError_t launch_emit(int dtype, const JobArgs& a, uint32_t lds_bytes, hipStream_t st);
@@ -65,8 +66,9 @@ hipError_t launch_classify_batch(int dtype, const JobArgs* jobs, const uint64_t*
                                  uint64_t rows, hipStream_t st);
 hipError_t launch_count_batch(const JobArgs* jobs, const uint64_t* row_start, uint32_t njobs, uint64_t rows,
                               hipStream_t st);
-hipError_t launch_emit_batch(int dtype, const JobArgs* jobs, const uint64_t* tile_start, uint32_t njobs,
-                             uint64_t tiles, uint32_t lds_bytes, hipStream_t st);
+/* mode: 0 duplicated vertices, 1 PolygonNormal, 2 VertexNormal (shared vertices) */
+hipError_t launch_emit_batch(int dtype, int mode, const JobArgs* jobs, const uint64_t* unit_start, uint32_t njobs,
+                             uint64_t units, uint32_t lds_bytes, hipStream_t st);
 
 /* Two-phase use of a batch plan without a graph (kmc_batch.hip): plan, launch
  * the front (classify, count, scan, sizes), wait, read the sizes, point the

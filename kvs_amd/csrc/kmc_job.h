@@ -8,17 +8,22 @@
  *
  * Geometry of the work decomposition (all in reference scan order, x fastest):
  *   - a *band* is R consecutive node rows [y0, y0+R) spanning the full x range;
- *   - a *tile* (z, b) is band b of node slice z.  It owns the vertices of the
- *     crossing edges whose low node lies in the band at slice z, and the
- *     triangles of the cells of band b at cell slice z-1.  Tiles in (z, b)
- *     order enumerate vertices in node scan order x {x,y,z}-edge and triangles
- *     in cell scan order, which is exactly the order of calculate_isopoints
- *     and connect_isopoints / extract_surfaces_with_duplication in
+ *   - a *tile* (z, b) is band b at step z.  It emits the triangles of the
+ *     cells of band b at cell slice z-1 and the vertices of the crossing edges
+ *     whose low node lies in the band at node slice z (or z-1 with
+ *     VertexNormal, whose normals need the face normals of the cells on both
+ *     sides of a slice).  Tiles in (z, b) order enumerate vertices in node
+ *     scan order x {x,y,z}-edge and triangles in cell scan order, which is the
+ *     order of calculate_isopoints and connect_isopoints /
+ *     extract_surfaces_with_duplication in
  *     Source/Core/Visualization/Mapper/MarchingCubes.cpp:155-243,365-521;
- *   - a workgroup marches z through a *slab* of consecutive slices of one band.
+ *   - a *unit* is one band over a slab of `slab` consecutive steps; a
+ *     workgroup marches z through a unit, keeping a ring of node-slice
+ *     classification masks (one new slice loaded per step) and, with
+ *     VertexNormal, a ring of the face normals of the last two cell slices.
  *
  * Passes of one job: classify (stream the values once, write the bit volume),
- * count (per-tile vertex / triangle counts from the bit volume), scan (tile
+ * count (per-row vertex / triangle counts from the bit volume), scan (row
  * prefixes), emit (outputs at the prefixes; raw values are read again only at
  * the endpoints of crossing edges).
  *
@@ -45,14 +50,13 @@ struct JobArgs {
     uint32_t cshift;   // lanes are split into 64 >> cshift rows of (1 << cshift) chunk lanes
                        // (1 << cshift = the power of two >= min(NC, 64))
     uint32_t nlev;     // isolevels in this pass
-    uint32_t slab;     // slices marched by one workgroup
-    uint32_t nslabs;
+    uint32_t slab;     // steps marched by one unit
+    uint32_t nslabs;   // units per band, ceil(nz / slab)
     uint32_t dup;      // 1: duplicated vertices (per-triangle output)
     uint32_t vnormal;  // 1: VertexNormal (only without duplication)
     uint32_t queue_cap;
-    uint32_t cache_cap;   // window vertex coordinates held in LDS by the emit pass
-    uint32_t use_cache;   // 1: interpolate each window vertex once into LDS; 0: interpolate per use
-    uint64_t ntiles;   // nz * nb
+    uint32_t fn_cap;   // face normals held per cell slice (VertexNormal)
+    uint64_t nunits;   // nb * nslabs
     uint64_t nrows;    // nz * ny: counts and prefixes are kept per node row
     double minv, maxv; // clamp range of the interpolation (volume min/max)
     float minc[3];     // min object coordinate
@@ -83,40 +87,43 @@ struct LevelInfo {
     uint32_t* conn;
     uint64_t cap_v, cap_t;
     double iso;
-    uint64_t vbase[4];  // vertex prefix of tiles (z-1,b), (z-1,b+1), (z,b), (z,b+1)
-    uint64_t tbase;     // triangle prefix of tile (z,b)
+    uint64_t vbase[4];  // vertex prefix of node rows (z-1,y0), (z-1,y0+R), (z,y0), (z,y0+R)
+    uint64_t tbase;     // triangle prefix of cell row (y0, z-1)
 };
 
 /* LDS layout of the emit workgroup; sizes in bytes, offsets 16-byte aligned.
- * The window of tile (z, b) is node slices z-1, z, z+1 (window slices 0..2)
- * and node rows y0-1 .. y0+R (window rows 0..R+1).  Arrays are indexed
- * [slice][level][row][chunk] so that consecutive lanes (chunks) touch
- * consecutive words. */
+ * Node slices live in a ring of `nslots` mask slots (4 with VertexNormal:
+ * slices z-2 .. z+1; else 3: z-1 .. z+1), each holding node rows
+ * y0-1 .. y0+R+1.  Window vertex prefixes cover node slices z-1 and z and
+ * node rows y0-1 .. y0+R; cell data cover cell slice z-1, cell rows
+ * y0-1 .. y0+R-1 (row 0 is the halo row of the band above, used for face
+ * normals only), in a ring of 2 cell slices with VertexNormal. */
 struct LdsLayout {
-    uint32_t nrb;       // bit rows held per slice: y0-1 .. y0+R+1
-    uint32_t ncp;       // chunks per stored row (= NC)
-    uint32_t off_tri;   // uint64 [256]                     case table
+    uint32_t nrb;       // mask rows held per slice: y0-1 .. y0+R+1
+    uint32_t nslots;    // node-slice mask slots
+    uint32_t ncs;       // cell-slice slots
+    uint32_t off_tri;   // uint64 [256]                          case table
     uint32_t off_lev;   // LevelInfo [kMaxLevels]
-    uint32_t off_bits;  // uint64 [3][nlev][nrb][ncp]        classification masks
-    uint32_t off_wpv;   // uint32 [3][nlev][R+2][NC]         window vertex chunk prefixes
-    uint32_t off_act;   // uint64 [nlev][R][NC]              active cells, cell slice z-1, band rows
-    uint32_t off_pt;    // uint32 [nlev][R][NC]              triangle chunk prefixes
-    uint32_t off_misc;  // uint32 [64]                       totals, offsets, flags
-    uint32_t off_cache; // float  [cache_cap][3]             window vertex coordinates
-    uint32_t off_vq;    // uint32 [queue_cap]                vertex work queue
-    uint32_t off_tq;    // uint32 [queue_cap]                triangle work queue
+    uint32_t off_bits;  // uint64 [nslots][nlev][nrb][NC]         classification masks
+    uint32_t off_wpv;   // uint32 [2][nlev][R+2][NC]              window vertex chunk prefixes
+    uint32_t off_act;   // uint64 [ncs][nlev][R+1][NC]            active cells
+    uint32_t off_pt;    // uint32 [ncs][nlev][R+1][NC]            triangle chunk prefixes
+    uint32_t off_misc;  // uint32 [kMiscWords]                    totals, offsets, flags
+    uint32_t off_fn;    // float  [ncs][fn_cap][3]                face normals (VertexNormal)
+    uint32_t off_vq;    // uint32 [queue_cap]                     vertex work queue
+    uint32_t off_tq;    // uint32 [queue_cap]                     triangle work queue
     uint32_t bytes;
 };
 
 /* misc words */
 enum : uint32_t {
     kMiscVTot = 0,        // [kMaxLevels] tile vertex totals
-    kMiscTTot = 8,        // [kMaxLevels] tile triangle totals
+    kMiscTTot = 8,        // [kMaxLevels] tile triangle totals (halo row included)
     kMiscQV = 16,         // [kMaxLevels + 1] vertex queue offsets per level
     kMiscQT = 25,         // [kMaxLevels + 1] triangle queue offsets per level
-    kMiscWOff = 34,       // [3 * kMaxLevels] cache offsets of (window slice, level) -- needs 24
-    kMiscOverflow = 58,   // cache overflow flag
-    kMiscWTotal = 59,     // window vertices in all slices and levels
+    kMiscHalo = 34,       // [kMaxLevels] halo-row triangles per level
+    kMiscFnOff = 42,      // [2][kMaxLevels + 1] face-normal offsets per (cell slot, level)
+    kMiscFnOvf = 60,      // [2] face normals of the cell slot did not fit
     kMiscWords = 64
 };
 
@@ -128,22 +135,24 @@ inline uint32_t align16(uint32_t x) { return (x + 15u) & ~15u; }
 #if defined(__HIPCC__)
 __host__ __device__
 #endif
-inline LdsLayout lds_layout(uint32_t R, uint32_t NC, uint32_t nlev, uint32_t queue_cap, uint32_t cache_cap)
+inline LdsLayout lds_layout(const JobArgs& a)
 {
     LdsLayout L;
+    const uint32_t R = a.R, NC = a.NC, nlev = a.nlev;
     L.nrb = R + 3;
-    L.ncp = NC;
+    L.nslots = a.vnormal ? 4u : 3u;
+    L.ncs = a.vnormal ? 2u : 1u;
     uint32_t o = 0;
     L.off_tri = o;   o = align16(o + 256u * 8u);
     L.off_lev = o;   o = align16(o + kMaxLevels * (uint32_t)sizeof(LevelInfo));
-    L.off_bits = o;  o = align16(o + 3u * nlev * L.nrb * L.ncp * 8u);
-    L.off_wpv = o;   o = align16(o + 3u * nlev * (R + 2) * NC * 4u);
-    L.off_act = o;   o = align16(o + nlev * R * NC * 8u);
-    L.off_pt = o;    o = align16(o + nlev * R * NC * 4u);
+    L.off_bits = o;  o = align16(o + L.nslots * nlev * L.nrb * NC * 8u);
+    L.off_wpv = o;   o = align16(o + 2u * nlev * (R + 2) * NC * 4u);
+    L.off_act = o;   o = align16(o + L.ncs * nlev * (R + 1) * NC * 8u);
+    L.off_pt = o;    o = align16(o + L.ncs * nlev * (R + 1) * NC * 4u);
     L.off_misc = o;  o = align16(o + kMiscWords * 4u);
-    L.off_cache = o; o = align16(o + cache_cap * 12u);
-    L.off_vq = o;    o = align16(o + queue_cap * 4u);
-    L.off_tq = o;    o = align16(o + queue_cap * 4u);
+    L.off_fn = o;    o = align16(o + (a.vnormal ? L.ncs * a.fn_cap * 12u : 0u));
+    L.off_vq = o;    o = align16(o + a.queue_cap * 4u);
+    L.off_tq = o;    o = align16(o + a.queue_cap * 4u);
     L.bytes = o;
     return L;
 }

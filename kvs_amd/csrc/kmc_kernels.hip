@@ -20,6 +20,9 @@
  * -ffp-contract=off so that no multiply-add is fused.
  */
 #include <hip/hip_runtime.h>
+#include <algorithm>
+#include <map>
+#include <mutex>
 #include "kmc_job.h"
 #include "mc_tables.h"
 
@@ -28,6 +31,12 @@
 namespace kmc {
 
 __constant__ uint64_t d_tri[256] = { KMC_TABLE_WORDS };
+
+/* Emit modes, a template parameter so that each kernel carries only its own
+ * path: duplicated vertices, shared vertices with PolygonNormal, shared
+ * vertices with VertexNormal. */
+enum : int { kModeDup = 0, kModePolygon = 1, kModeVertex = 2 };
+__host__ __device__ inline int emit_mode(const JobArgs& a) { return a.dup ? kModeDup : (a.vnormal ? kModeVertex : kModePolygon); }
 
 /* ------------------------------------------------------------------------- */
 /* classification                                                              */
@@ -122,20 +131,20 @@ __device__ __forceinline__ void face_normal(const float v0[3], const float v1[3]
 struct Views {
     uint64_t* tri;      // [256] case table
     LevelInfo* lev;     // [kMaxLevels]
-    uint64_t* bits;     // [3][nlev][nrb][ncp]
-    uint32_t* wpv;      // [3][nlev][R+2][NC]
-    uint64_t* act;      // [nlev][R][NC]
-    uint32_t* pt;       // [nlev][R][NC]
+    uint64_t* bits;     // [nslots][nlev][nrb][NC]
+    uint32_t* wpv;      // [2][nlev][R+2][NC]
+    uint64_t* act;      // [ncs][nlev][R+1][NC]
+    uint32_t* pt;       // [ncs][nlev][R+1][NC]
     uint32_t* misc;     // [kMiscWords]
-    float* cache;       // [cache_cap][3]
+    float* fn;          // [ncs][fn_cap][3]
     uint32_t* vq;       // [queue_cap]
     uint32_t* tq;       // [queue_cap]
-    uint32_t nrb, ncp, R, NC, nlev;
+    uint32_t nrb, nslots, R, NC, nlev, fn_cap;
 };
 
 __device__ __forceinline__ Views make_views(const JobArgs& a, uint8_t* smem)
 {
-    const LdsLayout L = lds_layout(a.R, a.NC, a.nlev, a.queue_cap, a.cache_cap);
+    const LdsLayout L = lds_layout(a);
     Views v;
     v.tri = (uint64_t*)(smem + L.off_tri);
     v.lev = (LevelInfo*)(smem + L.off_lev);
@@ -144,30 +153,43 @@ __device__ __forceinline__ Views make_views(const JobArgs& a, uint8_t* smem)
     v.act = (uint64_t*)(smem + L.off_act);
     v.pt = (uint32_t*)(smem + L.off_pt);
     v.misc = (uint32_t*)(smem + L.off_misc);
-    v.cache = (float*)(smem + L.off_cache);
+    v.fn = (float*)(smem + L.off_fn);
     v.vq = (uint32_t*)(smem + L.off_vq);
     v.tq = (uint32_t*)(smem + L.off_tq);
     v.nrb = L.nrb;
-    v.ncp = L.ncp;
+    v.nslots = L.nslots;
     v.R = a.R;
     v.NC = a.NC;
     v.nlev = a.nlev;
+    v.fn_cap = a.fn_cap;
     return v;
 }
 
-/* Ring slot of node slice s (s >= -1): any three consecutive slices get
+/* Ring slot of node slice s (s >= -2): any nslots consecutive slices get
  * distinct slots. */
-__device__ __forceinline__ uint32_t bslot(int s) { return (uint32_t)(s + 1) % 3u; }
+__device__ __forceinline__ uint32_t bslot(const Views& v, int s)
+{
+    return v.nslots == 4u ? ((uint32_t)s & 3u) : (uint32_t)(s + 3) % 3u;
+}
 
-/* Row of chunk masks: slice slot sl, level l, LDS row r (row y = y0 - 1 + r). */
+/* Ring slot of cell slice zc (VertexNormal keeps two cell slices). */
+template <int M>
+__device__ __forceinline__ uint32_t cslot(int zc) { return M == kModeVertex ? ((uint32_t)zc & 1u) : 0u; }
+
+/* Row of chunk masks: slot sl, level l, LDS row r (node row y = y0 - 1 + r). */
 __device__ __forceinline__ uint64_t* brow(const Views& v, uint32_t sl, uint32_t l, uint32_t r)
 {
-    return v.bits + ((sl * v.nlev + l) * v.nrb + r) * v.ncp;
+    return v.bits + ((sl * v.nlev + l) * v.nrb + r) * v.NC;
 }
-/* Window vertex chunk prefixes of window slice ws (0..2), level l, window row w. */
+/* Window vertex chunk prefixes of window slice ws (0: node slice z-1, 1: z), level l, window row w. */
 __device__ __forceinline__ uint32_t* wrow(const Views& v, uint32_t ws, uint32_t l, uint32_t w)
 {
     return v.wpv + ((ws * v.nlev + l) * (v.R + 2) + w) * v.NC;
+}
+/* Cell data index: cell slot cs, level l, cell row j (cell row y0 - 1 + j), chunk c. */
+__device__ __forceinline__ uint32_t cidx(const Views& v, uint32_t cs, uint32_t l, uint32_t j, uint32_t c)
+{
+    return ((cs * v.nlev + l) * (v.R + 1) + j) * v.NC + c;
 }
 
 /* Mask of the nodes x < limit inside chunk c. */
@@ -188,8 +210,8 @@ __device__ __forceinline__ uint64_t shift_next(const uint64_t* row, uint32_t c, 
 struct Cross { uint64_t x, y, z; };
 
 /* Crossing edges owned by the nodes of chunk c of window row w (node row
- * y0-1+w) of window slice ws (node slice z-1+ws).  Slice z+1's z-edges are
- * outside the window (their far end is slice z+2) and count as none. */
+ * y0-1+w) of window slice ws (node slice z-1+ws).  z-edges reach slice
+ * z+ws, whose masks the ring holds. */
 __device__ __forceinline__ Cross window_cross(const JobArgs& a, const Views& v, int z, int y0,
                                               uint32_t ws, uint32_t w, uint32_t c, uint32_t l)
 {
@@ -197,22 +219,49 @@ __device__ __forceinline__ Cross window_cross(const JobArgs& a, const Views& v, 
     const int s = z - 1 + (int)ws;
     const int y = y0 - 1 + (int)w;
     if (s < 0 || s >= (int)a.nz || y < 0 || y >= (int)a.ny) { k.x = k.y = k.z = 0; return k; }
-    const uint64_t* row = brow(v, bslot(s), l, w);
+    const uint64_t* row = brow(v, bslot(v, s), l, w);
     const uint64_t b = row[c];
     k.x = (b ^ shift_next(row, c, a.NC)) & valid_mask(c, a.nx - 1);
-    k.y = (y + 1 < (int)a.ny) ? ((b ^ brow(v, bslot(s), l, w + 1)[c]) & valid_mask(c, a.nx)) : 0ull;
-    k.z = (ws < 2 && s + 1 < (int)a.nz) ? ((b ^ brow(v, bslot(s + 1), l, w)[c]) & valid_mask(c, a.nx)) : 0ull;
+    k.y = (y + 1 < (int)a.ny) ? ((b ^ brow(v, bslot(v, s), l, w + 1)[c]) & valid_mask(c, a.nx)) : 0ull;
+    k.z = (s + 1 < (int)a.nz) ? ((b ^ brow(v, bslot(v, s + 1), l, w)[c]) & valid_mask(c, a.nx)) : 0ull;
     return k;
 }
 
-/* Cube index of cell (cx, cy, cz) from the bit rows (y0: first band row). */
+/* The eight corner masks of the cells of chunk c, cell row cy (node rows cy,
+ * cy+1), cell slice cz (node slices cz, cz+1), shifted so that bit i is
+ * corner k of cell x = 64c + i (corner order of MarchingCubes.cpp:187-194). */
+struct Corners { uint64_t c[8]; };
+
+__device__ __forceinline__ Corners corners_at(const JobArgs& a, const Views& v, int y0, uint32_t c, int cy, int cz,
+                                              uint32_t l)
+{
+    const uint32_t r = (uint32_t)(cy - y0 + 1);
+    const uint64_t* r0 = brow(v, bslot(v, cz), l, r);
+    const uint64_t* r1 = brow(v, bslot(v, cz), l, r + 1);
+    const uint64_t* r2 = brow(v, bslot(v, cz + 1), l, r);
+    const uint64_t* r3 = brow(v, bslot(v, cz + 1), l, r + 1);
+    Corners k;
+    k.c[0] = r0[c]; k.c[1] = shift_next(r0, c, a.NC); k.c[3] = r1[c]; k.c[2] = shift_next(r1, c, a.NC);
+    k.c[4] = r2[c]; k.c[5] = shift_next(r2, c, a.NC); k.c[7] = r3[c]; k.c[6] = shift_next(r3, c, a.NC);
+    return k;
+}
+
+__device__ __forceinline__ uint32_t corner_index(const Corners& k, uint32_t i)
+{
+    uint32_t ci = 0;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) ci |= (uint32_t)((k.c[q] >> i) & 1ull) << q;
+    return ci;
+}
+
+/* Cube index of cell (cx, cy, cz) from the mask ring (y0: first band row). */
 __device__ __forceinline__ uint32_t cube_index_at(const Views& v, int y0, uint32_t cx, int cy, int cz, uint32_t l)
 {
     const uint32_t r = (uint32_t)(cy - y0 + 1);
-    const uint64_t* b00 = brow(v, bslot(cz), l, r);
-    const uint64_t* b01 = brow(v, bslot(cz), l, r + 1);
-    const uint64_t* b10 = brow(v, bslot(cz + 1), l, r);
-    const uint64_t* b11 = brow(v, bslot(cz + 1), l, r + 1);
+    const uint64_t* b00 = brow(v, bslot(v, cz), l, r);
+    const uint64_t* b01 = brow(v, bslot(v, cz), l, r + 1);
+    const uint64_t* b10 = brow(v, bslot(v, cz + 1), l, r);
+    const uint64_t* b11 = brow(v, bslot(v, cz + 1), l, r + 1);
     const uint32_t c0 = cx >> 6, i0 = cx & 63u;
     const uint32_t c1 = (cx + 1) >> 6, i1 = (cx + 1) & 63u;
     uint32_t ci = 0;
@@ -261,16 +310,16 @@ __device__ __forceinline__ uint32_t wave_exclusive_scan(uint32_t* p, uint32_t n)
 /* slice loading                                                               */
 /* ------------------------------------------------------------------------- */
 
-/* Bring the chunk masks of node rows y0-1 .. y0+R+1 of slices z-1, z, z+1
- * from the bit volume into LDS (slot bslot(s) of each slice); rows and
- * slices outside the volume are zero.  For one (slice, level) the rows are
+/* Bring the chunk masks of node rows y0-1 .. y0+R+1 of node slices
+ * s_lo .. s_hi from the bit volume into their ring slots; rows and slices
+ * outside the volume are zero.  For one (slice, level) the rows are
  * contiguous both in the bit volume and in LDS, so the copy is a set of flat
  * ranges; loads are issued in batches before their stores. */
-__device__ void load_window(const JobArgs& a, const Views& v, int z, int y0)
+__device__ void load_slices(const JobArgs& a, const Views& v, int s_lo, int s_hi, int y0)
 {
+    if (s_hi < s_lo) return;
     const uint32_t nseg = v.nrb * a.NC;              // elements of one (slice, level)
-    const uint32_t ng = 3u * a.nlev;
-    /* valid element range of every segment: rows y0-1 .. y0+R+1 clipped to the volume */
+    const uint32_t ng = (uint32_t)(s_hi - s_lo + 1) * a.nlev;
     const uint32_t e_lo = (y0 - 1 < 0) ? a.NC : 0u;
     const int rows_left = (int)a.ny - (y0 - 1);
     const uint32_t e_hi = (rows_left >= (int)v.nrb ? v.nrb : (uint32_t)(rows_left > 0 ? rows_left : 0)) * a.NC;
@@ -284,14 +333,14 @@ __device__ void load_window(const JobArgs& a, const Views& v, int z, int y0)
 #pragma unroll
         for (int u = 0; u < kB; ++u) {
             if (g < ng) {
-                const uint32_t ws = g / a.nlev, l = g - ws * a.nlev;
-                const int s = z - 1 + (int)ws;
+                const uint32_t si = g / a.nlev, l = g - si * a.nlev;
+                const int s = s_lo + (int)si;
                 const bool ok = s >= 0 && s < (int)a.nz && e >= e_lo && e < e_hi;
                 /* element e of the segment is node row y0-1 + e / NC of slice s */
                 const uint64_t idx = ok ? (((uint64_t)l * a.nz + (uint64_t)s) * a.ny + (uint64_t)(y0 - 1)) * a.NC + e
                                         : 0ull;
                 val[u] = ok ? a.bitvol[idx] : 0ull;
-                dst[u] = ((bslot(s) * a.nlev + l) * v.nrb) * a.NC + e;
+                dst[u] = ((bslot(v, s) * a.nlev + l) * v.nrb) * a.NC + e;
                 cnt = u + 1;
                 e += kBlock;
                 while (g < ng && e >= nseg) { ++g; e = threadIdx.x; }
@@ -304,118 +353,118 @@ __device__ void load_window(const JobArgs& a, const Views& v, int z, int y0)
 }
 
 /* ------------------------------------------------------------------------- */
-/* per-tile counts and prefixes (one wave per row, one lane per chunk)         */
+/* per-tile counts and prefixes (lanes on (row, chunk))                        */
 /* ------------------------------------------------------------------------- */
 
-/* Window vertex counts per chunk for window slices 0..2 and rows 0..R+1 (not
- * needed with duplication), and active cells / triangle counts of the band's
- * cells at cell slice z-1; then exclusive prefixes: window prefixes per
- * (slice, level), triangle prefixes per level.  Fills misc with the tile
- * totals and the cache offsets.  Lanes cover 64 >> cshift rows of
- * 1 << cshift chunks each. */
-__device__ void tile_prefixes(const JobArgs& a, const Views& v, int z, int y0)
+/* Tile z: window vertex counts per chunk of window slices 0, 1 (node slices
+ * z-1, z), rows 1..R+1 (for output ranks), and active cells / triangle
+ * counts of cell slice z-1, cell rows j0..R (j = 0: the halo row y0-1, only
+ * with VertexNormal); then exclusive prefixes -- window prefixes per
+ * (slice, level), triangle prefixes per level -- and the queue offsets.
+ * ws_v: window slice whose vertices the tile emits.  Work items are
+ * (level, row group) pairs spread over all waves. */
+template <int M>
+__device__ void tile_prefixes(const JobArgs& a, const Views& v, int z, int y0, uint32_t ws_v, bool ranks)
 {
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
     const uint32_t sub = lane >> a.cshift, cl = lane & ((1u << a.cshift) - 1u);
     const uint32_t G = 64u >> a.cshift, CS = 1u << a.cshift;
     const uint32_t nwr = a.R + 2;
-    /* Without the cache only the tile ranks are needed: window slices 0, 1
-     * (node slices z-1, z) and window rows 1..R+1. */
-    const uint32_t nws = a.use_cache ? 3u : 2u;
-    const uint32_t wfirst = a.use_cache ? 0u : 1u;
-    if (!a.dup) {
-        for (uint32_t ws = 0; ws < nws; ++ws) {
-            for (uint32_t l = 0; l < a.nlev; ++l) {
-                for (uint32_t w0 = wfirst + wave * G; w0 < nwr; w0 += kWaves * G) {
-                    const uint32_t w = w0 + sub;
-                    if (w >= nwr) continue;
-                    uint32_t* dst = wrow(v, ws, l, w);
-                    for (uint32_t c = cl; c < a.NC; c += CS) {
-                        const Cross k = window_cross(a, v, z, y0, ws, w, c, l);
-                        dst[c] = (uint32_t)(__popcll(k.x) + __popcll(k.y) + __popcll(k.z));
-                    }
-                }
+    constexpr bool halo = M == kModeVertex;
+    constexpr bool dup = M == kModeDup;
+    const uint32_t j0 = halo ? 0u : 1u;
+    const int zc = z - 1;
+    const uint32_t cs = cslot<M>(zc);
+    if (ranks && !dup) {
+        /* window rows 1..R+1 of window slices 0, 1 */
+        const uint32_t ngr = (a.R + 1 + G - 1) / G;              // row groups
+        const uint32_t items = 2u * a.nlev * ngr;
+        for (uint32_t it = wave; it < items; it += kWaves) {
+            const uint32_t ws = it / (a.nlev * ngr), rem = it - ws * a.nlev * ngr;
+            const uint32_t l = rem / ngr, grp = rem - l * ngr;
+            const uint32_t w = 1 + grp * G + sub;
+            if (w >= nwr) continue;
+            uint32_t* dst = wrow(v, ws, l, w);
+            for (uint32_t c = cl; c < a.NC; c += CS) {
+                const Cross k = window_cross(a, v, z, y0, ws, w, c, l);
+                dst[c] = (uint32_t)(__popcll(k.x) + __popcll(k.y) + __popcll(k.z));
             }
         }
     }
-    const int zc = z - 1;
-    for (uint32_t l = 0; l < a.nlev; ++l) {
-        for (uint32_t k0 = wave * G; k0 < a.R; k0 += kWaves * G) {
-            const uint32_t k = k0 + sub;
-            if (k >= a.R) continue;
-            const int y = y0 + (int)k;
+    {
+        const uint32_t nj = a.R + 1;
+        const uint32_t ngr = (nj + G - 1) / G;
+        const uint32_t items = a.nlev * ngr;
+        for (uint32_t it = wave; it < items; it += kWaves) {
+            const uint32_t l = it / ngr, grp = it - l * ngr;
+            const uint32_t j = grp * G + sub;
+            if (j >= nj) continue;
+            const int y = y0 - 1 + (int)j;
             for (uint32_t c = cl; c < a.NC; c += CS) {
                 uint64_t act = 0;
                 uint32_t cnt = 0;
-                if (zc >= 0 && y + 1 < (int)a.ny) {
-                    const uint64_t* r0 = brow(v, bslot(zc), l, k + 1);
-                    const uint64_t* r1 = brow(v, bslot(zc), l, k + 2);
-                    const uint64_t* r2 = brow(v, bslot(z), l, k + 1);
-                    const uint64_t* r3 = brow(v, bslot(z), l, k + 2);
-                    const uint64_t c0 = r0[c], c1 = shift_next(r0, c, a.NC), c3 = r1[c], c2 = shift_next(r1, c, a.NC);
-                    const uint64_t c4 = r2[c], c5 = shift_next(r2, c, a.NC), c7 = r3[c], c6 = shift_next(r3, c, a.NC);
-                    const uint64_t o = c0 | c1 | c2 | c3 | c4 | c5 | c6 | c7;
-                    const uint64_t an = c0 & c1 & c2 & c3 & c4 & c5 & c6 & c7;
+                if (j >= j0 && zc >= 0 && zc + 1 < (int)a.nz && y >= 0 && y + 1 < (int)a.ny) {
+                    const Corners k = corners_at(a, v, y0, c, y, zc, l);
+                    uint64_t o = 0, an = ~0ull;
+#pragma unroll
+                    for (int q = 0; q < 8; ++q) { o |= k.c[q]; an &= k.c[q]; }
                     act = (o & ~an) & valid_mask(c, a.nx - 1);
                     uint64_t m = act;
                     while (m) {
                         const uint32_t i = (uint32_t)__ffsll((unsigned long long)m) - 1u;
                         m &= m - 1ull;
-                        const uint32_t ci = (uint32_t)((c0 >> i) & 1ull) | (uint32_t)((c1 >> i) & 1ull) << 1 |
-                                            (uint32_t)((c2 >> i) & 1ull) << 2 | (uint32_t)((c3 >> i) & 1ull) << 3 |
-                                            (uint32_t)((c4 >> i) & 1ull) << 4 | (uint32_t)((c5 >> i) & 1ull) << 5 |
-                                            (uint32_t)((c6 >> i) & 1ull) << 6 | (uint32_t)((c7 >> i) & 1ull) << 7;
-                        cnt += (uint32_t)(v.tri[ci] >> 60);
+                        cnt += (uint32_t)(v.tri[corner_index(k, i)] >> 60);
                     }
                 }
-                v.act[(l * a.R + k) * a.NC + c] = act;
-                v.pt[(l * a.R + k) * a.NC + c] = cnt;
+                v.act[cidx(v, cs, l, j, c)] = act;
+                v.pt[cidx(v, cs, l, j, c)] = cnt;
             }
         }
     }
     __syncthreads();
     /* one wave per sequence */
-    const uint32_t nwseq = a.dup ? 0u : nws * a.nlev;
+    const uint32_t nwseq = (ranks && !dup) ? 2u * a.nlev : 0u;
     const uint32_t nseq = nwseq + a.nlev;
     for (uint32_t q = wave; q < nseq; q += kWaves) {
         if (q < nwseq) {
-            uint32_t* base = wrow(v, q / a.nlev, q % a.nlev, wfirst);
-            const uint32_t tot = wave_exclusive_scan(base, (nwr - wfirst) * a.NC);
-            if (lane == 0) v.misc[kMiscWOff + q] = tot;   // total for now; offsets below
+            (void)wave_exclusive_scan(wrow(v, q / a.nlev, q % a.nlev, 1), (nwr - 1) * a.NC);
         } else {
             const uint32_t l = q - nwseq;
-            const uint32_t tot = wave_exclusive_scan(v.pt + l * a.R * a.NC, a.R * a.NC);
+            const uint32_t tot = wave_exclusive_scan(v.pt + cidx(v, cs, l, 0, 0), (a.R + 1) * a.NC);
             if (lane == 0) v.misc[kMiscTTot + l] = tot;
         }
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
-        uint32_t off = 0;
-        for (uint32_t q = 0; q < nwseq; ++q) {
-            const uint32_t t = v.misc[kMiscWOff + q];
-            v.misc[kMiscWOff + q] = off;
-            off += t;
-        }
-        v.misc[kMiscWTotal] = off;
-        v.misc[kMiscOverflow] = (a.dup || !a.use_cache || off > a.cache_cap) ? 1u : 0u;
-        uint32_t ov = 0, ot = 0;
-        for (uint32_t l = 0; l < a.nlev; ++l) {
-            /* tile vertices: window slice 1, window rows 1..R */
-            const uint32_t vt = a.dup ? 0u : (wrow(v, 1, l, a.R + 1)[0] - wrow(v, 1, l, 1)[0]);
+    if (threadIdx.x < 64) {
+        /* queue offsets: exclusive scans over levels (nlev <= 8 lanes) */
+        const uint32_t l = lane;
+        const bool in = l < a.nlev;
+        const uint32_t vt = (in && !dup && ranks) ? (wrow(v, ws_v, l, a.R + 1)[0] - wrow(v, ws_v, l, 1)[0]) : 0u;
+        const uint32_t tt = in ? v.misc[kMiscTTot + l] : 0u;
+        const uint32_t hl = (in && halo) ? v.pt[cidx(v, cs, l, 1, 0)] : 0u;
+        const uint32_t iv = wave_incl_scan(vt), it = wave_incl_scan(tt);
+        if (in) {
             v.misc[kMiscVTot + l] = vt;
-            v.misc[kMiscQV + l] = ov;
-            v.misc[kMiscQT + l] = ot;
-            ov += vt;
-            ot += v.misc[kMiscTTot + l];
+            v.misc[kMiscQV + l] = iv - vt;
+            v.misc[kMiscQT + l] = it - tt;
+            v.misc[kMiscHalo + l] = hl;
+            if (halo) v.misc[kMiscFnOff + cs * 9 + l] = it - tt;
         }
-        v.misc[kMiscQV + a.nlev] = ov;
-        v.misc[kMiscQT + a.nlev] = ot;
+        const uint32_t tv = __shfl(iv, (int)a.nlev - 1), ttot = __shfl(it, (int)a.nlev - 1);
+        if (lane == 0) {
+            v.misc[kMiscQV + a.nlev] = tv;
+            v.misc[kMiscQT + a.nlev] = ttot;
+            if (halo) {
+                v.misc[kMiscFnOff + cs * 9 + a.nlev] = ttot;
+                v.misc[kMiscFnOvf + cs] = ttot > a.fn_cap ? 1u : 0u;
+            }
+        }
     }
     __syncthreads();
 }
 
 /* ------------------------------------------------------------------------- */
-/* vertex coordinates                                                          */
+/* vertex coordinates and normals                                              */
 /* ------------------------------------------------------------------------- */
 
 /* Coordinates of the shared vertex on the edge of node (x,y,z) along axis,
@@ -428,131 +477,75 @@ __device__ __forceinline__ void node_edge_coords(const JobArgs& a, const T* vol,
 }
 
 /* Owner node offset (dx, dy, dz) and axis of local edge e, packed as
- * dx | dy << 1 | dz << 2 | axis << 3 (connect_isopoints, :493-504). */
+ * dx | dy << 1 | dz << 2 | axis << 3 (connect_isopoints, :493-504), five
+ * bits per edge. */
 __device__ __forceinline__ uint32_t edge_owner(uint32_t e)
 {
-    switch (e) {
-    case 0: return 0x00u;                 // (0,0,0) x
-    case 1: return 0x01u | (1u << 3);     // (1,0,0) y
-    case 2: return 0x02u;                 // (0,1,0) x
-    case 3: return 0x00u | (1u << 3);     // (0,0,0) y
-    case 4: return 0x04u;                 // (0,0,1) x
-    case 5: return 0x05u | (1u << 3);     // (1,0,1) y
-    case 6: return 0x06u;                 // (0,1,1) x
-    case 7: return 0x04u | (1u << 3);     // (0,0,1) y
-    case 8: return 0x00u | (2u << 3);     // (0,0,0) z
-    case 9: return 0x01u | (2u << 3);     // (1,0,0) z
-    case 10: return 0x03u | (2u << 3);    // (1,1,0) z
-    default: return 0x02u | (2u << 3);    // (0,1,0) z
-    }
+    constexpr uint64_t kOwner =
+        (0x00ull << 0) | (0x09ull << 5) | (0x02ull << 10) | (0x08ull << 15) | (0x04ull << 20) | (0x0Dull << 25) |
+        (0x06ull << 30) | (0x0Cull << 35) | (0x10ull << 40) | (0x11ull << 45) | (0x13ull << 50) | (0x12ull << 55);
+    return (uint32_t)(kOwner >> (5 * e)) & 31u;
 }
 
 /* Table-orientation endpoints of local edge e as two corner offsets packed
  * bit0 x, bit1 y, bit2 z, first endpoint in the low nibble
- * (MarchingCubesTable.cpp:276-290). */
+ * (MarchingCubesTable.cpp:276-290), eight bits per edge. */
 __device__ __forceinline__ uint32_t edge_ends(uint32_t e)
 {
-    switch (e) {
-    case 0: return 0x10u; case 1: return 0x31u; case 2: return 0x23u; case 3: return 0x02u;
-    case 4: return 0x54u; case 5: return 0x75u; case 6: return 0x67u; case 7: return 0x46u;
-    case 8: return 0x40u; case 9: return 0x51u; case 10: return 0x73u; default: return 0x62u;
-    }
+    constexpr uint64_t kLo = 0x4667755402233110ull;   // edges 0..7
+    constexpr uint32_t kHi = 0x62735140u;             // edges 8..11
+    return e < 8 ? (uint32_t)(kLo >> (8 * e)) & 0xFFu : (kHi >> (8 * (e - 8))) & 0xFFu;
 }
 
-/* Index in the window cache of the vertex on (node x of node row yy of node
- * slice zz, axis); the node must lie in the window. */
-__device__ __forceinline__ uint32_t cache_index(const JobArgs& a, const Views& v, int z, int y0,
-                                                uint32_t x, int yy, int zz, uint32_t axis, uint32_t l)
+/* Rank within its tile row range of the vertex on (node x of band row k (0..R)
+ * of node slice s in {z-1, z}, axis): rows 0..R-1 rank from the band's first
+ * row, row R (first row of the next band) from its own start. */
+__device__ __forceinline__ uint32_t tile_rank(const JobArgs& a, const Views& v, int z, int y0,
+                                              uint32_t x, uint32_t k, int s, uint32_t axis, uint32_t l)
 {
-    const uint32_t ws = (uint32_t)(zz - z + 1), w = (uint32_t)(yy - y0 + 1);
+    const uint32_t ws = (uint32_t)(s - z + 1), w = k + 1;
     const uint32_t c = x >> 6, i = x & 63u;
-    const Cross k = window_cross(a, v, z, y0, ws, w, c, l);
+    const Cross e = window_cross(a, v, z, y0, ws, w, c, l);
     const uint64_t lo = (1ull << i) - 1ull;
-    uint32_t r = v.misc[kMiscWOff + ws * a.nlev + l] + wrow(v, ws, l, w)[c];
-    r += __popcll(k.x & lo) + __popcll(k.y & lo) + __popcll(k.z & lo);
-    if (axis >= 1) r += (uint32_t)((k.x >> i) & 1ull);
-    if (axis >= 2) r += (uint32_t)((k.y >> i) & 1ull);
+    const uint32_t start = wrow(v, ws, l, k == a.R ? a.R + 1 : 1)[0];
+    uint32_t r = wrow(v, ws, l, w)[c] - start;
+    r += __popcll(e.x & lo) + __popcll(e.y & lo) + __popcll(e.z & lo);
+    if (axis >= 1) r += (uint32_t)((e.x >> i) & 1ull);
+    if (axis >= 2) r += (uint32_t)((e.y >> i) & 1ull);
     return r;
 }
 
-/* Coordinates of the shared vertex on (x, yy, zz, axis): from the window
- * cache, or interpolated when the window overflowed the cache. */
-template <typename T>
-__device__ __forceinline__ void vertex_coords(const JobArgs& a, const Views& v, const T* vol, double iso, int z, int y0,
-                                              uint32_t x, int yy, int zz, uint32_t axis, uint32_t l, float out[3])
+/* The four cells around the edge of node (x, y, zv) along axis, in scan
+ * order: offsets across the edge are -1/0 along the other two axes (low axis
+ * first); the edge's local id in cell q is the q-th nibble of `around`. */
+__device__ __forceinline__ void around_cell(uint32_t axis, int q, int x, int y, int zv, int& cx, int& cy, int& cz)
 {
-    if (!v.misc[kMiscOverflow]) {
-        const float* p = v.cache + 3u * cache_index(a, v, z, y0, x, yy, zz, axis, l);
-        out[0] = p[0]; out[1] = p[1]; out[2] = p[2];
-    } else {
-        node_edge_coords<T>(a, vol, iso, x, (uint32_t)yy, (uint32_t)zz, axis, out);
-    }
+    const int lo = (q & 1) ? 0 : -1, hi = (q & 2) ? 0 : -1;
+    cx = x + (axis == 0 ? 0 : lo);
+    cy = y + (axis == 0 ? lo : (axis == 1 ? 0 : hi));
+    cz = zv + (axis == 2 ? 0 : hi);
 }
 
-/* Fill the window cache: every crossing edge of the window is interpolated
- * once, one thread per vertex. */
-template <typename T>
-__device__ void fill_cache(const JobArgs& a, const Views& v, const T* vol, int z, int y0)
+__device__ __forceinline__ uint32_t around_nibbles(uint32_t axis)
 {
-    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
-    const uint32_t sub = lane >> a.cshift, cl = lane & ((1u << a.cshift) - 1u);
-    const uint32_t G = 64u >> a.cshift, CS = 1u << a.cshift;
-    const uint32_t nwr = a.R + 2;
-    for (uint32_t ws = 0; ws < 3; ++ws) {
-        for (uint32_t l = 0; l < a.nlev; ++l) {
-            const uint32_t off = v.misc[kMiscWOff + ws * a.nlev + l];
-            for (uint32_t w0 = wave * G; w0 < nwr; w0 += kWaves * G) {
-                const uint32_t w = w0 + sub;
-                if (w >= nwr) continue;
-                const uint32_t* pre = wrow(v, ws, l, w);
-                for (uint32_t c = cl; c < a.NC; c += CS) {
-                    const Cross k = window_cross(a, v, z, y0, ws, w, c, l);
-                    uint64_t m = k.x | k.y | k.z;
-                    uint32_t r = off + pre[c];
-                    while (m) {
-                        const uint32_t i = (uint32_t)__ffsll((unsigned long long)m) - 1u;
-                        m &= m - 1ull;
-                        const uint32_t base = (c * 64u + i) | (w << 20) | (ws << 25) | (l << 27);
-                        if ((k.x >> i) & 1ull) v.vq[r++] = base;
-                        if ((k.y >> i) & 1ull) v.vq[r++] = base | (1u << 30);
-                        if ((k.z >> i) & 1ull) v.vq[r++] = base | (2u << 30);
-                    }
-                }
-            }
-        }
-    }
-    __syncthreads();
-    const uint32_t n = v.misc[kMiscWTotal];
-    for (uint32_t q = threadIdx.x; q < n; q += kBlock) {
-        const uint32_t e = v.vq[q];
-        const uint32_t x = e & 0xFFFFFu, w = (e >> 20) & 31u, ws = (e >> 25) & 3u, l = (e >> 27) & 7u, axis = e >> 30;
-        float p[3];
-        node_edge_coords<T>(a, vol, v.lev[l].iso, x, (uint32_t)(y0 - 1 + (int)w), (uint32_t)(z - 1 + (int)ws), axis, p);
-        v.cache[3 * q + 0] = p[0];
-        v.cache[3 * q + 1] = p[1];
-        v.cache[3 * q + 2] = p[2];
-    }
-    __syncthreads();
+    return axis == 0 ? 0x0246u : (axis == 1 ? 0x3175u : 0x89BAu);
 }
 
-/* VertexNormal of the vertex on (x,y,z,axis): the float sum, in triangle order,
- * of the unnormalized face normals of the triangles that use it -- the order of
- * calculate_normals_on_vertex (:569-605).  Incident cells are visited in cell
- * scan order and triangles in table order. */
+/* VertexNormal of the vertex on (x,y,zv,axis): the float sum, in triangle
+ * order, of the unnormalized face normals of the triangles that use it --
+ * the order of calculate_normals_on_vertex (:569-605).  Incident cells are
+ * visited in cell scan order and triangles in table order; every face normal
+ * is recomputed from interpolated coordinates (used when the face normals of
+ * a cell slice did not fit in LDS). */
 template <typename T>
-__device__ void vertex_normal(const JobArgs& a, const Views& v, const T* vol, double iso, int z, int y0,
-                              uint32_t x, uint32_t y, uint32_t axis, uint32_t l, float n[3])
+__device__ void vertex_normal_direct(const JobArgs& a, const Views& v, const T* vol, double iso, int y0,
+                                     uint32_t x, uint32_t y, int zv, uint32_t axis, uint32_t l, float n[3])
 {
     n[0] = 0.0f; n[1] = 0.0f; n[2] = 0.0f;
-    /* The four cells around the edge in scan order: the two offsets across the
-     * edge are -1/0 along the other two axes (low axis first), and the edge's
-     * local id in each cell is the q-th nibble of `around`. */
-    const uint32_t around = axis == 0 ? 0x0246u : (axis == 1 ? 0x3175u : 0x89BAu);
+    const uint32_t around = around_nibbles(axis);
     for (int q = 0; q < 4; ++q) {
-        const int lo = (q & 1) ? 0 : -1, hi = (q & 2) ? 0 : -1;
-        const int cx = (int)x + (axis == 0 ? 0 : lo);
-        const int cy = (int)y + (axis == 0 ? lo : (axis == 1 ? 0 : hi));
-        const int cz = z + (axis == 2 ? 0 : hi);
+        int cx, cy, cz;
+        around_cell(axis, q, (int)x, (int)y, zv, cx, cy, cz);
         if (cx < 0 || cy < 0 || cz < 0) continue;
         if (cx + 1 >= (int)a.nx || cy + 1 >= (int)a.ny || cz + 1 >= (int)a.nz) continue;
         const uint32_t ci = cube_index_at(v, y0, (uint32_t)cx, cy, cz, l);
@@ -569,8 +562,8 @@ __device__ void vertex_normal(const JobArgs& a, const Views& v, const T* vol, do
 #pragma unroll
             for (int j = 0; j < 3; ++j) {
                 const uint32_t o = edge_owner(es[j]);
-                vertex_coords<T>(a, v, vol, iso, z, y0, (uint32_t)cx + (o & 1u), cy + (int)((o >> 1) & 1u),
-                                 cz + (int)((o >> 2) & 1u), o >> 3, l, p[j]);
+                node_edge_coords<T>(a, vol, iso, (uint32_t)cx + (o & 1u), (uint32_t)(cy + (int)((o >> 1) & 1u)),
+                                    (uint32_t)(cz + (int)((o >> 2) & 1u)), o >> 3, p[j]);
             }
             face_normal(p[0], p[1], p[2], fn);
             n[0] = n[0] + fn[0];
@@ -580,173 +573,223 @@ __device__ void vertex_normal(const JobArgs& a, const Views& v, const T* vol, do
     }
 }
 
-/* Rank within its tile of the vertex on (node x of band row k (0..R) of node
- * slice s in {z-1, z}, axis): rows 0..R-1 rank from the tile's first row,
- * row R (first row of the next band) from its own start. */
-__device__ __forceinline__ uint32_t tile_rank(const JobArgs& a, const Views& v, int z, int y0,
-                                              uint32_t x, uint32_t k, int s, uint32_t axis, uint32_t l)
+/* The same sum from the face normals the triangle phase kept in LDS for cell
+ * slices zv-1 and zv (band rows and the halo row above the band).  The
+ * triangle of (cell, t) sits at its cell slice's face-normal offset of the
+ * level + the triangle prefix of its (cell row, chunk) + the triangles of
+ * the active cells before it in the chunk + t. */
+__device__ void vertex_normal_ring(const JobArgs& a, const Views& v, int y0, uint32_t x, uint32_t y, int zv,
+                                   uint32_t axis, uint32_t l, float n[3])
 {
-    const uint32_t ws = (uint32_t)(s - z + 1), w = k + 1;
-    const uint32_t c = x >> 6, i = x & 63u;
-    const Cross e = window_cross(a, v, z, y0, ws, w, c, l);
-    const uint64_t lo = (1ull << i) - 1ull;
-    const uint32_t start = wrow(v, ws, l, k == a.R ? a.R + 1 : 1)[0];
-    uint32_t r = wrow(v, ws, l, w)[c] - start;
-    r += __popcll(e.x & lo) + __popcll(e.y & lo) + __popcll(e.z & lo);
-    if (axis >= 1) r += (uint32_t)((e.x >> i) & 1ull);
-    if (axis >= 2) r += (uint32_t)((e.y >> i) & 1ull);
-    return r;
+    n[0] = 0.0f; n[1] = 0.0f; n[2] = 0.0f;
+    const uint32_t around = around_nibbles(axis);
+    for (int q = 0; q < 4; ++q) {
+        int cx, cy, cz;
+        around_cell(axis, q, (int)x, (int)y, zv, cx, cy, cz);
+        if (cx < 0 || cy < 0 || cz < 0) continue;
+        if (cx + 1 >= (int)a.nx || cy + 1 >= (int)a.ny || cz + 1 >= (int)a.nz) continue;
+        const uint32_t cs = (uint32_t)cz & 1u, j = (uint32_t)(cy - y0 + 1);
+        const uint32_t c = (uint32_t)cx >> 6, i = (uint32_t)cx & 63u;
+        const Corners k = corners_at(a, v, y0, c, cy, cz, l);
+        const uint32_t ci = corner_index(k, i);
+        const uint64_t w = v.tri[ci];
+        const int nt = (int)(w >> 60);
+        uint32_t base = v.misc[kMiscFnOff + cs * 9 + l] + v.pt[cidx(v, cs, l, j, c)];
+        uint64_t m = v.act[cidx(v, cs, l, j, c)] & ((1ull << i) - 1ull);
+        while (m) {
+            const uint32_t b = (uint32_t)__ffsll((unsigned long long)m) - 1u;
+            m &= m - 1ull;
+            base += (uint32_t)(v.tri[corner_index(k, b)] >> 60);
+        }
+        const uint32_t me = (around >> (4 * q)) & 0xFu;
+        const float* f = v.fn + (size_t)cs * v.fn_cap * 3u;
+        for (int t = 0; t < nt; ++t) {
+            const uint32_t e0 = (uint32_t)((w >> (12 * t)) & 0xF);
+            const uint32_t e1 = (uint32_t)((w >> (12 * t + 8)) & 0xF);
+            const uint32_t e2 = (uint32_t)((w >> (12 * t + 4)) & 0xF);
+            if (e0 != me && e1 != me && e2 != me) continue;
+            const float* g = f + 3u * (base + (uint32_t)t);
+            n[0] = n[0] + g[0];
+            n[1] = n[1] + g[1];
+            n[2] = n[2] + g[2];
+        }
+    }
 }
 
-/* Emit tile (z, b): the vertices of node slice z (shared-vertex modes) and the
- * triangles of cell slice z-1, all levels, through two LDS work queues whose
- * entry index is the output rank. */
-template <typename T>
-__device__ void emit_tile(const JobArgs& a, const Views& v, const T* vol, int z, int y0)
+/* Emit tile (z, b): the vertices of window slice ws_v (shared-vertex modes,
+ * when out_vert) and the triangles of cell slice z-1 (when out_tri), all
+ * levels, through two LDS work queues whose entry index is the output rank.
+ * With VertexNormal the triangles of the halo row are processed too and
+ * every face normal is kept in the cell slice's ring slot. */
+template <typename T, int M>
+__device__ void emit_tile(const JobArgs& a, const Views& v, const T* vol, int z, int y0, uint32_t ws_v,
+                          bool out_vert, bool out_tri)
 {
+    constexpr bool vn = M == kModeVertex;
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
     const uint32_t sub = lane >> a.cshift, cl = lane & ((1u << a.cshift) - 1u);
     const uint32_t G = 64u >> a.cshift, CS = 1u << a.cshift;
-    const uint32_t nv_all = v.misc[kMiscQV + a.nlev];
+    const uint32_t nv_all = out_vert ? v.misc[kMiscQV + a.nlev] : 0u;
     const uint32_t nt_all = v.misc[kMiscQT + a.nlev];
     const uint32_t Q = a.queue_cap;
     const int zc = z - 1;
-    for (uint32_t w0 = 0; w0 < nv_all || w0 < nt_all; w0 += Q) {
-        /* fill: lanes on (band row, chunk), set bits in scan order */
-        for (uint32_t l = 0; l < a.nlev; ++l) {
-            for (uint32_t k0 = wave * G; k0 < a.R; k0 += kWaves * G) {
-                const uint32_t k = k0 + sub;
-                const int y = y0 + (int)k;
-                if (k >= a.R || y >= (int)a.ny) continue;
+    const int zv = z - 1 + (int)ws_v;
+    const uint32_t cs = cslot<M>(zc);
+    const bool keep_fn = vn && !v.misc[kMiscFnOvf + cs];
+
+    /* triangles of cell slice z-1 (with VertexNormal first, so that their face
+     * normals are in LDS before the vertices below read them) */
+    {
+        const uint32_t ngr = (a.R + 1 + G - 1) / G;          // row groups of cell rows 0..R
+        const uint32_t items = a.nlev * ngr;
+        for (uint32_t w0 = 0; w0 < nt_all; w0 += Q) {
+            for (uint32_t it = wave; it < items; it += kWaves) {
+                const uint32_t l = it / ngr, grp = it - l * ngr;
+                const uint32_t j = grp * G + sub;
+                if (j > a.R) continue;
                 for (uint32_t c = cl; c < a.NC; c += CS) {
-                    if (!a.dup) {
-                        const Cross e = window_cross(a, v, z, y0, 1, k + 1, c, l);
-                        uint64_t m = e.x | e.y | e.z;
-                        uint32_t r = wrow(v, 1, l, k + 1)[c] - wrow(v, 1, l, 1)[0] + v.misc[kMiscQV + l];
-                        while (m) {
-                            const uint32_t i = (uint32_t)__ffsll((unsigned long long)m) - 1u;
-                            m &= m - 1ull;
-                            const uint32_t base = (c * 64u + i) | (k << 20) | (l << 28);
-                            if ((e.x >> i) & 1ull) { if (r >= w0 && r - w0 < Q) v.vq[r - w0] = base; ++r; }
-                            if ((e.y >> i) & 1ull) { if (r >= w0 && r - w0 < Q) v.vq[r - w0] = base | (1u << 26); ++r; }
-                            if ((e.z >> i) & 1ull) { if (r >= w0 && r - w0 < Q) v.vq[r - w0] = base | (2u << 26); ++r; }
-                        }
-                    }
-                    uint64_t m = v.act[(l * a.R + k) * a.NC + c];
-                    if (m) {
-                        const uint64_t* r0 = brow(v, bslot(zc), l, k + 1);
-                        const uint64_t* r1 = brow(v, bslot(zc), l, k + 2);
-                        const uint64_t* r2 = brow(v, bslot(z), l, k + 1);
-                        const uint64_t* r3 = brow(v, bslot(z), l, k + 2);
-                        const uint64_t c0 = r0[c], c1 = shift_next(r0, c, a.NC), c3 = r1[c], c2 = shift_next(r1, c, a.NC);
-                        const uint64_t c4 = r2[c], c5 = shift_next(r2, c, a.NC), c7 = r3[c], c6 = shift_next(r3, c, a.NC);
-                        uint32_t r = v.pt[(l * a.R + k) * a.NC + c] + v.misc[kMiscQT + l];
-                        while (m) {
-                            const uint32_t i = (uint32_t)__ffsll((unsigned long long)m) - 1u;
-                            m &= m - 1ull;
-                            const uint32_t ci = (uint32_t)((c0 >> i) & 1ull) | (uint32_t)((c1 >> i) & 1ull) << 1 |
-                                                (uint32_t)((c2 >> i) & 1ull) << 2 | (uint32_t)((c3 >> i) & 1ull) << 3 |
-                                                (uint32_t)((c4 >> i) & 1ull) << 4 | (uint32_t)((c5 >> i) & 1ull) << 5 |
-                                                (uint32_t)((c6 >> i) & 1ull) << 6 | (uint32_t)((c7 >> i) & 1ull) << 7;
-                            const uint32_t nt = (uint32_t)(v.tri[ci] >> 60);
-                            const uint32_t x = c * 64u + i;
-                            for (uint32_t tt = 0; tt < nt; ++tt, ++r)
-                                if (r >= w0 && r - w0 < Q) v.tq[r - w0] = x | (k << 20) | (tt << 26) | (l << 29);
-                        }
+                    uint64_t m = v.act[cidx(v, cs, l, j, c)];
+                    if (!m) continue;
+                    const Corners k = corners_at(a, v, y0, c, y0 - 1 + (int)j, zc, l);
+                    uint32_t r = v.pt[cidx(v, cs, l, j, c)] + v.misc[kMiscQT + l];
+                    while (m) {
+                        const uint32_t i = (uint32_t)__ffsll((unsigned long long)m) - 1u;
+                        m &= m - 1ull;
+                        const uint32_t nt = (uint32_t)(v.tri[corner_index(k, i)] >> 60);
+                        const uint32_t x = c * 64u + i;
+                        for (uint32_t tt = 0; tt < nt; ++tt, ++r)
+                            if (r >= w0 && r - w0 < Q) v.tq[r - w0] = x | (j << 20) | (tt << 25) | (l << 28);
                     }
                 }
             }
-        }
-        __syncthreads();
-        /* process: vertices */
-        const uint32_t nvq = nv_all > w0 ? ((nv_all - w0) < Q ? (nv_all - w0) : Q) : 0u;
-        for (uint32_t q = threadIdx.x; q < nvq; q += kBlock) {
-            const uint32_t e = v.vq[q];
-            const uint32_t x = e & 0xFFFFFu, k = (e >> 20) & 0x3Fu, axis = (e >> 26) & 3u, l = e >> 28;
-            const LevelInfo& L = v.lev[l];
-            const uint64_t vid = L.vbase[2] + (w0 + q - v.misc[kMiscQV + l]);
-            if (vid >= L.cap_v) continue;
-            const uint32_t y = (uint32_t)y0 + k;
-            float p[3];
-            vertex_coords<T>(a, v, vol, L.iso, z, y0, x, (int)y, z, axis, l, p);
-            L.coords[3 * vid + 0] = p[0];
-            L.coords[3 * vid + 1] = p[1];
-            L.coords[3 * vid + 2] = p[2];
-            if (a.vnormal) {
-                float nn[3];
-                vertex_normal<T>(a, v, vol, L.iso, z, y0, x, y, axis, l, nn);
-                L.normals[3 * vid + 0] = nn[0];
-                L.normals[3 * vid + 1] = nn[1];
-                L.normals[3 * vid + 2] = nn[2];
-            }
-        }
-        /* process: triangles */
-        const uint32_t ntq = nt_all > w0 ? ((nt_all - w0) < Q ? (nt_all - w0) : Q) : 0u;
-        for (uint32_t q = threadIdx.x; q < ntq; q += kBlock) {
-            const uint32_t e = v.tq[q];
-            const uint32_t x = e & 0xFFFFFu, k = (e >> 20) & 0x3Fu, t = (e >> 26) & 7u, l = e >> 29;
-            const LevelInfo& L = v.lev[l];
-            const uint64_t tri = L.tbase + (w0 + q - v.misc[kMiscQT + l]);
-            if (tri >= L.cap_t) continue;
-            const int y = y0 + (int)k;
-            const uint32_t ci = cube_index_at(v, y0, x, y, zc, l);
-            const uint64_t w = v.tri[ci];
-            const uint32_t eds[3] = { (uint32_t)((w >> (12 * t)) & 0xF), (uint32_t)((w >> (12 * t + 8)) & 0xF),
-                                      (uint32_t)((w >> (12 * t + 4)) & 0xF) };
-            float p[3][3], nn[3];
-            if (a.dup) {
-                /* extract_surfaces_with_duplication: table orientation (:206-228) */
+            __syncthreads();
+            const uint32_t ntq = (nt_all - w0) < Q ? (nt_all - w0) : Q;
+            for (uint32_t q = threadIdx.x; q < ntq; q += kBlock) {
+                const uint32_t e = v.tq[q];
+                const uint32_t x = e & 0xFFFFFu, j = (e >> 20) & 0x1Fu, t = (e >> 25) & 7u, l = e >> 28;
+                const LevelInfo& L = v.lev[l];
+                const uint32_t rank = w0 + q;                           // rank in the tile, halo row included
+                const uint64_t tri = L.tbase + (rank - v.misc[kMiscQT + l] - v.misc[kMiscHalo + l]);
+                const bool write = out_tri && j >= 1 && tri < L.cap_t;
+                if (!write && !keep_fn) continue;
+                const int y = y0 - 1 + (int)j;
+                const uint32_t ci = cube_index_at(v, y0, x, y, zc, l);
+                const uint64_t w = v.tri[ci];
+                const uint32_t eds[3] = { (uint32_t)((w >> (12 * t)) & 0xF), (uint32_t)((w >> (12 * t + 8)) & 0xF),
+                                          (uint32_t)((w >> (12 * t + 4)) & 0xF) };
+                float p[3][3], nn[3];
+                if constexpr (M == kModeDup) {
+                    /* extract_surfaces_with_duplication: table orientation (:206-228) */
 #pragma unroll
-                for (int j = 0; j < 3; ++j) {
-                    const uint32_t qq = edge_ends(eds[j]);
-                    interpolate<T>(a, vol, L.iso,
-                                   x + (qq & 1u), (uint32_t)y + ((qq >> 1) & 1u), (uint32_t)zc + ((qq >> 2) & 1u),
-                                   x + ((qq >> 4) & 1u), (uint32_t)y + ((qq >> 5) & 1u), (uint32_t)zc + ((qq >> 6) & 1u),
-                                   p[j]);
-                }
-                face_normal(p[0], p[1], p[2], nn);
-                float* co = L.coords + 9 * tri;
-#pragma unroll
-                for (int j = 0; j < 3; ++j) {
-                    co[3 * j + 0] = p[j][0];
-                    co[3 * j + 1] = p[j][1];
-                    co[3 * j + 2] = p[j][2];
-                }
-                L.normals[3 * tri + 0] = nn[0];
-                L.normals[3 * tri + 1] = nn[1];
-                L.normals[3 * tri + 2] = nn[2];
-            } else {
-                /* connect_isopoints: vertex ids of the three edges (:493-514) */
-                uint32_t id[3];
-#pragma unroll
-                for (int j = 0; j < 3; ++j) {
-                    const uint32_t o = edge_owner(eds[j]);
-                    const uint32_t dx = o & 1u, dy = (o >> 1) & 1u, dz = (o >> 2) & 1u, ax = o >> 3;
-                    const uint32_t kk = k + dy;
-                    const uint32_t rank = tile_rank(a, v, z, y0, x + dx, kk, zc + (int)dz, ax, l);
-                    const uint32_t sel = (dz << 1) | (kk == a.R ? 1u : 0u);
-                    const uint64_t b = sel == 0 ? L.vbase[0] : sel == 1 ? L.vbase[1] : sel == 2 ? L.vbase[2] : L.vbase[3];
-                    id[j] = (uint32_t)(b + rank);
-                }
-                L.conn[3 * tri + 0] = id[0];
-                L.conn[3 * tri + 1] = id[1];
-                L.conn[3 * tri + 2] = id[2];
-                if (!a.vnormal) {
-                    /* calculate_normals_on_polygon (:531-559) */
-#pragma unroll
-                    for (int j = 0; j < 3; ++j) {
-                        const uint32_t o = edge_owner(eds[j]);
-                        vertex_coords<T>(a, v, vol, L.iso, z, y0, x + (o & 1u), y + (int)((o >> 1) & 1u),
-                                         zc + (int)((o >> 2) & 1u), o >> 3, l, p[j]);
+                    for (int jj = 0; jj < 3; ++jj) {
+                        const uint32_t qq = edge_ends(eds[jj]);
+                        interpolate<T>(a, vol, L.iso,
+                                       x + (qq & 1u), (uint32_t)y + ((qq >> 1) & 1u), (uint32_t)zc + ((qq >> 2) & 1u),
+                                       x + ((qq >> 4) & 1u), (uint32_t)y + ((qq >> 5) & 1u),
+                                       (uint32_t)zc + ((qq >> 6) & 1u), p[jj]);
                     }
                     face_normal(p[0], p[1], p[2], nn);
+                    float* co = L.coords + 9 * tri;
+#pragma unroll
+                    for (int jj = 0; jj < 3; ++jj) {
+                        co[3 * jj + 0] = p[jj][0];
+                        co[3 * jj + 1] = p[jj][1];
+                        co[3 * jj + 2] = p[jj][2];
+                    }
+                    L.normals[3 * tri + 0] = nn[0];
+                    L.normals[3 * tri + 1] = nn[1];
+                    L.normals[3 * tri + 2] = nn[2];
+                    continue;
+                }
+                if (write) {
+                    /* connect_isopoints: vertex ids of the three edges (:493-514) */
+                    uint32_t id[3];
+#pragma unroll
+                    for (int jj = 0; jj < 3; ++jj) {
+                        const uint32_t o = edge_owner(eds[jj]);
+                        const uint32_t dx = o & 1u, dy = (o >> 1) & 1u, dz = (o >> 2) & 1u, ax = o >> 3;
+                        const uint32_t kk = j - 1 + dy;
+                        const uint32_t rk = tile_rank(a, v, z, y0, x + dx, kk, zc + (int)dz, ax, l);
+                        const uint32_t sel = (dz << 1) | (kk == a.R ? 1u : 0u);
+                        const uint64_t b = sel == 0 ? L.vbase[0] : sel == 1 ? L.vbase[1] : sel == 2 ? L.vbase[2] : L.vbase[3];
+                        id[jj] = (uint32_t)(b + rk);
+                    }
+                    L.conn[3 * tri + 0] = id[0];
+                    L.conn[3 * tri + 1] = id[1];
+                    L.conn[3 * tri + 2] = id[2];
+                }
+                if (vn && !keep_fn) continue;
+                /* calculate_normals_on_polygon (:531-559); with VertexNormal the same face
+                 * normal is what calculate_normals_on_vertex sums */
+#pragma unroll
+                for (int jj = 0; jj < 3; ++jj) {
+                    const uint32_t o = edge_owner(eds[jj]);
+                    node_edge_coords<T>(a, vol, L.iso, x + (o & 1u), (uint32_t)(y + (int)((o >> 1) & 1u)),
+                                        (uint32_t)(zc + (int)((o >> 2) & 1u)), o >> 3, p[jj]);
+                }
+                face_normal(p[0], p[1], p[2], nn);
+                if constexpr (vn) {
+                    float* g = v.fn + ((size_t)cs * v.fn_cap + rank) * 3u;
+                    g[0] = nn[0]; g[1] = nn[1]; g[2] = nn[2];
+                } else {
                     L.normals[3 * tri + 0] = nn[0];
                     L.normals[3 * tri + 1] = nn[1];
                     L.normals[3 * tri + 2] = nn[2];
                 }
             }
+            __syncthreads();
         }
-        __syncthreads();
+    }
+
+    /* vertices of window slice ws_v, band rows */
+    if (nv_all) {
+        const bool ring_ok = vn && !v.misc[kMiscFnOvf + 0] && !v.misc[kMiscFnOvf + 1];
+        const uint32_t ngr = (a.R + G - 1) / G;              // row groups of band rows 0..R-1
+        const uint32_t items = a.nlev * ngr;
+        for (uint32_t w0 = 0; w0 < nv_all; w0 += Q) {
+            for (uint32_t it = wave; it < items; it += kWaves) {
+                const uint32_t l = it / ngr, grp = it - l * ngr;
+                const uint32_t k = grp * G + sub;
+                if (k >= a.R || y0 + (int)k >= (int)a.ny) continue;
+                for (uint32_t c = cl; c < a.NC; c += CS) {
+                    const Cross e = window_cross(a, v, z, y0, ws_v, k + 1, c, l);
+                    uint64_t m = e.x | e.y | e.z;
+                    uint32_t r = wrow(v, ws_v, l, k + 1)[c] - wrow(v, ws_v, l, 1)[0] + v.misc[kMiscQV + l];
+                    while (m) {
+                        const uint32_t i = (uint32_t)__ffsll((unsigned long long)m) - 1u;
+                        m &= m - 1ull;
+                        const uint32_t base = (c * 64u + i) | (k << 20) | (l << 28);
+                        if ((e.x >> i) & 1ull) { if (r >= w0 && r - w0 < Q) v.vq[r - w0] = base; ++r; }
+                        if ((e.y >> i) & 1ull) { if (r >= w0 && r - w0 < Q) v.vq[r - w0] = base | (1u << 26); ++r; }
+                        if ((e.z >> i) & 1ull) { if (r >= w0 && r - w0 < Q) v.vq[r - w0] = base | (2u << 26); ++r; }
+                    }
+                }
+            }
+            __syncthreads();
+            const uint32_t nvq = (nv_all - w0) < Q ? (nv_all - w0) : Q;
+            for (uint32_t q = threadIdx.x; q < nvq; q += kBlock) {
+                const uint32_t e = v.vq[q];
+                const uint32_t x = e & 0xFFFFFu, k = (e >> 20) & 0x3Fu, axis = (e >> 26) & 3u, l = e >> 28;
+                const LevelInfo& L = v.lev[l];
+                const uint64_t vid = L.vbase[ws_v == 1 ? 2 : 0] + (w0 + q - v.misc[kMiscQV + l]);
+                if (vid >= L.cap_v) continue;
+                const uint32_t y = (uint32_t)y0 + k;
+                float p[3];
+                node_edge_coords<T>(a, vol, L.iso, x, y, (uint32_t)zv, axis, p);
+                L.coords[3 * vid + 0] = p[0];
+                L.coords[3 * vid + 1] = p[1];
+                L.coords[3 * vid + 2] = p[2];
+                if constexpr (vn) {
+                    float nn[3];
+                    if (ring_ok) vertex_normal_ring(a, v, y0, x, y, zv, axis, l, nn);
+                    else vertex_normal_direct<T>(a, v, vol, L.iso, y0, x, y, zv, axis, l, nn);
+                    L.normals[3 * vid + 0] = nn[0];
+                    L.normals[3 * vid + 1] = nn[1];
+                    L.normals[3 * vid + 2] = nn[2];
+                }
+            }
+            __syncthreads();
+        }
     }
 }
 
@@ -1005,51 +1048,99 @@ __global__ __launch_bounds__(kBlock) void k_count_batch(const JobArgs* jobs, con
 /* emit pass                                                                   */
 /* ------------------------------------------------------------------------- */
 
-/* One workgroup per tile (z, b): the bit rows of slices z-1, z, z+1 are
- * loaded once, so tiles are independent and the grid needs no ordering; the
- * outputs land at the per-row prefixes of a.prefix. */
-template <typename T>
-__device__ void emit_tile_job(const JobArgs& a, uint64_t tile)
+/* Output bases of tile (z, band): vertex prefixes of node rows (z-1, y0),
+ * (z-1, y0+R), (z, y0), (z, y0+R) and the triangle prefix of cell row
+ * (y0, z-1), relative to the first row of their (level, kind) segment. */
+__device__ __forceinline__ void tile_bases(const JobArgs& a, const Views& v, int z, int y0)
 {
-    extern __shared__ __align__(16) uint8_t smem[];
-    const Views v = make_views(a, smem);
-    const T* vol = (const T*)a.values;
-    const int z = (int)(tile / a.nb);
-    const uint32_t band = (uint32_t)(tile - (uint64_t)z * a.nb);
-    const int y0 = (int)(band * a.R);
-
-    stage_tables(a, v);
-    load_window(a, v, z, y0);
     if (threadIdx.x < a.nlev) {
         const uint32_t l = threadIdx.x;
         const uint64_t* pre_v = a.prefix + (uint64_t)(l * 2 + 0) * a.nrows;
         const uint64_t* pre_t = a.prefix + (uint64_t)(l * 2 + 1) * a.nrows;
-        const uint64_t row = (uint64_t)z * a.ny + (uint64_t)y0;
+        const bool in = z < (int)a.nz;
+        const uint64_t row = in ? (uint64_t)z * a.ny + (uint64_t)y0 : 0ull;
         const bool next = (uint32_t)y0 + a.R < a.ny;
-        /* prefixes are relative to the first row of the (level, kind) segment */
         const uint64_t bv = pre_v[0], bt = pre_t[0];
         LevelInfo& L = v.lev[l];
-        L.tbase = pre_t[row] - bt;
-        L.vbase[2] = pre_v[row] - bv;
-        L.vbase[3] = next ? pre_v[row + a.R] - bv : 0;
-        L.vbase[0] = z > 0 ? pre_v[row - a.ny] - bv : 0;
-        L.vbase[1] = (z > 0 && next) ? pre_v[row - a.ny + a.R] - bv : 0;
+        L.tbase = in ? pre_t[row] - bt : 0;
+        L.vbase[2] = in ? pre_v[row] - bv : 0;
+        L.vbase[3] = (in && next) ? pre_v[row + a.R] - bv : 0;
+        const uint64_t prow = (uint64_t)(z - 1) * a.ny + (uint64_t)y0;
+        L.vbase[0] = z > 0 ? pre_v[prow] - bv : 0;
+        L.vbase[1] = (z > 0 && next) ? pre_v[prow + a.R] - bv : 0;
     }
-    __syncthreads();
-    tile_prefixes(a, v, z, y0);
-    if (!v.misc[kMiscOverflow]) fill_cache<T>(a, v, vol, z, y0);
-    emit_tile<T>(a, v, vol, z, y0);
 }
 
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_emit(JobArgs a) { emit_tile_job<T>(a, blockIdx.x); }
-
-/* Emit pass of a batch of jobs sharing the value type T. */
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_emit_batch(const JobArgs* jobs, const uint64_t* tile_start, uint32_t njobs)
+/* One unit: band b over steps zs .. ze-1 (with VertexNormal one more step
+ * after the last node slice).  Tile z emits the triangles of cell slice z-1
+ * and the vertices of node slice z, or of node slice z-1 with VertexNormal:
+ * their normals sum face normals of cell slices z-2 and z-1, both then in
+ * the face-normal ring (a first unit step without output fills in z-2).
+ * The mask ring is filled once at the start and takes one new slice per
+ * step. */
+template <typename T, int M>
+__device__ void emit_unit(const JobArgs& a, const Views& v, uint64_t unit)
 {
-    const uint32_t j = find_job(tile_start, njobs, blockIdx.x);
-    emit_tile_job<T>(jobs[j], blockIdx.x - tile_start[j]);
+    const T* vol = (const T*)a.values;
+    const uint32_t slab = (uint32_t)(unit / a.nb);
+    const uint32_t band = (uint32_t)(unit - (uint64_t)slab * a.nb);
+    const int y0 = (int)(band * a.R);
+    const int zs = (int)(slab * a.slab);
+    const int ze = min(zs + (int)a.slab, (int)a.nz);
+    constexpr bool vn = M == kModeVertex;
+    const int zend = (vn && ze == (int)a.nz) ? ze + 1 : ze;
+    const uint32_t ws_v = vn ? 0u : 1u;
+
+    load_slices(a, v, vn ? zs - 2 : zs - 1, zs + 1, y0);
+    if (threadIdx.x < 2) v.misc[kMiscFnOvf + threadIdx.x] = 0u;
+    __syncthreads();
+    if (vn && zs >= 2) {
+        tile_prefixes<M>(a, v, zs - 1, y0, ws_v, false);
+        emit_tile<T, M>(a, v, vol, zs - 1, y0, ws_v, false, false);
+        __syncthreads();
+    }
+    for (int z = zs; z < zend; ++z) {
+        if (z > zs) load_slices(a, v, z + 1, z + 1, y0);
+        tile_bases(a, v, z, y0);
+        __syncthreads();
+        tile_prefixes<M>(a, v, z, y0, ws_v, true);
+        emit_tile<T, M>(a, v, vol, z, y0, ws_v, true, true);
+        __syncthreads();   // the next step overwrites a mask slot and the cell data
+    }
+}
+
+/* Persistent blocks: each block stages the tables once and takes units
+ * blockIdx.x, blockIdx.x + gridDim.x, ...; consecutive blocks take
+ * neighbouring bands of the same slab. */
+template <typename T, int M>
+__global__ __launch_bounds__(kBlock) void k_emit(JobArgs a)
+{
+    extern __shared__ __align__(16) uint8_t smem[];
+    const Views v = make_views(a, smem);
+    stage_tables(a, v);
+    for (uint64_t u = blockIdx.x; u < a.nunits; u += gridDim.x) emit_unit<T, M>(a, v, u);
+}
+
+/* Emit pass of a batch of jobs sharing the value type T: the units of all
+ * jobs in one sequence, taken by persistent blocks; the tables are staged
+ * again when a block moves to another job. */
+template <typename T, int M>
+__global__ __launch_bounds__(kBlock) void k_emit_batch(const JobArgs* jobs, const uint64_t* unit_start, uint32_t njobs)
+{
+    extern __shared__ __align__(16) uint8_t smem[];
+    const uint64_t total = unit_start[njobs];
+    uint32_t staged = ~0u;
+    for (uint64_t g = blockIdx.x; g < total; g += gridDim.x) {
+        const uint32_t j = find_job(unit_start, njobs, g);
+        const JobArgs& a = jobs[j];
+        const Views v = make_views(a, smem);
+        if (j != staged) {
+            __syncthreads();
+            stage_tables(a, v);
+            staged = j;
+        }
+        emit_unit<T, M>(a, v, g - unit_start[j]);
+    }
 }
 
 /* ------------------------------------------------------------------------- */
@@ -1241,10 +1332,56 @@ __global__ void k_minmax_final(const T* vol, const Extreme<T>* parts, uint32_t n
 /* host-side launch wrappers                                                   */
 /* ------------------------------------------------------------------------- */
 
+/* Blocks that fit on the device at once for this LDS size (persistent grid);
+ * remembered per (kernel, LDS size). */
+template <typename K>
+uint32_t resident_blocks(K kernel, uint32_t lds_bytes)
+{
+    static std::mutex mu;
+    static std::map<std::pair<const void*, uint32_t>, uint32_t> known;
+    const std::pair<const void*, uint32_t> key((const void*)kernel, lds_bytes);
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        auto it = known.find(key);
+        if (it != known.end()) return it->second;
+    }
+    int cus = 0, dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0) {
+        (void)hipGetLastError();
+        cus = 256;
+    }
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kBlock, lds_bytes) != hipSuccess || per_cu <= 0) {
+        (void)hipGetLastError();
+        per_cu = 1;
+    }
+    const uint32_t n = (uint32_t)(cus * per_cu);
+    std::lock_guard<std::mutex> lk(mu);
+    known[key] = n;
+    return n;
+}
+
 template <typename T>
 hipError_t launch_emit_t(const JobArgs& a, uint32_t lds_bytes, hipStream_t st)
 {
-    hipLaunchKernelGGL((k_emit<T>), dim3((uint32_t)a.ntiles), dim3(kBlock), lds_bytes, st, a);
+    switch (emit_mode(a)) {
+    case kModeDup: {
+        const uint64_t grid = std::min<uint64_t>(a.nunits, resident_blocks(k_emit<T, kModeDup>, lds_bytes));
+        hipLaunchKernelGGL((k_emit<T, kModeDup>), dim3((uint32_t)grid), dim3(kBlock), lds_bytes, st, a);
+        break;
+    }
+    case kModePolygon: {
+        const uint64_t grid = std::min<uint64_t>(a.nunits, resident_blocks(k_emit<T, kModePolygon>, lds_bytes));
+        hipLaunchKernelGGL((k_emit<T, kModePolygon>), dim3((uint32_t)grid), dim3(kBlock), lds_bytes, st, a);
+        break;
+    }
+    default: {
+        const uint64_t grid = std::min<uint64_t>(a.nunits, resident_blocks(k_emit<T, kModeVertex>, lds_bytes));
+        hipLaunchKernelGGL((k_emit<T, kModeVertex>), dim3((uint32_t)grid), dim3(kBlock), lds_bytes, st, a);
+        break;
+    }
+    }
     return hipGetLastError();
 }
 
@@ -1342,28 +1479,41 @@ hipError_t launch_count_batch(const JobArgs* jobs, const uint64_t* row_start, ui
     return hipGetLastError();
 }
 
-template <typename T>
-hipError_t launch_emit_batch_t(const JobArgs* jobs, const uint64_t* tile_start, uint32_t njobs, uint64_t tiles,
+template <typename T, int M>
+hipError_t launch_emit_batch_m(const JobArgs* jobs, const uint64_t* unit_start, uint32_t njobs, uint64_t units,
                                uint32_t lds_bytes, hipStream_t st)
 {
-    hipLaunchKernelGGL((k_emit_batch<T>), dim3((uint32_t)tiles), dim3(kBlock), lds_bytes, st, jobs, tile_start, njobs);
+    const uint64_t grid = std::min<uint64_t>(units, resident_blocks(k_emit_batch<T, M>, lds_bytes));
+    hipLaunchKernelGGL((k_emit_batch<T, M>), dim3((uint32_t)grid), dim3(kBlock), lds_bytes, st, jobs, unit_start, njobs);
     return hipGetLastError();
 }
 
-hipError_t launch_emit_batch(int dtype, const JobArgs* jobs, const uint64_t* tile_start, uint32_t njobs,
-                             uint64_t tiles, uint32_t lds_bytes, hipStream_t st)
+/* Every pass of a batch group shares the value type T and the emit mode. */
+template <typename T>
+hipError_t launch_emit_batch_t(int mode, const JobArgs* jobs, const uint64_t* unit_start, uint32_t njobs,
+                               uint64_t units, uint32_t lds_bytes, hipStream_t st)
+{
+    switch (mode) {
+    case kModeDup: return launch_emit_batch_m<T, kModeDup>(jobs, unit_start, njobs, units, lds_bytes, st);
+    case kModePolygon: return launch_emit_batch_m<T, kModePolygon>(jobs, unit_start, njobs, units, lds_bytes, st);
+    default: return launch_emit_batch_m<T, kModeVertex>(jobs, unit_start, njobs, units, lds_bytes, st);
+    }
+}
+
+hipError_t launch_emit_batch(int dtype, int mode, const JobArgs* jobs, const uint64_t* unit_start, uint32_t njobs,
+                             uint64_t units, uint32_t lds_bytes, hipStream_t st)
 {
     switch (dtype) {
-    case 0: return launch_emit_batch_t<int8_t>(jobs, tile_start, njobs, tiles, lds_bytes, st);
-    case 1: return launch_emit_batch_t<int16_t>(jobs, tile_start, njobs, tiles, lds_bytes, st);
-    case 2: return launch_emit_batch_t<int32_t>(jobs, tile_start, njobs, tiles, lds_bytes, st);
-    case 3: return launch_emit_batch_t<int64_t>(jobs, tile_start, njobs, tiles, lds_bytes, st);
-    case 4: return launch_emit_batch_t<uint8_t>(jobs, tile_start, njobs, tiles, lds_bytes, st);
-    case 5: return launch_emit_batch_t<uint16_t>(jobs, tile_start, njobs, tiles, lds_bytes, st);
-    case 6: return launch_emit_batch_t<uint32_t>(jobs, tile_start, njobs, tiles, lds_bytes, st);
-    case 7: return launch_emit_batch_t<uint64_t>(jobs, tile_start, njobs, tiles, lds_bytes, st);
-    case 8: return launch_emit_batch_t<float>(jobs, tile_start, njobs, tiles, lds_bytes, st);
-    default: return launch_emit_batch_t<double>(jobs, tile_start, njobs, tiles, lds_bytes, st);
+    case 0: return launch_emit_batch_t<int8_t>(mode, jobs, unit_start, njobs, units, lds_bytes, st);
+    case 1: return launch_emit_batch_t<int16_t>(mode, jobs, unit_start, njobs, units, lds_bytes, st);
+    case 2: return launch_emit_batch_t<int32_t>(mode, jobs, unit_start, njobs, units, lds_bytes, st);
+    case 3: return launch_emit_batch_t<int64_t>(mode, jobs, unit_start, njobs, units, lds_bytes, st);
+    case 4: return launch_emit_batch_t<uint8_t>(mode, jobs, unit_start, njobs, units, lds_bytes, st);
+    case 5: return launch_emit_batch_t<uint16_t>(mode, jobs, unit_start, njobs, units, lds_bytes, st);
+    case 6: return launch_emit_batch_t<uint32_t>(mode, jobs, unit_start, njobs, units, lds_bytes, st);
+    case 7: return launch_emit_batch_t<uint64_t>(mode, jobs, unit_start, njobs, units, lds_bytes, st);
+    case 8: return launch_emit_batch_t<float>(mode, jobs, unit_start, njobs, units, lds_bytes, st);
+    default: return launch_emit_batch_t<double>(mode, jobs, unit_start, njobs, units, lds_bytes, st);
     }
 }
 

@@ -364,7 +364,7 @@ kmc_status kmc_server_register_volume(kmc_server* S, const kmc_volume* v, uint32
 {
     if (!S || !volume_id) return KMC_ERR_INVALID_ARGUMENT;
     kmc_status s = validate_volume(v);
-    if (s == KMC_OK && levels_per_pass(v) == 0) s = KMC_ERR_INVALID_ARGUMENT;   // rows too long for a tile
+    if (s == KMC_OK && levels_per_pass(v, 1) == 0) s = KMC_ERR_INVALID_ARGUMENT;   // rows too long for a tile
     if (s == KMC_OK) s = kmc_init(S->cfg.device);
     if (s != KMC_OK) { report(s); return s; }
     Resident r;
