@@ -181,10 +181,13 @@ __device__ __forceinline__ uint64_t* brow(const Views& v, uint32_t sl, uint32_t 
 {
     return v.bits + ((sl * v.nlev + l) * v.nrb + r) * v.NC;
 }
-/* Window vertex chunk prefixes of window slice ws (0: node slice z-1, 1: z), level l, window row w. */
-__device__ __forceinline__ uint32_t* wrow(const Views& v, uint32_t ws, uint32_t l, uint32_t w)
+/* Window vertex chunk prefixes of window slice ws (0: node slice z-1, 1: z)
+ * of tile z, level l, window row w.  The two slots hold node slices by
+ * parity, so slice z's prefixes of step z are slice z-1's of step z+1. */
+__device__ __forceinline__ uint32_t* wrow(const Views& v, int z, uint32_t ws, uint32_t l, uint32_t w)
 {
-    return v.wpv + ((ws * v.nlev + l) * (v.R + 2) + w) * v.NC;
+    const uint32_t slot = (uint32_t)(z - 1 + (int)ws) & 1u;
+    return v.wpv + ((slot * v.nlev + l) * (v.R + 2) + w) * v.NC;
 }
 /* Cell data index: cell slot cs, level l, cell row j (cell row y0 - 1 + j), chunk c. */
 __device__ __forceinline__ uint32_t cidx(const Views& v, uint32_t cs, uint32_t l, uint32_t j, uint32_t c)
@@ -364,7 +367,7 @@ __device__ void load_slices(const JobArgs& a, const Views& v, int s_lo, int s_hi
  * ws_v: window slice whose vertices the tile emits.  Work items are
  * (level, row group) pairs spread over all waves. */
 template <int M>
-__device__ void tile_prefixes(const JobArgs& a, const Views& v, int z, int y0, uint32_t ws_v, bool ranks)
+__device__ void tile_prefixes(const JobArgs& a, const Views& v, int z, int y0, uint32_t ws_v, bool ranks, bool first)
 {
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
     const uint32_t sub = lane >> a.cshift, cl = lane & ((1u << a.cshift) - 1u);
@@ -376,15 +379,17 @@ __device__ void tile_prefixes(const JobArgs& a, const Views& v, int z, int y0, u
     const int zc = z - 1;
     const uint32_t cs = cslot<M>(zc);
     if (ranks && !dup) {
-        /* window rows 1..R+1 of window slices 0, 1 */
+        /* window rows 1..R+1 of window slice 1 (and of slice 0 at a unit's first
+         * step; later it is the previous step's slice 1) */
+        const uint32_t ws0 = first ? 0u : 1u;
         const uint32_t ngr = (a.R + 1 + G - 1) / G;              // row groups
-        const uint32_t items = 2u * a.nlev * ngr;
+        const uint32_t items = (2u - ws0) * a.nlev * ngr;
         for (uint32_t it = wave; it < items; it += kWaves) {
-            const uint32_t ws = it / (a.nlev * ngr), rem = it - ws * a.nlev * ngr;
+            const uint32_t ws = ws0 + it / (a.nlev * ngr), rem = it - (ws - ws0) * a.nlev * ngr;
             const uint32_t l = rem / ngr, grp = rem - l * ngr;
             const uint32_t w = 1 + grp * G + sub;
             if (w >= nwr) continue;
-            uint32_t* dst = wrow(v, ws, l, w);
+            uint32_t* dst = wrow(v, z, ws, l, w);
             for (uint32_t c = cl; c < a.NC; c += CS) {
                 const Cross k = window_cross(a, v, z, y0, ws, w, c, l);
                 dst[c] = (uint32_t)(__popcll(k.x) + __popcll(k.y) + __popcll(k.z));
@@ -423,11 +428,12 @@ __device__ void tile_prefixes(const JobArgs& a, const Views& v, int z, int y0, u
     }
     __syncthreads();
     /* one wave per sequence */
-    const uint32_t nwseq = (ranks && !dup) ? 2u * a.nlev : 0u;
+    const uint32_t nws = first ? 2u : 1u;
+    const uint32_t nwseq = (ranks && !dup) ? nws * a.nlev : 0u;
     const uint32_t nseq = nwseq + a.nlev;
     for (uint32_t q = wave; q < nseq; q += kWaves) {
         if (q < nwseq) {
-            (void)wave_exclusive_scan(wrow(v, q / a.nlev, q % a.nlev, 1), (nwr - 1) * a.NC);
+            (void)wave_exclusive_scan(wrow(v, z, (2u - nws) + q / a.nlev, q % a.nlev, 1), (nwr - 1) * a.NC);
         } else {
             const uint32_t l = q - nwseq;
             const uint32_t tot = wave_exclusive_scan(v.pt + cidx(v, cs, l, 0, 0), (a.R + 1) * a.NC);
@@ -439,7 +445,7 @@ __device__ void tile_prefixes(const JobArgs& a, const Views& v, int z, int y0, u
         /* queue offsets: exclusive scans over levels (nlev <= 8 lanes) */
         const uint32_t l = lane;
         const bool in = l < a.nlev;
-        const uint32_t vt = (in && !dup && ranks) ? (wrow(v, ws_v, l, a.R + 1)[0] - wrow(v, ws_v, l, 1)[0]) : 0u;
+        const uint32_t vt = (in && !dup && ranks) ? (wrow(v, z, ws_v, l, a.R + 1)[0] - wrow(v, z, ws_v, l, 1)[0]) : 0u;
         const uint32_t tt = in ? v.misc[kMiscTTot + l] : 0u;
         const uint32_t hl = (in && halo) ? v.pt[cidx(v, cs, l, 1, 0)] : 0u;
         const uint32_t iv = wave_incl_scan(vt), it = wave_incl_scan(tt);
@@ -507,8 +513,8 @@ __device__ __forceinline__ uint32_t tile_rank(const JobArgs& a, const Views& v, 
     const uint32_t c = x >> 6, i = x & 63u;
     const Cross e = window_cross(a, v, z, y0, ws, w, c, l);
     const uint64_t lo = (1ull << i) - 1ull;
-    const uint32_t start = wrow(v, ws, l, k == a.R ? a.R + 1 : 1)[0];
-    uint32_t r = wrow(v, ws, l, w)[c] - start;
+    const uint32_t start = wrow(v, z, ws, l, k == a.R ? a.R + 1 : 1)[0];
+    uint32_t r = wrow(v, z, ws, l, w)[c] - start;
     r += __popcll(e.x & lo) + __popcll(e.y & lo) + __popcll(e.z & lo);
     if (axis >= 1) r += (uint32_t)((e.x >> i) & 1ull);
     if (axis >= 2) r += (uint32_t)((e.y >> i) & 1ull);
@@ -754,7 +760,7 @@ __device__ void emit_tile(const JobArgs& a, const Views& v, const T* vol, int z,
                 for (uint32_t c = cl; c < a.NC; c += CS) {
                     const Cross e = window_cross(a, v, z, y0, ws_v, k + 1, c, l);
                     uint64_t m = e.x | e.y | e.z;
-                    uint32_t r = wrow(v, ws_v, l, k + 1)[c] - wrow(v, ws_v, l, 1)[0] + v.misc[kMiscQV + l];
+                    uint32_t r = wrow(v, z, ws_v, l, k + 1)[c] - wrow(v, z, ws_v, l, 1)[0] + v.misc[kMiscQV + l];
                     while (m) {
                         const uint32_t i = (uint32_t)__ffsll((unsigned long long)m) - 1u;
                         m &= m - 1ull;
@@ -940,8 +946,9 @@ __global__ __launch_bounds__(kBlock) void k_classify_batch(const JobArgs* jobs, 
  * per wave iteration, one lane per 64-node chunk; masks are read straight
  * from the bit volume (coalesced, neighbours from L1/L2), so there is no LDS
  * window and no barrier.  ntri: triangle count per cube index (LDS). */
-__device__ __forceinline__ void count_rows(const JobArgs& a, const uint8_t* ntri, uint64_t r0, uint64_t r1,
-                                           uint64_t wave_first, uint64_t wave_step)
+template <int NLEV>
+__device__ __forceinline__ void count_rows_t(const JobArgs& a, const uint8_t* ntri, uint64_t r0, uint64_t r1,
+                                             uint64_t wave_first, uint64_t wave_step)
 {
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t sub = lane >> a.cshift, cl = lane & ((1u << a.cshift) - 1u);
@@ -954,7 +961,9 @@ __device__ __forceinline__ void count_rows(const JobArgs& a, const uint8_t* ntri
         const uint32_t z = ok ? (uint32_t)(r / a.ny) : 0u;
         const uint32_t y = ok ? (uint32_t)(r - (uint64_t)z * a.ny) : 0u;
         const bool has_up = y + 1 < a.ny, has_next = z + 1 < a.nz, has_cells = z >= 1 && has_up;
-        for (uint32_t l = 0; l < a.nlev; ++l) {
+        const uint32_t nl = NLEV ? (uint32_t)NLEV : a.nlev;
+#pragma unroll
+        for (uint32_t l = 0; l < nl; ++l) {
             uint32_t vc = 0, tc = 0;
             if (ok) {
                 const uint64_t* lev = a.bitvol + (uint64_t)l * level_words;
@@ -1006,6 +1015,20 @@ __device__ __forceinline__ void count_rows(const JobArgs& a, const uint8_t* ntri
                 a.counts[(uint64_t)(l * 2 + 1) * a.nrows + r] = tc;
             }
         }
+    }
+}
+
+/* Level count as a compile-time constant where it is small, so that the loads
+ * of all levels can be in flight together. */
+__device__ __forceinline__ void count_rows(const JobArgs& a, const uint8_t* ntri, uint64_t r0, uint64_t r1,
+                                           uint64_t wave_first, uint64_t wave_step)
+{
+    switch (a.nlev) {
+    case 1: count_rows_t<1>(a, ntri, r0, r1, wave_first, wave_step); break;
+    case 2: count_rows_t<2>(a, ntri, r0, r1, wave_first, wave_step); break;
+    case 3: count_rows_t<3>(a, ntri, r0, r1, wave_first, wave_step); break;
+    case 4: count_rows_t<4>(a, ntri, r0, r1, wave_first, wave_step); break;
+    default: count_rows_t<0>(a, ntri, r0, r1, wave_first, wave_step); break;
     }
 }
 
@@ -1095,7 +1118,7 @@ __device__ void emit_unit(const JobArgs& a, const Views& v, uint64_t unit)
     if (threadIdx.x < 2) v.misc[kMiscFnOvf + threadIdx.x] = 0u;
     __syncthreads();
     if (vn && zs >= 2) {
-        tile_prefixes<M>(a, v, zs - 1, y0, ws_v, false);
+        tile_prefixes<M>(a, v, zs - 1, y0, ws_v, false, true);
         emit_tile<T, M>(a, v, vol, zs - 1, y0, ws_v, false, false);
         __syncthreads();
     }
@@ -1103,7 +1126,7 @@ __device__ void emit_unit(const JobArgs& a, const Views& v, uint64_t unit)
         if (z > zs) load_slices(a, v, z + 1, z + 1, y0);
         tile_bases(a, v, z, y0);
         __syncthreads();
-        tile_prefixes<M>(a, v, z, y0, ws_v, true);
+        tile_prefixes<M>(a, v, z, y0, ws_v, true, z == zs);
         emit_tile<T, M>(a, v, vol, z, y0, ws_v, true, true);
         __syncthreads();   // the next step overwrites a mask slot and the cell data
     }
@@ -1272,6 +1295,8 @@ template <> __device__ __forceinline__ bool is_nan_val<double>(double v) { retur
 
 /* Running extreme with the reference's tie rule: on equal values the earlier
  * index wins (only observable for float +0/-0). */
+constexpr uint32_t kMinMaxParts = 2048;   // partial results of the first pass
+
 template <typename T>
 struct Extreme {
     T mn, mx;
@@ -1289,43 +1314,84 @@ __device__ __forceinline__ void merge_extreme(Extreme<T>& a, const Extreme<T>& b
 }
 
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_minmax_partial(const T* vol, uint64_t n, Extreme<T>* parts)
+__device__ __forceinline__ void take_value(Extreme<T>& e, T v, uint64_t i)
+{
+    if (is_nan_val<T>(v)) return;
+    if (!e.valid) { e.mn = v; e.mx = v; e.imn = i; e.imx = i; e.valid = 1; return; }
+    if (v < e.mn) { e.mn = v; e.imn = i; }
+    if (v > e.mx) { e.mx = v; e.imx = i; }
+}
+
+/* Block-wide merge of the threads' extremes; thread 0 gets the result. */
+template <typename T>
+__device__ __forceinline__ Extreme<T> block_extreme(Extreme<T> e)
 {
     __shared__ Extreme<T> sh[kBlock];
-    Extreme<T> e;
-    e.mn = T(0); e.mx = T(0); e.imn = 0; e.imx = 0; e.valid = 0;
-    for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (uint64_t)gridDim.x * kBlock) {
-        const T v = vol[i];
-        if (is_nan_val<T>(v)) continue;
-        if (!e.valid) { e.mn = v; e.mx = v; e.imn = i; e.imx = i; e.valid = 1; }
-        else {
-            if (v < e.mn) { e.mn = v; e.imn = i; }
-            if (v > e.mx) { e.mx = v; e.imx = i; }
-        }
-    }
     sh[threadIdx.x] = e;
     __syncthreads();
     for (uint32_t s = kBlock / 2; s > 0; s >>= 1) {
         if (threadIdx.x < s) {
-            Extreme<T> a = sh[threadIdx.x];
-            merge_extreme(a, sh[threadIdx.x + s]);
-            sh[threadIdx.x] = a;
+            Extreme<T> x = sh[threadIdx.x];
+            merge_extreme(x, sh[threadIdx.x + s]);
+            sh[threadIdx.x] = x;
         }
         __syncthreads();
     }
-    if (threadIdx.x == 0) parts[blockIdx.x] = sh[0];
+    return sh[0];
+}
+
+/* Streams the volume with 16-byte loads, kU of them in flight per thread;
+ * each thread visits its elements in increasing index order, so strict
+ * comparisons keep the first occurrence, and merges break ties by index. */
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_minmax_partial(const T* vol, uint64_t n, Extreme<T>* parts)
+{
+    constexpr uint32_t V = 16 / sizeof(T);
+    constexpr uint32_t kU = 4;
+    Extreme<T> e;
+    e.mn = T(0); e.mx = T(0); e.imn = 0; e.imx = 0; e.valid = 0;
+    const uint64_t tid = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+    const uint64_t nth = (uint64_t)gridDim.x * kBlock;
+    const bool aligned = ((uintptr_t)vol & 15u) == 0;
+    const uint64_t nvec = aligned ? n / V : 0;
+    const uint4* vp = (const uint4*)vol;
+    for (uint64_t b = tid; b < nvec; b += nth * kU) {
+        uint4 r[kU];
+#pragma unroll
+        for (uint32_t u = 0; u < kU; ++u) {
+            const uint64_t idx = b + u * nth;
+            r[u] = idx < nvec ? vp[idx] : make_uint4(0, 0, 0, 0);
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < kU; ++u) {
+            const uint64_t idx = b + u * nth;
+            if (idx < nvec) {
+                T vals[V];
+                __builtin_memcpy(vals, &r[u], 16);
+#pragma unroll
+                for (uint32_t k = 0; k < V; ++k) take_value(e, vals[k], idx * V + k);
+            }
+        }
+    }
+    for (uint64_t i = nvec * V + tid; i < n; i += nth) take_value(e, vol[i], i);
+    const Extreme<T> r = block_extreme(e);
+    if (threadIdx.x == 0) parts[blockIdx.x] = r;
 }
 
 template <typename T>
-__global__ void k_minmax_final(const T* vol, const Extreme<T>* parts, uint32_t nparts, double* out)
+__global__ __launch_bounds__(kBlock) void k_minmax_final(const T* vol, const Extreme<T>* parts, uint32_t nparts, double* out)
 {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    Extreme<T> e = parts[0];
-    for (uint32_t i = 1; i < nparts; ++i) merge_extreme(e, parts[i]);
-    const T first = vol[0];
-    if (is_nan_val<T>(first) || !e.valid) { e.mn = first; e.mx = first; }
-    out[0] = (double)e.mn;
-    out[1] = (double)e.mx;
+    Extreme<T> e;
+    e.mn = T(0); e.mx = T(0); e.imn = 0; e.imx = 0; e.valid = 0;
+    for (uint32_t i = threadIdx.x; i < nparts; i += kBlock) merge_extreme(e, parts[i]);
+    const Extreme<T> r = block_extreme(e);
+    if (threadIdx.x == 0) {
+        Extreme<T> f = r;
+        const T first = vol[0];
+        if (is_nan_val<T>(first) || !f.valid) { f.mn = first; f.mx = first; }
+        out[0] = (double)f.mn;
+        out[1] = (double)f.mx;
+    }
 }
 
 /* ------------------------------------------------------------------------- */
@@ -1536,10 +1602,10 @@ uint32_t scan_partials_needed(uint64_t n, uint32_t narr)
 template <typename T>
 hipError_t launch_minmax_t(const void* vol, uint64_t n, void* scratch, double* out, hipStream_t st)
 {
-    const uint32_t nparts = 1024;
+    const uint32_t nparts = kMinMaxParts;
     Extreme<T>* parts = (Extreme<T>*)scratch;
     hipLaunchKernelGGL((k_minmax_partial<T>), dim3(nparts), dim3(kBlock), 0, st, (const T*)vol, n, parts);
-    hipLaunchKernelGGL((k_minmax_final<T>), dim3(1), dim3(64), 0, st, (const T*)vol, parts, nparts, out);
+    hipLaunchKernelGGL((k_minmax_final<T>), dim3(1), dim3(kBlock), 0, st, (const T*)vol, parts, nparts, out);
     return hipGetLastError();
 }
 
@@ -1559,6 +1625,6 @@ hipError_t launch_minmax(int dtype, const void* vol, uint64_t n, void* scratch, 
     }
 }
 
-size_t minmax_scratch_bytes() { return 1024 * sizeof(Extreme<double>) + 64; }
+size_t minmax_scratch_bytes() { return kMinMaxParts * sizeof(Extreme<double>) + 64; }
 
 }  // namespace kmc
