@@ -55,6 +55,7 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=5)
     ap.add_argument("--serve-volumes", type=int, default=32, help="resident volumes of the serving leg (0: skip it)")
     ap.add_argument("--serve-requests", type=int, default=300)
+    ap.add_argument("--no-ct", action="store_true", help="skip the 512^3 uint8 CT-like leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: every MI355X box here has a single GPU and the "
@@ -152,6 +153,12 @@ def main():
     serving = None
     if args.serve_volumes > 0:
         serving = serving_leg(lib, kmc, datasets, torch, args)
+        torch.cuda.empty_cache()
+
+    ct = None
+    if not args.no_ct:
+        ct = ct_leg(lib, kmc, datasets, torch, args)
+    fp64 = fp64_reevaluation(kmc, datasets)
 
     out = {
         "metric": METRIC,
@@ -188,6 +195,8 @@ def main():
         "copy_ceiling": copy,
         "batched": batched,
         "serving": serving,
+        "ct_512": ct,
+        "fp64_reevaluation": fp64,
     }
     print(json.dumps(out), flush=True)
 
@@ -454,6 +463,105 @@ def serving_leg(lib, kmc, datasets, torch, args):
         "speedup_vs_plain": round(plain_s / sat["span_s"], 2),
         "poisson_sweep": sweep,
     }
+
+
+def ct_leg(lib, kmc, datasets, torch, args):
+    """512^3 uint8 CT-like volume (value noise + 20 ellipsoids), isolevel 100,
+    VertexNormal: one kmc_extract on the HBM-resident volume (device outputs),
+    and the PCIe-inclusive drop-in case (host volume in, host surface out);
+    the reference's own kvs::MarchingCubes on the same bytes, 1 thread."""
+    import numpy as np
+
+    import oracle
+
+    n, iso = 512, 100.0
+    print("[bench] CT leg: generating 512^3 uint8 volume", file=sys.stderr, flush=True)
+    t = datasets.ct_like_torch(n, seed=1001, nellipsoids=20, dtype=np.uint8, device="cuda")
+    torch.cuda.synchronize()
+    v = kmc.make_volume(t.data_ptr(), dims=(n, n, n), memory=kmc.MEMORY_DEVICE, dtype=np.uint8)
+    mn, mx = ctypes.c_double(), ctypes.c_double()
+    assert lib.kmc_volume_min_max(ctypes.byref(v), None, ctypes.byref(mn), ctypes.byref(mx)) == 0
+    v.has_min_max_values, v.min_value, v.max_value = 1, mn.value, mx.value
+    out = kmc.Surface()
+
+    def once(vol, memory):
+        st = lib.kmc_extract(ctypes.byref(vol), iso, kmc.VERTEX_NORMAL, 0, memory, None, ctypes.byref(out))
+        if st != 0:
+            raise kmc.KmcError(st)
+        r = (int(out.n_vertices), int(out.n_triangles))
+        lib.kmc_surface_release(ctypes.byref(out))
+        return r
+
+    once(v, kmc.MEMORY_DEVICE)
+    reps = 5
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        nv, nt = once(v, kmc.MEMORY_DEVICE)
+    dev_ms = (time.perf_counter() - t0) / reps * 1e3
+    host = t.cpu().numpy()
+    hv = kmc.make_volume(host, min_max=(mn.value, mx.value))
+    once(hv, kmc.MEMORY_HOST)
+    t0 = time.perf_counter()
+    for _ in range(2):
+        once(hv, kmc.MEMORY_HOST)
+    e2e_ms = (time.perf_counter() - t0) / 2 * 1e3
+    cpu = None
+    if not args.no_cpu_baseline and oracle.ref_available():
+        print("[bench] CT leg: reference CPU path", file=sys.stderr, flush=True)
+        r = oracle.ref_marching_cubes(host, iso, oracle.VERTEX_NORMAL, False, min_max_values=(mn.value, mx.value))
+        assert r["connections"].size // 3 == nt, (r["connections"].size // 3, nt)
+        cpu = {"ms": round(r["seconds"] * 1e3, 1), "cores": 1, "kind": "reference",
+               "speedup_device": round(r["seconds"] * 1e3 / dev_ms, 1)}
+    return {
+        "workload": "512^3 uint8 CT-like (32^3 value-noise lattice + 20 ellipsoids), isolevel 100, VertexNormal",
+        "vertices": nv, "triangles": nt,
+        "device_ms": round(dev_ms, 3),
+        "gvoxels_per_s": round(n ** 3 / (dev_ms / 1e3) / 1e9, 2),
+        "mtriangles_per_s": round(nt / (dev_ms / 1e3) / 1e6, 1),
+        "end_to_end_ms": round(e2e_ms, 2),
+        "end_to_end_note": "host volume in, host (pinned) surface out: PCIe both ways, drop-in case",
+        "cpu_reference": cpu,
+    }
+
+
+def fp64_reevaluation(kmc, datasets):
+    """Largest deviation of the surface coordinates from an FP64 re-evaluation
+    of the interpolation (64^3 blob field, isolevel 0.5, shared vertices): the
+    crossing edges are enumerated in node scan order x (x, y, z) edges, the
+    order of the returned vertices, and interpolated in float64 (the same
+    clamp and ratio, no rounding to float until the end).  The coordinates are
+    the reference's bit for bit; this measures how far that float rounding is
+    from exact arithmetic."""
+    import numpy as np
+
+    v = datasets.blobs(64, seed=1000)
+    iso = 0.5
+    s = kmc.extract(v, iso, kmc.POLYGON_NORMAL, False)
+    coords = s["coords"].reshape(-1, 3).astype(np.float64)
+    d = v.astype(np.float64)
+    lo, hi = float(v.min()), float(v.max())
+    nz, ny, nx = d.shape
+    above = d > iso
+    pts = []
+    for axis, (dz, dy, dx) in enumerate(((0, 0, 1), (0, 1, 0), (1, 0, 0))):
+        a = d[:nz - dz, :ny - dy, :nx - dx]
+        b = d[dz:, dy:, dx:]
+        cross = above[:nz - dz, :ny - dy, :nx - dx] != above[dz:, dy:, dx:]
+        zz, yy, xx = np.nonzero(cross)
+        va = np.clip(a[zz, yy, xx], lo, hi)
+        vb = np.clip(b[zz, yy, xx], lo, hi)
+        r = np.abs((iso - va) / (vb - va))
+        p = np.stack([xx + r * dx, yy + r * dy, zz + r * dz], axis=1).astype(np.float64)
+        key = (zz * ny + yy) * nx + xx
+        pts.append((key * 3 + axis, p))
+    keys = np.concatenate([k for k, _ in pts])
+    exact = np.concatenate([p for _, p in pts])[np.argsort(keys, kind="stable")]
+    assert exact.shape == coords.shape, (exact.shape, coords.shape)
+    err = np.abs(coords - exact)
+    rel = err / np.maximum(np.abs(exact), 1e-30)
+    return {"workload": "64^3 blobs, isolevel 0.5, shared vertices", "vertices": int(coords.shape[0]),
+            "max_abs": float(err.max()), "max_rel": float(rel[np.abs(exact) > 1e-6].max())}
 
 
 def cpu_baseline(vol, slices, vmin, vmax):
