@@ -56,7 +56,11 @@ def main():
     ap.add_argument("--serve-volumes", type=int, default=32, help="resident volumes of the serving leg (0: skip it)")
     ap.add_argument("--serve-requests", type=int, default=300)
     ap.add_argument("--no-ct", action="store_true", help="skip the 512^3 uint8 CT-like leg")
+    ap.add_argument("--only-main", action="store_true",
+                    help="only the 1024^3 step (no CPU baseline, batched, serving, CT or FP64 legs): for profiling")
     args = ap.parse_args()
+    if args.only_main:
+        args.no_cpu_baseline, args.no_ct, args.batch_jobs, args.serve_volumes = True, True, 0, 0
     if args.gpus != 1:
         print("bench.py measures one GPU only: every MI355X box here has a single GPU and the "
               "path shards by z-slab (see DESIGN.md); run with --gpus 1", file=sys.stderr)
@@ -158,7 +162,7 @@ def main():
     ct = None
     if not args.no_ct:
         ct = ct_leg(lib, kmc, datasets, torch, args)
-    fp64 = fp64_reevaluation(kmc, datasets)
+    fp64 = None if args.only_main else fp64_reevaluation(kmc, datasets)
 
     out = {
         "metric": METRIC,
@@ -237,7 +241,7 @@ def kernel_rooflines(prof, n, nlev, counts):
         "scan": 2 * nlev * rows * (4 + 8),
         "emit": masks + 8 * nv + 24 * nv + 12 * nt,
     }
-    names = {"classify": "k_classify<float, 3>", "count": "k_count", "scan": "k_scan_*", "emit": "k_emit<float>"}
+    names = {"classify": "k_classify<float, 3>", "count": "k_count", "scan": "k_scan_*", "emit": "k_emit<float, 2>"}
     traffic = measured_traffic()
     out = {}
     for k, ms in prof.items():
