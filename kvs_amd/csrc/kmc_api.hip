@@ -294,42 +294,58 @@ kmc_status min_max_on_device(const kmc_volume* v, const void* dvals, hipStream_t
 
 bool plan_geometry(JobArgs& a, uint32_t* lds_bytes, uint32_t r_max, uint32_t queue_cap)
 {
-    /* tuning overrides: KMC_TILE_ROWS caps R, KMC_QUEUE_CAP sets the work-queue size,
-     * KMC_FN_CAP the face normals kept per cell slice (VertexNormal) */
+    /* tuning overrides: KMC_TILE_ROWS sets the largest R, KMC_QUEUE_CAP the work-queue
+     * size, KMC_FN_CAP the face normals kept per cell slice (VertexNormal), KMC_TEAM_LDS
+     * the LDS budget of one team, KMC_SLAB the steps per unit */
     static const char* env_r = std::getenv("KMC_TILE_ROWS");
     static const char* env_q = std::getenv("KMC_QUEUE_CAP");
     static const char* env_f = std::getenv("KMC_FN_CAP");
-    if (env_r) r_max = std::min<uint32_t>(r_max, (uint32_t)std::max(1, std::atoi(env_r)));
-    if (queue_cap == 0) queue_cap = a.vnormal ? 512u : 1024u;
+    static const char* env_b = std::getenv("KMC_TEAM_LDS");
+    static const char* env_s = std::getenv("KMC_SLAB");
+    if (env_r) r_max = (uint32_t)std::max(1, std::atoi(env_r));
+    r_max = std::min<uint32_t>(r_max, 31);                 // cell rows are 5-bit queue fields
+    if (queue_cap == 0) queue_cap = a.vnormal ? 112u : 128u;
     if (env_q) queue_cap = (uint32_t)std::max(64, std::atoi(env_q));
     a.NC = (a.nx + 63) / 64;
     a.cshift = 0;
     while ((1u << a.cshift) < a.NC && a.cshift < 6) ++a.cshift;
     a.queue_cap = queue_cap;
-    a.fn_cap = a.vnormal ? (env_f ? (uint32_t)std::max(64, std::atoi(env_f)) : 512u) : 0u;
+    a.fn_cap = a.vnormal ? (env_f ? (uint32_t)std::max(64, std::atoi(env_f)) : 112u) : 0u;
     a.nrows = (uint64_t)a.nz * a.ny;
-    const uint32_t candidates[] = { 16, 8, 4, 2, 1 };
-    for (uint32_t R : candidates) {
-        if (R > r_max) continue;
+    /* the tallest band whose team fits the budget (VertexNormal: four 4-team
+     * workgroups per CU, else five); for very long rows R = 1 with as many
+     * teams as a 64 KiB workgroup holds */
+    const uint32_t budget = env_b ? (uint32_t)std::max(1024, std::atoi(env_b))
+                                  : (a.vnormal ? 40960u : 32256u) / kEmitTeams - kTableBytes / kEmitTeams;
+    uint32_t pick = 0, pick_bytes = 0;
+    for (uint32_t R = r_max; R >= 1 && !pick; --R) {
         a.R = R;
-        a.nb = (a.ny + R - 1) / R;
         const LdsLayout L = lds_layout(a);
-        if (L.bytes <= 60 * 1024) {
-            *lds_bytes = L.bytes;
-            /* steps per unit: about 8192 units when the volume allows (load balance over
-             * persistent blocks), at most 64 steps; VertexNormal needs 2 (a unit's first
-             * step only prepares the face normals of the slice below) */
-            static const char* env_s = std::getenv("KMC_SLAB");
-            const uint64_t want = ((uint64_t)a.nz * a.nb) / 8192;
-            const uint64_t lo = a.vnormal ? 2 : 1;
-            a.slab = env_s ? (uint32_t)std::max(1, std::atoi(env_s))
-                           : (uint32_t)std::min<uint64_t>(64, std::max<uint64_t>(lo, want));
-            a.nslabs = (a.nz + a.slab - 1) / a.slab;
-            a.nunits = (uint64_t)a.nb * a.nslabs;
-            return true;
-        }
+        if (L.bytes <= budget) { pick = R; pick_bytes = L.bytes; }
     }
-    return false;
+    a.teams = kEmitTeams;
+    if (!pick) {
+        a.R = 1;
+        const LdsLayout L = lds_layout(a);
+        a.teams = std::min<uint32_t>(kEmitTeams, (65536u - kTableBytes) / L.bytes);
+        if (a.teams == 0) return false;
+        pick = 1;
+        pick_bytes = L.bytes;
+    }
+    a.R = pick;
+    a.nb = (a.ny + pick - 1) / pick;
+    a.team_stride = pick_bytes;
+    *lds_bytes = emit_lds_bytes(pick_bytes, a.teams);
+    /* steps per unit: about 65536 units when the volume allows (load balance over
+     * some 4096 resident teams), at most 64 steps; VertexNormal needs 2 (a unit's
+     * first step only prepares the face normals of the slice below) */
+    const uint64_t want = ((uint64_t)a.nz * a.nb * std::max(1u, a.nlev)) / 65536;
+    const uint64_t lo = a.vnormal ? 2 : 1;
+    a.slab = env_s ? (uint32_t)std::max(1, std::atoi(env_s))
+                   : (uint32_t)std::min<uint64_t>(64, std::max<uint64_t>(lo, want));
+    a.nslabs = (a.nz + a.slab - 1) / a.slab;
+    a.nunits = (uint64_t)a.nlev * a.nb * a.nslabs;
+    return true;
 }
 
 uint32_t levels_per_pass(const kmc_volume* v, int vnormal)

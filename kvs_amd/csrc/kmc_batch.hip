@@ -183,6 +183,7 @@ struct Group {
     uint32_t p0, p1;             // passes [p0, p1)
     uint64_t rows, units;
     uint32_t emit_lds;
+    uint32_t teams;                // emit teams per workgroup
     uint64_t* row_start_d;       // p1 - p0 + 1 entries
     uint64_t* unit_start_d;
     std::vector<uint64_t> row_start_h, unit_start_h;   // upload sources, kept while the copies may run
@@ -287,14 +288,23 @@ kmc_status plan_tiles(kmc_batch* b, uint32_t r_max, uint32_t queue_cap)
     for (Group& g : b->groups) {
         std::vector<uint64_t> rs, ts;
         uint64_t r = 0, t = 0;
-        g.emit_lds = 0;
+        uint32_t stride = 0, teams = kEmitTeams;
         for (uint32_t i = g.p0; i < g.p1; ++i) {
             rs.push_back(r);
             ts.push_back(t);
             r += b->passes[i].a.nrows;
             t += b->passes[i].a.nunits;
-            g.emit_lds = std::max(g.emit_lds, b->passes[i].lds);
+            stride = std::max(stride, b->passes[i].a.team_stride);
         }
+        /* one emit launch per group: every team region of a workgroup is as
+         * large as the group's largest, and as many teams as 64 KiB hold */
+        teams = std::min<uint32_t>(kEmitTeams, (65536u - kTableBytes) / std::max(1u, stride));
+        for (uint32_t i = g.p0; i < g.p1; ++i) {
+            b->passes[i].a.team_stride = stride;
+            b->passes[i].a.teams = teams;
+        }
+        g.teams = teams;
+        g.emit_lds = emit_lds_bytes(stride, teams);
         rs.push_back(r);
         ts.push_back(t);
         g.rows = r;
@@ -349,7 +359,8 @@ hipError_t record_emit(kmc_batch* b, hipStream_t st)
 {
     hipError_t e = hipSuccess;
     for (const Group& g : b->groups) {
-        e = launch_emit_batch(g.dtype, g.mode, b->jobs_d + g.p0, g.unit_start_d, g.p1 - g.p0, g.units, g.emit_lds, st);
+        e = launch_emit_batch(g.dtype, g.mode, b->jobs_d + g.p0, g.unit_start_d, g.p1 - g.p0, g.units, g.emit_lds,
+                              g.teams, st);
         if (e != hipSuccess) return e;
     }
     return e;
@@ -451,7 +462,7 @@ kmc_status run(kmc_batch* b, hipStream_t st)
 kmc_status autotune(kmc_batch* b, hipStream_t st)
 {
     struct Cand { uint32_t r, q; };
-    const Cand cands[] = { { 16, 1024 }, { 8, 1024 }, { 4, 1024 }, { 16, 512 }, { 8, 512 }, { 8, 2048 } };
+    const Cand cands[] = { { 16, 128 }, { 8, 128 }, { 4, 128 }, { 2, 128 }, { 16, 256 }, { 8, 64 } };
     double best = 1e30;
     Cand best_c = { b->r_max, b->queue_cap };
     for (const Cand& c : cands) {
@@ -489,7 +500,7 @@ kmc_status create_impl(const kmc_job* jobs, uint32_t n_jobs, uint32_t flags, hip
     (void)hipGetDevice(&b->device);
     b->flags = flags;
     b->r_max = 16;
-    b->queue_cap = 1024;
+    b->queue_cap = 128;
     b->upload_stream = st;
 
     /* jobs: device values, min/max, object coordinates */

@@ -44,13 +44,14 @@ void init_surface(kmc_surface* s, double iso, int normal_type, int dup, double m
 void init_job(JobArgs& a, const kmc_volume* v, const void* dvals, const double* iso, uint32_t nlev, int normal_type,
               int dup, double mn, double mx, const float minc[3], const float maxc[3]);
 
-/* Tile geometry and emit LDS size: the largest band height R <= r_max whose
- * LDS state fits (queue_cap 0: the mode's default).  False if none does. */
+/* Tile geometry and emit LDS size (a.nlev set): the tallest band R <= r_max
+ * whose team LDS state fits the per-team budget (queue_cap 0: the default).
+ * False if not even R = 1 fits a workgroup. */
 bool plan_geometry(JobArgs& a, uint32_t* lds_bytes, uint32_t r_max = 16, uint32_t queue_cap = 0);
 
-/* The most isolevels (<= kMaxLevels) one pass over this volume can hold in
- * the emit LDS at the default sizes (for VertexNormal when vnormal); 0 if
- * not even one fits. */
+/* Isolevels per pass over this volume (emit teams work on one level each, so
+ * this is kMaxLevels whenever a team's LDS state fits at all; 0 if it does
+ * not, for VertexNormal when vnormal). */
 uint32_t levels_per_pass(const kmc_volume* v, int vnormal);
 
 /* Kernel launchers (kmc_kernels.hip). */
@@ -68,7 +69,7 @@ hipError_t launch_count_batch(const JobArgs* jobs, const uint64_t* row_start, ui
                               hipStream_t st);
 /* mode: 0 duplicated vertices, 1 PolygonNormal, 2 VertexNormal (shared vertices) */
 hipError_t launch_emit_batch(int dtype, int mode, const JobArgs* jobs, const uint64_t* unit_start, uint32_t njobs,
-                             uint64_t units, uint32_t lds_bytes, hipStream_t st);
+                             uint64_t units, uint32_t lds_bytes, uint32_t teams, hipStream_t st);
 
 /* Two-phase use of a batch plan without a graph (kmc_batch.hip): plan, launch
  * the front (classify, count, scan, sizes), wait, read the sizes, point the

@@ -17,10 +17,12 @@
  *     order of calculate_isopoints and connect_isopoints /
  *     extract_surfaces_with_duplication in
  *     Source/Core/Visualization/Mapper/MarchingCubes.cpp:155-243,365-521;
- *   - a *unit* is one band over a slab of `slab` consecutive steps; a
- *     workgroup marches z through a unit, keeping a ring of node-slice
- *     classification masks (one new slice loaded per step) and, with
- *     VertexNormal, a ring of the face normals of the last two cell slices.
+ *   - a *unit* is one level of one band over a slab of `slab` consecutive
+ *     steps.  The emit pass runs one unit at a time per wave (a *team*): the
+ *     wave marches z through its unit with its own LDS region, keeping a ring
+ *     of node-slice classification masks (one new slice loaded per step) and,
+ *     with VertexNormal, a ring of the face normals of the last two cell
+ *     slices.  Teams never wait for each other.
  *
  * Passes of one job: classify (stream the values once, write the bit volume),
  * count (per-row vertex / triangle counts from the bit volume), scan (row
@@ -38,8 +40,11 @@
 namespace kmc {
 
 constexpr int kMaxLevels = 8;
-constexpr int kBlock = 256;          // threads per workgroup (4 waves of 64)
+constexpr int kBlock = 256;          // threads per workgroup of the dense passes (4 waves of 64)
 constexpr int kWaves = kBlock / 64;
+constexpr int kEmitTeams = 4;        // emit workgroup: 4 independent one-wave teams
+constexpr int kEmitBlock = 64 * kEmitTeams;
+constexpr uint32_t kTableBytes = 256 * 8;   // case table at the start of the emit workgroup's LDS
 
 struct JobArgs {
     const void* values;
@@ -56,7 +61,9 @@ struct JobArgs {
     uint32_t vnormal;  // 1: VertexNormal (only without duplication)
     uint32_t queue_cap;
     uint32_t fn_cap;   // face normals held per cell slice (VertexNormal)
-    uint64_t nunits;   // nb * nslabs
+    uint32_t team_stride;  // bytes of LDS per team (>= the team layout's size)
+    uint32_t teams;        // teams per emit workgroup (<= kEmitTeams; fewer when a team's LDS is large)
+    uint64_t nunits;   // nlev * nb * nslabs
     uint64_t nrows;    // nz * ny: counts and prefixes are kept per node row
     double minv, maxv; // clamp range of the interpolation (volume min/max)
     float minc[3];     // min object coordinate
@@ -79,52 +86,27 @@ struct JobArgs {
     uint64_t cap_triangles[kMaxLevels];
 };
 
-/* Per-level parameters staged in LDS so that emission can index them by a
- * runtime level without touching private memory. */
-struct LevelInfo {
-    float* coords;
-    float* normals;
-    uint32_t* conn;
-    uint64_t cap_v, cap_t;
-    double iso;
-    uint64_t vbase[4];  // vertex prefix of node rows (z-1,y0), (z-1,y0+R), (z,y0), (z,y0+R)
-    uint64_t tbase;     // triangle prefix of cell row (y0, z-1)
-};
-
-/* LDS layout of the emit workgroup; sizes in bytes, offsets 16-byte aligned.
- * Node slices live in a ring of `nslots` mask slots (4 with VertexNormal:
- * slices z-2 .. z+1; else 3: z-1 .. z+1), each holding node rows
- * y0-1 .. y0+R+1.  Window vertex prefixes cover node slices z-1 and z and
- * node rows y0-1 .. y0+R; cell data cover cell slice z-1, cell rows
- * y0-1 .. y0+R-1 (row 0 is the halo row of the band above, used for face
- * normals only), in a ring of 2 cell slices with VertexNormal. */
+/* LDS layout of one team (one level); sizes in bytes, offsets 16-byte
+ * aligned, relative to the team's region.  Node slices live in a ring of
+ * 3 mask slots (slices z-1 .. z+1), each holding node rows y0-1 .. y0+R+1.  Window vertex
+ * prefixes cover node slices z-1 and z and node rows y0-1 .. y0+R; cell data
+ * cover cell slice z-1, cell rows y0-1 .. y0+R-1 (row 0 is the halo row of
+ * the band above, used for face normals only), in a ring of 2 cell slices
+ * with VertexNormal. */
 struct LdsLayout {
     uint32_t nrb;       // mask rows held per slice: y0-1 .. y0+R+1
     uint32_t nslots;    // node-slice mask slots
     uint32_t ncs;       // cell-slice slots
-    uint32_t off_tri;   // uint64 [256]                          case table
-    uint32_t off_lev;   // LevelInfo [kMaxLevels]
-    uint32_t off_bits;  // uint64 [nslots][nlev][nrb][NC]         classification masks
-    uint32_t off_wpv;   // uint32 [2][nlev][R+2][NC]              window vertex chunk prefixes
-    uint32_t off_act;   // uint64 [ncs][nlev][R+1][NC]            active cells
-    uint32_t off_pt;    // uint32 [ncs][nlev][R+1][NC]            triangle chunk prefixes
-    uint32_t off_misc;  // uint32 [kMiscWords]                    totals, offsets, flags
-    uint32_t off_fn;    // float  [ncs][fn_cap][3]                face normals (VertexNormal)
-    uint32_t off_vq;    // uint32 [queue_cap]                     vertex work queue
-    uint32_t off_tq;    // uint32 [queue_cap]                     triangle work queue
+    uint32_t off_bits;  // uint64 [nslots][nrb][NC]         classification masks
+    uint32_t off_wpv;   // uint32 [2][R+2][NC]              window vertex chunk prefixes
+    uint32_t off_act;   // uint64 [ncs][R+1][NC]            active cells
+    uint32_t off_pt;    // uint32 [ncs][R+1][NC]            triangle chunk prefixes
+    uint32_t off_ap;    // uint32 [ncs][R+1][NC]            active-cell chunk prefixes (VertexNormal)
+    uint32_t off_crec;  // uint32 [ncs][fn_cap]             per active cell: first triangle << 8 | cube index
+    uint32_t off_fn;    // float  [ncs][fn_cap][3]          face normals (VertexNormal)
+    uint32_t off_vq;    // uint32 [queue_cap]               vertex work queue
+    uint32_t off_tq;    // uint32 [queue_cap]               triangle work queue
     uint32_t bytes;
-};
-
-/* misc words */
-enum : uint32_t {
-    kMiscVTot = 0,        // [kMaxLevels] tile vertex totals
-    kMiscTTot = 8,        // [kMaxLevels] tile triangle totals (halo row included)
-    kMiscQV = 16,         // [kMaxLevels + 1] vertex queue offsets per level
-    kMiscQT = 25,         // [kMaxLevels + 1] triangle queue offsets per level
-    kMiscHalo = 34,       // [kMaxLevels] halo-row triangles per level
-    kMiscFnOff = 42,      // [2][kMaxLevels + 1] face-normal offsets per (cell slot, level)
-    kMiscFnOvf = 60,      // [2] face normals of the cell slot did not fit
-    kMiscWords = 64
 };
 
 #if defined(__HIPCC__)
@@ -138,23 +120,28 @@ __host__ __device__
 inline LdsLayout lds_layout(const JobArgs& a)
 {
     LdsLayout L;
-    const uint32_t R = a.R, NC = a.NC, nlev = a.nlev;
+    const uint32_t R = a.R, NC = a.NC;
     L.nrb = R + 3;
-    L.nslots = a.vnormal ? 4u : 3u;
+    L.nslots = 3u;
     L.ncs = a.vnormal ? 2u : 1u;
     uint32_t o = 0;
-    L.off_tri = o;   o = align16(o + 256u * 8u);
-    L.off_lev = o;   o = align16(o + kMaxLevels * (uint32_t)sizeof(LevelInfo));
-    L.off_bits = o;  o = align16(o + L.nslots * nlev * L.nrb * NC * 8u);
-    L.off_wpv = o;   o = align16(o + 2u * nlev * (R + 2) * NC * 4u);
-    L.off_act = o;   o = align16(o + L.ncs * nlev * (R + 1) * NC * 8u);
-    L.off_pt = o;    o = align16(o + L.ncs * nlev * (R + 1) * NC * 4u);
-    L.off_misc = o;  o = align16(o + kMiscWords * 4u);
+    L.off_bits = o;  o = align16(o + L.nslots * L.nrb * NC * 8u);
+    L.off_wpv = o;   o = align16(o + 2u * (R + 2) * NC * 4u);
+    L.off_act = o;   o = align16(o + L.ncs * (R + 1) * NC * 8u);
+    L.off_pt = o;    o = align16(o + L.ncs * (R + 1) * NC * 4u);
+    L.off_ap = o;    o = align16(o + (a.vnormal ? L.ncs * (R + 1) * NC * 4u : 0u));
+    L.off_crec = o;  o = align16(o + (a.vnormal ? L.ncs * a.fn_cap * 4u : 0u));
     L.off_fn = o;    o = align16(o + (a.vnormal ? L.ncs * a.fn_cap * 12u : 0u));
     L.off_vq = o;    o = align16(o + a.queue_cap * 4u);
     L.off_tq = o;    o = align16(o + a.queue_cap * 4u);
     L.bytes = o;
     return L;
 }
+
+/* LDS of an emit workgroup of `teams` teams, team_stride bytes apart. */
+#if defined(__HIPCC__)
+__host__ __device__
+#endif
+inline uint32_t emit_lds_bytes(uint32_t team_stride, uint32_t teams) { return kTableBytes + teams * team_stride; }
 
 }  // namespace kmc

@@ -36,6 +36,7 @@ __constant__ uint64_t d_tri[256] = { KMC_TABLE_WORDS };
  * path: duplicated vertices, shared vertices with PolygonNormal, shared
  * vertices with VertexNormal. */
 enum : int { kModeDup = 0, kModePolygon = 1, kModeVertex = 2 };
+
 __host__ __device__ inline int emit_mode(const JobArgs& a) { return a.dup ? kModeDup : (a.vnormal ? kModeVertex : kModePolygon); }
 
 /* ------------------------------------------------------------------------- */
@@ -125,74 +126,134 @@ __device__ __forceinline__ void face_normal(const float v0[3], const float v1[3]
 }
 
 /* ------------------------------------------------------------------------- */
-/* LDS views                                                                   */
+/* emit teams                                                                  */
 /* ------------------------------------------------------------------------- */
 
-struct Views {
-    uint64_t* tri;      // [256] case table
-    LevelInfo* lev;     // [kMaxLevels]
-    uint64_t* bits;     // [nslots][nlev][nrb][NC]
-    uint32_t* wpv;      // [2][nlev][R+2][NC]
-    uint64_t* act;      // [ncs][nlev][R+1][NC]
-    uint32_t* pt;       // [ncs][nlev][R+1][NC]
-    uint32_t* misc;     // [kMiscWords]
-    float* fn;          // [ncs][fn_cap][3]
-    uint32_t* vq;       // [queue_cap]
-    uint32_t* tq;       // [queue_cap]
-    uint32_t nrb, nslots, R, NC, nlev, fn_cap;
+/* The emit pass runs one independent team per wave: a team marches its own
+ * unit (level, band, slab) in its own LDS region, so nothing in the emit pass
+ * waits for another wave.  The LDS operations of one wave complete in issue
+ * order; wave_sync only keeps the compiler from moving memory operations
+ * across the points where lanes start reading what other lanes wrote. */
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ uint32_t uniform(uint32_t x) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)x); }
+
+__device__ __forceinline__ uint64_t readlane64(uint64_t x, int k)
+{
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)x, k);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(x >> 32), k);
+    return (uint64_t)hi << 32 | lo;
+}
+
+/* LDS arrays of one team (layout of lds_layout) and the shared case table. */
+struct Team {
+    const uint64_t* tri;  // [256] case table
+    uint64_t* bits;       // [nslots][nrb][NC]
+    uint32_t* wpv;        // [2][R+2][NC]
+    uint64_t* act;        // [ncs][R+1][NC]
+    uint32_t* pt;         // [ncs][R+1][NC]
+    uint32_t* ap;         // [ncs][R+1][NC]
+    uint32_t* crec;       // [ncs][fn_cap]
+    float* fn;            // [ncs][fn_cap][3]
+    uint32_t* vq;         // [queue_cap]
+    uint32_t* tq;         // [queue_cap]
+    uint32_t nrb, R, NC, fn_cap;
 };
 
-__device__ __forceinline__ Views make_views(const JobArgs& a, uint8_t* smem)
+__device__ __forceinline__ Team make_team(const JobArgs& a, uint8_t* smem, uint32_t wave)
 {
     const LdsLayout L = lds_layout(a);
-    Views v;
-    v.tri = (uint64_t*)(smem + L.off_tri);
-    v.lev = (LevelInfo*)(smem + L.off_lev);
-    v.bits = (uint64_t*)(smem + L.off_bits);
-    v.wpv = (uint32_t*)(smem + L.off_wpv);
-    v.act = (uint64_t*)(smem + L.off_act);
-    v.pt = (uint32_t*)(smem + L.off_pt);
-    v.misc = (uint32_t*)(smem + L.off_misc);
-    v.fn = (float*)(smem + L.off_fn);
-    v.vq = (uint32_t*)(smem + L.off_vq);
-    v.tq = (uint32_t*)(smem + L.off_tq);
+    uint8_t* base = smem + kTableBytes + wave * a.team_stride;
+    Team v;
+    v.tri = (const uint64_t*)smem;
+    v.bits = (uint64_t*)(base + L.off_bits);
+    v.wpv = (uint32_t*)(base + L.off_wpv);
+    v.act = (uint64_t*)(base + L.off_act);
+    v.pt = (uint32_t*)(base + L.off_pt);
+    v.ap = (uint32_t*)(base + L.off_ap);
+    v.crec = (uint32_t*)(base + L.off_crec);
+    v.fn = (float*)(base + L.off_fn);
+    v.vq = (uint32_t*)(base + L.off_vq);
+    v.tq = (uint32_t*)(base + L.off_tq);
     v.nrb = L.nrb;
-    v.nslots = L.nslots;
     v.R = a.R;
     v.NC = a.NC;
-    v.nlev = a.nlev;
     v.fn_cap = a.fn_cap;
     return v;
 }
 
-/* Ring slot of node slice s (s >= -2): any nslots consecutive slices get
- * distinct slots. */
-__device__ __forceinline__ uint32_t bslot(const Views& v, int s)
+/* The level a unit works on: its bit volume, prefix segments and outputs. */
+struct Level {
+    const uint64_t* bits;    // [nz][ny][NC] chunk masks of the level
+    const uint64_t* pre_v;   // prefixes of per-node-row vertex counts
+    const uint64_t* pre_t;   // prefixes of per-cell-row triangle counts
+    uint64_t bv, bt;         // first prefix of each segment (outputs index from 0)
+    double iso;
+    float* coords;
+    float* normals;
+    uint32_t* conn;
+    uint64_t cap_v, cap_t;
+};
+
+__device__ __forceinline__ Level level_of(const JobArgs& a, uint32_t l)
 {
-    return v.nslots == 4u ? ((uint32_t)s & 3u) : (uint32_t)(s + 3) % 3u;
+    Level L;
+    L.bits = a.bitvol + (uint64_t)l * ((uint64_t)a.nz * a.ny * a.NC);
+    L.pre_v = a.prefix + (uint64_t)(l * 2 + 0) * a.nrows;
+    L.pre_t = a.prefix + (uint64_t)(l * 2 + 1) * a.nrows;
+    L.bv = L.pre_v[0];
+    L.bt = L.pre_t[0];
+    L.iso = a.iso[0];
+    L.coords = a.coords[0];
+    L.normals = a.normals[0];
+    L.conn = a.conn[0];
+    L.cap_v = a.cap_vertices[0];
+    L.cap_t = a.cap_triangles[0];
+#pragma unroll
+    for (int k = 1; k < kMaxLevels; ++k) {
+        if (k == (int)l) {
+            L.iso = a.iso[k];
+            L.coords = a.coords[k];
+            L.normals = a.normals[k];
+            L.conn = a.conn[k];
+            L.cap_v = a.cap_vertices[k];
+            L.cap_t = a.cap_triangles[k];
+        }
+    }
+    return L;
 }
+
+/* Ring slot of node slice s (s >= -2): any 3 consecutive slices get distinct
+ * slots. */
+template <int M>
+__device__ __forceinline__ uint32_t bslot(int s) { return (uint32_t)(s + 3) % 3u; }
 
 /* Ring slot of cell slice zc (VertexNormal keeps two cell slices). */
 template <int M>
 __device__ __forceinline__ uint32_t cslot(int zc) { return M == kModeVertex ? ((uint32_t)zc & 1u) : 0u; }
 
-/* Row of chunk masks: slot sl, level l, LDS row r (node row y = y0 - 1 + r). */
-__device__ __forceinline__ uint64_t* brow(const Views& v, uint32_t sl, uint32_t l, uint32_t r)
+/* Row of chunk masks: slot sl, LDS row r (node row y = y0 - 1 + r). */
+__device__ __forceinline__ const uint64_t* brow(const Team& v, uint32_t sl, uint32_t r)
 {
-    return v.bits + ((sl * v.nlev + l) * v.nrb + r) * v.NC;
+    return v.bits + (sl * v.nrb + r) * v.NC;
 }
 /* Window vertex chunk prefixes of window slice ws (0: node slice z-1, 1: z)
- * of tile z, level l, window row w.  The two slots hold node slices by
- * parity, so slice z's prefixes of step z are slice z-1's of step z+1. */
-__device__ __forceinline__ uint32_t* wrow(const Views& v, int z, uint32_t ws, uint32_t l, uint32_t w)
+ * of tile z, window row w.  The two slots hold node slices by parity, so
+ * slice z's prefixes of step z are slice z-1's of step z+1. */
+__device__ __forceinline__ uint32_t* wrow(const Team& v, int z, uint32_t ws, uint32_t w)
 {
     const uint32_t slot = (uint32_t)(z - 1 + (int)ws) & 1u;
-    return v.wpv + ((slot * v.nlev + l) * (v.R + 2) + w) * v.NC;
+    return v.wpv + (slot * (v.R + 2) + w) * v.NC;
 }
-/* Cell data index: cell slot cs, level l, cell row j (cell row y0 - 1 + j), chunk c. */
-__device__ __forceinline__ uint32_t cidx(const Views& v, uint32_t cs, uint32_t l, uint32_t j, uint32_t c)
+/* Cell data index: cell slot cs, cell row j (cell row y0 - 1 + j), chunk c. */
+__device__ __forceinline__ uint32_t cidx(const Team& v, uint32_t cs, uint32_t j, uint32_t c)
 {
-    return ((cs * v.nlev + l) * (v.R + 1) + j) * v.NC + c;
+    return (cs * (v.R + 1) + j) * v.NC + c;
 }
 
 /* Mask of the nodes x < limit inside chunk c. */
@@ -215,18 +276,19 @@ struct Cross { uint64_t x, y, z; };
 /* Crossing edges owned by the nodes of chunk c of window row w (node row
  * y0-1+w) of window slice ws (node slice z-1+ws).  z-edges reach slice
  * z+ws, whose masks the ring holds. */
-__device__ __forceinline__ Cross window_cross(const JobArgs& a, const Views& v, int z, int y0,
-                                              uint32_t ws, uint32_t w, uint32_t c, uint32_t l)
+template <int M>
+__device__ __forceinline__ Cross window_cross(const JobArgs& a, const Team& v, int z, int y0,
+                                              uint32_t ws, uint32_t w, uint32_t c)
 {
     Cross k;
     const int s = z - 1 + (int)ws;
     const int y = y0 - 1 + (int)w;
     if (s < 0 || s >= (int)a.nz || y < 0 || y >= (int)a.ny) { k.x = k.y = k.z = 0; return k; }
-    const uint64_t* row = brow(v, bslot(v, s), l, w);
+    const uint64_t* row = brow(v, bslot<M>(s), w);
     const uint64_t b = row[c];
     k.x = (b ^ shift_next(row, c, a.NC)) & valid_mask(c, a.nx - 1);
-    k.y = (y + 1 < (int)a.ny) ? ((b ^ brow(v, bslot(v, s), l, w + 1)[c]) & valid_mask(c, a.nx)) : 0ull;
-    k.z = (s + 1 < (int)a.nz) ? ((b ^ brow(v, bslot(v, s + 1), l, w)[c]) & valid_mask(c, a.nx)) : 0ull;
+    k.y = (y + 1 < (int)a.ny) ? ((b ^ brow(v, bslot<M>(s), w + 1)[c]) & valid_mask(c, a.nx)) : 0ull;
+    k.z = (s + 1 < (int)a.nz) ? ((b ^ brow(v, bslot<M>(s + 1), w)[c]) & valid_mask(c, a.nx)) : 0ull;
     return k;
 }
 
@@ -235,14 +297,14 @@ __device__ __forceinline__ Cross window_cross(const JobArgs& a, const Views& v, 
  * corner k of cell x = 64c + i (corner order of MarchingCubes.cpp:187-194). */
 struct Corners { uint64_t c[8]; };
 
-__device__ __forceinline__ Corners corners_at(const JobArgs& a, const Views& v, int y0, uint32_t c, int cy, int cz,
-                                              uint32_t l)
+template <int M>
+__device__ __forceinline__ Corners corners_at(const JobArgs& a, const Team& v, int y0, uint32_t c, int cy, int cz)
 {
     const uint32_t r = (uint32_t)(cy - y0 + 1);
-    const uint64_t* r0 = brow(v, bslot(v, cz), l, r);
-    const uint64_t* r1 = brow(v, bslot(v, cz), l, r + 1);
-    const uint64_t* r2 = brow(v, bslot(v, cz + 1), l, r);
-    const uint64_t* r3 = brow(v, bslot(v, cz + 1), l, r + 1);
+    const uint64_t* r0 = brow(v, bslot<M>(cz), r);
+    const uint64_t* r1 = brow(v, bslot<M>(cz), r + 1);
+    const uint64_t* r2 = brow(v, bslot<M>(cz + 1), r);
+    const uint64_t* r3 = brow(v, bslot<M>(cz + 1), r + 1);
     Corners k;
     k.c[0] = r0[c]; k.c[1] = shift_next(r0, c, a.NC); k.c[3] = r1[c]; k.c[2] = shift_next(r1, c, a.NC);
     k.c[4] = r2[c]; k.c[5] = shift_next(r2, c, a.NC); k.c[7] = r3[c]; k.c[6] = shift_next(r3, c, a.NC);
@@ -258,13 +320,36 @@ __device__ __forceinline__ uint32_t corner_index(const Corners& k, uint32_t i)
 }
 
 /* Cube index of cell (cx, cy, cz) from the mask ring (y0: first band row). */
-__device__ __forceinline__ uint32_t cube_index_at(const Views& v, int y0, uint32_t cx, int cy, int cz, uint32_t l)
+template <int M>
+__device__ __forceinline__ uint32_t cube_index_at(const Team& v, int y0, uint32_t cx, int cy, int cz)
 {
     const uint32_t r = (uint32_t)(cy - y0 + 1);
-    const uint64_t* b00 = brow(v, bslot(v, cz), l, r);
-    const uint64_t* b01 = brow(v, bslot(v, cz), l, r + 1);
-    const uint64_t* b10 = brow(v, bslot(v, cz + 1), l, r);
-    const uint64_t* b11 = brow(v, bslot(v, cz + 1), l, r + 1);
+    const uint64_t* b00 = brow(v, bslot<M>(cz), r);
+    const uint64_t* b01 = brow(v, bslot<M>(cz), r + 1);
+    const uint64_t* b10 = brow(v, bslot<M>(cz + 1), r);
+    const uint64_t* b11 = brow(v, bslot<M>(cz + 1), r + 1);
+    const uint32_t c0 = cx >> 6, i0 = cx & 63u;
+    const uint32_t c1 = (cx + 1) >> 6, i1 = (cx + 1) & 63u;
+    uint32_t ci = 0;
+    ci |= (uint32_t)((b00[c0] >> i0) & 1ull) << 0;
+    ci |= (uint32_t)((b00[c1] >> i1) & 1ull) << 1;
+    ci |= (uint32_t)((b01[c1] >> i1) & 1ull) << 2;
+    ci |= (uint32_t)((b01[c0] >> i0) & 1ull) << 3;
+    ci |= (uint32_t)((b10[c0] >> i0) & 1ull) << 4;
+    ci |= (uint32_t)((b10[c1] >> i1) & 1ull) << 5;
+    ci |= (uint32_t)((b11[c1] >> i1) & 1ull) << 6;
+    ci |= (uint32_t)((b11[c0] >> i0) & 1ull) << 7;
+    return ci;
+}
+
+/* Cube index of cell (cx, cy, cz) read from the level's bit volume in memory
+ * (for cells whose node slices have left the mask ring). */
+__device__ __forceinline__ uint32_t cube_index_mem(const JobArgs& a, const uint64_t* bits, uint32_t cx, int cy, int cz)
+{
+    const uint64_t* b00 = bits + ((uint64_t)cz * a.ny + (uint64_t)cy) * a.NC;
+    const uint64_t* b01 = b00 + a.NC;
+    const uint64_t* b10 = b00 + (uint64_t)a.ny * a.NC;
+    const uint64_t* b11 = b10 + a.NC;
     const uint32_t c0 = cx >> 6, i0 = cx & 63u;
     const uint32_t c1 = (cx + 1) >> 6, i1 = (cx + 1) & 63u;
     uint32_t ci = 0;
@@ -294,7 +379,8 @@ __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x)
     return x;
 }
 
-/* In-place exclusive scan of n u32 values by one wave; returns the total. */
+/* In-place exclusive scan of n u32 values in LDS by one wave; returns the
+ * total (the same in every lane). */
 __device__ __forceinline__ uint32_t wave_exclusive_scan(uint32_t* p, uint32_t n)
 {
     const uint32_t lane = threadIdx.x & 63u;
@@ -314,44 +400,42 @@ __device__ __forceinline__ uint32_t wave_exclusive_scan(uint32_t* p, uint32_t n)
 /* ------------------------------------------------------------------------- */
 
 /* Bring the chunk masks of node rows y0-1 .. y0+R+1 of node slices
- * s_lo .. s_hi from the bit volume into their ring slots; rows and slices
- * outside the volume are zero.  For one (slice, level) the rows are
- * contiguous both in the bit volume and in LDS, so the copy is a set of flat
- * ranges; loads are issued in batches before their stores. */
-__device__ void load_slices(const JobArgs& a, const Views& v, int s_lo, int s_hi, int y0)
+ * s_lo .. s_hi of the level into their ring slots; rows and slices outside
+ * the volume are zero.  For one slice the rows are contiguous both in the
+ * bit volume and in LDS, so the copy is a flat range per slice; a lane's
+ * loads are all issued before its stores. */
+template <int M>
+__device__ void load_slices(const JobArgs& a, const Team& v, const Level& L, int s_lo, int s_hi, int y0)
 {
     if (s_hi < s_lo) return;
-    const uint32_t nseg = v.nrb * a.NC;              // elements of one (slice, level)
-    const uint32_t ng = (uint32_t)(s_hi - s_lo + 1) * a.nlev;
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t nseg = v.nrb * a.NC;              // elements of one slice
+    const uint32_t total = (uint32_t)(s_hi - s_lo + 1) * nseg;
     const uint32_t e_lo = (y0 - 1 < 0) ? a.NC : 0u;
     const int rows_left = (int)a.ny - (y0 - 1);
     const uint32_t e_hi = (rows_left >= (int)v.nrb ? v.nrb : (uint32_t)(rows_left > 0 ? rows_left : 0)) * a.NC;
-    uint32_t g = 0, e = threadIdx.x;
-    while (g < ng && e >= nseg) { ++g; e = threadIdx.x; }
-    while (g < ng) {
-        constexpr int kB = 8;
+    const uint64_t slice_words = (uint64_t)a.ny * a.NC;
+    const int64_t row0 = (int64_t)(y0 - 1) * a.NC;
+    constexpr int kB = 4;
+    /* element i = b0 + 64u + lane is element e of slice s_lo + si */
+    uint32_t si = 0, e = lane;
+    while (e >= nseg) { e -= nseg; ++si; }
+    for (uint32_t b0 = 0; b0 < total; b0 += 64u * kB) {
         uint64_t val[kB];
         uint32_t dst[kB];
-        int cnt = 0;
 #pragma unroll
         for (int u = 0; u < kB; ++u) {
-            if (g < ng) {
-                const uint32_t si = g / a.nlev, l = g - si * a.nlev;
-                const int s = s_lo + (int)si;
-                const bool ok = s >= 0 && s < (int)a.nz && e >= e_lo && e < e_hi;
-                /* element e of the segment is node row y0-1 + e / NC of slice s */
-                const uint64_t idx = ok ? (((uint64_t)l * a.nz + (uint64_t)s) * a.ny + (uint64_t)(y0 - 1)) * a.NC + e
-                                        : 0ull;
-                val[u] = ok ? a.bitvol[idx] : 0ull;
-                dst[u] = ((bslot(v, s) * a.nlev + l) * v.nrb) * a.NC + e;
-                cnt = u + 1;
-                e += kBlock;
-                while (g < ng && e >= nseg) { ++g; e = threadIdx.x; }
-            }
+            const uint32_t i = b0 + 64u * u + lane;
+            const int s = s_lo + (int)si;
+            const bool ok = i < total && s >= 0 && s < (int)a.nz && e >= e_lo && e < e_hi;
+            val[u] = ok ? L.bits[(uint64_t)((int64_t)s * (int64_t)slice_words + row0 + (int64_t)e)] : 0ull;
+            dst[u] = i < total ? bslot<M>(s) * nseg + e : ~0u;
+            e += 64u;
+            while (e >= nseg) { e -= nseg; ++si; }
         }
 #pragma unroll
         for (int u = 0; u < kB; ++u)
-            if (u < cnt) v.bits[dst[u]] = val[u];
+            if (dst[u] != ~0u) v.bits[dst[u]] = val[u];
     }
 }
 
@@ -359,17 +443,21 @@ __device__ void load_slices(const JobArgs& a, const Views& v, int s_lo, int s_hi
 /* per-tile counts and prefixes (lanes on (row, chunk))                        */
 /* ------------------------------------------------------------------------- */
 
+/* Totals of a tile: vertices it emits, triangles of its cell slice (halo row
+ * included) and triangles of the halo row. */
+struct TileTotals { uint32_t nv, nt, halo; };
+
 /* Tile z: window vertex counts per chunk of window slices 0, 1 (node slices
  * z-1, z), rows 1..R+1 (for output ranks), and active cells / triangle
  * counts of cell slice z-1, cell rows j0..R (j = 0: the halo row y0-1, only
- * with VertexNormal); then exclusive prefixes -- window prefixes per
- * (slice, level), triangle prefixes per level -- and the queue offsets.
- * ws_v: window slice whose vertices the tile emits.  Work items are
- * (level, row group) pairs spread over all waves. */
+ * with VertexNormal); then exclusive prefixes -- window prefixes per slice
+ * and triangle prefixes.  ws_v: window slice whose vertices the tile emits.
+ * Lanes cover 64 >> cshift rows of 1 << cshift chunk lanes. */
 template <int M>
-__device__ void tile_prefixes(const JobArgs& a, const Views& v, int z, int y0, uint32_t ws_v, bool ranks, bool first)
+__device__ TileTotals tile_prefixes(const JobArgs& a, const Team& v, int z, int y0, uint32_t ws_v, bool ranks,
+                                    bool first)
 {
-    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    const uint32_t lane = threadIdx.x & 63u;
     const uint32_t sub = lane >> a.cshift, cl = lane & ((1u << a.cshift) - 1u);
     const uint32_t G = 64u >> a.cshift, CS = 1u << a.cshift;
     const uint32_t nwr = a.R + 2;
@@ -378,95 +466,56 @@ __device__ void tile_prefixes(const JobArgs& a, const Views& v, int z, int y0, u
     const uint32_t j0 = halo ? 0u : 1u;
     const int zc = z - 1;
     const uint32_t cs = cslot<M>(zc);
-    if (ranks && !dup) {
-        /* window rows 1..R+1 of window slice 1 (and of slice 0 at a unit's first
-         * step; later it is the previous step's slice 1) */
-        const uint32_t ws0 = first ? 0u : 1u;
-        const uint32_t ngr = (a.R + 1 + G - 1) / G;              // row groups
-        const uint32_t items = (2u - ws0) * a.nlev * ngr;
-        for (uint32_t it = wave; it < items; it += kWaves) {
-            const uint32_t ws = ws0 + it / (a.nlev * ngr), rem = it - (ws - ws0) * a.nlev * ngr;
-            const uint32_t l = rem / ngr, grp = rem - l * ngr;
-            const uint32_t w = 1 + grp * G + sub;
-            if (w >= nwr) continue;
-            uint32_t* dst = wrow(v, z, ws, l, w);
-            for (uint32_t c = cl; c < a.NC; c += CS) {
-                const Cross k = window_cross(a, v, z, y0, ws, w, c, l);
-                dst[c] = (uint32_t)(__popcll(k.x) + __popcll(k.y) + __popcll(k.z));
-            }
-        }
-    }
-    {
-        const uint32_t nj = a.R + 1;
-        const uint32_t ngr = (nj + G - 1) / G;
-        const uint32_t items = a.nlev * ngr;
-        for (uint32_t it = wave; it < items; it += kWaves) {
-            const uint32_t l = it / ngr, grp = it - l * ngr;
-            const uint32_t j = grp * G + sub;
-            if (j >= nj) continue;
-            const int y = y0 - 1 + (int)j;
-            for (uint32_t c = cl; c < a.NC; c += CS) {
-                uint64_t act = 0;
-                uint32_t cnt = 0;
-                if (j >= j0 && zc >= 0 && zc + 1 < (int)a.nz && y >= 0 && y + 1 < (int)a.ny) {
-                    const Corners k = corners_at(a, v, y0, c, y, zc, l);
-                    uint64_t o = 0, an = ~0ull;
-#pragma unroll
-                    for (int q = 0; q < 8; ++q) { o |= k.c[q]; an &= k.c[q]; }
-                    act = (o & ~an) & valid_mask(c, a.nx - 1);
-                    uint64_t m = act;
-                    while (m) {
-                        const uint32_t i = (uint32_t)__ffsll((unsigned long long)m) - 1u;
-                        m &= m - 1ull;
-                        cnt += (uint32_t)(v.tri[corner_index(k, i)] >> 60);
-                    }
+    const bool wranks = ranks && !dup;
+    /* window rows 1..R+1 of window slice 1 (and of slice 0 at a unit's first
+     * step; later it is the previous step's slice 1) */
+    if (wranks) {
+        for (uint32_t ws = first ? 0u : 1u; ws < 2u; ++ws) {
+            for (uint32_t w = 1 + sub; w < nwr; w += G) {
+                uint32_t* dst = wrow(v, z, ws, w);
+                for (uint32_t c = cl; c < a.NC; c += CS) {
+                    const Cross k = window_cross<M>(a, v, z, y0, ws, w, c);
+                    dst[c] = (uint32_t)(__popcll(k.x) + __popcll(k.y) + __popcll(k.z));
                 }
-                v.act[cidx(v, cs, l, j, c)] = act;
-                v.pt[cidx(v, cs, l, j, c)] = cnt;
             }
         }
     }
-    __syncthreads();
-    /* one wave per sequence */
-    const uint32_t nws = first ? 2u : 1u;
-    const uint32_t nwseq = (ranks && !dup) ? nws * a.nlev : 0u;
-    const uint32_t nseq = nwseq + a.nlev;
-    for (uint32_t q = wave; q < nseq; q += kWaves) {
-        if (q < nwseq) {
-            (void)wave_exclusive_scan(wrow(v, z, (2u - nws) + q / a.nlev, q % a.nlev, 1), (nwr - 1) * a.NC);
-        } else {
-            const uint32_t l = q - nwseq;
-            const uint32_t tot = wave_exclusive_scan(v.pt + cidx(v, cs, l, 0, 0), (a.R + 1) * a.NC);
-            if (lane == 0) v.misc[kMiscTTot + l] = tot;
-        }
-    }
-    __syncthreads();
-    if (threadIdx.x < 64) {
-        /* queue offsets: exclusive scans over levels (nlev <= 8 lanes) */
-        const uint32_t l = lane;
-        const bool in = l < a.nlev;
-        const uint32_t vt = (in && !dup && ranks) ? (wrow(v, z, ws_v, l, a.R + 1)[0] - wrow(v, z, ws_v, l, 1)[0]) : 0u;
-        const uint32_t tt = in ? v.misc[kMiscTTot + l] : 0u;
-        const uint32_t hl = (in && halo) ? v.pt[cidx(v, cs, l, 1, 0)] : 0u;
-        const uint32_t iv = wave_incl_scan(vt), it = wave_incl_scan(tt);
-        if (in) {
-            v.misc[kMiscVTot + l] = vt;
-            v.misc[kMiscQV + l] = iv - vt;
-            v.misc[kMiscQT + l] = it - tt;
-            v.misc[kMiscHalo + l] = hl;
-            if (halo) v.misc[kMiscFnOff + cs * 9 + l] = it - tt;
-        }
-        const uint32_t tv = __shfl(iv, (int)a.nlev - 1), ttot = __shfl(it, (int)a.nlev - 1);
-        if (lane == 0) {
-            v.misc[kMiscQV + a.nlev] = tv;
-            v.misc[kMiscQT + a.nlev] = ttot;
-            if (halo) {
-                v.misc[kMiscFnOff + cs * 9 + a.nlev] = ttot;
-                v.misc[kMiscFnOvf + cs] = ttot > a.fn_cap ? 1u : 0u;
+    for (uint32_t j = sub; j <= a.R; j += G) {
+        const int y = y0 - 1 + (int)j;
+        const bool live = j >= j0 && zc >= 0 && zc + 1 < (int)a.nz && y >= 0 && y + 1 < (int)a.ny;
+        for (uint32_t c = cl; c < a.NC; c += CS) {
+            uint64_t act = 0;
+            uint32_t cnt = 0;
+            if (live) {
+                const Corners k = corners_at<M>(a, v, y0, c, y, zc);
+                uint64_t o = 0, an = ~0ull;
+#pragma unroll
+                for (int q = 0; q < 8; ++q) { o |= k.c[q]; an &= k.c[q]; }
+                act = (o & ~an) & valid_mask(c, a.nx - 1);
+                uint64_t m = act;
+                while (m) {
+                    const uint32_t i = (uint32_t)__ffsll((unsigned long long)m) - 1u;
+                    m &= m - 1ull;
+                    cnt += (uint32_t)(v.tri[corner_index(k, i)] >> 60);
+                }
             }
+            v.act[cidx(v, cs, j, c)] = act;
+            v.pt[cidx(v, cs, j, c)] = cnt;
+            if (halo) v.ap[cidx(v, cs, j, c)] = (uint32_t)__popcll(act);
         }
     }
-    __syncthreads();
+    wave_sync();
+    if (halo) (void)wave_exclusive_scan(v.ap + cidx(v, cs, 0, 0), (a.R + 1) * a.NC);
+    if (wranks) {
+        for (uint32_t ws = first ? 0u : 1u; ws < 2u; ++ws)
+            (void)wave_exclusive_scan(wrow(v, z, ws, 1), (nwr - 1) * a.NC);
+    }
+    TileTotals t;
+    t.nt = uniform(wave_exclusive_scan(v.pt + cidx(v, cs, 0, 0), (a.R + 1) * a.NC));
+    wave_sync();
+    t.nv = wranks ? uniform(wrow(v, z, ws_v, a.R + 1)[0] - wrow(v, z, ws_v, 1)[0]) : 0u;
+    t.halo = halo ? uniform(v.pt[cidx(v, cs, 1, 0)]) : 0u;
+    return t;
 }
 
 /* ------------------------------------------------------------------------- */
@@ -503,18 +552,48 @@ __device__ __forceinline__ uint32_t edge_ends(uint32_t e)
     return e < 8 ? (uint32_t)(kLo >> (8 * e)) & 0xFFu : (kHi >> (8 * (e - 8))) & 0xFFu;
 }
 
+/* Output bases of tile (z, band): vertex prefixes of node rows (z-1, y0),
+ * (z-1, y0+R), (z, y0), (z, y0+R) and the triangle prefix of cell row
+ * (y0, z-1), relative to the first row of their segment; one lane loads each
+ * and every lane gets all five. */
+struct Bases { uint64_t v[4]; uint64_t t; };
+
+__device__ __forceinline__ Bases tile_bases(const JobArgs& a, const Level& L, int z, int y0)
+{
+    const uint32_t lane = threadIdx.x & 63u;
+    const bool in = z < (int)a.nz, next = (uint32_t)y0 + a.R < a.ny;
+    const uint64_t row = (uint64_t)z * a.ny + (uint64_t)y0;
+    const uint64_t prow = (uint64_t)(z - 1) * a.ny + (uint64_t)y0;
+    const uint64_t* p = nullptr;
+    uint64_t b = 0;
+    if (lane == 0 && z > 0) { p = L.pre_v + prow; b = L.bv; }
+    if (lane == 1 && z > 0 && next) { p = L.pre_v + prow + a.R; b = L.bv; }
+    if (lane == 2 && in) { p = L.pre_v + row; b = L.bv; }
+    if (lane == 3 && in && next) { p = L.pre_v + row + a.R; b = L.bv; }
+    if (lane == 4 && in) { p = L.pre_t + row; b = L.bt; }
+    const uint64_t x = p ? *p - b : 0ull;
+    Bases B;
+    B.v[0] = readlane64(x, 0);
+    B.v[1] = readlane64(x, 1);
+    B.v[2] = readlane64(x, 2);
+    B.v[3] = readlane64(x, 3);
+    B.t = readlane64(x, 4);
+    return B;
+}
+
 /* Rank within its tile row range of the vertex on (node x of band row k (0..R)
  * of node slice s in {z-1, z}, axis): rows 0..R-1 rank from the band's first
  * row, row R (first row of the next band) from its own start. */
-__device__ __forceinline__ uint32_t tile_rank(const JobArgs& a, const Views& v, int z, int y0,
-                                              uint32_t x, uint32_t k, int s, uint32_t axis, uint32_t l)
+template <int M>
+__device__ __forceinline__ uint32_t tile_rank(const JobArgs& a, const Team& v, int z, int y0,
+                                              uint32_t x, uint32_t k, int s, uint32_t axis)
 {
     const uint32_t ws = (uint32_t)(s - z + 1), w = k + 1;
     const uint32_t c = x >> 6, i = x & 63u;
-    const Cross e = window_cross(a, v, z, y0, ws, w, c, l);
+    const Cross e = window_cross<M>(a, v, z, y0, ws, w, c);
     const uint64_t lo = (1ull << i) - 1ull;
-    const uint32_t start = wrow(v, z, ws, l, k == a.R ? a.R + 1 : 1)[0];
-    uint32_t r = wrow(v, z, ws, l, w)[c] - start;
+    const uint32_t start = wrow(v, z, ws, k == a.R ? a.R + 1 : 1)[0];
+    uint32_t r = wrow(v, z, ws, w)[c] - start;
     r += __popcll(e.x & lo) + __popcll(e.y & lo) + __popcll(e.z & lo);
     if (axis >= 1) r += (uint32_t)((e.x >> i) & 1ull);
     if (axis >= 2) r += (uint32_t)((e.y >> i) & 1ull);
@@ -540,12 +619,13 @@ __device__ __forceinline__ uint32_t around_nibbles(uint32_t axis)
 /* VertexNormal of the vertex on (x,y,zv,axis): the float sum, in triangle
  * order, of the unnormalized face normals of the triangles that use it --
  * the order of calculate_normals_on_vertex (:569-605).  Incident cells are
- * visited in cell scan order and triangles in table order; every face normal
- * is recomputed from interpolated coordinates (used when the face normals of
- * a cell slice did not fit in LDS). */
+ * visited in cell scan order and triangles in table order; cube indices come
+ * from the bit volume and every face normal is recomputed from interpolated
+ * coordinates (used when the face normals of a cell slice did not fit in
+ * LDS). */
 template <typename T>
-__device__ void vertex_normal_direct(const JobArgs& a, const Views& v, const T* vol, double iso, int y0,
-                                     uint32_t x, uint32_t y, int zv, uint32_t axis, uint32_t l, float n[3])
+__device__ void vertex_normal_direct(const JobArgs& a, const Team& v, const Level& L, const T* vol,
+                                     uint32_t x, uint32_t y, int zv, uint32_t axis, float n[3])
 {
     n[0] = 0.0f; n[1] = 0.0f; n[2] = 0.0f;
     const uint32_t around = around_nibbles(axis);
@@ -554,7 +634,7 @@ __device__ void vertex_normal_direct(const JobArgs& a, const Views& v, const T* 
         around_cell(axis, q, (int)x, (int)y, zv, cx, cy, cz);
         if (cx < 0 || cy < 0 || cz < 0) continue;
         if (cx + 1 >= (int)a.nx || cy + 1 >= (int)a.ny || cz + 1 >= (int)a.nz) continue;
-        const uint32_t ci = cube_index_at(v, y0, (uint32_t)cx, cy, cz, l);
+        const uint32_t ci = cube_index_mem(a, L.bits, (uint32_t)cx, cy, cz);
         const uint64_t w = v.tri[ci];
         const int nt = (int)(w >> 60);
         const uint32_t me = (around >> (4 * q)) & 0xFu;
@@ -568,7 +648,7 @@ __device__ void vertex_normal_direct(const JobArgs& a, const Views& v, const T* 
 #pragma unroll
             for (int j = 0; j < 3; ++j) {
                 const uint32_t o = edge_owner(es[j]);
-                node_edge_coords<T>(a, vol, iso, (uint32_t)cx + (o & 1u), (uint32_t)(cy + (int)((o >> 1) & 1u)),
+                node_edge_coords<T>(a, vol, L.iso, (uint32_t)cx + (o & 1u), (uint32_t)(cy + (int)((o >> 1) & 1u)),
                                     (uint32_t)(cz + (int)((o >> 2) & 1u)), o >> 3, p[j]);
             }
             face_normal(p[0], p[1], p[2], fn);
@@ -580,12 +660,13 @@ __device__ void vertex_normal_direct(const JobArgs& a, const Views& v, const T* 
 }
 
 /* The same sum from the face normals the triangle phase kept in LDS for cell
- * slices zv-1 and zv (band rows and the halo row above the band).  The
- * triangle of (cell, t) sits at its cell slice's face-normal offset of the
- * level + the triangle prefix of its (cell row, chunk) + the triangles of
- * the active cells before it in the chunk + t. */
-__device__ void vertex_normal_ring(const JobArgs& a, const Views& v, int y0, uint32_t x, uint32_t y, int zv,
-                                   uint32_t axis, uint32_t l, float n[3])
+ * slices zv-1 and zv (band rows and the halo row above the band).  Every cell
+ * around a crossing edge is active; its record, at its active-cell rank (the
+ * active-cell prefix of its (cell row, chunk) + the active cells before it
+ * in the chunk), holds its cube index and the rank of its first triangle. */
+template <int M>
+__device__ void vertex_normal_ring(const JobArgs& a, const Team& v, int y0, uint32_t x, uint32_t y, int zv,
+                                   uint32_t axis, float n[3])
 {
     n[0] = 0.0f; n[1] = 0.0f; n[2] = 0.0f;
     const uint32_t around = around_nibbles(axis);
@@ -596,17 +677,12 @@ __device__ void vertex_normal_ring(const JobArgs& a, const Views& v, int y0, uin
         if (cx + 1 >= (int)a.nx || cy + 1 >= (int)a.ny || cz + 1 >= (int)a.nz) continue;
         const uint32_t cs = (uint32_t)cz & 1u, j = (uint32_t)(cy - y0 + 1);
         const uint32_t c = (uint32_t)cx >> 6, i = (uint32_t)cx & 63u;
-        const Corners k = corners_at(a, v, y0, c, cy, cz, l);
-        const uint32_t ci = corner_index(k, i);
-        const uint64_t w = v.tri[ci];
+        const uint32_t d = cidx(v, cs, j, c);
+        const uint32_t rank = v.ap[d] + (uint32_t)__popcll(v.act[d] & ((1ull << i) - 1ull));
+        const uint32_t rc = v.crec[cs * v.fn_cap + rank];
+        const uint32_t base = rc >> 8;
+        const uint64_t w = v.tri[rc & 0xFFu];
         const int nt = (int)(w >> 60);
-        uint32_t base = v.misc[kMiscFnOff + cs * 9 + l] + v.pt[cidx(v, cs, l, j, c)];
-        uint64_t m = v.act[cidx(v, cs, l, j, c)] & ((1ull << i) - 1ull);
-        while (m) {
-            const uint32_t b = (uint32_t)__ffsll((unsigned long long)m) - 1u;
-            m &= m - 1ull;
-            base += (uint32_t)(v.tri[corner_index(k, b)] >> 60);
-        }
         const uint32_t me = (around >> (4 * q)) & 0xFu;
         const float* f = v.fn + (size_t)cs * v.fn_cap * 3u;
         for (int t = 0; t < nt; ++t) {
@@ -622,202 +698,180 @@ __device__ void vertex_normal_ring(const JobArgs& a, const Views& v, int y0, uin
     }
 }
 
-/* Emit tile (z, b): the vertices of window slice ws_v (shared-vertex modes,
- * when out_vert) and the triangles of cell slice z-1 (when out_tri), all
- * levels, through two LDS work queues whose entry index is the output rank.
- * With VertexNormal the triangles of the halo row are processed too and
- * every face normal is kept in the cell slice's ring slot. */
+/* Emit tile (z, band): the vertices of window slice ws_v (shared-vertex
+ * modes, when out_vert) and the triangles of cell slice z-1 (when out_tri),
+ * through two LDS work queues whose entry index is the output rank.  With
+ * VertexNormal the triangles of the halo row are processed too and, when
+ * keep_fn, every face normal is kept in the cell slice's ring slot; ring_ok:
+ * both slots hold all their face normals. */
 template <typename T, int M>
-__device__ void emit_tile(const JobArgs& a, const Views& v, const T* vol, int z, int y0, uint32_t ws_v,
-                          bool out_vert, bool out_tri)
+__device__ void emit_tile(const JobArgs& a, const Team& v, const Level& L, const Bases& B, const T* vol, int z,
+                          int y0, uint32_t ws_v, bool out_vert, bool out_tri, const TileTotals& tt, bool keep_fn,
+                          bool ring_ok)
 {
     constexpr bool vn = M == kModeVertex;
-    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    const uint32_t lane = threadIdx.x & 63u;
     const uint32_t sub = lane >> a.cshift, cl = lane & ((1u << a.cshift) - 1u);
     const uint32_t G = 64u >> a.cshift, CS = 1u << a.cshift;
-    const uint32_t nv_all = out_vert ? v.misc[kMiscQV + a.nlev] : 0u;
-    const uint32_t nt_all = v.misc[kMiscQT + a.nlev];
+    const uint32_t nv_all = out_vert ? tt.nv : 0u;
+    const uint32_t nt_all = tt.nt;
     const uint32_t Q = a.queue_cap;
     const int zc = z - 1;
     const int zv = z - 1 + (int)ws_v;
     const uint32_t cs = cslot<M>(zc);
-    const bool keep_fn = vn && !v.misc[kMiscFnOvf + cs];
 
     /* triangles of cell slice z-1 (with VertexNormal first, so that their face
      * normals are in LDS before the vertices below read them) */
-    {
-        const uint32_t ngr = (a.R + 1 + G - 1) / G;          // row groups of cell rows 0..R
-        const uint32_t items = a.nlev * ngr;
-        for (uint32_t w0 = 0; w0 < nt_all; w0 += Q) {
-            for (uint32_t it = wave; it < items; it += kWaves) {
-                const uint32_t l = it / ngr, grp = it - l * ngr;
-                const uint32_t j = grp * G + sub;
-                if (j > a.R) continue;
-                for (uint32_t c = cl; c < a.NC; c += CS) {
-                    uint64_t m = v.act[cidx(v, cs, l, j, c)];
-                    if (!m) continue;
-                    const Corners k = corners_at(a, v, y0, c, y0 - 1 + (int)j, zc, l);
-                    uint32_t r = v.pt[cidx(v, cs, l, j, c)] + v.misc[kMiscQT + l];
-                    while (m) {
-                        const uint32_t i = (uint32_t)__ffsll((unsigned long long)m) - 1u;
-                        m &= m - 1ull;
-                        const uint32_t nt = (uint32_t)(v.tri[corner_index(k, i)] >> 60);
-                        const uint32_t x = c * 64u + i;
-                        for (uint32_t tt = 0; tt < nt; ++tt, ++r)
-                            if (r >= w0 && r - w0 < Q) v.tq[r - w0] = x | (j << 20) | (tt << 25) | (l << 28);
-                    }
+    for (uint32_t w0 = 0; w0 < nt_all; w0 += Q) {
+        for (uint32_t j = sub; j <= a.R; j += G) {
+            for (uint32_t c = cl; c < a.NC; c += CS) {
+                uint64_t m = v.act[cidx(v, cs, j, c)];
+                if (!m) continue;
+                const Corners k = corners_at<M>(a, v, y0, c, y0 - 1 + (int)j, zc);
+                uint32_t r = v.pt[cidx(v, cs, j, c)];
+                /* with the face normals kept, each active cell also records its first
+                 * triangle and cube index at its active-cell rank */
+                const bool rec = vn && keep_fn && w0 == 0;
+                uint32_t ai = rec ? v.ap[cidx(v, cs, j, c)] : 0u;
+                while (m) {
+                    const uint32_t i = (uint32_t)__ffsll((unsigned long long)m) - 1u;
+                    m &= m - 1ull;
+                    const uint32_t ci = corner_index(k, i);
+                    const uint32_t nt = (uint32_t)(v.tri[ci] >> 60);
+                    const uint32_t x = c * 64u + i;
+                    if (rec) v.crec[cs * v.fn_cap + ai++] = r << 8 | ci;
+                    for (uint32_t t = 0; t < nt; ++t, ++r)
+                        if (r >= w0 && r - w0 < Q) v.tq[r - w0] = x | (j << 20) | (t << 25);
                 }
             }
-            __syncthreads();
-            const uint32_t ntq = (nt_all - w0) < Q ? (nt_all - w0) : Q;
-            for (uint32_t q = threadIdx.x; q < ntq; q += kBlock) {
-                const uint32_t e = v.tq[q];
-                const uint32_t x = e & 0xFFFFFu, j = (e >> 20) & 0x1Fu, t = (e >> 25) & 7u, l = e >> 28;
-                const LevelInfo& L = v.lev[l];
-                const uint32_t rank = w0 + q;                           // rank in the tile, halo row included
-                const uint64_t tri = L.tbase + (rank - v.misc[kMiscQT + l] - v.misc[kMiscHalo + l]);
-                const bool write = out_tri && j >= 1 && tri < L.cap_t;
-                if (!write && !keep_fn) continue;
-                const int y = y0 - 1 + (int)j;
-                const uint32_t ci = cube_index_at(v, y0, x, y, zc, l);
-                const uint64_t w = v.tri[ci];
-                const uint32_t eds[3] = { (uint32_t)((w >> (12 * t)) & 0xF), (uint32_t)((w >> (12 * t + 8)) & 0xF),
-                                          (uint32_t)((w >> (12 * t + 4)) & 0xF) };
-                float p[3][3], nn[3];
-                if constexpr (M == kModeDup) {
-                    /* extract_surfaces_with_duplication: table orientation (:206-228) */
+        }
+        wave_sync();
+        const uint32_t ntq = (nt_all - w0) < Q ? (nt_all - w0) : Q;
+        for (uint32_t q = lane; q < ntq; q += 64u) {
+            const uint32_t e = v.tq[q];
+            const uint32_t x = e & 0xFFFFFu, j = (e >> 20) & 0x1Fu, t = (e >> 25) & 7u;
+            const uint32_t rank = w0 + q;                           // rank in the tile, halo row included
+            const uint64_t tri = B.t + (uint64_t)(rank - tt.halo);
+            const bool write = out_tri && j >= 1 && tri < L.cap_t;
+            if (!write && !keep_fn) continue;
+            const int y = y0 - 1 + (int)j;
+            const uint32_t ci = cube_index_at<M>(v, y0, x, y, zc);
+            const uint64_t w = v.tri[ci];
+            const uint32_t eds[3] = { (uint32_t)((w >> (12 * t)) & 0xF), (uint32_t)((w >> (12 * t + 8)) & 0xF),
+                                      (uint32_t)((w >> (12 * t + 4)) & 0xF) };
+            float p[3][3], nn[3];
+            if constexpr (M == kModeDup) {
+                /* extract_surfaces_with_duplication: table orientation (:206-228) */
 #pragma unroll
-                    for (int jj = 0; jj < 3; ++jj) {
-                        const uint32_t qq = edge_ends(eds[jj]);
-                        interpolate<T>(a, vol, L.iso,
-                                       x + (qq & 1u), (uint32_t)y + ((qq >> 1) & 1u), (uint32_t)zc + ((qq >> 2) & 1u),
-                                       x + ((qq >> 4) & 1u), (uint32_t)y + ((qq >> 5) & 1u),
-                                       (uint32_t)zc + ((qq >> 6) & 1u), p[jj]);
-                    }
-                    face_normal(p[0], p[1], p[2], nn);
-                    float* co = L.coords + 9 * tri;
-#pragma unroll
-                    for (int jj = 0; jj < 3; ++jj) {
-                        co[3 * jj + 0] = p[jj][0];
-                        co[3 * jj + 1] = p[jj][1];
-                        co[3 * jj + 2] = p[jj][2];
-                    }
-                    L.normals[3 * tri + 0] = nn[0];
-                    L.normals[3 * tri + 1] = nn[1];
-                    L.normals[3 * tri + 2] = nn[2];
-                    continue;
+                for (int jj = 0; jj < 3; ++jj) {
+                    const uint32_t qq = edge_ends(eds[jj]);
+                    interpolate<T>(a, vol, L.iso,
+                                   x + (qq & 1u), (uint32_t)y + ((qq >> 1) & 1u), (uint32_t)zc + ((qq >> 2) & 1u),
+                                   x + ((qq >> 4) & 1u), (uint32_t)y + ((qq >> 5) & 1u),
+                                   (uint32_t)zc + ((qq >> 6) & 1u), p[jj]);
                 }
-                if (write) {
-                    /* connect_isopoints: vertex ids of the three edges (:493-514) */
-                    uint32_t id[3];
+                face_normal(p[0], p[1], p[2], nn);
+                float* co = L.coords + 9 * tri;
 #pragma unroll
-                    for (int jj = 0; jj < 3; ++jj) {
-                        const uint32_t o = edge_owner(eds[jj]);
-                        const uint32_t dx = o & 1u, dy = (o >> 1) & 1u, dz = (o >> 2) & 1u, ax = o >> 3;
-                        const uint32_t kk = j - 1 + dy;
-                        const uint32_t rk = tile_rank(a, v, z, y0, x + dx, kk, zc + (int)dz, ax, l);
-                        const uint32_t sel = (dz << 1) | (kk == a.R ? 1u : 0u);
-                        const uint64_t b = sel == 0 ? L.vbase[0] : sel == 1 ? L.vbase[1] : sel == 2 ? L.vbase[2] : L.vbase[3];
-                        id[jj] = (uint32_t)(b + rk);
-                    }
-                    L.conn[3 * tri + 0] = id[0];
-                    L.conn[3 * tri + 1] = id[1];
-                    L.conn[3 * tri + 2] = id[2];
+                for (int jj = 0; jj < 3; ++jj) {
+                    co[3 * jj + 0] = p[jj][0];
+                    co[3 * jj + 1] = p[jj][1];
+                    co[3 * jj + 2] = p[jj][2];
                 }
-                if (vn && !keep_fn) continue;
-                /* calculate_normals_on_polygon (:531-559); with VertexNormal the same face
-                 * normal is what calculate_normals_on_vertex sums */
+                L.normals[3 * tri + 0] = nn[0];
+                L.normals[3 * tri + 1] = nn[1];
+                L.normals[3 * tri + 2] = nn[2];
+                continue;
+            }
+            if (write) {
+                /* connect_isopoints: vertex ids of the three edges (:493-514) */
+                uint32_t id[3];
 #pragma unroll
                 for (int jj = 0; jj < 3; ++jj) {
                     const uint32_t o = edge_owner(eds[jj]);
-                    node_edge_coords<T>(a, vol, L.iso, x + (o & 1u), (uint32_t)(y + (int)((o >> 1) & 1u)),
-                                        (uint32_t)(zc + (int)((o >> 2) & 1u)), o >> 3, p[jj]);
+                    const uint32_t dx = o & 1u, dy = (o >> 1) & 1u, dz = (o >> 2) & 1u, ax = o >> 3;
+                    const uint32_t kk = j - 1 + dy;
+                    const uint32_t rk = tile_rank<M>(a, v, z, y0, x + dx, kk, zc + (int)dz, ax);
+                    const uint32_t sel = (dz << 1) | (kk == a.R ? 1u : 0u);
+                    const uint64_t b = sel == 0 ? B.v[0] : sel == 1 ? B.v[1] : sel == 2 ? B.v[2] : B.v[3];
+                    id[jj] = (uint32_t)(b + rk);
                 }
-                face_normal(p[0], p[1], p[2], nn);
-                if constexpr (vn) {
-                    float* g = v.fn + ((size_t)cs * v.fn_cap + rank) * 3u;
-                    g[0] = nn[0]; g[1] = nn[1]; g[2] = nn[2];
-                } else {
-                    L.normals[3 * tri + 0] = nn[0];
-                    L.normals[3 * tri + 1] = nn[1];
-                    L.normals[3 * tri + 2] = nn[2];
-                }
+                L.conn[3 * tri + 0] = id[0];
+                L.conn[3 * tri + 1] = id[1];
+                L.conn[3 * tri + 2] = id[2];
             }
-            __syncthreads();
+            if (vn && !keep_fn) continue;
+            /* calculate_normals_on_polygon (:531-559); with VertexNormal the same face
+             * normal is what calculate_normals_on_vertex sums */
+#pragma unroll
+            for (int jj = 0; jj < 3; ++jj) {
+                const uint32_t o = edge_owner(eds[jj]);
+                node_edge_coords<T>(a, vol, L.iso, x + (o & 1u), (uint32_t)(y + (int)((o >> 1) & 1u)),
+                                    (uint32_t)(zc + (int)((o >> 2) & 1u)), o >> 3, p[jj]);
+            }
+            face_normal(p[0], p[1], p[2], nn);
+            if constexpr (vn) {
+                float* g = v.fn + ((size_t)cs * v.fn_cap + rank) * 3u;
+                g[0] = nn[0]; g[1] = nn[1]; g[2] = nn[2];
+            } else {
+                L.normals[3 * tri + 0] = nn[0];
+                L.normals[3 * tri + 1] = nn[1];
+                L.normals[3 * tri + 2] = nn[2];
+            }
         }
+        wave_sync();
     }
 
     /* vertices of window slice ws_v, band rows */
-    if (nv_all) {
-        const bool ring_ok = vn && !v.misc[kMiscFnOvf + 0] && !v.misc[kMiscFnOvf + 1];
-        const uint32_t ngr = (a.R + G - 1) / G;              // row groups of band rows 0..R-1
-        const uint32_t items = a.nlev * ngr;
-        for (uint32_t w0 = 0; w0 < nv_all; w0 += Q) {
-            for (uint32_t it = wave; it < items; it += kWaves) {
-                const uint32_t l = it / ngr, grp = it - l * ngr;
-                const uint32_t k = grp * G + sub;
-                if (k >= a.R || y0 + (int)k >= (int)a.ny) continue;
-                for (uint32_t c = cl; c < a.NC; c += CS) {
-                    const Cross e = window_cross(a, v, z, y0, ws_v, k + 1, c, l);
-                    uint64_t m = e.x | e.y | e.z;
-                    uint32_t r = wrow(v, z, ws_v, l, k + 1)[c] - wrow(v, z, ws_v, l, 1)[0] + v.misc[kMiscQV + l];
-                    while (m) {
-                        const uint32_t i = (uint32_t)__ffsll((unsigned long long)m) - 1u;
-                        m &= m - 1ull;
-                        const uint32_t base = (c * 64u + i) | (k << 20) | (l << 28);
-                        if ((e.x >> i) & 1ull) { if (r >= w0 && r - w0 < Q) v.vq[r - w0] = base; ++r; }
-                        if ((e.y >> i) & 1ull) { if (r >= w0 && r - w0 < Q) v.vq[r - w0] = base | (1u << 26); ++r; }
-                        if ((e.z >> i) & 1ull) { if (r >= w0 && r - w0 < Q) v.vq[r - w0] = base | (2u << 26); ++r; }
-                    }
+    for (uint32_t w0 = 0; w0 < nv_all; w0 += Q) {
+        const uint32_t start = wrow(v, z, ws_v, 1)[0];
+        for (uint32_t k = sub; k < a.R && y0 + (int)k < (int)a.ny; k += G) {
+            for (uint32_t c = cl; c < a.NC; c += CS) {
+                const Cross e = window_cross<M>(a, v, z, y0, ws_v, k + 1, c);
+                uint64_t m = e.x | e.y | e.z;
+                uint32_t r = wrow(v, z, ws_v, k + 1)[c] - start;
+                while (m) {
+                    const uint32_t i = (uint32_t)__ffsll((unsigned long long)m) - 1u;
+                    m &= m - 1ull;
+                    const uint32_t base = (c * 64u + i) | (k << 20);
+                    if ((e.x >> i) & 1ull) { if (r >= w0 && r - w0 < Q) v.vq[r - w0] = base; ++r; }
+                    if ((e.y >> i) & 1ull) { if (r >= w0 && r - w0 < Q) v.vq[r - w0] = base | (1u << 26); ++r; }
+                    if ((e.z >> i) & 1ull) { if (r >= w0 && r - w0 < Q) v.vq[r - w0] = base | (2u << 26); ++r; }
                 }
             }
-            __syncthreads();
-            const uint32_t nvq = (nv_all - w0) < Q ? (nv_all - w0) : Q;
-            for (uint32_t q = threadIdx.x; q < nvq; q += kBlock) {
-                const uint32_t e = v.vq[q];
-                const uint32_t x = e & 0xFFFFFu, k = (e >> 20) & 0x3Fu, axis = (e >> 26) & 3u, l = e >> 28;
-                const LevelInfo& L = v.lev[l];
-                const uint64_t vid = L.vbase[ws_v == 1 ? 2 : 0] + (w0 + q - v.misc[kMiscQV + l]);
-                if (vid >= L.cap_v) continue;
-                const uint32_t y = (uint32_t)y0 + k;
-                float p[3];
-                node_edge_coords<T>(a, vol, L.iso, x, y, (uint32_t)zv, axis, p);
-                L.coords[3 * vid + 0] = p[0];
-                L.coords[3 * vid + 1] = p[1];
-                L.coords[3 * vid + 2] = p[2];
-                if constexpr (vn) {
-                    float nn[3];
-                    if (ring_ok) vertex_normal_ring(a, v, y0, x, y, zv, axis, l, nn);
-                    else vertex_normal_direct<T>(a, v, vol, L.iso, y0, x, y, zv, axis, l, nn);
-                    L.normals[3 * vid + 0] = nn[0];
-                    L.normals[3 * vid + 1] = nn[1];
-                    L.normals[3 * vid + 2] = nn[2];
-                }
-            }
-            __syncthreads();
         }
+        wave_sync();
+        const uint32_t nvq = (nv_all - w0) < Q ? (nv_all - w0) : Q;
+        for (uint32_t q = lane; q < nvq; q += 64u) {
+            const uint32_t e = v.vq[q];
+            const uint32_t x = e & 0xFFFFFu, k = (e >> 20) & 0x3Fu, axis = (e >> 26) & 3u;
+            const uint64_t vid = (ws_v == 1 ? B.v[2] : B.v[0]) + (uint64_t)(w0 + q);
+            if (vid >= L.cap_v) continue;
+            const uint32_t y = (uint32_t)y0 + k;
+            float p[3];
+            node_edge_coords<T>(a, vol, L.iso, x, y, (uint32_t)zv, axis, p);
+            L.coords[3 * vid + 0] = p[0];
+            L.coords[3 * vid + 1] = p[1];
+            L.coords[3 * vid + 2] = p[2];
+            if constexpr (vn) {
+                float nn[3];
+                if (ring_ok) vertex_normal_ring<M>(a, v, y0, x, y, zv, axis, nn);
+                else vertex_normal_direct<T>(a, v, L, vol, x, y, zv, axis, nn);
+                L.normals[3 * vid + 0] = nn[0];
+                L.normals[3 * vid + 1] = nn[1];
+                L.normals[3 * vid + 2] = nn[2];
+            }
+        }
+        wave_sync();
     }
 }
 
-/* Stage the case table and the per-level parameters in LDS. */
-__device__ void stage_tables(const JobArgs& a, const Views& v)
+/* Stage the case table in the workgroup's LDS (shared by its teams). */
+__device__ void stage_table(uint8_t* smem)
 {
-    for (uint32_t i = threadIdx.x; i < 256; i += kBlock) v.tri[i] = d_tri[i];
-    if (threadIdx.x < kMaxLevels) {
-        const uint32_t l = threadIdx.x;
-        LevelInfo L;
-        L.coords = nullptr; L.normals = nullptr; L.conn = nullptr;
-        L.cap_v = 0; L.cap_t = 0; L.iso = 0.0; L.tbase = 0;
-        L.vbase[0] = L.vbase[1] = L.vbase[2] = L.vbase[3] = 0;
-#pragma unroll
-        for (int m = 0; m < kMaxLevels; ++m) {
-            if (m == (int)l) {
-                L.coords = a.coords[m]; L.normals = a.normals[m]; L.conn = a.conn[m];
-                L.cap_v = a.cap_vertices[m]; L.cap_t = a.cap_triangles[m]; L.iso = a.iso[m];
-            }
-        }
-        v.lev[l] = L;
-    }
+    uint64_t* tri = (uint64_t*)smem;
+    for (uint32_t i = threadIdx.x; i < 256; i += kEmitBlock) tri[i] = d_tri[i];
+    __syncthreads();
 }
 
 /* ------------------------------------------------------------------------- */
@@ -1071,97 +1125,93 @@ __global__ __launch_bounds__(kBlock) void k_count_batch(const JobArgs* jobs, con
 /* emit pass                                                                   */
 /* ------------------------------------------------------------------------- */
 
-/* Output bases of tile (z, band): vertex prefixes of node rows (z-1, y0),
- * (z-1, y0+R), (z, y0), (z, y0+R) and the triangle prefix of cell row
- * (y0, z-1), relative to the first row of their (level, kind) segment. */
-__device__ __forceinline__ void tile_bases(const JobArgs& a, const Views& v, int z, int y0)
-{
-    if (threadIdx.x < a.nlev) {
-        const uint32_t l = threadIdx.x;
-        const uint64_t* pre_v = a.prefix + (uint64_t)(l * 2 + 0) * a.nrows;
-        const uint64_t* pre_t = a.prefix + (uint64_t)(l * 2 + 1) * a.nrows;
-        const bool in = z < (int)a.nz;
-        const uint64_t row = in ? (uint64_t)z * a.ny + (uint64_t)y0 : 0ull;
-        const bool next = (uint32_t)y0 + a.R < a.ny;
-        const uint64_t bv = pre_v[0], bt = pre_t[0];
-        LevelInfo& L = v.lev[l];
-        L.tbase = in ? pre_t[row] - bt : 0;
-        L.vbase[2] = in ? pre_v[row] - bv : 0;
-        L.vbase[3] = (in && next) ? pre_v[row + a.R] - bv : 0;
-        const uint64_t prow = (uint64_t)(z - 1) * a.ny + (uint64_t)y0;
-        L.vbase[0] = z > 0 ? pre_v[prow] - bv : 0;
-        L.vbase[1] = (z > 0 && next) ? pre_v[prow + a.R] - bv : 0;
-    }
-}
-
-/* One unit: band b over steps zs .. ze-1 (with VertexNormal one more step
- * after the last node slice).  Tile z emits the triangles of cell slice z-1
- * and the vertices of node slice z, or of node slice z-1 with VertexNormal:
- * their normals sum face normals of cell slices z-2 and z-1, both then in
- * the face-normal ring (a first unit step without output fills in z-2).
- * The mask ring is filled once at the start and takes one new slice per
- * step. */
+/* One unit: level l of band b over steps zs .. ze-1 (with VertexNormal one
+ * more step after the last node slice).  Tile z emits the triangles of cell
+ * slice z-1 and the vertices of node slice z, or of node slice z-1 with
+ * VertexNormal: their normals sum face normals of cell slices z-2 and z-1,
+ * both then in the face-normal ring (a first unit step without output fills
+ * in z-2).  The mask ring holds node slices z-1 .. z+1 and takes one new
+ * slice per step. */
 template <typename T, int M>
-__device__ void emit_unit(const JobArgs& a, const Views& v, uint64_t unit)
+__device__ void emit_unit(const JobArgs& a, const Team& v, uint64_t unit)
 {
     const T* vol = (const T*)a.values;
-    const uint32_t slab = (uint32_t)(unit / a.nb);
-    const uint32_t band = (uint32_t)(unit - (uint64_t)slab * a.nb);
+    const uint64_t per_level = (uint64_t)a.nb * a.nslabs;
+    const uint32_t l = (uint32_t)(unit / per_level);
+    const uint64_t rem = unit - (uint64_t)l * per_level;
+    const uint32_t slab = (uint32_t)(rem / a.nb);
+    const uint32_t band = (uint32_t)(rem - (uint64_t)slab * a.nb);
     const int y0 = (int)(band * a.R);
     const int zs = (int)(slab * a.slab);
     const int ze = min(zs + (int)a.slab, (int)a.nz);
     constexpr bool vn = M == kModeVertex;
     const int zend = (vn && ze == (int)a.nz) ? ze + 1 : ze;
     const uint32_t ws_v = vn ? 0u : 1u;
+    const Level L = level_of(a, l);
 
-    load_slices(a, v, vn ? zs - 2 : zs - 1, zs + 1, y0);
-    if (threadIdx.x < 2) v.misc[kMiscFnOvf + threadIdx.x] = 0u;
-    __syncthreads();
+    /* VertexNormal first prepares cell slice zs-2 (node slices zs-2, zs-1) */
+    load_slices<M>(a, v, L, vn ? zs - 2 : zs - 1, vn ? zs : zs + 1, y0);
+    uint32_t ovf = 0;   // bit cs: the face normals of cell slot cs did not fit
+    wave_sync();
     if (vn && zs >= 2) {
-        tile_prefixes<M>(a, v, zs - 1, y0, ws_v, false, true);
-        emit_tile<T, M>(a, v, vol, zs - 1, y0, ws_v, false, false);
-        __syncthreads();
+        const TileTotals t = tile_prefixes<M>(a, v, zs - 1, y0, ws_v, false, true);
+        const bool keep = t.nt <= v.fn_cap;
+        ovf = keep ? 0u : (1u << cslot<M>(zs - 2));
+        Bases B;
+        B.v[0] = B.v[1] = B.v[2] = B.v[3] = 0;
+        B.t = 0;
+        emit_tile<T, M>(a, v, L, B, vol, zs - 1, y0, ws_v, false, false, t, keep, false);
+        wave_sync();
     }
     for (int z = zs; z < zend; ++z) {
-        if (z > zs) load_slices(a, v, z + 1, z + 1, y0);
-        tile_bases(a, v, z, y0);
-        __syncthreads();
-        tile_prefixes<M>(a, v, z, y0, ws_v, true, z == zs);
-        emit_tile<T, M>(a, v, vol, z, y0, ws_v, true, true);
-        __syncthreads();   // the next step overwrites a mask slot and the cell data
+        if (vn || z > zs) load_slices<M>(a, v, L, z + 1, z + 1, y0);
+        const Bases B = tile_bases(a, L, z, y0);
+        wave_sync();
+        const TileTotals t = tile_prefixes<M>(a, v, z, y0, ws_v, true, z == zs);
+        bool keep = false;
+        if (vn) {
+            const uint32_t bit = 1u << cslot<M>(z - 1);
+            keep = t.nt <= v.fn_cap;
+            ovf = keep ? (ovf & ~bit) : (ovf | bit);
+        }
+        emit_tile<T, M>(a, v, L, B, vol, z, y0, ws_v, true, true, t, keep, ovf == 0);
+        wave_sync();   // the next step overwrites a mask slot and the cell data
     }
 }
 
-/* Persistent blocks: each block stages the tables once and takes units
- * blockIdx.x, blockIdx.x + gridDim.x, ...; consecutive blocks take
- * neighbouring bands of the same slab. */
+/* Persistent workgroups of `teams` one-wave teams (the other waves only help
+ * stage the table): team w of workgroup g takes units g * teams + w, then
+ * strides by all teams of the grid; neighbouring teams take neighbouring
+ * bands of the same slab and level. */
 template <typename T, int M>
-__global__ __launch_bounds__(kBlock) void k_emit(JobArgs a)
+__global__ __launch_bounds__(kEmitBlock) void k_emit(JobArgs a)
 {
     extern __shared__ __align__(16) uint8_t smem[];
-    const Views v = make_views(a, smem);
-    stage_tables(a, v);
-    for (uint64_t u = blockIdx.x; u < a.nunits; u += gridDim.x) emit_unit<T, M>(a, v, u);
+    stage_table(smem);
+    const uint32_t wave = uniform(threadIdx.x >> 6);
+    if (wave >= a.teams) return;
+    const Team v = make_team(a, smem, wave);
+    const uint64_t step = (uint64_t)gridDim.x * a.teams;
+    for (uint64_t u = (uint64_t)blockIdx.x * a.teams + wave; u < a.nunits; u += step) emit_unit<T, M>(a, v, u);
 }
 
-/* Emit pass of a batch of jobs sharing the value type T: the units of all
- * jobs in one sequence, taken by persistent blocks; the tables are staged
- * again when a block moves to another job. */
+/* Emit pass of a batch of jobs sharing the value type T and the mode: the
+ * units of all jobs in one sequence, taken by persistent teams.  Every job of
+ * the launch has the same team_stride. */
 template <typename T, int M>
-__global__ __launch_bounds__(kBlock) void k_emit_batch(const JobArgs* jobs, const uint64_t* unit_start, uint32_t njobs)
+__global__ __launch_bounds__(kEmitBlock) void k_emit_batch(const JobArgs* jobs, const uint64_t* unit_start, uint32_t njobs,
+                                                            uint32_t teams)
 {
     extern __shared__ __align__(16) uint8_t smem[];
+    stage_table(smem);
+    const uint32_t wave = uniform(threadIdx.x >> 6);
+    if (wave >= teams) return;
     const uint64_t total = unit_start[njobs];
-    uint32_t staged = ~0u;
-    for (uint64_t g = blockIdx.x; g < total; g += gridDim.x) {
-        const uint32_t j = find_job(unit_start, njobs, g);
+    const uint64_t step = (uint64_t)gridDim.x * teams;
+    for (uint64_t g = (uint64_t)blockIdx.x * teams + wave; g < total; g += step) {
+        const uint32_t j = uniform(find_job(unit_start, njobs, g));
         const JobArgs& a = jobs[j];
-        const Views v = make_views(a, smem);
-        if (j != staged) {
-            __syncthreads();
-            stage_tables(a, v);
-            staged = j;
-        }
+        const Team v = make_team(a, smem, wave);
         emit_unit<T, M>(a, v, g - unit_start[j]);
     }
 }
@@ -1398,8 +1448,8 @@ __global__ __launch_bounds__(kBlock) void k_minmax_final(const T* vol, const Ext
 /* host-side launch wrappers                                                   */
 /* ------------------------------------------------------------------------- */
 
-/* Blocks that fit on the device at once for this LDS size (persistent grid);
- * remembered per (kernel, LDS size). */
+/* Emit workgroups that fit on the device at once for this LDS size
+ * (persistent grid); remembered per (kernel, LDS size). */
 template <typename K>
 uint32_t resident_blocks(K kernel, uint32_t lds_bytes)
 {
@@ -1418,7 +1468,7 @@ uint32_t resident_blocks(K kernel, uint32_t lds_bytes)
         cus = 256;
     }
     int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kBlock, lds_bytes) != hipSuccess || per_cu <= 0) {
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kEmitBlock, lds_bytes) != hipSuccess || per_cu <= 0) {
         (void)hipGetLastError();
         per_cu = 1;
     }
@@ -1431,20 +1481,21 @@ uint32_t resident_blocks(K kernel, uint32_t lds_bytes)
 template <typename T>
 hipError_t launch_emit_t(const JobArgs& a, uint32_t lds_bytes, hipStream_t st)
 {
+    const uint64_t teams = (a.nunits + a.teams - 1) / a.teams;
     switch (emit_mode(a)) {
     case kModeDup: {
-        const uint64_t grid = std::min<uint64_t>(a.nunits, resident_blocks(k_emit<T, kModeDup>, lds_bytes));
-        hipLaunchKernelGGL((k_emit<T, kModeDup>), dim3((uint32_t)grid), dim3(kBlock), lds_bytes, st, a);
+        const uint64_t grid = std::min<uint64_t>(teams, resident_blocks(k_emit<T, kModeDup>, lds_bytes));
+        hipLaunchKernelGGL((k_emit<T, kModeDup>), dim3((uint32_t)grid), dim3(kEmitBlock), lds_bytes, st, a);
         break;
     }
     case kModePolygon: {
-        const uint64_t grid = std::min<uint64_t>(a.nunits, resident_blocks(k_emit<T, kModePolygon>, lds_bytes));
-        hipLaunchKernelGGL((k_emit<T, kModePolygon>), dim3((uint32_t)grid), dim3(kBlock), lds_bytes, st, a);
+        const uint64_t grid = std::min<uint64_t>(teams, resident_blocks(k_emit<T, kModePolygon>, lds_bytes));
+        hipLaunchKernelGGL((k_emit<T, kModePolygon>), dim3((uint32_t)grid), dim3(kEmitBlock), lds_bytes, st, a);
         break;
     }
     default: {
-        const uint64_t grid = std::min<uint64_t>(a.nunits, resident_blocks(k_emit<T, kModeVertex>, lds_bytes));
-        hipLaunchKernelGGL((k_emit<T, kModeVertex>), dim3((uint32_t)grid), dim3(kBlock), lds_bytes, st, a);
+        const uint64_t grid = std::min<uint64_t>(teams, resident_blocks(k_emit<T, kModeVertex>, lds_bytes));
+        hipLaunchKernelGGL((k_emit<T, kModeVertex>), dim3((uint32_t)grid), dim3(kEmitBlock), lds_bytes, st, a);
         break;
     }
     }
@@ -1547,39 +1598,41 @@ hipError_t launch_count_batch(const JobArgs* jobs, const uint64_t* row_start, ui
 
 template <typename T, int M>
 hipError_t launch_emit_batch_m(const JobArgs* jobs, const uint64_t* unit_start, uint32_t njobs, uint64_t units,
-                               uint32_t lds_bytes, hipStream_t st)
+                               uint32_t lds_bytes, uint32_t teams, hipStream_t st)
 {
-    const uint64_t grid = std::min<uint64_t>(units, resident_blocks(k_emit_batch<T, M>, lds_bytes));
-    hipLaunchKernelGGL((k_emit_batch<T, M>), dim3((uint32_t)grid), dim3(kBlock), lds_bytes, st, jobs, unit_start, njobs);
+    const uint64_t grid = std::min<uint64_t>((units + teams - 1) / teams,
+                                             resident_blocks(k_emit_batch<T, M>, lds_bytes));
+    hipLaunchKernelGGL((k_emit_batch<T, M>), dim3((uint32_t)grid), dim3(kEmitBlock), lds_bytes, st, jobs, unit_start,
+                       njobs, teams);
     return hipGetLastError();
 }
 
 /* Every pass of a batch group shares the value type T and the emit mode. */
 template <typename T>
 hipError_t launch_emit_batch_t(int mode, const JobArgs* jobs, const uint64_t* unit_start, uint32_t njobs,
-                               uint64_t units, uint32_t lds_bytes, hipStream_t st)
+                               uint64_t units, uint32_t lds_bytes, uint32_t teams, hipStream_t st)
 {
     switch (mode) {
-    case kModeDup: return launch_emit_batch_m<T, kModeDup>(jobs, unit_start, njobs, units, lds_bytes, st);
-    case kModePolygon: return launch_emit_batch_m<T, kModePolygon>(jobs, unit_start, njobs, units, lds_bytes, st);
-    default: return launch_emit_batch_m<T, kModeVertex>(jobs, unit_start, njobs, units, lds_bytes, st);
+    case kModeDup: return launch_emit_batch_m<T, kModeDup>(jobs, unit_start, njobs, units, lds_bytes, teams, st);
+    case kModePolygon: return launch_emit_batch_m<T, kModePolygon>(jobs, unit_start, njobs, units, lds_bytes, teams, st);
+    default: return launch_emit_batch_m<T, kModeVertex>(jobs, unit_start, njobs, units, lds_bytes, teams, st);
     }
 }
 
 hipError_t launch_emit_batch(int dtype, int mode, const JobArgs* jobs, const uint64_t* unit_start, uint32_t njobs,
-                             uint64_t units, uint32_t lds_bytes, hipStream_t st)
+                             uint64_t units, uint32_t lds_bytes, uint32_t teams, hipStream_t st)
 {
     switch (dtype) {
-    case 0: return launch_emit_batch_t<int8_t>(mode, jobs, unit_start, njobs, units, lds_bytes, st);
-    case 1: return launch_emit_batch_t<int16_t>(mode, jobs, unit_start, njobs, units, lds_bytes, st);
-    case 2: return launch_emit_batch_t<int32_t>(mode, jobs, unit_start, njobs, units, lds_bytes, st);
-    case 3: return launch_emit_batch_t<int64_t>(mode, jobs, unit_start, njobs, units, lds_bytes, st);
-    case 4: return launch_emit_batch_t<uint8_t>(mode, jobs, unit_start, njobs, units, lds_bytes, st);
-    case 5: return launch_emit_batch_t<uint16_t>(mode, jobs, unit_start, njobs, units, lds_bytes, st);
-    case 6: return launch_emit_batch_t<uint32_t>(mode, jobs, unit_start, njobs, units, lds_bytes, st);
-    case 7: return launch_emit_batch_t<uint64_t>(mode, jobs, unit_start, njobs, units, lds_bytes, st);
-    case 8: return launch_emit_batch_t<float>(mode, jobs, unit_start, njobs, units, lds_bytes, st);
-    default: return launch_emit_batch_t<double>(mode, jobs, unit_start, njobs, units, lds_bytes, st);
+    case 0: return launch_emit_batch_t<int8_t>(mode, jobs, unit_start, njobs, units, lds_bytes, teams, st);
+    case 1: return launch_emit_batch_t<int16_t>(mode, jobs, unit_start, njobs, units, lds_bytes, teams, st);
+    case 2: return launch_emit_batch_t<int32_t>(mode, jobs, unit_start, njobs, units, lds_bytes, teams, st);
+    case 3: return launch_emit_batch_t<int64_t>(mode, jobs, unit_start, njobs, units, lds_bytes, teams, st);
+    case 4: return launch_emit_batch_t<uint8_t>(mode, jobs, unit_start, njobs, units, lds_bytes, teams, st);
+    case 5: return launch_emit_batch_t<uint16_t>(mode, jobs, unit_start, njobs, units, lds_bytes, teams, st);
+    case 6: return launch_emit_batch_t<uint32_t>(mode, jobs, unit_start, njobs, units, lds_bytes, teams, st);
+    case 7: return launch_emit_batch_t<uint64_t>(mode, jobs, unit_start, njobs, units, lds_bytes, teams, st);
+    case 8: return launch_emit_batch_t<float>(mode, jobs, unit_start, njobs, units, lds_bytes, teams, st);
+    default: return launch_emit_batch_t<double>(mode, jobs, unit_start, njobs, units, lds_bytes, teams, st);
     }
 }
 
