@@ -388,6 +388,8 @@ void object_coords(const kmc_volume* v, float minc[3], float maxc[3])
     }
 }
 
+static inline size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
 /* Per-job fields of a pass of up to kMaxLevels levels. */
 void init_job(JobArgs& a, const kmc_volume* v, const void* dvals, const double* iso, uint32_t nlev, int normal_type,
               int dup, double mn, double mx, const float minc[3], const float maxc[3])
@@ -466,13 +468,24 @@ kmc_status run_pass(const kmc_volume* v, const void* dvals, const double* iso, u
     uint32_t* counts = (uint32_t*)device_alloc(counts_bytes);
     uint64_t* prefix = (uint64_t*)device_alloc(prefix_bytes);
     uint64_t* aux = (uint64_t*)device_alloc((size_t)(nparts + narr) * sizeof(uint64_t));
-    const size_t bitvol_bytes = (size_t)nlev * a.nz * a.ny * a.NC * sizeof(uint64_t);
-    uint64_t* bitvol = (uint64_t*)device_alloc(bitvol_bytes);
+    const size_t chunks = (size_t)nlev * a.nz * a.ny * a.NC;
+    uint64_t* bitvol = (uint64_t*)device_alloc(chunks * sizeof(uint64_t));
+    /* flat emit (compaction records + one thread per item) unless the records'
+     * 20-bit y / z fields do not fit or KMC_EMIT=team asks for the tile teams */
+    static const char* env_emit = std::getenv("KMC_EMIT");
+    const bool flat = a.ny <= (1u << 20) && a.nz <= (1u << 20) && !(env_emit && std::strcmp(env_emit, "team") == 0);
+    uint32_t* cpre = flat ? (uint32_t*)device_alloc(2 * chunks * sizeof(uint32_t)) : nullptr;
+    void* recs = nullptr;
     auto cleanup = [&]() {
-        device_free(counts); device_free(prefix); device_free(aux); device_free(bitvol);
+        device_free(counts); device_free(prefix); device_free(aux); device_free(bitvol); device_free(cpre);
+        device_free(recs);
     };
-    if (!counts || !prefix || !aux || !bitvol) { cleanup(); return KMC_ERR_OUT_OF_MEMORY; }
+    if (!counts || !prefix || !aux || !bitvol || (flat && !cpre)) { cleanup(); return KMC_ERR_OUT_OF_MEMORY; }
     a.bitvol = bitvol;
+    if (flat) {
+        a.vpc = cpre;
+        a.tpc = cpre + chunks;
+    }
     uint64_t* partials = aux;
     uint64_t* totals = aux + nparts;
     a.counts = counts;
@@ -529,11 +542,34 @@ kmc_status run_pass(const kmc_volume* v, const void* dvals, const double* iso, u
         a.cap_vertices[l] = verts;
         a.cap_triangles[l] = nt;
     }
+    if (status == KMC_OK && flat) {
+        /* records: vertex keys, triangle records and cube indices, face normals */
+        a.voff[0] = a.toff[0] = 0;
+        for (uint32_t l = 0; l < nlev; ++l) {
+            a.voff[l + 1] = a.voff[l] + (dup ? 0 : tot[l * 2 + 0]);
+            a.toff[l + 1] = a.toff[l] + tot[l * 2 + 1];
+        }
+        const uint64_t nv = a.voff[nlev], nt = a.toff[nlev];
+        const size_t bytes = align_up(nv * 8, 256) + align_up(nt * 8, 256) + align_up(nt, 256) +
+                             (a.vnormal ? nt * 12 : 0);
+        if (bytes) {
+            recs = device_alloc(bytes);
+            if (!recs) status = KMC_ERR_OUT_OF_MEMORY;
+        }
+        if (recs) {
+            uint8_t* p = (uint8_t*)recs;
+            a.vkey = (uint64_t*)p; p += align_up(nv * 8, 256);
+            a.trec = (uint64_t*)p; p += align_up(nt * 8, 256);
+            a.tci = p; p += align_up(nt, 256);
+            a.fn = a.vnormal ? (float*)p : nullptr;
+        }
+    }
     if (status == KMC_OK) {
         bool any = false;
         for (uint32_t l = 0; l < nlev; ++l) any = any || outs[l].n_vertices > 0;
         timer.mark(4);
-        if (any && launch_emit(v->type, a, lds, st) != hipSuccess) status = KMC_ERR_DEVICE;
+        if (any && (flat ? launch_flat_emit(v->type, a, st) : launch_emit(v->type, a, lds, st)) != hipSuccess)
+            status = KMC_ERR_DEVICE;
         timer.mark(5);
         if (status == KMC_OK && hipStreamSynchronize(st) != hipSuccess) status = KMC_ERR_DEVICE;
         if (status == KMC_OK) timer.accumulate();

@@ -56,6 +56,10 @@ uint32_t levels_per_pass(const kmc_volume* v, int vnormal);
 
 /* Kernel launchers (kmc_kernels.hip). */
 hipError_t launch_emit(int dtype, const JobArgs& a, uint32_t lds_bytes, hipStream_t st);
+/* Flat emit of a pass (a.vpc / a.tpc written by its count pass, a.vkey /
+ * a.trec / a.tci / a.fn and a.voff / a.toff set): compaction records, then
+ * one thread per vertex, per triangle and (VertexNormal) per vertex normal. */
+hipError_t launch_flat_emit(int dtype, const JobArgs& a, hipStream_t st);
 hipError_t launch_classify(int dtype, const JobArgs& a, hipStream_t st);
 hipError_t launch_count(const JobArgs& a, hipStream_t st);
 hipError_t launch_scan(const uint32_t* counts, uint64_t n, uint32_t narr, uint64_t* partials,

@@ -78,6 +78,21 @@ struct JobArgs {
     // classification bit volume: bitvol[((l * nz + z) * ny + y) * NC + c] is the
     // 64-bit mask of nodes x in [64c, 64c+64) of row y, slice z, level l
     uint64_t* bitvol;
+    // optional, same layout as bitvol: per chunk the exclusive prefix within its
+    // row of the vertex counts (vpc) and of the triangle counts of cell row
+    // (y, z-1) (tpc); written by the count pass when non-null
+    uint32_t* vpc;
+    uint32_t* tpc;
+    // records of the compaction pass (flat emit): per vertex, in output order,
+    // x | y << 20 | z << 40 | axis << 60; per triangle x | y << 20 | z << 40 |
+    // t << 60 of its cell and the cell's cube index; per triangle its face normal
+    // (VertexNormal); level l's items start at voff[l] / toff[l]
+    uint64_t* vkey;
+    uint64_t* trec;
+    uint8_t* tci;
+    float* fn;
+    uint64_t voff[kMaxLevels + 1];
+    uint64_t toff[kMaxLevels + 1];
     // per-level outputs
     float* coords[kMaxLevels];
     float* normals[kMaxLevels];

@@ -992,14 +992,76 @@ __global__ __launch_bounds__(kBlock) void k_classify_batch(const JobArgs* jobs, 
 /* count pass: per-row counts from the bit volume                              */
 /* ------------------------------------------------------------------------- */
 
-/* Per-row counts of tile (z, b): for node row y of slice z the crossing edges
- * it owns, and for cell row (y, z-1) its triangles.  LDS holds only the chunk
- * masks of slices z-1, z, z+1 (rows y0-1 .. y0+R+1) and the case table; a
- * wave takes one (level, row) at a time with its lanes on the row's chunks. */
+/* Crossing edges and active cells of chunk c of node row (y, z) of one level,
+ * read straight from the bit volume `lev` (coalesced across the lanes of a
+ * row, neighbours from L1/L2).  x/y/z: crossing edges owned by the row's
+ * nodes; act: active cells of cell row (y, z-1) with their eight corner
+ * masks (bit i of k[q] is corner q of cell x = 64c + i). */
+struct RowChunk {
+    uint64_t x, y, z, act;
+    uint64_t k[8];
+};
+
+__device__ __forceinline__ void row_chunk(const JobArgs& a, const uint64_t* lev, uint32_t y, uint32_t z, uint32_t c,
+                                          bool cells, RowChunk& o)
+{
+    const uint64_t slice_words = (uint64_t)a.ny * a.NC;
+    const bool has_up = y + 1 < a.ny, has_next = z + 1 < a.nz, has_cells = cells && z >= 1 && has_up;
+    const uint64_t* row = lev + (uint64_t)z * slice_words + (uint64_t)y * a.NC;
+    const uint64_t* up = row + a.NC;
+    const uint64_t* nz = row + slice_words;
+    const bool nxt = c + 1 < a.NC;
+    const uint64_t b = row[c];
+    const uint64_t b1 = nxt ? row[c + 1] : 0ull;
+    const uint64_t u = has_up ? up[c] : 0ull;
+    const uint64_t u1 = (has_up && nxt) ? up[c + 1] : 0ull;
+    const uint64_t n0 = has_next ? nz[c] : 0ull;
+    const uint64_t c4 = b, c5 = (b >> 1) | (b1 << 63);
+    const uint64_t c7 = u, c6 = (u >> 1) | (u1 << 63);
+    o.x = (b ^ c5) & valid_mask(c, a.nx - 1);
+    o.y = has_up ? ((b ^ u) & valid_mask(c, a.nx)) : 0ull;
+    o.z = has_next ? ((b ^ n0) & valid_mask(c, a.nx)) : 0ull;
+    o.act = 0ull;
+    if (has_cells) {
+        const uint64_t* pr = row - slice_words;
+        const uint64_t* pu = pr + a.NC;
+        const uint64_t p = pr[c], p1 = nxt ? pr[c + 1] : 0ull;
+        const uint64_t q = pu[c], q1 = nxt ? pu[c + 1] : 0ull;
+        o.k[0] = p; o.k[1] = (p >> 1) | (p1 << 63);
+        o.k[3] = q; o.k[2] = (q >> 1) | (q1 << 63);
+        o.k[4] = c4; o.k[5] = c5; o.k[6] = c6; o.k[7] = c7;
+        uint64_t or_ = 0, an = ~0ull;
+#pragma unroll
+        for (int s = 0; s < 8; ++s) { or_ |= o.k[s]; an &= o.k[s]; }
+        o.act = (or_ & ~an) & valid_mask(c, a.nx - 1);
+    }
+}
+
+__device__ __forceinline__ uint32_t corner_bits(const uint64_t k[8], uint32_t i)
+{
+    uint32_t ci = 0;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) ci |= (uint32_t)((k[q] >> i) & 1ull) << q;
+    return ci;
+}
+
+/* Exclusive scan of x over the CS lanes of each row group (CS a power of two). */
+__device__ __forceinline__ uint32_t group_excl_scan(uint32_t x, uint32_t cl, uint32_t CS, uint32_t& total)
+{
+    uint32_t inc = x;
+    for (uint32_t d = 1; d < CS; d <<= 1) {
+        const uint32_t t = __shfl_up(inc, d, (int)CS);
+        if (cl >= d) inc += t;
+    }
+    total = __shfl(inc, (int)CS - 1, (int)CS);
+    return inc - x;
+}
+
 /* Counts of node rows [r0, r1) of a job: one row group of 64 >> cshift rows
- * per wave iteration, one lane per 64-node chunk; masks are read straight
- * from the bit volume (coalesced, neighbours from L1/L2), so there is no LDS
- * window and no barrier.  ntri: triangle count per cube index (LDS). */
+ * per wave iteration, one lane per 64-node chunk, no LDS window and no
+ * barrier.  Per row: the crossing edges its nodes own and the triangles of
+ * cell row (y, z-1); with a.vpc / a.tpc also each chunk's exclusive prefix of
+ * both within its row.  ntri: triangle count per cube index (LDS). */
 template <int NLEV>
 __device__ __forceinline__ void count_rows_t(const JobArgs& a, const uint8_t* ntri, uint64_t r0, uint64_t r1,
                                              uint64_t wave_first, uint64_t wave_step)
@@ -1007,62 +1069,54 @@ __device__ __forceinline__ void count_rows_t(const JobArgs& a, const uint8_t* nt
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t sub = lane >> a.cshift, cl = lane & ((1u << a.cshift) - 1u);
     const uint32_t G = 64u >> a.cshift, CS = 1u << a.cshift;
-    const uint64_t slice_words = (uint64_t)a.ny * a.NC;
-    const uint64_t level_words = (uint64_t)a.nz * slice_words;
+    const uint64_t level_words = (uint64_t)a.nz * a.ny * a.NC;
+    const bool chunk_prefixes = a.vpc != nullptr;
     for (uint64_t g = r0 + wave_first * G; g < r1; g += wave_step * G) {
         const uint64_t r = g + sub;
         const bool ok = r < r1;
         const uint32_t z = ok ? (uint32_t)(r / a.ny) : 0u;
         const uint32_t y = ok ? (uint32_t)(r - (uint64_t)z * a.ny) : 0u;
-        const bool has_up = y + 1 < a.ny, has_next = z + 1 < a.nz, has_cells = z >= 1 && has_up;
         const uint32_t nl = NLEV ? (uint32_t)NLEV : a.nlev;
 #pragma unroll
         for (uint32_t l = 0; l < nl; ++l) {
+            const uint64_t* lev = a.bitvol + (uint64_t)l * level_words;
             uint32_t vc = 0, tc = 0;
-            if (ok) {
-                const uint64_t* lev = a.bitvol + (uint64_t)l * level_words;
-                const uint64_t* row = lev + (uint64_t)z * slice_words + (uint64_t)y * a.NC;
-                const uint64_t* up = row + a.NC;
-                const uint64_t* nz = row + slice_words;
-                const uint64_t* pr = row - slice_words;
-                const uint64_t* pu = pr + a.NC;
-                for (uint32_t c = cl; c < a.NC; c += CS) {
-                    const bool nxt = c + 1 < a.NC;
-                    const uint64_t b = row[c];
-                    const uint64_t b1 = nxt ? row[c + 1] : 0ull;
-                    const uint64_t u = has_up ? up[c] : 0ull;
-                    const uint64_t u1 = (has_up && nxt) ? up[c + 1] : 0ull;
-                    const uint64_t n0 = has_next ? nz[c] : 0ull;
-                    const uint64_t c4 = b, c5 = (b >> 1) | (b1 << 63);
-                    const uint64_t c7 = u, c6 = (u >> 1) | (u1 << 63);
-                    const uint64_t X = (b ^ c5) & valid_mask(c, a.nx - 1);
-                    const uint64_t Y = has_up ? ((b ^ u) & valid_mask(c, a.nx)) : 0ull;
-                    const uint64_t Zm = has_next ? ((b ^ n0) & valid_mask(c, a.nx)) : 0ull;
-                    vc += (uint32_t)(__popcll(X) + __popcll(Y) + __popcll(Zm));
-                    if (has_cells) {
-                        const uint64_t p = pr[c], p1 = nxt ? pr[c + 1] : 0ull;
-                        const uint64_t q = pu[c], q1 = nxt ? pu[c + 1] : 0ull;
-                        const uint64_t c0 = p, c1 = (p >> 1) | (p1 << 63);
-                        const uint64_t c3 = q, c2 = (q >> 1) | (q1 << 63);
-                        const uint64_t o = c0 | c1 | c2 | c3 | c4 | c5 | c6 | c7;
-                        const uint64_t an = c0 & c1 & c2 & c3 & c4 & c5 & c6 & c7;
-                        uint64_t m = (o & ~an) & valid_mask(c, a.nx - 1);
-                        while (m) {
-                            const uint32_t i = (uint32_t)__ffsll((unsigned long long)m) - 1u;
-                            m &= m - 1ull;
-                            const uint32_t ci = (uint32_t)((c0 >> i) & 1ull) | (uint32_t)((c1 >> i) & 1ull) << 1 |
-                                                (uint32_t)((c2 >> i) & 1ull) << 2 | (uint32_t)((c3 >> i) & 1ull) << 3 |
-                                                (uint32_t)((c4 >> i) & 1ull) << 4 | (uint32_t)((c5 >> i) & 1ull) << 5 |
-                                                (uint32_t)((c6 >> i) & 1ull) << 6 | (uint32_t)((c7 >> i) & 1ull) << 7;
-                            tc += ntri[ci];
-                        }
+            for (uint32_t c0 = 0; c0 < a.NC; c0 += CS) {
+                const uint32_t c = c0 + cl;
+                uint32_t cv = 0, ct = 0;
+                if (ok && c < a.NC) {
+                    RowChunk o;
+                    row_chunk(a, lev, y, z, c, true, o);
+                    cv = (uint32_t)(__popcll(o.x) + __popcll(o.y) + __popcll(o.z));
+                    uint64_t m = o.act;
+                    while (m) {
+                        const uint32_t i = (uint32_t)__ffsll((unsigned long long)m) - 1u;
+                        m &= m - 1ull;
+                        ct += ntri[corner_bits(o.k, i)];
                     }
                 }
+                if (chunk_prefixes) {
+                    uint32_t sv, st;
+                    const uint32_t ev = group_excl_scan(cv, cl, CS, sv);
+                    const uint32_t et = group_excl_scan(ct, cl, CS, st);
+                    if (ok && c < a.NC) {
+                        const uint64_t w = (uint64_t)l * level_words + r * a.NC + c;
+                        a.vpc[w] = vc + ev;
+                        a.tpc[w] = tc + et;
+                    }
+                    vc += sv;
+                    tc += st;
+                } else {
+                    vc += cv;
+                    tc += ct;
+                }
             }
-            /* sum over the CS lanes of each row group */
-            for (uint32_t d = CS >> 1; d >= 1; d >>= 1) {
-                vc += __shfl_xor(vc, (int)d);
-                tc += __shfl_xor(tc, (int)d);
+            if (!chunk_prefixes) {
+                /* sum over the CS lanes of each row group */
+                for (uint32_t d = CS >> 1; d >= 1; d >>= 1) {
+                    vc += __shfl_xor(vc, (int)d);
+                    tc += __shfl_xor(tc, (int)d);
+                }
             }
             if (ok && cl == 0) {
                 a.counts[(uint64_t)(l * 2 + 0) * a.nrows + r] = vc;
@@ -1213,6 +1267,281 @@ __global__ __launch_bounds__(kEmitBlock) void k_emit_batch(const JobArgs* jobs, 
         const JobArgs& a = jobs[j];
         const Team v = make_team(a, smem, wave);
         emit_unit<T, M>(a, v, g - unit_start[j]);
+    }
+}
+
+/* ------------------------------------------------------------------------- */
+/* flat emit: compaction records, then one thread per output item              */
+/* ------------------------------------------------------------------------- */
+
+/* Level of output item i given the per-level start offsets (nlev <= 8). */
+__device__ __forceinline__ uint32_t level_at(const uint64_t* off, uint32_t nlev, uint64_t i)
+{
+    uint32_t l = 0;
+    for (uint32_t k = 1; k < nlev; ++k) l += i >= off[k] ? 1u : 0u;
+    return l;
+}
+
+/* Compaction: every crossing edge writes its key at its vertex id and every
+ * triangle its cell and table index t (plus the cell's cube index) at its
+ * triangle id -- row prefix + chunk prefix within the row + rank within the
+ * chunk, i.e. vertices in node scan order x {x,y,z}-edge and triangles in
+ * cell scan order x table order (calculate_isopoints :365-437,
+ * connect_isopoints :439-521, extract_surfaces_with_duplication :155-243).
+ * Same row groups and lanes as the count pass. */
+__device__ __forceinline__ void compact_rows(const JobArgs& a, const uint8_t* ntri, uint64_t r0, uint64_t r1,
+                                             uint64_t wave_first, uint64_t wave_step)
+{
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t sub = lane >> a.cshift, cl = lane & ((1u << a.cshift) - 1u);
+    const uint32_t G = 64u >> a.cshift, CS = 1u << a.cshift;
+    const uint64_t level_words = (uint64_t)a.nz * a.ny * a.NC;
+    const bool verts = !a.dup;
+    for (uint64_t g = r0 + wave_first * G; g < r1; g += wave_step * G) {
+        const uint64_t r = g + sub;
+        if (r >= r1) continue;
+        const uint32_t z = (uint32_t)(r / a.ny);
+        const uint32_t y = (uint32_t)(r - (uint64_t)z * a.ny);
+        for (uint32_t l = 0; l < a.nlev; ++l) {
+            const uint64_t* lev = a.bitvol + (uint64_t)l * level_words;
+            const uint64_t* pre_v = a.prefix + (uint64_t)(l * 2 + 0) * a.nrows;
+            const uint64_t* pre_t = a.prefix + (uint64_t)(l * 2 + 1) * a.nrows;
+            const uint64_t vb = a.voff[l] + (pre_v[r] - pre_v[0]);
+            const uint64_t tb = a.toff[l] + (pre_t[r] - pre_t[0]);
+            for (uint32_t c = cl; c < a.NC; c += CS) {
+                RowChunk o;
+                row_chunk(a, lev, y, z, c, true, o);
+                const uint64_t w = (uint64_t)l * level_words + r * a.NC + c;
+                if (verts) {
+                    uint64_t m = o.x | o.y | o.z;
+                    if (m) {
+                        uint64_t id = vb + a.vpc[w];
+                        const uint64_t rowkey = (uint64_t)y << 20 | (uint64_t)z << 40;
+                        while (m) {
+                            const uint32_t i = (uint32_t)__ffsll((unsigned long long)m) - 1u;
+                            m &= m - 1ull;
+                            const uint64_t key = rowkey | (c * 64u + i);
+                            if ((o.x >> i) & 1ull) a.vkey[id++] = key;
+                            if ((o.y >> i) & 1ull) a.vkey[id++] = key | 1ull << 60;
+                            if ((o.z >> i) & 1ull) a.vkey[id++] = key | 2ull << 60;
+                        }
+                    }
+                }
+                uint64_t m = o.act;
+                if (m) {
+                    uint64_t id = tb + a.tpc[w];
+                    const uint64_t cellkey = (uint64_t)y << 20 | (uint64_t)(z - 1) << 40;
+                    while (m) {
+                        const uint32_t i = (uint32_t)__ffsll((unsigned long long)m) - 1u;
+                        m &= m - 1ull;
+                        const uint32_t ci = corner_bits(o.k, i);
+                        const uint32_t nt = ntri[ci];
+                        const uint64_t key = cellkey | (c * 64u + i);
+                        for (uint32_t t = 0; t < nt; ++t, ++id) {
+                            a.trec[id] = key | (uint64_t)t << 60;
+                            a.tci[id] = (uint8_t)ci;
+                        }
+                    }
+                }
+            }
+        }
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_compact(JobArgs a)
+{
+    __shared__ uint8_t ntri[256];
+    stage_ntri(ntri);
+    const uint64_t wave_id = ((uint64_t)blockIdx.x * kBlock + threadIdx.x) >> 6;
+    const uint64_t nwaves = ((uint64_t)gridDim.x * kBlock) >> 6;
+    compact_rows(a, ntri, 0, a.nrows, wave_id, nwaves);
+}
+
+/* Vertex id of the shared vertex on the edge of node (x, y, z) along axis:
+ * the row's prefix + the chunk's prefix within the row + the crossing edges
+ * of the chunk's nodes before x (and of x's lower axes). */
+__device__ __forceinline__ uint64_t vertex_id(const JobArgs& a, const uint64_t* lev, const uint64_t* pre_v,
+                                              const uint32_t* vpc_l, uint32_t x, uint32_t y, uint32_t z, uint32_t axis)
+{
+    RowChunk o;
+    const uint32_t c = x >> 6, i = x & 63u;
+    row_chunk(a, lev, y, z, c, false, o);
+    const uint64_t row = (uint64_t)z * a.ny + y;
+    const uint64_t lo = (1ull << i) - 1ull;
+    uint64_t r = vpc_l[row * a.NC + c];
+    r += (uint64_t)(__popcll(o.x & lo) + __popcll(o.y & lo) + __popcll(o.z & lo));
+    if (axis >= 1) r += (o.x >> i) & 1ull;
+    if (axis >= 2) r += (o.y >> i) & 1ull;
+    return (pre_v[row] - pre_v[0]) + r;
+}
+
+/* Coordinates of every vertex (calculate_isopoints, :365-437). */
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_vertices(JobArgs a)
+{
+    const T* vol = (const T*)a.values;
+    const uint64_t n = a.voff[a.nlev];
+    for (uint64_t g = (uint64_t)blockIdx.x * kBlock + threadIdx.x; g < n; g += (uint64_t)gridDim.x * kBlock) {
+        const uint32_t l = level_at(a.voff, a.nlev, g);
+        const uint64_t key = a.vkey[g];
+        const uint32_t x = (uint32_t)(key & 0xFFFFFu), y = (uint32_t)((key >> 20) & 0xFFFFFu);
+        const uint32_t z = (uint32_t)((key >> 40) & 0xFFFFFu), axis = (uint32_t)(key >> 60);
+        float p[3];
+        node_edge_coords<T>(a, vol, iso_of(a, (int)l), x, y, z, axis, p);
+        float* co = nullptr;
+#pragma unroll
+        for (int k = 0; k < kMaxLevels; ++k) if (k == (int)l) co = a.coords[k];
+        const uint64_t id = g - a.voff[l];
+        co[3 * id + 0] = p[0];
+        co[3 * id + 1] = p[1];
+        co[3 * id + 2] = p[2];
+    }
+}
+
+__device__ __forceinline__ void stage_table_block(uint64_t* tri)
+{
+    for (uint32_t i = threadIdx.x; i < 256; i += kBlock) tri[i] = d_tri[i];
+    __syncthreads();
+}
+
+/* Every triangle: with duplication its three vertices in table orientation and
+ * its face normal (extract_surfaces_with_duplication, :155-243); otherwise its
+ * connectivity (connect_isopoints, :439-521) and the face normal of the
+ * shared vertices' coordinates -- the PolygonNormal output
+ * (calculate_normals_on_polygon, :531-559) or, with VertexNormal, kept in
+ * a.fn for the per-vertex sums. */
+template <typename T, int M>
+__global__ __launch_bounds__(kBlock) void k_triangles(JobArgs a)
+{
+    __shared__ uint64_t tri[256];
+    stage_table_block(tri);
+    const T* vol = (const T*)a.values;
+    const uint64_t n = a.toff[a.nlev];
+    const uint64_t level_words = (uint64_t)a.nz * a.ny * a.NC;
+    for (uint64_t g = (uint64_t)blockIdx.x * kBlock + threadIdx.x; g < n; g += (uint64_t)gridDim.x * kBlock) {
+        const uint32_t l = level_at(a.toff, a.nlev, g);
+        const uint64_t rec = a.trec[g];
+        const uint32_t x = (uint32_t)(rec & 0xFFFFFu), y = (uint32_t)((rec >> 20) & 0xFFFFFu);
+        const uint32_t zc = (uint32_t)((rec >> 40) & 0xFFFFFu), t = (uint32_t)(rec >> 60);
+        const uint64_t w = tri[a.tci[g]];
+        const uint32_t eds[3] = { (uint32_t)((w >> (12 * t)) & 0xF), (uint32_t)((w >> (12 * t + 8)) & 0xF),
+                                  (uint32_t)((w >> (12 * t + 4)) & 0xF) };
+        float* coords = nullptr;
+        float* normals = nullptr;
+        uint32_t* conn = nullptr;
+#pragma unroll
+        for (int k = 0; k < kMaxLevels; ++k)
+            if (k == (int)l) { coords = a.coords[k]; normals = a.normals[k]; conn = a.conn[k]; }
+        const uint64_t id = g - a.toff[l];
+        float p[3][3], nn[3];
+        if constexpr (M == kModeDup) {
+            const double iso = iso_of(a, (int)l);
+#pragma unroll
+            for (int jj = 0; jj < 3; ++jj) {
+                const uint32_t qq = edge_ends(eds[jj]);
+                interpolate<T>(a, vol, iso, x + (qq & 1u), y + ((qq >> 1) & 1u), zc + ((qq >> 2) & 1u),
+                               x + ((qq >> 4) & 1u), y + ((qq >> 5) & 1u), zc + ((qq >> 6) & 1u), p[jj]);
+            }
+            face_normal(p[0], p[1], p[2], nn);
+            float* co = coords + 9 * id;
+#pragma unroll
+            for (int jj = 0; jj < 3; ++jj) {
+                co[3 * jj + 0] = p[jj][0];
+                co[3 * jj + 1] = p[jj][1];
+                co[3 * jj + 2] = p[jj][2];
+            }
+            normals[3 * id + 0] = nn[0];
+            normals[3 * id + 1] = nn[1];
+            normals[3 * id + 2] = nn[2];
+        } else {
+            const uint64_t* lev = a.bitvol + (uint64_t)l * level_words;
+            const uint64_t* pre_v = a.prefix + (uint64_t)(l * 2 + 0) * a.nrows;
+            const uint32_t* vpc_l = a.vpc + (uint64_t)l * level_words;
+            uint64_t vid[3];
+#pragma unroll
+            for (int jj = 0; jj < 3; ++jj) {
+                const uint32_t o = edge_owner(eds[jj]);
+                vid[jj] = vertex_id(a, lev, pre_v, vpc_l, x + (o & 1u), y + ((o >> 1) & 1u), zc + ((o >> 2) & 1u), o >> 3);
+            }
+            conn[3 * id + 0] = (uint32_t)vid[0];
+            conn[3 * id + 1] = (uint32_t)vid[1];
+            conn[3 * id + 2] = (uint32_t)vid[2];
+#pragma unroll
+            for (int jj = 0; jj < 3; ++jj) {
+                p[jj][0] = coords[3 * vid[jj] + 0];
+                p[jj][1] = coords[3 * vid[jj] + 1];
+                p[jj][2] = coords[3 * vid[jj] + 2];
+            }
+            face_normal(p[0], p[1], p[2], nn);
+            float* dst = M == kModeVertex ? a.fn + 3 * g : normals + 3 * id;
+            dst[0] = nn[0];
+            dst[1] = nn[1];
+            dst[2] = nn[2];
+        }
+    }
+}
+
+/* VertexNormal of every vertex: the float sum, in triangle order, of the face
+ * normals of the triangles that use it (calculate_normals_on_vertex,
+ * :569-605).  The four cells around the edge are visited in cell scan order;
+ * a cell's triangles are contiguous records inside its chunk's range of its
+ * cell row, found by a binary search on x, and taken in table order. */
+__global__ __launch_bounds__(kBlock) void k_vertex_normals(JobArgs a)
+{
+    __shared__ uint64_t tri[256];
+    stage_table_block(tri);
+    const uint64_t n = a.voff[a.nlev];
+    const uint64_t level_words = (uint64_t)a.nz * a.ny * a.NC;
+    for (uint64_t g = (uint64_t)blockIdx.x * kBlock + threadIdx.x; g < n; g += (uint64_t)gridDim.x * kBlock) {
+        const uint32_t l = level_at(a.voff, a.nlev, g);
+        const uint64_t key = a.vkey[g];
+        const int x = (int)(key & 0xFFFFFu), y = (int)((key >> 20) & 0xFFFFFu);
+        const int zv = (int)((key >> 40) & 0xFFFFFu);
+        const uint32_t axis = (uint32_t)(key >> 60);
+        const uint64_t* pre_t = a.prefix + (uint64_t)(l * 2 + 1) * a.nrows;
+        const uint32_t* cnt_t = a.counts + (uint64_t)(l * 2 + 1) * a.nrows;
+        const uint32_t* tpc_l = a.tpc + (uint64_t)l * level_words;
+        const uint64_t tl = a.toff[l] - pre_t[0];
+        const uint32_t around = around_nibbles(axis);
+        float nv[3] = { 0.0f, 0.0f, 0.0f };
+        for (int q = 0; q < 4; ++q) {
+            int cx, cy, cz;
+            around_cell(axis, q, x, y, zv, cx, cy, cz);
+            if (cx < 0 || cy < 0 || cz < 0) continue;
+            if (cx + 1 >= (int)a.nx || cy + 1 >= (int)a.ny || cz + 1 >= (int)a.nz) continue;
+            /* the triangles of cell row (cy, cz) are counted at node row (cy, cz + 1) */
+            const uint64_t row = (uint64_t)(cz + 1) * a.ny + (uint64_t)cy;
+            const uint32_t c = (uint32_t)cx >> 6;
+            const uint64_t rb = tl + pre_t[row];
+            uint64_t lo = rb + tpc_l[row * a.NC + c];
+            uint64_t hi = (c + 1 < a.NC) ? rb + tpc_l[row * a.NC + c + 1] : rb + cnt_t[row];
+            const uint64_t end = hi;
+            while (lo < hi) {   // first record with x >= cx
+                const uint64_t mid = lo + ((hi - lo) >> 1);
+                if ((int)(a.trec[mid] & 0xFFFFFu) < cx) lo = mid + 1; else hi = mid;
+            }
+            if (lo >= end) continue;   // (every cell around a crossing edge is active)
+            const uint64_t w = tri[a.tci[lo]];
+            const int nt = (int)(w >> 60);
+            const uint32_t me = (around >> (4 * q)) & 0xFu;
+            for (int t = 0; t < nt; ++t) {
+                const uint32_t e0 = (uint32_t)((w >> (12 * t)) & 0xF);
+                const uint32_t e1 = (uint32_t)((w >> (12 * t + 8)) & 0xF);
+                const uint32_t e2 = (uint32_t)((w >> (12 * t + 4)) & 0xF);
+                if (e0 != me && e1 != me && e2 != me) continue;
+                const float* f = a.fn + 3 * (lo + (uint64_t)t);
+                nv[0] = nv[0] + f[0];
+                nv[1] = nv[1] + f[1];
+                nv[2] = nv[2] + f[2];
+            }
+        }
+        float* normals = nullptr;
+#pragma unroll
+        for (int k = 0; k < kMaxLevels; ++k) if (k == (int)l) normals = a.normals[k];
+        const uint64_t id = g - a.voff[l];
+        normals[3 * id + 0] = nv[0];
+        normals[3 * id + 1] = nv[1];
+        normals[3 * id + 2] = nv[2];
     }
 }
 
@@ -1542,6 +1871,51 @@ hipError_t launch_count(const JobArgs& a, hipStream_t st)
     if (blocks > 16384) blocks = 16384;
     hipLaunchKernelGGL(k_count, dim3((uint32_t)blocks), dim3(kBlock), 0, st, a);
     return hipGetLastError();
+}
+
+/* Grid of a flat per-item kernel: up to 8 resident workgroups per CU. */
+static uint32_t flat_blocks(uint64_t items)
+{
+    const uint64_t b = (items + kBlock - 1) / kBlock;
+    return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(b, 256 * 8));
+}
+
+template <typename T>
+hipError_t launch_flat_t(const JobArgs& a, hipStream_t st)
+{
+    const uint64_t nv = a.voff[a.nlev], nt = a.toff[a.nlev];
+    if (!a.dup && nv) hipLaunchKernelGGL((k_vertices<T>), dim3(flat_blocks(nv)), dim3(kBlock), 0, st, a);
+    if (nt) {
+        switch (emit_mode(a)) {
+        case kModeDup: hipLaunchKernelGGL((k_triangles<T, kModeDup>), dim3(flat_blocks(nt)), dim3(kBlock), 0, st, a); break;
+        case kModePolygon:
+            hipLaunchKernelGGL((k_triangles<T, kModePolygon>), dim3(flat_blocks(nt)), dim3(kBlock), 0, st, a);
+            break;
+        default: hipLaunchKernelGGL((k_triangles<T, kModeVertex>), dim3(flat_blocks(nt)), dim3(kBlock), 0, st, a); break;
+        }
+    }
+    if (a.vnormal && nv) hipLaunchKernelGGL(k_vertex_normals, dim3(flat_blocks(nv)), dim3(kBlock), 0, st, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_flat_emit(int dtype, const JobArgs& a, hipStream_t st)
+{
+    const uint64_t G = 64u >> a.cshift;
+    uint64_t blocks = ((a.nrows + G - 1) / G + kWaves - 1) / kWaves;
+    if (blocks > 16384) blocks = 16384;
+    hipLaunchKernelGGL(k_compact, dim3((uint32_t)blocks), dim3(kBlock), 0, st, a);
+    switch (dtype) {
+    case 0: return launch_flat_t<int8_t>(a, st);
+    case 1: return launch_flat_t<int16_t>(a, st);
+    case 2: return launch_flat_t<int32_t>(a, st);
+    case 3: return launch_flat_t<int64_t>(a, st);
+    case 4: return launch_flat_t<uint8_t>(a, st);
+    case 5: return launch_flat_t<uint16_t>(a, st);
+    case 6: return launch_flat_t<uint32_t>(a, st);
+    case 7: return launch_flat_t<uint64_t>(a, st);
+    case 8: return launch_flat_t<float>(a, st);
+    default: return launch_flat_t<double>(a, st);
+    }
 }
 
 hipError_t launch_emit(int dtype, const JobArgs& a, uint32_t lds_bytes, hipStream_t st)
