@@ -25,11 +25,43 @@
 
 namespace kmc {
 
-/* Where one surface of a run landed (element offsets into the arenas). */
+/* Where one surface of a run landed (element offsets into the arenas); the
+ * entry after the last surface holds the totals, the overflow flag (nv) and
+ * the record bytes the flat emit needs (rb). */
 struct SegOut {
     uint64_t nv, nt;          // vertices (node-shared) and triangles counted
     uint64_t oc, on, ok;      // offsets of coords (floats), normals (floats), connections (uint32)
+    uint64_t rb;
 };
+
+/* Flat-emit records of a pass: vertex keys, triangle records, cube indices,
+ * face normals (VertexNormal), each 256-byte aligned. */
+__host__ __device__ inline uint64_t align256(uint64_t x) { return (x + 255) & ~255ull; }
+__host__ __device__ inline uint64_t record_bytes(uint64_t nv, uint64_t nt, bool vn)
+{
+    return align256(nv * 8) + align256(nt * 8) + align256(nt) + (vn ? align256(nt * 12) : 0);
+}
+
+/* Point pass a's records at base (or at nothing when base is null) and set its
+ * per-level item offsets from the level totals nv[l] / nt[l]. */
+__host__ __device__ inline void place_records(JobArgs& a, const uint64_t* nv, const uint64_t* nt, uint8_t* base)
+{
+    a.voff[0] = a.toff[0] = 0;
+    for (uint32_t l = 0; l < a.nlev; ++l) {
+        a.voff[l + 1] = a.voff[l] + (base && !a.dup ? nv[l] : 0);
+        a.toff[l + 1] = a.toff[l] + (base ? nt[l] : 0);
+    }
+    if (!base) {
+        a.vkey = nullptr; a.trec = nullptr; a.tci = nullptr; a.fn = nullptr;
+        return;
+    }
+    const uint64_t V = a.voff[a.nlev], T = a.toff[a.nlev];
+    uint8_t* p = base;
+    a.vkey = (uint64_t*)p; p += align256(V * 8);
+    a.trec = (uint64_t*)p; p += align256(T * 8);
+    a.tci = p; p += align256(T);
+    a.fn = a.vnormal ? (float*)p : nullptr;
+}
 
 constexpr uint32_t kLayoutBlock = 256;
 constexpr uint32_t kLayoutPer = 8;
@@ -73,7 +105,8 @@ __device__ void block_scan3(uint64_t v[3], uint64_t tot[3])
 __global__ __launch_bounds__(kLayoutBlock) void k_batch_layout(JobArgs* jobs, const uint32_t* seg_map, uint32_t nseg,
                                                                float* arena_c, float* arena_n, uint32_t* arena_k,
                                                                uint64_t cap_c, uint64_t cap_n, uint64_t cap_k,
-                                                               SegOut* out)
+                                                               SegOut* out, uint32_t npass, uint8_t* arena_r,
+                                                               uint64_t cap_r, uint64_t* istart)
 {
     __shared__ uint32_t overflow;
     /* one sweep per kLayoutBlock * kLayoutPer surfaces, carrying the offsets */
@@ -118,9 +151,59 @@ __global__ __launch_bounds__(kLayoutBlock) void k_batch_layout(JobArgs* jobs, co
         carry[0] += tot[0]; carry[1] += tot[1]; carry[2] += tot[2];
         __syncthreads();
     }
-    if (threadIdx.x == 0) overflow = (carry[0] > cap_c || carry[1] > cap_n || carry[2] > cap_k) ? 1u : 0u;
+    /* flat-emit records of every pass (istart: null for the tile teams) */
+    uint64_t rcarry[3] = { 0, 0, 0 };
+    if (istart) {
+        for (uint32_t b0 = 0; b0 < npass; b0 += kLayoutBlock * kLayoutPer) {
+            const uint32_t p0 = b0 + threadIdx.x * kLayoutPer;
+            uint64_t sz[kLayoutPer][3];
+            uint64_t v[3] = { 0, 0, 0 };
+            for (uint32_t i = 0; i < kLayoutPer; ++i) {
+                sz[i][0] = sz[i][1] = sz[i][2] = 0;
+                if (p0 + i < npass) {
+                    const JobArgs& a = jobs[p0 + i];
+                    for (uint32_t l = 0; l < a.nlev; ++l) {
+                        sz[i][1] += a.dup ? 0 : level_total(a, l, 0);
+                        sz[i][2] += level_total(a, l, 1);
+                    }
+                    sz[i][0] = record_bytes(sz[i][1], sz[i][2], a.vnormal != 0);
+                }
+                v[0] += sz[i][0]; v[1] += sz[i][1]; v[2] += sz[i][2];
+            }
+            uint64_t tot[3];
+            block_scan3(v, tot);
+            for (uint32_t i = 0; i < kLayoutPer; ++i) {
+                if (p0 + i < npass) {
+                    istart[p0 + i] = rcarry[1] + v[1];                 // vertex items
+                    istart[npass + 1 + p0 + i] = rcarry[2] + v[2];     // triangle items
+                    jobs[p0 + i].voff[0] = rcarry[0] + v[0];           // record byte offset (placed below)
+                }
+                v[0] += sz[i][0]; v[1] += sz[i][1]; v[2] += sz[i][2];
+            }
+            rcarry[0] += tot[0]; rcarry[1] += tot[1]; rcarry[2] += tot[2];
+            __syncthreads();
+        }
+    }
+    if (threadIdx.x == 0)
+        overflow = (carry[0] > cap_c || carry[1] > cap_n || carry[2] > cap_k || (istart && rcarry[0] > cap_r)) ? 1u : 0u;
     __syncthreads();
     const bool fits = overflow == 0;
+    if (istart) {
+        for (uint32_t p = threadIdx.x; p < npass; p += kLayoutBlock) {
+            JobArgs& a = jobs[p];
+            uint64_t nv[kMaxLevels], nt[kMaxLevels];
+            for (uint32_t l = 0; l < a.nlev; ++l) {
+                nv[l] = level_total(a, l, 0);
+                nt[l] = level_total(a, l, 1);
+            }
+            place_records(a, nv, nt, fits ? arena_r + a.voff[0] : nullptr);
+            if (!fits) istart[p] = istart[npass + 1 + p] = 0;
+        }
+        if (threadIdx.x == 0) {
+            istart[npass] = fits ? rcarry[1] : 0;
+            istart[2 * npass + 1] = fits ? rcarry[2] : 0;
+        }
+    }
     for (uint32_t s = threadIdx.x; s < nseg; s += kLayoutBlock) {
         const uint32_t m = seg_map[s];
         JobArgs& a = jobs[m >> 3];
@@ -140,6 +223,7 @@ __global__ __launch_bounds__(kLayoutBlock) void k_batch_layout(JobArgs* jobs, co
         t.oc = carry[0];
         t.on = carry[1];
         t.ok = carry[2];
+        t.rb = rcarry[0];
         out[nseg] = t;
     }
 }
@@ -214,6 +298,13 @@ struct kmc_batch {
     uint32_t scan_parts;
     uint64_t* bitvol_d;
     uint64_t bitvol_words;
+    bool flat;                              // flat emit (else the tile teams)
+    bool flat_ok;                           // every pass's rows fit the flat records' 20-bit y / z
+    uint32_t* cpre_d;                       // in-row chunk prefixes of the count pass (flat)
+    uint64_t* istart_d;                     // per-pass first vertex / triangle item (flat)
+    std::vector<uint64_t> istart_h;
+    uint8_t* arena_r;                       // flat-emit records
+    uint64_t cap_r;
     SegOut* segout_d;
     SegOut* segout_h;                       // pinned
     float* arena_c;
@@ -265,6 +356,9 @@ void destroy(kmc_batch* b)
     device_free(b->prefix_d);
     device_free(b->scan_aux_d);
     device_free(b->bitvol_d);
+    device_free(b->cpre_d);
+    device_free(b->istart_d);
+    device_free(b->arena_r);
     device_free(b->segout_d);
     device_free(b->arena_c);
     device_free(b->arena_n);
@@ -321,6 +415,10 @@ kmc_status plan_tiles(kmc_batch* b, uint32_t r_max, uint32_t queue_cap)
                            b->upload_stream) != hipSuccess)
             return KMC_ERR_DEVICE;
     }
+    for (Pass& p : b->passes) {
+        p.a.vpc = b->flat ? b->cpre_d + p.bitvol_off : nullptr;
+        p.a.tpc = b->flat ? b->cpre_d + b->bitvol_words + p.bitvol_off : nullptr;
+    }
     b->jobs_h.resize(b->passes.size());
     for (size_t i = 0; i < b->passes.size(); ++i) b->jobs_h[i] = b->passes[i].a;
     if (!b->jobs_h.empty() && hipMemcpyAsync(b->jobs_d, b->jobs_h.data(), b->jobs_h.size() * sizeof(JobArgs),
@@ -349,7 +447,8 @@ hipError_t record_front(kmc_batch* b, hipStream_t st, bool copy_sizes)
     e = launch_scan(b->counts_d, b->n_counts, 1, partials, b->prefix_d, totals, st);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_batch_layout, dim3(1), dim3(kLayoutBlock), 0, st, b->jobs_d, b->seg_map_d, nseg, b->arena_c,
-                       b->arena_n, b->arena_k, b->cap_c, b->cap_n, b->cap_k, b->segout_d);
+                       b->arena_n, b->arena_k, b->cap_c, b->cap_n, b->cap_k, b->segout_d, (uint32_t)b->passes.size(),
+                       b->arena_r, b->cap_r, b->flat ? b->istart_d : nullptr);
     e = hipGetLastError();
     if (e != hipSuccess || !copy_sizes) return e;
     return hipMemcpyAsync(b->segout_h, b->segout_d, (nseg + 1) * sizeof(SegOut), hipMemcpyDeviceToHost, st);
@@ -358,6 +457,17 @@ hipError_t record_front(kmc_batch* b, hipStream_t st, bool copy_sizes)
 hipError_t record_emit(kmc_batch* b, hipStream_t st)
 {
     hipError_t e = hipSuccess;
+    if (b->flat) {
+        const uint32_t np = (uint32_t)b->passes.size();
+        for (const Group& g : b->groups) {
+            e = launch_compact_batch(b->jobs_d + g.p0, g.row_start_d, g.p1 - g.p0, g.rows, st);
+            if (e == hipSuccess)
+                e = launch_flat_batch(g.dtype, g.mode, b->jobs_d + g.p0, b->istart_d + g.p0, b->istart_d + np + 1 + g.p0,
+                                      g.p1 - g.p0, st);
+            if (e != hipSuccess) return e;
+        }
+        return e;
+    }
     for (const Group& g : b->groups) {
         e = launch_emit_batch(g.dtype, g.mode, b->jobs_d + g.p0, g.unit_start_d, g.p1 - g.p0, g.units, g.emit_lds,
                               g.teams, st);
@@ -414,7 +524,7 @@ kmc_status capture(kmc_batch* b)
     return KMC_OK;
 }
 
-kmc_status grow_arena(kmc_batch* b, uint64_t need_c, uint64_t need_n, uint64_t need_k)
+kmc_status grow_arena(kmc_batch* b, uint64_t need_c, uint64_t need_n, uint64_t need_k, uint64_t need_r)
 {
     auto grow = [](void*& p, uint64_t& cap, uint64_t need, size_t esz) -> bool {
         if (need <= cap) return true;
@@ -425,8 +535,10 @@ kmc_status grow_arena(kmc_batch* b, uint64_t need_c, uint64_t need_n, uint64_t n
         return p != nullptr;
     };
     void* c = b->arena_c; void* n = b->arena_n; void* k = b->arena_k;
-    const bool ok = grow(c, b->cap_c, need_c, 4) && grow(n, b->cap_n, need_n, 4) && grow(k, b->cap_k, need_k, 4);
-    b->arena_c = (float*)c; b->arena_n = (float*)n; b->arena_k = (uint32_t*)k;
+    void* r = b->arena_r;
+    const bool ok = grow(c, b->cap_c, need_c, 4) && grow(n, b->cap_n, need_n, 4) && grow(k, b->cap_k, need_k, 4) &&
+                    grow(r, b->cap_r, need_r, 1);
+    b->arena_c = (float*)c; b->arena_n = (float*)n; b->arena_k = (uint32_t*)k; b->arena_r = (uint8_t*)r;
     return ok ? KMC_OK : KMC_ERR_OUT_OF_MEMORY;
 }
 
@@ -451,21 +563,25 @@ kmc_status run(kmc_batch* b, hipStream_t st)
         b->last_ms = ms;
         const SegOut& t = b->segout_h[nseg];
         if (t.nv == 0) { b->have_results = true; return KMC_OK; }
-        const kmc_status s = grow_arena(b, t.oc, t.on, t.ok);
+        const kmc_status s = grow_arena(b, t.oc, t.on, t.ok, t.rb);
         if (s != KMC_OK) return s;
         destroy_graph(b);
     }
     return KMC_ERR_DEVICE;
 }
 
-/* Time the candidate tile shapes on this batch; keep the fastest. */
+/* Time the candidate emit forms on this batch -- the flat emit and the tile
+ * teams at several band heights and queue sizes; keep the fastest. */
 kmc_status autotune(kmc_batch* b, hipStream_t st)
 {
-    struct Cand { uint32_t r, q; };
-    const Cand cands[] = { { 16, 128 }, { 8, 128 }, { 4, 128 }, { 2, 128 }, { 16, 256 }, { 8, 64 } };
+    struct Cand { uint32_t r, q; bool flat; };
+    const Cand cands[] = { { 16, 128, true }, { 16, 128, false }, { 8, 128, false }, { 4, 128, false },
+                           { 2, 128, false }, { 16, 256, false } };
     double best = 1e30;
-    Cand best_c = { b->r_max, b->queue_cap };
+    Cand best_c = { b->r_max, b->queue_cap, b->flat };
     for (const Cand& c : cands) {
+        if (c.flat && !b->flat_ok) continue;
+        b->flat = c.flat;
         kmc_status s = plan_tiles(b, c.r, c.q);
         if (s == KMC_OK) { destroy_graph(b); s = run(b, st); }
         if (s != KMC_OK) continue;          // a shape whose LDS does not fit is skipped
@@ -476,6 +592,7 @@ kmc_status autotune(kmc_batch* b, hipStream_t st)
         }
         if (s == KMC_OK && t < best) { best = t; best_c = c; }
     }
+    b->flat = best_c.flat;
     kmc_status s = plan_tiles(b, best_c.r, best_c.q);
     if (s == KMC_OK) { destroy_graph(b); s = capture(b); }
     return s;
@@ -590,6 +707,19 @@ kmc_status create_impl(const kmc_job* jobs, uint32_t n_jobs, uint32_t flags, hip
         b->prefix_d = (uint64_t*)device_alloc(coff * 8);
         b->scan_aux_d = (uint64_t*)device_alloc((b->scan_parts + 1) * 8);
         b->bitvol_d = (uint64_t*)device_alloc(boff * 8);
+        static const char* env_emit = std::getenv("KMC_EMIT");
+        b->flat_ok = !(env_emit && std::strcmp(env_emit, "team") == 0);
+        for (const Pass& p : b->passes) b->flat_ok = b->flat_ok && p.a.ny <= (1u << 20) && p.a.nz <= (1u << 20);
+        b->flat = b->flat_ok;
+        if (b->flat_ok) {
+            b->cpre_d = (uint32_t*)device_alloc(2 * boff * 4);
+            b->istart_d = (uint64_t*)device_alloc((2 * b->passes.size() + 2) * 8);
+            if (!b->cpre_d || !b->istart_d) {
+                (void)hipGetLastError();
+                destroy(b);
+                return KMC_ERR_OUT_OF_MEMORY;
+            }
+        }
         b->jobs_d = (JobArgs*)device_alloc(b->passes.size() * sizeof(JobArgs));
         b->seg_map_d = (uint32_t*)device_alloc(nseg * 4);
         b->segout_d = (SegOut*)device_alloc((nseg + 1) * sizeof(SegOut));
@@ -688,6 +818,48 @@ void batch_set_outputs(kmc_batch* b, uint32_t i, float* coords, float* normals, 
 hipError_t batch_launch_emit(kmc_batch* b, hipStream_t st)
 {
     if (b->passes.empty()) return hipSuccess;
+    if (b->flat) {
+        /* records of every pass from the sizes of the front phase */
+        const uint32_t np = (uint32_t)b->passes.size();
+        std::vector<uint64_t> off(np);
+        uint64_t bytes = 0;
+        b->istart_h.assign(2 * np + 2, 0);
+        uint64_t vsum = 0, tsum = 0;
+        for (uint32_t i = 0; i < np; ++i) {
+            const Pass& p = b->passes[i];
+            uint64_t V = 0, T = 0;
+            for (uint32_t l = 0; l < p.nlev; ++l) {
+                V += p.a.dup ? 0 : b->segout_h[p.seg0 + l].nv;
+                T += b->segout_h[p.seg0 + l].nt;
+            }
+            off[i] = bytes;
+            bytes += record_bytes(V, T, p.a.vnormal != 0);
+            b->istart_h[i] = vsum;
+            b->istart_h[np + 1 + i] = tsum;
+            vsum += V;
+            tsum += T;
+        }
+        b->istart_h[np] = vsum;
+        b->istart_h[2 * np + 1] = tsum;
+        if (bytes > b->cap_r) {
+            device_free(b->arena_r);
+            b->arena_r = (uint8_t*)device_alloc(bytes);
+            b->cap_r = b->arena_r ? bytes : 0;
+            if (!b->arena_r) return hipErrorOutOfMemory;
+        }
+        for (uint32_t i = 0; i < np; ++i) {
+            Pass& p = b->passes[i];
+            uint64_t nv[kMaxLevels], nt[kMaxLevels];
+            for (uint32_t l = 0; l < p.nlev; ++l) {
+                nv[l] = b->segout_h[p.seg0 + l].nv;
+                nt[l] = b->segout_h[p.seg0 + l].nt;
+            }
+            place_records(p.a, nv, nt, b->arena_r + off[i]);
+        }
+        const hipError_t e = hipMemcpyAsync(b->istart_d, b->istart_h.data(), b->istart_h.size() * 8,
+                                            hipMemcpyHostToDevice, st);
+        if (e != hipSuccess) return e;
+    }
     b->jobs_h.resize(b->passes.size());
     for (size_t i = 0; i < b->passes.size(); ++i) b->jobs_h[i] = b->passes[i].a;
     const hipError_t e = hipMemcpyAsync(b->jobs_d, b->jobs_h.data(), b->jobs_h.size() * sizeof(JobArgs),
@@ -696,11 +868,22 @@ hipError_t batch_launch_emit(kmc_batch* b, hipStream_t st)
     return record_emit(b, st);
 }
 
-/* Device bytes the plan holds besides the volumes (counts, prefixes, masks, tables). */
+/* Device bytes the plan holds besides the volumes (counts, prefixes, masks,
+ * chunk prefixes, tables). */
 uint64_t batch_work_bytes(const kmc_batch* b)
 {
-    return b->n_counts * 12 + b->bitvol_words * 8 + b->passes.size() * sizeof(JobArgs) +
+    return b->n_counts * 12 + b->bitvol_words * (b->flat_ok ? 16 : 8) + b->passes.size() * (sizeof(JobArgs) + 16) +
            b->seg_pass_level.size() * (4 + sizeof(SegOut)) + (uint64_t)(b->scan_parts + 1) * 8;
+}
+
+/* Flat-emit record bytes surface i adds once its sizes are known. */
+uint64_t batch_record_bytes(const kmc_batch* b, uint32_t i)
+{
+    const uint32_t sg = b->surface_seg[i];
+    if (!b->flat || sg == ~0u) return 0;
+    const Pass& p = b->passes[b->seg_pass_level[sg] >> 3];
+    const SegOut& so = b->segout_h[sg];
+    return record_bytes(p.a.dup ? 0 : so.nv, so.nt, p.a.vnormal != 0);
 }
 
 void batch_destroy(kmc_batch* b) { destroy(b); }

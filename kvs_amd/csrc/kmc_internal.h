@@ -60,6 +60,12 @@ hipError_t launch_emit(int dtype, const JobArgs& a, uint32_t lds_bytes, hipStrea
  * a.trec / a.tci / a.fn and a.voff / a.toff set): compaction records, then
  * one thread per vertex, per triangle and (VertexNormal) per vertex normal. */
 hipError_t launch_flat_emit(int dtype, const JobArgs& a, hipStream_t st);
+/* Flat emit of a batch group: compaction over the rows of its jobs, then the
+ * per-item kernels over the items start[j] .. start[j + 1] of each job. */
+hipError_t launch_compact_batch(const JobArgs* jobs, const uint64_t* row_start, uint32_t njobs, uint64_t rows,
+                                hipStream_t st);
+hipError_t launch_flat_batch(int dtype, int mode, const JobArgs* jobs, const uint64_t* vstart, const uint64_t* tstart,
+                             uint32_t njobs, hipStream_t st);
 hipError_t launch_classify(int dtype, const JobArgs& a, hipStream_t st);
 hipError_t launch_count(const JobArgs& a, hipStream_t st);
 hipError_t launch_scan(const uint32_t* counts, uint64_t n, uint32_t narr, uint64_t* partials,
@@ -83,6 +89,8 @@ hipError_t batch_launch_front(kmc_batch* b, hipStream_t st);
 uint32_t batch_surfaces(const kmc_batch* b);
 void batch_surface_sizes(const kmc_batch* b, uint32_t i, kmc_surface* o);
 void batch_set_outputs(kmc_batch* b, uint32_t i, float* coords, float* normals, uint32_t* conn);
+/* Flat-emit record bytes surface i adds once its sizes are known. */
+uint64_t batch_record_bytes(const kmc_batch* b, uint32_t i);
 hipError_t batch_launch_emit(kmc_batch* b, hipStream_t st);
 uint64_t batch_work_bytes(const kmc_batch* b);
 void batch_destroy(kmc_batch* b);

@@ -1297,6 +1297,7 @@ __device__ __forceinline__ void compact_rows(const JobArgs& a, const uint8_t* nt
     const uint32_t G = 64u >> a.cshift, CS = 1u << a.cshift;
     const uint64_t level_words = (uint64_t)a.nz * a.ny * a.NC;
     const bool verts = !a.dup;
+    if (!a.trec) return;   // the records did not fit (batched form): nothing to write
     for (uint64_t g = r0 + wave_first * G; g < r1; g += wave_step * G) {
         const uint64_t r = g + sub;
         if (r >= r1) continue;
@@ -1357,6 +1358,27 @@ __global__ __launch_bounds__(kBlock) void k_compact(JobArgs a)
     compact_rows(a, ntri, 0, a.nrows, wave_id, nwaves);
 }
 
+/* Compaction of a batch of jobs: the rows of all jobs in one sequence, split
+ * as the count pass splits them. */
+__global__ __launch_bounds__(kBlock) void k_compact_batch(const JobArgs* jobs, const uint64_t* row_start, uint32_t njobs)
+{
+    __shared__ uint8_t ntri[256];
+    stage_ntri(ntri);
+    const uint64_t total = row_start[njobs];
+    const uint64_t nwaves = ((uint64_t)gridDim.x * kBlock) >> 6;
+    const uint64_t wave_id = ((uint64_t)blockIdx.x * kBlock + threadIdx.x) >> 6;
+    const uint64_t per = (total + nwaves - 1) / nwaves;
+    const uint64_t begin = wave_id * per;
+    const uint64_t end = begin + per < total ? begin + per : total;
+    uint64_t g = begin;
+    while (g < end) {
+        const uint32_t j = __builtin_amdgcn_readfirstlane(find_job(row_start, njobs, g));
+        const uint64_t je = row_start[j + 1] < end ? row_start[j + 1] : end;
+        compact_rows(jobs[j], ntri, g - row_start[j], je - row_start[j], 0, 1);
+        g = je;
+    }
+}
+
 /* Vertex id of the shared vertex on the edge of node (x, y, z) along axis:
  * the row's prefix + the chunk's prefix within the row + the crossing edges
  * of the chunk's nodes before x (and of x's lower axes). */
@@ -1375,27 +1397,32 @@ __device__ __forceinline__ uint64_t vertex_id(const JobArgs& a, const uint64_t* 
     return (pre_v[row] - pre_v[0]) + r;
 }
 
-/* Coordinates of every vertex (calculate_isopoints, :365-437). */
+/* Coordinates of vertex g of the job's items (calculate_isopoints, :365-437). */
+template <typename T>
+__device__ __forceinline__ void vertex_item(const JobArgs& a, uint64_t g)
+{
+    const uint32_t l = level_at(a.voff, a.nlev, g);
+    const uint64_t key = a.vkey[g];
+    const uint32_t x = (uint32_t)(key & 0xFFFFFu), y = (uint32_t)((key >> 20) & 0xFFFFFu);
+    const uint32_t z = (uint32_t)((key >> 40) & 0xFFFFFu), axis = (uint32_t)(key >> 60);
+    float p[3];
+    node_edge_coords<T>(a, (const T*)a.values, iso_of(a, (int)l), x, y, z, axis, p);
+    float* co = nullptr;
+#pragma unroll
+    for (int k = 0; k < kMaxLevels; ++k) if (k == (int)l) co = a.coords[k];
+    if (!co) return;   // a level without outputs (not admitted)
+    const uint64_t id = g - a.voff[l];
+    co[3 * id + 0] = p[0];
+    co[3 * id + 1] = p[1];
+    co[3 * id + 2] = p[2];
+}
+
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_vertices(JobArgs a)
 {
-    const T* vol = (const T*)a.values;
     const uint64_t n = a.voff[a.nlev];
-    for (uint64_t g = (uint64_t)blockIdx.x * kBlock + threadIdx.x; g < n; g += (uint64_t)gridDim.x * kBlock) {
-        const uint32_t l = level_at(a.voff, a.nlev, g);
-        const uint64_t key = a.vkey[g];
-        const uint32_t x = (uint32_t)(key & 0xFFFFFu), y = (uint32_t)((key >> 20) & 0xFFFFFu);
-        const uint32_t z = (uint32_t)((key >> 40) & 0xFFFFFu), axis = (uint32_t)(key >> 60);
-        float p[3];
-        node_edge_coords<T>(a, vol, iso_of(a, (int)l), x, y, z, axis, p);
-        float* co = nullptr;
-#pragma unroll
-        for (int k = 0; k < kMaxLevels; ++k) if (k == (int)l) co = a.coords[k];
-        const uint64_t id = g - a.voff[l];
-        co[3 * id + 0] = p[0];
-        co[3 * id + 1] = p[1];
-        co[3 * id + 2] = p[2];
-    }
+    for (uint64_t g = (uint64_t)blockIdx.x * kBlock + threadIdx.x; g < n; g += (uint64_t)gridDim.x * kBlock)
+        vertex_item<T>(a, g);
 }
 
 __device__ __forceinline__ void stage_table_block(uint64_t* tri)
@@ -1411,74 +1438,80 @@ __device__ __forceinline__ void stage_table_block(uint64_t* tri)
  * (calculate_normals_on_polygon, :531-559) or, with VertexNormal, kept in
  * a.fn for the per-vertex sums. */
 template <typename T, int M>
+__device__ __forceinline__ void triangle_item(const JobArgs& a, const uint64_t* tri, uint64_t g)
+{
+    const T* vol = (const T*)a.values;
+    const uint64_t level_words = (uint64_t)a.nz * a.ny * a.NC;
+    const uint32_t l = level_at(a.toff, a.nlev, g);
+    const uint64_t rec = a.trec[g];
+    const uint32_t x = (uint32_t)(rec & 0xFFFFFu), y = (uint32_t)((rec >> 20) & 0xFFFFFu);
+    const uint32_t zc = (uint32_t)((rec >> 40) & 0xFFFFFu), t = (uint32_t)(rec >> 60);
+    const uint64_t w = tri[a.tci[g]];
+    const uint32_t eds[3] = { (uint32_t)((w >> (12 * t)) & 0xF), (uint32_t)((w >> (12 * t + 8)) & 0xF),
+                              (uint32_t)((w >> (12 * t + 4)) & 0xF) };
+    float* coords = nullptr;
+    float* normals = nullptr;
+    uint32_t* conn = nullptr;
+#pragma unroll
+    for (int k = 0; k < kMaxLevels; ++k)
+        if (k == (int)l) { coords = a.coords[k]; normals = a.normals[k]; conn = a.conn[k]; }
+    if (!coords) return;   // a level without outputs (not admitted)
+    const uint64_t id = g - a.toff[l];
+    float p[3][3], nn[3];
+    if constexpr (M == kModeDup) {
+        const double iso = iso_of(a, (int)l);
+#pragma unroll
+        for (int jj = 0; jj < 3; ++jj) {
+            const uint32_t qq = edge_ends(eds[jj]);
+            interpolate<T>(a, vol, iso, x + (qq & 1u), y + ((qq >> 1) & 1u), zc + ((qq >> 2) & 1u),
+                           x + ((qq >> 4) & 1u), y + ((qq >> 5) & 1u), zc + ((qq >> 6) & 1u), p[jj]);
+        }
+        face_normal(p[0], p[1], p[2], nn);
+        float* co = coords + 9 * id;
+#pragma unroll
+        for (int jj = 0; jj < 3; ++jj) {
+            co[3 * jj + 0] = p[jj][0];
+            co[3 * jj + 1] = p[jj][1];
+            co[3 * jj + 2] = p[jj][2];
+        }
+        normals[3 * id + 0] = nn[0];
+        normals[3 * id + 1] = nn[1];
+        normals[3 * id + 2] = nn[2];
+    } else {
+        const uint64_t* lev = a.bitvol + (uint64_t)l * level_words;
+        const uint64_t* pre_v = a.prefix + (uint64_t)(l * 2 + 0) * a.nrows;
+        const uint32_t* vpc_l = a.vpc + (uint64_t)l * level_words;
+        uint64_t vid[3];
+#pragma unroll
+        for (int jj = 0; jj < 3; ++jj) {
+            const uint32_t o = edge_owner(eds[jj]);
+            vid[jj] = vertex_id(a, lev, pre_v, vpc_l, x + (o & 1u), y + ((o >> 1) & 1u), zc + ((o >> 2) & 1u), o >> 3);
+        }
+        conn[3 * id + 0] = (uint32_t)vid[0];
+        conn[3 * id + 1] = (uint32_t)vid[1];
+        conn[3 * id + 2] = (uint32_t)vid[2];
+#pragma unroll
+        for (int jj = 0; jj < 3; ++jj) {
+            p[jj][0] = coords[3 * vid[jj] + 0];
+            p[jj][1] = coords[3 * vid[jj] + 1];
+            p[jj][2] = coords[3 * vid[jj] + 2];
+        }
+        face_normal(p[0], p[1], p[2], nn);
+        float* dst = M == kModeVertex ? a.fn + 3 * g : normals + 3 * id;
+        dst[0] = nn[0];
+        dst[1] = nn[1];
+        dst[2] = nn[2];
+    }
+}
+
+template <typename T, int M>
 __global__ __launch_bounds__(kBlock) void k_triangles(JobArgs a)
 {
     __shared__ uint64_t tri[256];
     stage_table_block(tri);
-    const T* vol = (const T*)a.values;
     const uint64_t n = a.toff[a.nlev];
-    const uint64_t level_words = (uint64_t)a.nz * a.ny * a.NC;
-    for (uint64_t g = (uint64_t)blockIdx.x * kBlock + threadIdx.x; g < n; g += (uint64_t)gridDim.x * kBlock) {
-        const uint32_t l = level_at(a.toff, a.nlev, g);
-        const uint64_t rec = a.trec[g];
-        const uint32_t x = (uint32_t)(rec & 0xFFFFFu), y = (uint32_t)((rec >> 20) & 0xFFFFFu);
-        const uint32_t zc = (uint32_t)((rec >> 40) & 0xFFFFFu), t = (uint32_t)(rec >> 60);
-        const uint64_t w = tri[a.tci[g]];
-        const uint32_t eds[3] = { (uint32_t)((w >> (12 * t)) & 0xF), (uint32_t)((w >> (12 * t + 8)) & 0xF),
-                                  (uint32_t)((w >> (12 * t + 4)) & 0xF) };
-        float* coords = nullptr;
-        float* normals = nullptr;
-        uint32_t* conn = nullptr;
-#pragma unroll
-        for (int k = 0; k < kMaxLevels; ++k)
-            if (k == (int)l) { coords = a.coords[k]; normals = a.normals[k]; conn = a.conn[k]; }
-        const uint64_t id = g - a.toff[l];
-        float p[3][3], nn[3];
-        if constexpr (M == kModeDup) {
-            const double iso = iso_of(a, (int)l);
-#pragma unroll
-            for (int jj = 0; jj < 3; ++jj) {
-                const uint32_t qq = edge_ends(eds[jj]);
-                interpolate<T>(a, vol, iso, x + (qq & 1u), y + ((qq >> 1) & 1u), zc + ((qq >> 2) & 1u),
-                               x + ((qq >> 4) & 1u), y + ((qq >> 5) & 1u), zc + ((qq >> 6) & 1u), p[jj]);
-            }
-            face_normal(p[0], p[1], p[2], nn);
-            float* co = coords + 9 * id;
-#pragma unroll
-            for (int jj = 0; jj < 3; ++jj) {
-                co[3 * jj + 0] = p[jj][0];
-                co[3 * jj + 1] = p[jj][1];
-                co[3 * jj + 2] = p[jj][2];
-            }
-            normals[3 * id + 0] = nn[0];
-            normals[3 * id + 1] = nn[1];
-            normals[3 * id + 2] = nn[2];
-        } else {
-            const uint64_t* lev = a.bitvol + (uint64_t)l * level_words;
-            const uint64_t* pre_v = a.prefix + (uint64_t)(l * 2 + 0) * a.nrows;
-            const uint32_t* vpc_l = a.vpc + (uint64_t)l * level_words;
-            uint64_t vid[3];
-#pragma unroll
-            for (int jj = 0; jj < 3; ++jj) {
-                const uint32_t o = edge_owner(eds[jj]);
-                vid[jj] = vertex_id(a, lev, pre_v, vpc_l, x + (o & 1u), y + ((o >> 1) & 1u), zc + ((o >> 2) & 1u), o >> 3);
-            }
-            conn[3 * id + 0] = (uint32_t)vid[0];
-            conn[3 * id + 1] = (uint32_t)vid[1];
-            conn[3 * id + 2] = (uint32_t)vid[2];
-#pragma unroll
-            for (int jj = 0; jj < 3; ++jj) {
-                p[jj][0] = coords[3 * vid[jj] + 0];
-                p[jj][1] = coords[3 * vid[jj] + 1];
-                p[jj][2] = coords[3 * vid[jj] + 2];
-            }
-            face_normal(p[0], p[1], p[2], nn);
-            float* dst = M == kModeVertex ? a.fn + 3 * g : normals + 3 * id;
-            dst[0] = nn[0];
-            dst[1] = nn[1];
-            dst[2] = nn[2];
-        }
-    }
+    for (uint64_t g = (uint64_t)blockIdx.x * kBlock + threadIdx.x; g < n; g += (uint64_t)gridDim.x * kBlock)
+        triangle_item<T, M>(a, tri, g);
 }
 
 /* VertexNormal of every vertex: the float sum, in triangle order, of the face
@@ -1486,64 +1519,105 @@ __global__ __launch_bounds__(kBlock) void k_triangles(JobArgs a)
  * :569-605).  The four cells around the edge are visited in cell scan order;
  * a cell's triangles are contiguous records inside its chunk's range of its
  * cell row, found by a binary search on x, and taken in table order. */
+__device__ __forceinline__ void vertex_normal_item(const JobArgs& a, const uint64_t* tri, uint64_t g)
+{
+    const uint64_t level_words = (uint64_t)a.nz * a.ny * a.NC;
+    const uint32_t l = level_at(a.voff, a.nlev, g);
+    float* normals = nullptr;
+#pragma unroll
+    for (int k = 0; k < kMaxLevels; ++k) if (k == (int)l) normals = a.normals[k];
+    if (!normals) return;   // a level without outputs (not admitted)
+    const uint64_t key = a.vkey[g];
+    const int x = (int)(key & 0xFFFFFu), y = (int)((key >> 20) & 0xFFFFFu);
+    const int zv = (int)((key >> 40) & 0xFFFFFu);
+    const uint32_t axis = (uint32_t)(key >> 60);
+    const uint64_t* pre_t = a.prefix + (uint64_t)(l * 2 + 1) * a.nrows;
+    const uint32_t* cnt_t = a.counts + (uint64_t)(l * 2 + 1) * a.nrows;
+    const uint32_t* tpc_l = a.tpc + (uint64_t)l * level_words;
+    const uint64_t tl = a.toff[l] - pre_t[0];
+    const uint32_t around = around_nibbles(axis);
+    float nv[3] = { 0.0f, 0.0f, 0.0f };
+    for (int q = 0; q < 4; ++q) {
+        int cx, cy, cz;
+        around_cell(axis, q, x, y, zv, cx, cy, cz);
+        if (cx < 0 || cy < 0 || cz < 0) continue;
+        if (cx + 1 >= (int)a.nx || cy + 1 >= (int)a.ny || cz + 1 >= (int)a.nz) continue;
+        /* the triangles of cell row (cy, cz) are counted at node row (cy, cz + 1) */
+        const uint64_t row = (uint64_t)(cz + 1) * a.ny + (uint64_t)cy;
+        const uint32_t c = (uint32_t)cx >> 6;
+        const uint64_t rb = tl + pre_t[row];
+        uint64_t lo = rb + tpc_l[row * a.NC + c];
+        uint64_t hi = (c + 1 < a.NC) ? rb + tpc_l[row * a.NC + c + 1] : rb + cnt_t[row];
+        const uint64_t end = hi;
+        while (lo < hi) {   // first record with x >= cx
+            const uint64_t mid = lo + ((hi - lo) >> 1);
+            if ((int)(a.trec[mid] & 0xFFFFFu) < cx) lo = mid + 1; else hi = mid;
+        }
+        if (lo >= end) continue;   // (every cell around a crossing edge is active)
+        const uint64_t w = tri[a.tci[lo]];
+        const int nt = (int)(w >> 60);
+        const uint32_t me = (around >> (4 * q)) & 0xFu;
+        for (int t = 0; t < nt; ++t) {
+            const uint32_t e0 = (uint32_t)((w >> (12 * t)) & 0xF);
+            const uint32_t e1 = (uint32_t)((w >> (12 * t + 8)) & 0xF);
+            const uint32_t e2 = (uint32_t)((w >> (12 * t + 4)) & 0xF);
+            if (e0 != me && e1 != me && e2 != me) continue;
+            const float* f = a.fn + 3 * (lo + (uint64_t)t);
+            nv[0] = nv[0] + f[0];
+            nv[1] = nv[1] + f[1];
+            nv[2] = nv[2] + f[2];
+        }
+    }
+    const uint64_t id = g - a.voff[l];
+    normals[3 * id + 0] = nv[0];
+    normals[3 * id + 1] = nv[1];
+    normals[3 * id + 2] = nv[2];
+}
+
 __global__ __launch_bounds__(kBlock) void k_vertex_normals(JobArgs a)
 {
     __shared__ uint64_t tri[256];
     stage_table_block(tri);
     const uint64_t n = a.voff[a.nlev];
-    const uint64_t level_words = (uint64_t)a.nz * a.ny * a.NC;
+    for (uint64_t g = (uint64_t)blockIdx.x * kBlock + threadIdx.x; g < n; g += (uint64_t)gridDim.x * kBlock)
+        vertex_normal_item(a, tri, g);
+}
+
+/* Items of a batch of jobs: start[j] is job j's first item in one sequence
+ * over the launch's jobs (start[njobs]: the end); persistent grid. */
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_vertices_batch(const JobArgs* jobs, const uint64_t* start, uint32_t njobs)
+{
+    const uint64_t base = start[0], n = start[njobs] - base;
     for (uint64_t g = (uint64_t)blockIdx.x * kBlock + threadIdx.x; g < n; g += (uint64_t)gridDim.x * kBlock) {
-        const uint32_t l = level_at(a.voff, a.nlev, g);
-        const uint64_t key = a.vkey[g];
-        const int x = (int)(key & 0xFFFFFu), y = (int)((key >> 20) & 0xFFFFFu);
-        const int zv = (int)((key >> 40) & 0xFFFFFu);
-        const uint32_t axis = (uint32_t)(key >> 60);
-        const uint64_t* pre_t = a.prefix + (uint64_t)(l * 2 + 1) * a.nrows;
-        const uint32_t* cnt_t = a.counts + (uint64_t)(l * 2 + 1) * a.nrows;
-        const uint32_t* tpc_l = a.tpc + (uint64_t)l * level_words;
-        const uint64_t tl = a.toff[l] - pre_t[0];
-        const uint32_t around = around_nibbles(axis);
-        float nv[3] = { 0.0f, 0.0f, 0.0f };
-        for (int q = 0; q < 4; ++q) {
-            int cx, cy, cz;
-            around_cell(axis, q, x, y, zv, cx, cy, cz);
-            if (cx < 0 || cy < 0 || cz < 0) continue;
-            if (cx + 1 >= (int)a.nx || cy + 1 >= (int)a.ny || cz + 1 >= (int)a.nz) continue;
-            /* the triangles of cell row (cy, cz) are counted at node row (cy, cz + 1) */
-            const uint64_t row = (uint64_t)(cz + 1) * a.ny + (uint64_t)cy;
-            const uint32_t c = (uint32_t)cx >> 6;
-            const uint64_t rb = tl + pre_t[row];
-            uint64_t lo = rb + tpc_l[row * a.NC + c];
-            uint64_t hi = (c + 1 < a.NC) ? rb + tpc_l[row * a.NC + c + 1] : rb + cnt_t[row];
-            const uint64_t end = hi;
-            while (lo < hi) {   // first record with x >= cx
-                const uint64_t mid = lo + ((hi - lo) >> 1);
-                if ((int)(a.trec[mid] & 0xFFFFFu) < cx) lo = mid + 1; else hi = mid;
-            }
-            if (lo >= end) continue;   // (every cell around a crossing edge is active)
-            const uint64_t w = tri[a.tci[lo]];
-            const int nt = (int)(w >> 60);
-            const uint32_t me = (around >> (4 * q)) & 0xFu;
-            for (int t = 0; t < nt; ++t) {
-                const uint32_t e0 = (uint32_t)((w >> (12 * t)) & 0xF);
-                const uint32_t e1 = (uint32_t)((w >> (12 * t + 8)) & 0xF);
-                const uint32_t e2 = (uint32_t)((w >> (12 * t + 4)) & 0xF);
-                if (e0 != me && e1 != me && e2 != me) continue;
-                const float* f = a.fn + 3 * (lo + (uint64_t)t);
-                nv[0] = nv[0] + f[0];
-                nv[1] = nv[1] + f[1];
-                nv[2] = nv[2] + f[2];
-            }
-        }
-        float* normals = nullptr;
-#pragma unroll
-        for (int k = 0; k < kMaxLevels; ++k) if (k == (int)l) normals = a.normals[k];
-        const uint64_t id = g - a.voff[l];
-        normals[3 * id + 0] = nv[0];
-        normals[3 * id + 1] = nv[1];
-        normals[3 * id + 2] = nv[2];
+        const uint32_t j = find_job(start, njobs, g + base);
+        vertex_item<T>(jobs[j], g + base - start[j]);
     }
 }
+
+template <typename T, int M>
+__global__ __launch_bounds__(kBlock) void k_triangles_batch(const JobArgs* jobs, const uint64_t* start, uint32_t njobs)
+{
+    __shared__ uint64_t tri[256];
+    stage_table_block(tri);
+    const uint64_t base = start[0], n = start[njobs] - base;
+    for (uint64_t g = (uint64_t)blockIdx.x * kBlock + threadIdx.x; g < n; g += (uint64_t)gridDim.x * kBlock) {
+        const uint32_t j = find_job(start, njobs, g + base);
+        triangle_item<T, M>(jobs[j], tri, g + base - start[j]);
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_vertex_normals_batch(const JobArgs* jobs, const uint64_t* start, uint32_t njobs)
+{
+    __shared__ uint64_t tri[256];
+    stage_table_block(tri);
+    const uint64_t base = start[0], n = start[njobs] - base;
+    for (uint64_t g = (uint64_t)blockIdx.x * kBlock + threadIdx.x; g < n; g += (uint64_t)gridDim.x * kBlock) {
+        const uint32_t j = find_job(start, njobs, g + base);
+        vertex_normal_item(jobs[j], tri, g + base - start[j]);
+    }
+}
+
 
 /* ------------------------------------------------------------------------- */
 /* tile-count scan                                                             */
@@ -1896,6 +1970,48 @@ hipError_t launch_flat_t(const JobArgs& a, hipStream_t st)
     }
     if (a.vnormal && nv) hipLaunchKernelGGL(k_vertex_normals, dim3(flat_blocks(nv)), dim3(kBlock), 0, st, a);
     return hipGetLastError();
+}
+
+hipError_t launch_compact_batch(const JobArgs* jobs, const uint64_t* row_start, uint32_t njobs, uint64_t rows,
+                                hipStream_t st)
+{
+    uint64_t blocks = (rows / 4 + kWaves - 1) / kWaves + 1;
+    if (blocks > 16384) blocks = 16384;
+    hipLaunchKernelGGL(k_compact_batch, dim3((uint32_t)blocks), dim3(kBlock), 0, st, jobs, row_start, njobs);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_flat_batch_t(int mode, const JobArgs* jobs, const uint64_t* vstart, const uint64_t* tstart,
+                               uint32_t njobs, hipStream_t st)
+{
+    /* item counts live on the device: persistent grids */
+    const dim3 g(256 * 8), b(kBlock);
+    if (mode != kModeDup) hipLaunchKernelGGL((k_vertices_batch<T>), g, b, 0, st, jobs, vstart, njobs);
+    switch (mode) {
+    case kModeDup: hipLaunchKernelGGL((k_triangles_batch<T, kModeDup>), g, b, 0, st, jobs, tstart, njobs); break;
+    case kModePolygon: hipLaunchKernelGGL((k_triangles_batch<T, kModePolygon>), g, b, 0, st, jobs, tstart, njobs); break;
+    default: hipLaunchKernelGGL((k_triangles_batch<T, kModeVertex>), g, b, 0, st, jobs, tstart, njobs); break;
+    }
+    if (mode == kModeVertex) hipLaunchKernelGGL(k_vertex_normals_batch, g, b, 0, st, jobs, vstart, njobs);
+    return hipGetLastError();
+}
+
+hipError_t launch_flat_batch(int dtype, int mode, const JobArgs* jobs, const uint64_t* vstart, const uint64_t* tstart,
+                             uint32_t njobs, hipStream_t st)
+{
+    switch (dtype) {
+    case 0: return launch_flat_batch_t<int8_t>(mode, jobs, vstart, tstart, njobs, st);
+    case 1: return launch_flat_batch_t<int16_t>(mode, jobs, vstart, tstart, njobs, st);
+    case 2: return launch_flat_batch_t<int32_t>(mode, jobs, vstart, tstart, njobs, st);
+    case 3: return launch_flat_batch_t<int64_t>(mode, jobs, vstart, tstart, njobs, st);
+    case 4: return launch_flat_batch_t<uint8_t>(mode, jobs, vstart, tstart, njobs, st);
+    case 5: return launch_flat_batch_t<uint16_t>(mode, jobs, vstart, tstart, njobs, st);
+    case 6: return launch_flat_batch_t<uint32_t>(mode, jobs, vstart, tstart, njobs, st);
+    case 7: return launch_flat_batch_t<uint64_t>(mode, jobs, vstart, tstart, njobs, st);
+    case 8: return launch_flat_batch_t<float>(mode, jobs, vstart, tstart, njobs, st);
+    default: return launch_flat_batch_t<double>(mode, jobs, vstart, tstart, njobs, st);
+    }
 }
 
 hipError_t launch_flat_emit(int dtype, const JobArgs& a, hipStream_t st)

@@ -185,7 +185,7 @@ void run_batch(kmc_server* S, hipStream_t st, const std::vector<uint64_t>& ids)
     if (s == KMC_OK) {
         for (size_t i = 0; i < n; ++i) {
             batch_surface_sizes(b, (uint32_t)i, &out[i]);
-            obytes[i] = output_bytes(out[i]);
+            obytes[i] = output_bytes(out[i]) + batch_record_bytes(b, (uint32_t)i);
         }
         /* reserve outputs in queue order; the rest go back to the queue */
         std::lock_guard<std::mutex> lk(S->mu);
