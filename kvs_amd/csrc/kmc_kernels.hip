@@ -1518,7 +1518,8 @@ __global__ __launch_bounds__(kBlock) void k_triangles(JobArgs a)
  * normals of the triangles that use it (calculate_normals_on_vertex,
  * :569-605).  The four cells around the edge are visited in cell scan order;
  * a cell's triangles are contiguous records inside its chunk's range of its
- * cell row, found by a binary search on x, and taken in table order. */
+ * cell row, found by a binary search on x, and taken in table order.  The
+ * four cells' loads and search steps are issued together. */
 __device__ __forceinline__ void vertex_normal_item(const JobArgs& a, const uint64_t* tri, uint64_t g)
 {
     const uint64_t level_words = (uint64_t)a.nz * a.ny * a.NC;
@@ -1536,25 +1537,46 @@ __device__ __forceinline__ void vertex_normal_item(const JobArgs& a, const uint6
     const uint32_t* tpc_l = a.tpc + (uint64_t)l * level_words;
     const uint64_t tl = a.toff[l] - pre_t[0];
     const uint32_t around = around_nibbles(axis);
-    float nv[3] = { 0.0f, 0.0f, 0.0f };
+    uint64_t lo[4], hi[4];
+    int cxs[4];
+#pragma unroll
     for (int q = 0; q < 4; ++q) {
         int cx, cy, cz;
         around_cell(axis, q, x, y, zv, cx, cy, cz);
-        if (cx < 0 || cy < 0 || cz < 0) continue;
-        if (cx + 1 >= (int)a.nx || cy + 1 >= (int)a.ny || cz + 1 >= (int)a.nz) continue;
+        cxs[q] = cx;
+        lo[q] = hi[q] = 0;
+        if (cx < 0 || cy < 0 || cz < 0 || cx + 1 >= (int)a.nx || cy + 1 >= (int)a.ny || cz + 1 >= (int)a.nz) continue;
         /* the triangles of cell row (cy, cz) are counted at node row (cy, cz + 1) */
         const uint64_t row = (uint64_t)(cz + 1) * a.ny + (uint64_t)cy;
         const uint32_t c = (uint32_t)cx >> 6;
         const uint64_t rb = tl + pre_t[row];
-        uint64_t lo = rb + tpc_l[row * a.NC + c];
-        uint64_t hi = (c + 1 < a.NC) ? rb + tpc_l[row * a.NC + c + 1] : rb + cnt_t[row];
-        const uint64_t end = hi;
-        while (lo < hi) {   // first record with x >= cx
-            const uint64_t mid = lo + ((hi - lo) >> 1);
-            if ((int)(a.trec[mid] & 0xFFFFFu) < cx) lo = mid + 1; else hi = mid;
+        lo[q] = rb + tpc_l[row * a.NC + c];
+        hi[q] = (c + 1 < a.NC) ? rb + tpc_l[row * a.NC + c + 1] : rb + cnt_t[row];
+    }
+    /* first record with x >= cx in each range (every cell around a crossing
+     * edge is active, so it is the cell's first triangle) */
+    uint64_t end[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) end[q] = hi[q];
+    for (;;) {
+        bool more = false;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            if (lo[q] < hi[q]) {
+                const uint64_t mid = lo[q] + ((hi[q] - lo[q]) >> 1);
+                if ((int)(a.trec[mid] & 0xFFFFFu) < cxs[q]) lo[q] = mid + 1; else hi[q] = mid;
+                more = more || lo[q] < hi[q];
+            }
         }
-        if (lo >= end) continue;   // (every cell around a crossing edge is active)
-        const uint64_t w = tri[a.tci[lo]];
+        if (!more) break;
+    }
+    uint32_t ci[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) ci[q] = lo[q] < end[q] ? a.tci[lo[q]] : 0u;
+    float nv[3] = { 0.0f, 0.0f, 0.0f };
+    for (int q = 0; q < 4; ++q) {
+        if (lo[q] >= end[q]) continue;
+        const uint64_t w = tri[ci[q]];
         const int nt = (int)(w >> 60);
         const uint32_t me = (around >> (4 * q)) & 0xFu;
         for (int t = 0; t < nt; ++t) {
@@ -1562,7 +1584,7 @@ __device__ __forceinline__ void vertex_normal_item(const JobArgs& a, const uint6
             const uint32_t e1 = (uint32_t)((w >> (12 * t + 8)) & 0xF);
             const uint32_t e2 = (uint32_t)((w >> (12 * t + 4)) & 0xF);
             if (e0 != me && e1 != me && e2 != me) continue;
-            const float* f = a.fn + 3 * (lo + (uint64_t)t);
+            const float* f = a.fn + 3 * (lo[q] + (uint64_t)t);
             nv[0] = nv[0] + f[0];
             nv[1] = nv[1] + f[1];
             nv[2] = nv[2] + f[2];
