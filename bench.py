@@ -133,7 +133,8 @@ def main():
 
     # per-kernel device time (HIP events between the launches, on the same stream)
     lib.kmc_set_timing(1)
-    prof = {"classify": 0.0, "count": 0.0, "scan": 0.0, "emit": 0.0}
+    prof = {"classify": 0.0, "count": 0.0, "scan": 0.0, "compact": 0.0, "vertices": 0.0, "triangles": 0.0,
+            "vertex_normals": 0.0, "emit": 0.0}
     for _ in range(args.profile_steps):
         step()
         t = kmc.Timing()
@@ -223,36 +224,55 @@ def measured_traffic():
 
 
 def kernel_rooflines(prof, n, nlev, counts):
-    """Algorithmic bytes and achieved bandwidth of each kernel of one step.
-    Per launch: classify reads the volume (4 B per node) and writes the
-    classification masks (nlev bits per node, 64-node words); count reads the
-    masks and writes 2 x 4 B per node row and level; scan reads those counts
-    and writes 8 B prefixes; emit reads the masks, the two endpoint values of
-    every crossing edge (2 x 4 B per vertex) and writes the surfaces (12 B
-    coordinates + 12 B normals per vertex, 12 B connections per triangle)."""
+    """Algorithmic bytes and achieved bandwidth of each kernel of one step
+    (one launch each; the emit stage is compact + vertices + triangles +
+    vertex_normals).  Per launch:
+      classify        reads the volume (4 B per node), writes the classification
+                      masks (nlev bits per node, in 64-node words);
+      count           reads the masks, writes 2 x 4 B per node row and level and
+                      the in-row chunk prefixes (2 x 4 B per 64-node chunk and level);
+      scan            reads those row counts, writes 8 B prefixes;
+      compact         reads the chunk prefixes and the masks, writes the records
+                      (8 B per vertex, 8 + 1 B per triangle);
+      vertices        reads each vertex key and its edge's two values, writes its
+                      coordinates: 8 + 8 + 12 B per vertex;
+      triangles       reads each record, cube index and its three vertices'
+                      coordinates, writes connections and face normal:
+                      8 + 1 + 36 + 12 + 12 B per triangle;
+      vertex_normals  reads each vertex key and the face normals of the triangles
+                      using it (3 per triangle, 12 B each), writes the normal:
+                      8 + 12 B per vertex + 36 B per triangle."""
     nc = (n + 63) // 64
     masks = nlev * n * n * nc * 8
+    chunks = nlev * n * n * nc
     rows = n * n
     nv = sum(v for v, _ in counts)
     nt = sum(t for _, t in counts)
     algo = {
         "classify": n ** 3 * 4 + masks,
-        "count": masks + 2 * nlev * rows * 4,
+        "count": masks + 2 * nlev * rows * 4 + 8 * chunks,
         "scan": 2 * nlev * rows * (4 + 8),
-        "emit": masks + 8 * nv + 24 * nv + 12 * nt,
+        "compact": 8 * chunks + masks + 8 * nv + 9 * nt,
+        "vertices": 28 * nv,
+        "triangles": 69 * nt,
+        "vertex_normals": 20 * nv + 36 * nt,
     }
-    names = {"classify": "k_classify<float, 3>", "count": "k_count", "scan": "k_scan_*", "emit": "k_emit<float, 2>"}
+    names = {"classify": "k_classify<float, 3>", "count": "k_count", "scan": "k_scan_*", "compact": "k_compact",
+             "vertices": "k_vertices<float>", "triangles": "k_triangles<float, 2>",
+             "vertex_normals": "k_vertex_normals"}
     traffic = measured_traffic()
     out = {}
-    for k, ms in prof.items():
+    for k in names:
+        ms = prof[k]
         gbs = algo[k] / (ms / 1e3) / 1e9 if ms > 0 else 0.0
         out[k] = {"kernel": names[k], "ms": round(ms, 4), "algorithmic_bytes": algo[k], "achieved_gbs": round(gbs, 1),
                   "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": traffic.get(names[k])}
+    out["emit_total_ms"] = round(prof["emit"], 4)
     return out
 
 
 def dominant_roofline(kernels):
-    k = max(kernels, key=lambda x: kernels[x]["ms"])
+    k = max((x for x in kernels if isinstance(kernels[x], dict)), key=lambda x: kernels[x]["ms"])
     e = kernels[k]
     return {"bound": "hbm", "kernel": e["kernel"], "achieved": e["achieved_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": e["frac"], "traffic": e["traffic"], "algorithmic_bytes": e["algorithmic_bytes"],

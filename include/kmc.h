@@ -133,10 +133,14 @@ void kmc_surface_release(kmc_surface* surface);
  */
 typedef struct kmc_timing {
     double classify_ms;          /* values -> classification masks */
-    double count_ms;             /* per-row vertex / triangle counts */
+    double count_ms;             /* per-row vertex / triangle counts (and in-row chunk prefixes) */
     double scan_ms;              /* row prefixes */
-    double emit_ms;              /* coordinates, connections, normals */
+    double emit_ms;              /* coordinates, connections, normals: the four below together */
     uint32_t passes;
+    double compact_ms;           /* vertex keys and triangle records at their output ranks */
+    double vertices_ms;          /* coordinates */
+    double triangles_ms;         /* connections and face normals (with duplication: coordinates too) */
+    double vertex_normals_ms;    /* VertexNormal sums */
 } kmc_timing;
 
 void kmc_set_timing(int32_t enable);

@@ -420,17 +420,20 @@ void init_job(JobArgs& a, const kmc_volume* v, const void* dvals, const double* 
 namespace {
 thread_local bool t_timing = false;
 thread_local kmc_timing t_last;
-thread_local hipEvent_t t_ev[6] = { nullptr, nullptr, nullptr, nullptr, nullptr, nullptr };
+thread_local hipEvent_t t_ev[9] = { nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr };
 }
 
 struct PassTimer {
     hipStream_t st;
     bool on;
+    bool flat = false;
     explicit PassTimer(hipStream_t s) : st(s), on(t_timing)
     {
-        for (int i = 0; i < 6 && on; ++i)
+        for (int i = 0; i < 9 && on; ++i)
             if (!t_ev[i] && hipEventCreate(&t_ev[i]) != hipSuccess) { (void)hipGetLastError(); on = false; }
     }
+    /* events between the flat emit's kernels (after compaction, vertices, triangles) */
+    hipEvent_t* flat_marks() { flat = on; return on ? &t_ev[6] : nullptr; }
     bool mark(int i)
     {
         if (on && hipEventRecord(t_ev[i], st) != hipSuccess) { (void)hipGetLastError(); on = false; }
@@ -446,6 +449,17 @@ struct PassTimer {
         t_last.scan_ms += ms[2];
         t_last.emit_ms += ms[4];
         t_last.passes += 1;
+        if (flat) {
+            float f[4] = { 0, 0, 0, 0 };
+            (void)hipEventElapsedTime(&f[0], t_ev[4], t_ev[6]);
+            (void)hipEventElapsedTime(&f[1], t_ev[6], t_ev[7]);
+            (void)hipEventElapsedTime(&f[2], t_ev[7], t_ev[8]);
+            (void)hipEventElapsedTime(&f[3], t_ev[8], t_ev[5]);
+            t_last.compact_ms += f[0];
+            t_last.vertices_ms += f[1];
+            t_last.triangles_ms += f[2];
+            t_last.vertex_normals_ms += f[3];
+        }
     }
 };
 
@@ -568,7 +582,8 @@ kmc_status run_pass(const kmc_volume* v, const void* dvals, const double* iso, u
         bool any = false;
         for (uint32_t l = 0; l < nlev; ++l) any = any || outs[l].n_vertices > 0;
         timer.mark(4);
-        if (any && (flat ? launch_flat_emit(v->type, a, st) : launch_emit(v->type, a, lds, st)) != hipSuccess)
+        if (any && (flat ? launch_flat_emit(v->type, a, st, timer.flat_marks()) : launch_emit(v->type, a, lds, st)) !=
+                       hipSuccess)
             status = KMC_ERR_DEVICE;
         timer.mark(5);
         if (status == KMC_OK && hipStreamSynchronize(st) != hipSuccess) status = KMC_ERR_DEVICE;

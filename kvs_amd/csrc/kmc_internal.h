@@ -58,8 +58,10 @@ uint32_t levels_per_pass(const kmc_volume* v, int vnormal);
 hipError_t launch_emit(int dtype, const JobArgs& a, uint32_t lds_bytes, hipStream_t st);
 /* Flat emit of a pass (a.vpc / a.tpc written by its count pass, a.vkey /
  * a.trec / a.tci / a.fn and a.voff / a.toff set): compaction records, then
- * one thread per vertex, per triangle and (VertexNormal) per vertex normal. */
-hipError_t launch_flat_emit(int dtype, const JobArgs& a, hipStream_t st);
+ * one thread per vertex, per triangle and (VertexNormal) per vertex normal.
+ * marks: three events recorded after the compaction, the vertices and the
+ * triangles (timing), or null. */
+hipError_t launch_flat_emit(int dtype, const JobArgs& a, hipStream_t st, hipEvent_t* marks = nullptr);
 /* Flat emit of a batch group: compaction over the rows of its jobs, then the
  * per-item kernels over the items start[j] .. start[j + 1] of each job. */
 hipError_t launch_compact_batch(const JobArgs* jobs, const uint64_t* row_start, uint32_t njobs, uint64_t rows,

@@ -1309,14 +1309,21 @@ __device__ __forceinline__ void compact_rows(const JobArgs& a, const uint8_t* nt
             const uint64_t* pre_t = a.prefix + (uint64_t)(l * 2 + 1) * a.nrows;
             const uint64_t vb = a.voff[l] + (pre_v[r] - pre_v[0]);
             const uint64_t tb = a.toff[l] + (pre_t[r] - pre_t[0]);
+            const uint32_t* vpc = a.vpc + (uint64_t)l * level_words + r * a.NC;
+            const uint32_t* tpc = a.tpc + (uint64_t)l * level_words + r * a.NC;
             for (uint32_t c = cl; c < a.NC; c += CS) {
+                /* chunks without crossing edges or active cells (most of them) are
+                 * skipped on the count pass's prefixes alone */
+                const uint32_t v0 = vpc[c], t0 = tpc[c];
+                const uint32_t v1 = c + 1 < a.NC ? vpc[c + 1] : a.counts[(uint64_t)(l * 2 + 0) * a.nrows + r];
+                const uint32_t t1 = c + 1 < a.NC ? tpc[c + 1] : a.counts[(uint64_t)(l * 2 + 1) * a.nrows + r];
+                if ((!verts || v1 == v0) && t1 == t0) continue;
                 RowChunk o;
                 row_chunk(a, lev, y, z, c, true, o);
-                const uint64_t w = (uint64_t)l * level_words + r * a.NC + c;
                 if (verts) {
                     uint64_t m = o.x | o.y | o.z;
                     if (m) {
-                        uint64_t id = vb + a.vpc[w];
+                        uint64_t id = vb + v0;
                         const uint64_t rowkey = (uint64_t)y << 20 | (uint64_t)z << 40;
                         while (m) {
                             const uint32_t i = (uint32_t)__ffsll((unsigned long long)m) - 1u;
@@ -1330,7 +1337,7 @@ __device__ __forceinline__ void compact_rows(const JobArgs& a, const uint8_t* nt
                 }
                 uint64_t m = o.act;
                 if (m) {
-                    uint64_t id = tb + a.tpc[w];
+                    uint64_t id = tb + t0;
                     const uint64_t cellkey = (uint64_t)y << 20 | (uint64_t)(z - 1) << 40;
                     while (m) {
                         const uint32_t i = (uint32_t)__ffsll((unsigned long long)m) - 1u;
@@ -1977,10 +1984,11 @@ static uint32_t flat_blocks(uint64_t items)
 }
 
 template <typename T>
-hipError_t launch_flat_t(const JobArgs& a, hipStream_t st)
+hipError_t launch_flat_t(const JobArgs& a, hipStream_t st, hipEvent_t* marks)
 {
     const uint64_t nv = a.voff[a.nlev], nt = a.toff[a.nlev];
     if (!a.dup && nv) hipLaunchKernelGGL((k_vertices<T>), dim3(flat_blocks(nv)), dim3(kBlock), 0, st, a);
+    if (marks) (void)hipEventRecord(marks[1], st);
     if (nt) {
         switch (emit_mode(a)) {
         case kModeDup: hipLaunchKernelGGL((k_triangles<T, kModeDup>), dim3(flat_blocks(nt)), dim3(kBlock), 0, st, a); break;
@@ -1990,6 +1998,7 @@ hipError_t launch_flat_t(const JobArgs& a, hipStream_t st)
         default: hipLaunchKernelGGL((k_triangles<T, kModeVertex>), dim3(flat_blocks(nt)), dim3(kBlock), 0, st, a); break;
         }
     }
+    if (marks) (void)hipEventRecord(marks[2], st);
     if (a.vnormal && nv) hipLaunchKernelGGL(k_vertex_normals, dim3(flat_blocks(nv)), dim3(kBlock), 0, st, a);
     return hipGetLastError();
 }
@@ -2036,23 +2045,24 @@ hipError_t launch_flat_batch(int dtype, int mode, const JobArgs* jobs, const uin
     }
 }
 
-hipError_t launch_flat_emit(int dtype, const JobArgs& a, hipStream_t st)
+hipError_t launch_flat_emit(int dtype, const JobArgs& a, hipStream_t st, hipEvent_t* marks)
 {
     const uint64_t G = 64u >> a.cshift;
     uint64_t blocks = ((a.nrows + G - 1) / G + kWaves - 1) / kWaves;
     if (blocks > 16384) blocks = 16384;
     hipLaunchKernelGGL(k_compact, dim3((uint32_t)blocks), dim3(kBlock), 0, st, a);
+    if (marks) (void)hipEventRecord(marks[0], st);
     switch (dtype) {
-    case 0: return launch_flat_t<int8_t>(a, st);
-    case 1: return launch_flat_t<int16_t>(a, st);
-    case 2: return launch_flat_t<int32_t>(a, st);
-    case 3: return launch_flat_t<int64_t>(a, st);
-    case 4: return launch_flat_t<uint8_t>(a, st);
-    case 5: return launch_flat_t<uint16_t>(a, st);
-    case 6: return launch_flat_t<uint32_t>(a, st);
-    case 7: return launch_flat_t<uint64_t>(a, st);
-    case 8: return launch_flat_t<float>(a, st);
-    default: return launch_flat_t<double>(a, st);
+    case 0: return launch_flat_t<int8_t>(a, st, marks);
+    case 1: return launch_flat_t<int16_t>(a, st, marks);
+    case 2: return launch_flat_t<int32_t>(a, st, marks);
+    case 3: return launch_flat_t<int64_t>(a, st, marks);
+    case 4: return launch_flat_t<uint8_t>(a, st, marks);
+    case 5: return launch_flat_t<uint16_t>(a, st, marks);
+    case 6: return launch_flat_t<uint32_t>(a, st, marks);
+    case 7: return launch_flat_t<uint64_t>(a, st, marks);
+    case 8: return launch_flat_t<float>(a, st, marks);
+    default: return launch_flat_t<double>(a, st, marks);
     }
 }
 

@@ -150,6 +150,10 @@ class Timing(ctypes.Structure):
         ("scan_ms", ctypes.c_double),
         ("emit_ms", ctypes.c_double),
         ("passes", ctypes.c_uint32),
+        ("compact_ms", ctypes.c_double),
+        ("vertices_ms", ctypes.c_double),
+        ("triangles_ms", ctypes.c_double),
+        ("vertex_normals_ms", ctypes.c_double),
     ]
 
 _lib = None
