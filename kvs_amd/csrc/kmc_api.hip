@@ -485,8 +485,9 @@ kmc_status run_pass(const kmc_volume* v, const void* dvals, const double* iso, u
     const size_t chunks = (size_t)nlev * a.nz * a.ny * a.NC;
     uint64_t* bitvol = (uint64_t*)device_alloc(chunks * sizeof(uint64_t));
     /* flat emit (compaction records + one thread per item) unless the records'
-     * 20-bit y / z fields do not fit or KMC_EMIT=team asks for the tile teams */
-    static const char* env_emit = std::getenv("KMC_EMIT");
+     * 20-bit y / z fields do not fit or KMC_EMIT=team asks for the tile teams
+     * (read per call) */
+    const char* env_emit = std::getenv("KMC_EMIT");
     const bool flat = a.ny <= (1u << 20) && a.nz <= (1u << 20) && !(env_emit && std::strcmp(env_emit, "team") == 0);
     uint32_t* cpre = flat ? (uint32_t*)device_alloc(2 * chunks * sizeof(uint32_t)) : nullptr;
     void* recs = nullptr;

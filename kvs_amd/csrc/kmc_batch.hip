@@ -707,7 +707,7 @@ kmc_status create_impl(const kmc_job* jobs, uint32_t n_jobs, uint32_t flags, hip
         b->prefix_d = (uint64_t*)device_alloc(coff * 8);
         b->scan_aux_d = (uint64_t*)device_alloc((b->scan_parts + 1) * 8);
         b->bitvol_d = (uint64_t*)device_alloc(boff * 8);
-        static const char* env_emit = std::getenv("KMC_EMIT");
+        const char* env_emit = std::getenv("KMC_EMIT");
         b->flat_ok = !(env_emit && std::strcmp(env_emit, "team") == 0);
         for (const Pass& p : b->passes) b->flat_ok = b->flat_ok && p.a.ny <= (1u << 20) && p.a.nz <= (1u << 20);
         b->flat = b->flat_ok;

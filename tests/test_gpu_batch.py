@@ -174,3 +174,17 @@ def test_batch_rejects_bad_job_like_plain_call():
 
 def test_empty_batch():
     assert kmc.batch_extract([]) == []
+
+
+def test_batch_on_tile_team_emit(monkeypatch):
+    # the batched form's tile-team emit (KMC_EMIT=team at creation) equals the plain calls
+    monkeypatch.setenv("KMC_EMIT", "team")
+    specs = mixed_specs(seed=29)
+    print("batch of", len(specs), "jobs on the tile teams", flush=True)
+    vols = [kmc.make_volume(v, min_max=mm, coords=co) for (v, _, _, _, mm, co) in specs]
+    res = kmc.batch_extract([(vol, lv, n, d) for vol, (v, lv, n, d, mm, co) in zip(vols, specs)])
+    monkeypatch.delenv("KMC_EMIT")
+    for j, ((v, levels, normal, dup, mm, co), surfs) in enumerate(zip(specs, res)):
+        assert len(surfs) == len(levels)
+        for lv, got in zip(levels, surfs):
+            assert_same_surface(got, plain(v, lv, normal, dup, mm, co), f"team job {j} level {lv}")

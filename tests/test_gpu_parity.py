@@ -160,3 +160,32 @@ def test_many_rows_across_groups():
     v = datasets.blobs((40, 67, 9), seed=13, nblobs=5)
     for normal, dup in MODES:
         check(v, 0.3, normal, dup)
+
+
+def test_tile_team_emit(monkeypatch):
+    # the tile-team emit (used for volumes whose y or z exceed the flat records'
+    # 20-bit fields, and an autotuner candidate of the batched form) on the same
+    # cases: KMC_EMIT=team selects it per call
+    monkeypatch.setenv("KMC_EMIT", "team")
+    rng = np.random.default_rng(31)
+    for dtype in (np.uint8, np.int16, np.float32, np.float64):
+        shape = tuple(int(s) for s in rng.integers(2, 30, 3))
+        v, iso = random_volume(rng, dtype, shape)
+        for normal, dup in MODES:
+            check(v, iso, normal, dup)
+    for nx in (63, 64, 65, 129):
+        v = rng.standard_normal((6, 5, nx)).astype(np.float32)
+        for normal, dup in MODES:
+            check(v, 0.1, normal, dup)
+    v = datasets.blobs((37, 29, 101), seed=5, nblobs=6)
+    for normal, dup in MODES:
+        check(v, 0.3, normal, dup)
+    v = rng.standard_normal((3, 4, 4100)).astype(np.float32)
+    for normal, dup in MODES:
+        check(v, 0.2, normal, dup)
+    v = datasets.blobs(48, seed=77)
+    levels = [0.2, 0.45, 0.7]
+    for normal, dup in MODES:
+        got = kmc.extract_levels(v, levels, normal, dup)
+        for lv, g in zip(levels, got):
+            assert_same_surface(g, oracle.extract(v, lv, normal, dup), f"team levels {lv} {normal} {dup}")
