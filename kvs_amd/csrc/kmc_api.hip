@@ -566,7 +566,7 @@ kmc_status run_pass(const kmc_volume* v, const void* dvals, const double* iso, u
         }
         const uint64_t nv = a.voff[nlev], nt = a.toff[nlev];
         const size_t bytes = align_up(nv * 8, 256) + align_up(nt * 8, 256) + align_up(nt, 256) +
-                             (a.vnormal ? nt * 12 : 0);
+                             (a.vnormal ? align_up(nt * 12, 256) + nv * 16 : 0);
         if (bytes) {
             recs = device_alloc(bytes);
             if (!recs) status = KMC_ERR_OUT_OF_MEMORY;
@@ -577,6 +577,8 @@ kmc_status run_pass(const kmc_volume* v, const void* dvals, const double* iso, u
             a.trec = (uint64_t*)p; p += align_up(nt * 8, 256);
             a.tci = p; p += align_up(nt, 256);
             a.fn = a.vnormal ? (float*)p : nullptr;
+            if (a.vnormal) p += align_up(nt * 12, 256);
+            a.vslot = a.vnormal ? (uint32_t*)p : nullptr;
         }
     }
     if (status == KMC_OK) {

@@ -35,11 +35,11 @@ struct SegOut {
 };
 
 /* Flat-emit records of a pass: vertex keys, triangle records, cube indices,
- * face normals (VertexNormal), each 256-byte aligned. */
+ * face normals and vertex slots (VertexNormal), each 256-byte aligned. */
 __host__ __device__ inline uint64_t align256(uint64_t x) { return (x + 255) & ~255ull; }
 __host__ __device__ inline uint64_t record_bytes(uint64_t nv, uint64_t nt, bool vn)
 {
-    return align256(nv * 8) + align256(nt * 8) + align256(nt) + (vn ? align256(nt * 12) : 0);
+    return align256(nv * 8) + align256(nt * 8) + align256(nt) + (vn ? align256(nt * 12) + align256(nv * 16) : 0);
 }
 
 /* Point pass a's records at base (or at nothing when base is null) and set its
@@ -52,7 +52,7 @@ __host__ __device__ inline void place_records(JobArgs& a, const uint64_t* nv, co
         a.toff[l + 1] = a.toff[l] + (base ? nt[l] : 0);
     }
     if (!base) {
-        a.vkey = nullptr; a.trec = nullptr; a.tci = nullptr; a.fn = nullptr;
+        a.vkey = nullptr; a.trec = nullptr; a.tci = nullptr; a.fn = nullptr; a.vslot = nullptr;
         return;
     }
     const uint64_t V = a.voff[a.nlev], T = a.toff[a.nlev];
@@ -61,6 +61,8 @@ __host__ __device__ inline void place_records(JobArgs& a, const uint64_t* nv, co
     a.trec = (uint64_t*)p; p += align256(T * 8);
     a.tci = p; p += align256(T);
     a.fn = a.vnormal ? (float*)p : nullptr;
+    if (a.vnormal) p += align256(T * 12);
+    a.vslot = a.vnormal ? (uint32_t*)p : nullptr;
 }
 
 constexpr uint32_t kLayoutBlock = 256;

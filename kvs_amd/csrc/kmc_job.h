@@ -91,6 +91,10 @@ struct JobArgs {
     uint64_t* trec;
     uint8_t* tci;
     float* fn;
+    // VertexNormal: per vertex, for each of the four cells around its edge (in
+    // cell scan order) the item index of that cell's first triangle record
+    // (~0u: no such cell); written by the triangles that use the edge
+    uint32_t* vslot;
     uint64_t voff[kMaxLevels + 1];
     uint64_t toff[kMaxLevels + 1];
     // per-level outputs

@@ -1422,6 +1422,7 @@ __device__ __forceinline__ void vertex_item(const JobArgs& a, uint64_t g)
     co[3 * id + 0] = p[0];
     co[3 * id + 1] = p[1];
     co[3 * id + 2] = p[2];
+    if (a.vslot) *(uint4*)(a.vslot + 4 * g) = make_uint4(~0u, ~0u, ~0u, ~0u);
 }
 
 template <typename T>
@@ -1497,6 +1498,27 @@ __device__ __forceinline__ void triangle_item(const JobArgs& a, const uint64_t* 
         conn[3 * id + 0] = (uint32_t)vid[0];
         conn[3 * id + 1] = (uint32_t)vid[1];
         conn[3 * id + 2] = (uint32_t)vid[2];
+        if constexpr (M == kModeVertex) {
+            /* the first triangle of this cell that uses an edge tells the edge's
+             * vertex where the cell's records start; the cell sits at position q
+             * around the edge (around_cell) */
+#pragma unroll
+            for (int jj = 0; jj < 3; ++jj) {
+                bool first = true;
+                for (uint32_t u = 0; u < t; ++u) {
+                    const uint32_t f0 = (uint32_t)((w >> (12 * u)) & 0xF), f1 = (uint32_t)((w >> (12 * u + 4)) & 0xF);
+                    const uint32_t f2 = (uint32_t)((w >> (12 * u + 8)) & 0xF);
+                    first = first && f0 != eds[jj] && f1 != eds[jj] && f2 != eds[jj];
+                }
+                if (!first) continue;
+                const uint32_t o = edge_owner(eds[jj]);
+                const uint32_t dx = o & 1u, dy = (o >> 1) & 1u, dz = (o >> 2) & 1u, ax = o >> 3;
+                const uint32_t q = ax == 0 ? ((dy ? 0u : 1u) | (dz ? 0u : 2u))
+                                 : ax == 1 ? ((dx ? 0u : 1u) | (dz ? 0u : 2u))
+                                           : ((dx ? 0u : 1u) | (dy ? 0u : 2u));
+                a.vslot[4 * (a.voff[l] + vid[jj]) + q] = (uint32_t)(g - t);
+            }
+        }
 #pragma unroll
         for (int jj = 0; jj < 3; ++jj) {
             p[jj][0] = coords[3 * vid[jj] + 0];
@@ -1523,66 +1545,26 @@ __global__ __launch_bounds__(kBlock) void k_triangles(JobArgs a)
 
 /* VertexNormal of every vertex: the float sum, in triangle order, of the face
  * normals of the triangles that use it (calculate_normals_on_vertex,
- * :569-605).  The four cells around the edge are visited in cell scan order;
- * a cell's triangles are contiguous records inside its chunk's range of its
- * cell row, found by a binary search on x, and taken in table order.  The
- * four cells' loads and search steps are issued together. */
+ * :569-605).  The four cells around the edge are visited in cell scan order
+ * (their first records come from the vertex's slots, written by the
+ * triangles) and each cell's triangles that use the edge in table order. */
 __device__ __forceinline__ void vertex_normal_item(const JobArgs& a, const uint64_t* tri, uint64_t g)
 {
-    const uint64_t level_words = (uint64_t)a.nz * a.ny * a.NC;
     const uint32_t l = level_at(a.voff, a.nlev, g);
     float* normals = nullptr;
 #pragma unroll
     for (int k = 0; k < kMaxLevels; ++k) if (k == (int)l) normals = a.normals[k];
     if (!normals) return;   // a level without outputs (not admitted)
-    const uint64_t key = a.vkey[g];
-    const int x = (int)(key & 0xFFFFFu), y = (int)((key >> 20) & 0xFFFFFu);
-    const int zv = (int)((key >> 40) & 0xFFFFFu);
-    const uint32_t axis = (uint32_t)(key >> 60);
-    const uint64_t* pre_t = a.prefix + (uint64_t)(l * 2 + 1) * a.nrows;
-    const uint32_t* cnt_t = a.counts + (uint64_t)(l * 2 + 1) * a.nrows;
-    const uint32_t* tpc_l = a.tpc + (uint64_t)l * level_words;
-    const uint64_t tl = a.toff[l] - pre_t[0];
+    const uint32_t axis = (uint32_t)(a.vkey[g] >> 60);
     const uint32_t around = around_nibbles(axis);
-    uint64_t lo[4], hi[4];
-    int cxs[4];
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        int cx, cy, cz;
-        around_cell(axis, q, x, y, zv, cx, cy, cz);
-        cxs[q] = cx;
-        lo[q] = hi[q] = 0;
-        if (cx < 0 || cy < 0 || cz < 0 || cx + 1 >= (int)a.nx || cy + 1 >= (int)a.ny || cz + 1 >= (int)a.nz) continue;
-        /* the triangles of cell row (cy, cz) are counted at node row (cy, cz + 1) */
-        const uint64_t row = (uint64_t)(cz + 1) * a.ny + (uint64_t)cy;
-        const uint32_t c = (uint32_t)cx >> 6;
-        const uint64_t rb = tl + pre_t[row];
-        lo[q] = rb + tpc_l[row * a.NC + c];
-        hi[q] = (c + 1 < a.NC) ? rb + tpc_l[row * a.NC + c + 1] : rb + cnt_t[row];
-    }
-    /* first record with x >= cx in each range (every cell around a crossing
-     * edge is active, so it is the cell's first triangle) */
-    uint64_t end[4];
-#pragma unroll
-    for (int q = 0; q < 4; ++q) end[q] = hi[q];
-    for (;;) {
-        bool more = false;
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            if (lo[q] < hi[q]) {
-                const uint64_t mid = lo[q] + ((hi[q] - lo[q]) >> 1);
-                if ((int)(a.trec[mid] & 0xFFFFFu) < cxs[q]) lo[q] = mid + 1; else hi[q] = mid;
-                more = more || lo[q] < hi[q];
-            }
-        }
-        if (!more) break;
-    }
+    const uint4 sl = *(const uint4*)(a.vslot + 4 * g);
+    const uint32_t first[4] = { sl.x, sl.y, sl.z, sl.w };
     uint32_t ci[4];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) ci[q] = lo[q] < end[q] ? a.tci[lo[q]] : 0u;
+    for (int q = 0; q < 4; ++q) ci[q] = first[q] != ~0u ? a.tci[first[q]] : 0u;
     float nv[3] = { 0.0f, 0.0f, 0.0f };
     for (int q = 0; q < 4; ++q) {
-        if (lo[q] >= end[q]) continue;
+        if (first[q] == ~0u) continue;
         const uint64_t w = tri[ci[q]];
         const int nt = (int)(w >> 60);
         const uint32_t me = (around >> (4 * q)) & 0xFu;
@@ -1591,7 +1573,7 @@ __device__ __forceinline__ void vertex_normal_item(const JobArgs& a, const uint6
             const uint32_t e1 = (uint32_t)((w >> (12 * t + 8)) & 0xF);
             const uint32_t e2 = (uint32_t)((w >> (12 * t + 4)) & 0xF);
             if (e0 != me && e1 != me && e2 != me) continue;
-            const float* f = a.fn + 3 * (lo[q] + (uint64_t)t);
+            const float* f = a.fn + 3 * ((uint64_t)first[q] + (uint64_t)t);
             nv[0] = nv[0] + f[0];
             nv[1] = nv[1] + f[1];
             nv[2] = nv[2] + f[2];
