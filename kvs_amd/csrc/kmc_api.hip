@@ -490,16 +490,21 @@ kmc_status run_pass(const kmc_volume* v, const void* dvals, const double* iso, u
     const char* env_emit = std::getenv("KMC_EMIT");
     const bool flat = a.ny <= (1u << 20) && a.nz <= (1u << 20) && !(env_emit && std::strcmp(env_emit, "team") == 0);
     uint32_t* cpre = flat ? (uint32_t*)device_alloc(2 * chunks * sizeof(uint32_t)) : nullptr;
+    ChunkCross* cross = (flat && !dup) ? (ChunkCross*)device_alloc(chunks * sizeof(ChunkCross)) : nullptr;
     void* recs = nullptr;
     auto cleanup = [&]() {
         device_free(counts); device_free(prefix); device_free(aux); device_free(bitvol); device_free(cpre);
-        device_free(recs);
+        device_free(cross); device_free(recs);
     };
-    if (!counts || !prefix || !aux || !bitvol || (flat && !cpre)) { cleanup(); return KMC_ERR_OUT_OF_MEMORY; }
+    if (!counts || !prefix || !aux || !bitvol || (flat && !cpre) || (flat && !dup && !cross)) {
+        cleanup();
+        return KMC_ERR_OUT_OF_MEMORY;
+    }
     a.bitvol = bitvol;
     if (flat) {
         a.vpc = cpre;
         a.tpc = cpre + chunks;
+        a.cross = cross;
     }
     uint64_t* partials = aux;
     uint64_t* totals = aux + nparts;

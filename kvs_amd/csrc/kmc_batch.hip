@@ -303,6 +303,7 @@ struct kmc_batch {
     bool flat;                              // flat emit (else the tile teams)
     bool flat_ok;                           // every pass's rows fit the flat records' 20-bit y / z
     uint32_t* cpre_d;                       // in-row chunk prefixes of the count pass (flat)
+    ChunkCross* cross_d;                    // chunk crossing records of the compaction (flat)
     uint64_t* istart_d;                     // per-pass first vertex / triangle item (flat)
     std::vector<uint64_t> istart_h;
     uint8_t* arena_r;                       // flat-emit records
@@ -359,6 +360,7 @@ void destroy(kmc_batch* b)
     device_free(b->scan_aux_d);
     device_free(b->bitvol_d);
     device_free(b->cpre_d);
+    device_free(b->cross_d);
     device_free(b->istart_d);
     device_free(b->arena_r);
     device_free(b->segout_d);
@@ -420,6 +422,7 @@ kmc_status plan_tiles(kmc_batch* b, uint32_t r_max, uint32_t queue_cap)
     for (Pass& p : b->passes) {
         p.a.vpc = b->flat ? b->cpre_d + p.bitvol_off : nullptr;
         p.a.tpc = b->flat ? b->cpre_d + b->bitvol_words + p.bitvol_off : nullptr;
+        p.a.cross = b->flat ? b->cross_d + p.bitvol_off : nullptr;
     }
     b->jobs_h.resize(b->passes.size());
     for (size_t i = 0; i < b->passes.size(); ++i) b->jobs_h[i] = b->passes[i].a;
@@ -715,8 +718,9 @@ kmc_status create_impl(const kmc_job* jobs, uint32_t n_jobs, uint32_t flags, hip
         b->flat = b->flat_ok;
         if (b->flat_ok) {
             b->cpre_d = (uint32_t*)device_alloc(2 * boff * 4);
+            b->cross_d = (ChunkCross*)device_alloc(boff * sizeof(ChunkCross));
             b->istart_d = (uint64_t*)device_alloc((2 * b->passes.size() + 2) * 8);
-            if (!b->cpre_d || !b->istart_d) {
+            if (!b->cpre_d || !b->cross_d || !b->istart_d) {
                 (void)hipGetLastError();
                 destroy(b);
                 return KMC_ERR_OUT_OF_MEMORY;
@@ -874,7 +878,7 @@ hipError_t batch_launch_emit(kmc_batch* b, hipStream_t st)
  * chunk prefixes, tables). */
 uint64_t batch_work_bytes(const kmc_batch* b)
 {
-    return b->n_counts * 12 + b->bitvol_words * (b->flat_ok ? 16 : 8) + b->passes.size() * (sizeof(JobArgs) + 16) +
+    return b->n_counts * 12 + b->bitvol_words * (b->flat_ok ? 48 : 8) + b->passes.size() * (sizeof(JobArgs) + 16) +
            b->seg_pass_level.size() * (4 + sizeof(SegOut)) + (uint64_t)(b->scan_parts + 1) * 8;
 }
 

@@ -46,6 +46,14 @@ constexpr int kEmitTeams = 4;        // emit workgroup: 4 independent one-wave t
 constexpr int kEmitBlock = 64 * kEmitTeams;
 constexpr uint32_t kTableBytes = 256 * 8;   // case table at the start of the emit workgroup's LDS
 
+/* Crossing edges of one 64-node chunk with a vertex (flat emit): the
+ * level-relative id of its first vertex and the x / y / z crossing-edge masks
+ * of its nodes. */
+struct alignas(32) ChunkCross {
+    uint64_t base;
+    uint64_t x, y, z;
+};
+
 struct JobArgs {
     const void* values;
     uint32_t nx, ny, nz;
@@ -91,6 +99,8 @@ struct JobArgs {
     uint64_t* trec;
     uint8_t* tci;
     float* fn;
+    // per chunk (bit volume layout), written by the compaction for chunks with a vertex
+    ChunkCross* cross;
     // VertexNormal: per vertex, for each of the four cells around its edge (in
     // cell scan order) the item index of that cell's first triangle record
     // (~0u: no such cell); written by the triangles that use the edge

@@ -1323,6 +1323,12 @@ __device__ __forceinline__ void compact_rows(const JobArgs& a, const uint8_t* nt
                 if (verts) {
                     uint64_t m = o.x | o.y | o.z;
                     if (m) {
+                        ChunkCross cc;
+                        cc.base = (pre_v[r] - pre_v[0]) + v0;
+                        cc.x = o.x;
+                        cc.y = o.y;
+                        cc.z = o.z;
+                        a.cross[(uint64_t)l * level_words + r * a.NC + c] = cc;
                         uint64_t id = vb + v0;
                         const uint64_t rowkey = (uint64_t)y << 20 | (uint64_t)z << 40;
                         while (m) {
@@ -1387,21 +1393,20 @@ __global__ __launch_bounds__(kBlock) void k_compact_batch(const JobArgs* jobs, c
 }
 
 /* Vertex id of the shared vertex on the edge of node (x, y, z) along axis:
- * the row's prefix + the chunk's prefix within the row + the crossing edges
- * of the chunk's nodes before x (and of x's lower axes). */
-__device__ __forceinline__ uint64_t vertex_id(const JobArgs& a, const uint64_t* lev, const uint64_t* pre_v,
-                                              const uint32_t* vpc_l, uint32_t x, uint32_t y, uint32_t z, uint32_t axis)
+ * the first vertex of its chunk (row prefix + chunk prefix within the row)
+ * + the crossing edges of the chunk's nodes before x (and of x's lower
+ * axes), from the chunk's compaction record. */
+__device__ __forceinline__ uint64_t vertex_id(const JobArgs& a, const ChunkCross* cross_l, uint32_t x, uint32_t y,
+                                              uint32_t z, uint32_t axis)
 {
-    RowChunk o;
     const uint32_t c = x >> 6, i = x & 63u;
-    row_chunk(a, lev, y, z, c, false, o);
-    const uint64_t row = (uint64_t)z * a.ny + y;
+    const ChunkCross o = cross_l[((uint64_t)z * a.ny + y) * a.NC + c];
     const uint64_t lo = (1ull << i) - 1ull;
-    uint64_t r = vpc_l[row * a.NC + c];
+    uint64_t r = o.base;
     r += (uint64_t)(__popcll(o.x & lo) + __popcll(o.y & lo) + __popcll(o.z & lo));
     if (axis >= 1) r += (o.x >> i) & 1ull;
     if (axis >= 2) r += (o.y >> i) & 1ull;
-    return (pre_v[row] - pre_v[0]) + r;
+    return r;
 }
 
 /* Coordinates of vertex g of the job's items (calculate_isopoints, :365-437). */
@@ -1486,14 +1491,12 @@ __device__ __forceinline__ void triangle_item(const JobArgs& a, const uint64_t* 
         normals[3 * id + 1] = nn[1];
         normals[3 * id + 2] = nn[2];
     } else {
-        const uint64_t* lev = a.bitvol + (uint64_t)l * level_words;
-        const uint64_t* pre_v = a.prefix + (uint64_t)(l * 2 + 0) * a.nrows;
-        const uint32_t* vpc_l = a.vpc + (uint64_t)l * level_words;
+        const ChunkCross* cross_l = a.cross + (uint64_t)l * level_words;
         uint64_t vid[3];
 #pragma unroll
         for (int jj = 0; jj < 3; ++jj) {
             const uint32_t o = edge_owner(eds[jj]);
-            vid[jj] = vertex_id(a, lev, pre_v, vpc_l, x + (o & 1u), y + ((o >> 1) & 1u), zc + ((o >> 2) & 1u), o >> 3);
+            vid[jj] = vertex_id(a, cross_l, x + (o & 1u), y + ((o >> 1) & 1u), zc + ((o >> 2) & 1u), o >> 3);
         }
         conn[3 * id + 0] = (uint32_t)vid[0];
         conn[3 * id + 1] = (uint32_t)vid[1];
