@@ -1282,93 +1282,127 @@ __device__ __forceinline__ uint32_t level_at(const uint64_t* off, uint32_t nlev,
     return l;
 }
 
-/* Compaction: every crossing edge writes its key at its vertex id and every
- * triangle its cell and table index t (plus the cell's cube index) at its
- * triangle id -- row prefix + chunk prefix within the row + rank within the
- * chunk, i.e. vertices in node scan order x {x,y,z}-edge and triangles in
- * cell scan order x table order (calculate_isopoints :365-437,
- * connect_isopoints :439-521, extract_surfaces_with_duplication :155-243).
- * Same row groups and lanes as the count pass. */
-__device__ __forceinline__ void compact_rows(const JobArgs& a, const uint8_t* ntri, uint64_t r0, uint64_t r1,
+/* One non-empty chunk of the compaction, queued as y | z << 20 | level << 40
+ * | chunk << 44. */
+__device__ __forceinline__ void compact_chunk(const JobArgs& a, const uint8_t* ntri, uint64_t e)
+{
+    const uint32_t y = (uint32_t)(e & 0xFFFFFu), z = (uint32_t)((e >> 20) & 0xFFFFFu);
+    const uint32_t l = (uint32_t)((e >> 40) & 0xFu), c = (uint32_t)(e >> 44);
+    const uint64_t level_words = (uint64_t)a.nz * a.ny * a.NC;
+    const uint64_t r = (uint64_t)z * a.ny + y;
+    const uint64_t w = (uint64_t)l * level_words + r * a.NC + c;
+    const uint64_t* pre_v = a.prefix + (uint64_t)(l * 2 + 0) * a.nrows;
+    const uint64_t* pre_t = a.prefix + (uint64_t)(l * 2 + 1) * a.nrows;
+    RowChunk o;
+    row_chunk(a, a.bitvol + (uint64_t)l * level_words, y, z, c, true, o);
+    if (!a.dup) {
+        uint64_t m = o.x | o.y | o.z;
+        if (m) {
+            ChunkCross cc;
+            cc.base = (pre_v[r] - pre_v[0]) + a.vpc[w];
+            cc.x = o.x;
+            cc.y = o.y;
+            cc.z = o.z;
+            a.cross[w] = cc;
+            uint64_t id = a.voff[l] + cc.base;
+            const uint64_t rowkey = (uint64_t)y << 20 | (uint64_t)z << 40;
+            while (m) {
+                const uint32_t i = (uint32_t)__ffsll((unsigned long long)m) - 1u;
+                m &= m - 1ull;
+                const uint64_t key = rowkey | (c * 64u + i);
+                if ((o.x >> i) & 1ull) a.vkey[id++] = key;
+                if ((o.y >> i) & 1ull) a.vkey[id++] = key | 1ull << 60;
+                if ((o.z >> i) & 1ull) a.vkey[id++] = key | 2ull << 60;
+            }
+        }
+    }
+    uint64_t m = o.act;
+    if (m) {
+        uint64_t id = a.toff[l] + (pre_t[r] - pre_t[0]) + a.tpc[w];
+        const uint64_t cellkey = (uint64_t)y << 20 | (uint64_t)(z - 1) << 40;
+        while (m) {
+            const uint32_t i = (uint32_t)__ffsll((unsigned long long)m) - 1u;
+            m &= m - 1ull;
+            const uint32_t ci = corner_bits(o.k, i);
+            const uint32_t nt = ntri[ci];
+            const uint64_t key = cellkey | (c * 64u + i);
+            for (uint32_t t = 0; t < nt; ++t, ++id) {
+                a.trec[id] = key | (uint64_t)t << 60;
+                a.tci[id] = (uint8_t)ci;
+            }
+        }
+    }
+}
+
+/* Compaction of node rows [r0, r1): every crossing edge writes its key at its
+ * vertex id and every triangle its cell and table index t (plus the cell's
+ * cube index) at its triangle id -- row prefix + chunk prefix within the row
+ * + rank within the chunk, i.e. vertices in node scan order x {x,y,z}-edge
+ * and triangles in cell scan order x table order (calculate_isopoints
+ * :365-437, connect_isopoints :439-521, extract_surfaces_with_duplication
+ * :155-243); chunks with a vertex also leave their crossing record.  Lanes
+ * first scan (row, chunk) pairs as the count pass does and queue the
+ * non-empty chunks (the chunk prefixes tell) in the wave's LDS queue q (128
+ * entries); the work on them runs 64 chunks at a time, one per lane. */
+__device__ __forceinline__ void compact_rows(const JobArgs& a, const uint8_t* ntri, uint64_t* q, uint64_t r0, uint64_t r1,
                                              uint64_t wave_first, uint64_t wave_step)
 {
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t sub = lane >> a.cshift, cl = lane & ((1u << a.cshift) - 1u);
     const uint32_t G = 64u >> a.cshift, CS = 1u << a.cshift;
     const uint64_t level_words = (uint64_t)a.nz * a.ny * a.NC;
-    const bool verts = !a.dup;
+    const uint64_t below = (1ull << lane) - 1ull;
     if (!a.trec) return;   // the records did not fit (batched form): nothing to write
+    uint32_t qn = 0;
     for (uint64_t g = r0 + wave_first * G; g < r1; g += wave_step * G) {
         const uint64_t r = g + sub;
-        if (r >= r1) continue;
-        const uint32_t z = (uint32_t)(r / a.ny);
-        const uint32_t y = (uint32_t)(r - (uint64_t)z * a.ny);
+        const bool ok = r < r1;
+        const uint32_t z = ok ? (uint32_t)(r / a.ny) : 0u;
+        const uint32_t y = ok ? (uint32_t)(r - (uint64_t)z * a.ny) : 0u;
         for (uint32_t l = 0; l < a.nlev; ++l) {
-            const uint64_t* lev = a.bitvol + (uint64_t)l * level_words;
-            const uint64_t* pre_v = a.prefix + (uint64_t)(l * 2 + 0) * a.nrows;
-            const uint64_t* pre_t = a.prefix + (uint64_t)(l * 2 + 1) * a.nrows;
-            const uint64_t vb = a.voff[l] + (pre_v[r] - pre_v[0]);
-            const uint64_t tb = a.toff[l] + (pre_t[r] - pre_t[0]);
-            const uint32_t* vpc = a.vpc + (uint64_t)l * level_words + r * a.NC;
-            const uint32_t* tpc = a.tpc + (uint64_t)l * level_words + r * a.NC;
-            for (uint32_t c = cl; c < a.NC; c += CS) {
-                /* chunks without crossing edges or active cells (most of them) are
-                 * skipped on the count pass's prefixes alone */
-                const uint32_t v0 = vpc[c], t0 = tpc[c];
-                const uint32_t v1 = c + 1 < a.NC ? vpc[c + 1] : a.counts[(uint64_t)(l * 2 + 0) * a.nrows + r];
-                const uint32_t t1 = c + 1 < a.NC ? tpc[c + 1] : a.counts[(uint64_t)(l * 2 + 1) * a.nrows + r];
-                if ((!verts || v1 == v0) && t1 == t0) continue;
-                RowChunk o;
-                row_chunk(a, lev, y, z, c, true, o);
-                if (verts) {
-                    uint64_t m = o.x | o.y | o.z;
-                    if (m) {
-                        ChunkCross cc;
-                        cc.base = (pre_v[r] - pre_v[0]) + v0;
-                        cc.x = o.x;
-                        cc.y = o.y;
-                        cc.z = o.z;
-                        a.cross[(uint64_t)l * level_words + r * a.NC + c] = cc;
-                        uint64_t id = vb + v0;
-                        const uint64_t rowkey = (uint64_t)y << 20 | (uint64_t)z << 40;
-                        while (m) {
-                            const uint32_t i = (uint32_t)__ffsll((unsigned long long)m) - 1u;
-                            m &= m - 1ull;
-                            const uint64_t key = rowkey | (c * 64u + i);
-                            if ((o.x >> i) & 1ull) a.vkey[id++] = key;
-                            if ((o.y >> i) & 1ull) a.vkey[id++] = key | 1ull << 60;
-                            if ((o.z >> i) & 1ull) a.vkey[id++] = key | 2ull << 60;
-                        }
-                    }
+            for (uint32_t c0 = 0; c0 < a.NC; c0 += CS) {
+                const uint32_t c = c0 + cl;
+                bool busy = false;
+                if (ok && c < a.NC) {
+                    /* chunks without crossing edges or active cells (most of them)
+                     * are skipped on the count pass's prefixes alone */
+                    const uint64_t w = (uint64_t)l * level_words + r * a.NC + c;
+                    const uint32_t v0 = a.vpc[w], t0 = a.tpc[w];
+                    const uint32_t v1 = c + 1 < a.NC ? a.vpc[w + 1] : a.counts[(uint64_t)(l * 2 + 0) * a.nrows + r];
+                    const uint32_t t1 = c + 1 < a.NC ? a.tpc[w + 1] : a.counts[(uint64_t)(l * 2 + 1) * a.nrows + r];
+                    busy = (!a.dup && v1 != v0) || t1 != t0;
                 }
-                uint64_t m = o.act;
-                if (m) {
-                    uint64_t id = tb + t0;
-                    const uint64_t cellkey = (uint64_t)y << 20 | (uint64_t)(z - 1) << 40;
-                    while (m) {
-                        const uint32_t i = (uint32_t)__ffsll((unsigned long long)m) - 1u;
-                        m &= m - 1ull;
-                        const uint32_t ci = corner_bits(o.k, i);
-                        const uint32_t nt = ntri[ci];
-                        const uint64_t key = cellkey | (c * 64u + i);
-                        for (uint32_t t = 0; t < nt; ++t, ++id) {
-                            a.trec[id] = key | (uint64_t)t << 60;
-                            a.tci[id] = (uint8_t)ci;
-                        }
-                    }
+                const uint64_t bm = __ballot(busy);
+                if (busy)
+                    q[qn + (uint32_t)__popcll(bm & below)] =
+                        (uint64_t)y | (uint64_t)z << 20 | (uint64_t)l << 40 | (uint64_t)c << 44;
+                qn += (uint32_t)__popcll(bm);
+                if (qn >= 64) {
+                    wave_sync();
+                    compact_chunk(a, ntri, q[lane]);
+                    const uint32_t rest = qn - 64;
+                    const uint64_t mv = lane < rest ? q[64 + lane] : 0ull;
+                    wave_sync();
+                    if (lane < rest) q[lane] = mv;
+                    qn = rest;
+                    wave_sync();
                 }
             }
         }
     }
+    wave_sync();
+    if (lane < qn) compact_chunk(a, ntri, q[lane]);
+    wave_sync();   // the queue is reused by the next call
 }
 
 __global__ __launch_bounds__(kBlock) void k_compact(JobArgs a)
 {
     __shared__ uint8_t ntri[256];
+    __shared__ uint64_t queue[kWaves][128];
     stage_ntri(ntri);
     const uint64_t wave_id = ((uint64_t)blockIdx.x * kBlock + threadIdx.x) >> 6;
     const uint64_t nwaves = ((uint64_t)gridDim.x * kBlock) >> 6;
-    compact_rows(a, ntri, 0, a.nrows, wave_id, nwaves);
+    compact_rows(a, ntri, queue[threadIdx.x >> 6], 0, a.nrows, wave_id, nwaves);
 }
 
 /* Compaction of a batch of jobs: the rows of all jobs in one sequence, split
@@ -1376,6 +1410,7 @@ __global__ __launch_bounds__(kBlock) void k_compact(JobArgs a)
 __global__ __launch_bounds__(kBlock) void k_compact_batch(const JobArgs* jobs, const uint64_t* row_start, uint32_t njobs)
 {
     __shared__ uint8_t ntri[256];
+    __shared__ uint64_t queue[kWaves][128];
     stage_ntri(ntri);
     const uint64_t total = row_start[njobs];
     const uint64_t nwaves = ((uint64_t)gridDim.x * kBlock) >> 6;
@@ -1387,7 +1422,7 @@ __global__ __launch_bounds__(kBlock) void k_compact_batch(const JobArgs* jobs, c
     while (g < end) {
         const uint32_t j = __builtin_amdgcn_readfirstlane(find_job(row_start, njobs, g));
         const uint64_t je = row_start[j + 1] < end ? row_start[j + 1] : end;
-        compact_rows(jobs[j], ntri, g - row_start[j], je - row_start[j], 0, 1);
+        compact_rows(jobs[j], ntri, queue[threadIdx.x >> 6], g - row_start[j], je - row_start[j], 0, 1);
         g = je;
     }
 }
