@@ -233,15 +233,19 @@ def kernel_rooflines(prof, n, nlev, counts):
                       the in-row chunk prefixes (2 x 4 B per 64-node chunk and level);
       scan            reads those row counts, writes 8 B prefixes;
       compact         reads the chunk prefixes and the masks, writes the records
-                      (8 B per vertex, 8 + 1 B per triangle);
+                      (8 B per vertex, 8 + 1 B per triangle; the 32-byte crossing
+                      records of chunks with a vertex are not counted);
       vertices        reads each vertex key and its edge's two values, writes its
-                      coordinates: 8 + 8 + 12 B per vertex;
+                      coordinates and clears its four cell slots: 8 + 8 + 12 + 16 B
+                      per vertex;
       triangles       reads each record, cube index and its three vertices'
                       coordinates, writes connections and face normal:
-                      8 + 1 + 36 + 12 + 12 B per triangle;
-      vertex_normals  reads each vertex key and the face normals of the triangles
-                      using it (3 per triangle, 12 B each), writes the normal:
-                      8 + 12 B per vertex + 36 B per triangle."""
+                      8 + 1 + 36 + 12 + 12 B per triangle (the cell-slot writes,
+                      about 8 B per triangle, are not counted);
+      vertex_normals  reads each vertex key, its four cell slots and the cells'
+                      cube indices, and the face normals of the triangles using it
+                      (3 per triangle, 12 B each), writes the normal: 8 + 16 + 4 + 12
+                      B per vertex + 36 B per triangle."""
     nc = (n + 63) // 64
     masks = nlev * n * n * nc * 8
     chunks = nlev * n * n * nc
@@ -253,9 +257,9 @@ def kernel_rooflines(prof, n, nlev, counts):
         "count": masks + 2 * nlev * rows * 4 + 8 * chunks,
         "scan": 2 * nlev * rows * (4 + 8),
         "compact": 8 * chunks + masks + 8 * nv + 9 * nt,
-        "vertices": 28 * nv,
+        "vertices": 44 * nv,
         "triangles": 69 * nt,
-        "vertex_normals": 20 * nv + 36 * nt,
+        "vertex_normals": 40 * nv + 36 * nt,
     }
     names = {"classify": "k_classify<float, 3>", "count": "k_count", "scan": "k_scan_*", "compact": "k_compact",
              "vertices": "k_vertices<float>", "triangles": "k_triangles<float, 2>",
