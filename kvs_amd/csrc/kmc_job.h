@@ -25,9 +25,11 @@
  *     slices.  Teams never wait for each other.
  *
  * Passes of one job: classify (stream the values once, write the bit volume),
- * count (per-row vertex / triangle counts from the bit volume), scan (row
- * prefixes), emit (outputs at the prefixes; raw values are read again only at
- * the endpoints of crossing edges).
+ * count (per-row vertex / triangle counts and in-row chunk prefixes from the
+ * bit volume), scan (row prefixes), emit (outputs at the prefixes; raw values
+ * are read again only at the endpoints of crossing edges).  The emit is flat
+ * (compaction records, then one thread per output item) unless y or z exceeds
+ * 2^20, when the tile teams below do it.
  *
  * Within a slice, the classification bits of 64 consecutive nodes of a row
  * are held as one 64-bit mask per level (a *chunk*), built with one wave

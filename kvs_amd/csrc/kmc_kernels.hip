@@ -12,6 +12,13 @@
  * Dense per-node work is therefore one load, one compare per level and one
  * ballot per level; only crossing edges and active cells do more.
  *
+ * Passes: classify (values -> masks), count (per-row counts and in-row chunk
+ * prefixes), scan (row prefixes), then the flat emit: a compaction writes
+ * every vertex's key and every triangle's record at its output rank, and one
+ * thread per vertex / triangle / vertex normal produces the outputs.  The
+ * tile-team emit (each wave marching a band of rows through z with LDS rings
+ * of masks) remains for volumes whose rows exceed the records' fields.
+ *
  * Arithmetic follows Source/Core/Visualization/Mapper/MarchingCubes.cpp bit for
  * bit: the comparison double(v) > isolevel (:299-316), the FP64 clamp / ratio /
  * lerp of interpolate_vertex (:326-356), the float (p + min) * scale of the
@@ -126,7 +133,7 @@ __device__ __forceinline__ void face_normal(const float v0[3], const float v1[3]
 }
 
 /* ------------------------------------------------------------------------- */
-/* emit teams                                                                  */
+/* tile-team emit                                                              */
 /* ------------------------------------------------------------------------- */
 
 /* The emit pass runs one independent team per wave: a team marches its own
@@ -1176,7 +1183,7 @@ __global__ __launch_bounds__(kBlock) void k_count_batch(const JobArgs* jobs, con
 }
 
 /* ------------------------------------------------------------------------- */
-/* emit pass                                                                   */
+/* tile-team emit pass                                                         */
 /* ------------------------------------------------------------------------- */
 
 /* One unit: level l of band b over steps zs .. ze-1 (with VertexNormal one
@@ -1669,7 +1676,7 @@ __global__ __launch_bounds__(kBlock) void k_vertex_normals_batch(const JobArgs* 
 
 
 /* ------------------------------------------------------------------------- */
-/* tile-count scan                                                             */
+/* row-count scan                                                              */
 /* ------------------------------------------------------------------------- */
 
 constexpr uint32_t kScanPer = 16;                      // elements per thread
