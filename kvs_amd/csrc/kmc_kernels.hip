@@ -1052,6 +1052,23 @@ __device__ __forceinline__ uint32_t corner_bits(const uint64_t k[8], uint32_t i)
     return ci;
 }
 
+/* The cube index with bits 2, 3 and bits 6, 7 swapped (an involution). */
+__device__ __forceinline__ uint32_t swap_pairs(uint32_t c)
+{
+    return (c & 0x33u) | ((c & 0x44u) << 1) | ((c & 0x88u) >> 1);
+}
+
+/* swap_pairs(cube index) of cell i of a chunk: for i < 63 the corners (x, x+1)
+ * of each of the cell's four node rows are two adjacent bits of that row's
+ * mask (k[0], k[3], k[4], k[7]), taken as one 2-bit window. */
+__device__ __forceinline__ uint32_t corner_pairs(const uint64_t k[8], uint32_t i)
+{
+    if (i == 63u) return swap_pairs(corner_bits(k, i));
+    const uint32_t p0 = (uint32_t)(k[0] >> i) & 3u, p3 = (uint32_t)(k[3] >> i) & 3u;
+    const uint32_t p4 = (uint32_t)(k[4] >> i) & 3u, p7 = (uint32_t)(k[7] >> i) & 3u;
+    return p0 | p3 << 2 | p4 << 4 | p7 << 6;
+}
+
 /* Exclusive scan of x over the CS lanes of each row group (CS a power of two). */
 __device__ __forceinline__ uint32_t group_excl_scan(uint32_t x, uint32_t cl, uint32_t CS, uint32_t& total)
 {
@@ -1099,7 +1116,7 @@ __device__ __forceinline__ void count_rows_t(const JobArgs& a, const uint8_t* nt
                     while (m) {
                         const uint32_t i = (uint32_t)__ffsll((unsigned long long)m) - 1u;
                         m &= m - 1ull;
-                        ct += ntri[corner_bits(o.k, i)];
+                        ct += ntri[corner_pairs(o.k, i)];
                     }
                 }
                 if (chunk_prefixes) {
@@ -1147,9 +1164,10 @@ __device__ __forceinline__ void count_rows(const JobArgs& a, const uint8_t* ntri
     }
 }
 
+/* Triangle count per swap_pairs(cube index) (the index corner_pairs gives). */
 __device__ __forceinline__ void stage_ntri(uint8_t* ntri)
 {
-    for (uint32_t i = threadIdx.x; i < 256; i += kBlock) ntri[i] = (uint8_t)(d_tri[i] >> 60);
+    for (uint32_t i = threadIdx.x; i < 256; i += kBlock) ntri[i] = (uint8_t)(d_tri[swap_pairs(i)] >> 60);
     __syncthreads();
 }
 
@@ -1330,8 +1348,9 @@ __device__ __forceinline__ void compact_chunk(const JobArgs& a, const uint8_t* n
         while (m) {
             const uint32_t i = (uint32_t)__ffsll((unsigned long long)m) - 1u;
             m &= m - 1ull;
-            const uint32_t ci = corner_bits(o.k, i);
-            const uint32_t nt = ntri[ci];
+            const uint32_t pi = corner_pairs(o.k, i);
+            const uint32_t ci = swap_pairs(pi);
+            const uint32_t nt = ntri[pi];
             const uint64_t key = cellkey | (c * 64u + i);
             for (uint32_t t = 0; t < nt; ++t, ++id) {
                 a.trec[id] = key | (uint64_t)t << 60;

@@ -62,9 +62,12 @@ struct Resident {
 /* Device bytes of the masks, counts and prefixes of one single-level request. */
 uint64_t request_scratch(const kmc_volume& v)
 {
+    /* per node row: counts and prefixes of vertices and triangles (2 x 12 B);
+     * per 64-node chunk: its mask (8 B), its two in-row prefixes (8 B) and its
+     * crossing record (32 B) */
     const uint64_t nx = v.resolution[0], ny = v.resolution[1], nz = v.resolution[2];
     const uint64_t nrows = ny * nz, NC = (nx + 63) / 64;
-    return nrows * 2 * 12 + nrows * NC * 8 + 4096;
+    return nrows * 2 * 12 + nrows * NC * 48 + 4096;
 }
 
 uint64_t output_bytes(const kmc_surface& s)
