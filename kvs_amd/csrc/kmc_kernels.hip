@@ -891,12 +891,7 @@ __device__ __forceinline__ bool above(T v, const JobArgs& a, int l) { return (do
 template <>
 __device__ __forceinline__ bool above<float>(float v, const JobArgs& a, int l) { return v > a.thr[l]; }
 
-/* Streams the volume once.  Each wave owns whole rows: its lanes load 64
- * consecutive nodes per chunk (one coalesced access), the comparison against
- * level l is a ballot (v_cmp into a lane mask), and the mask is parked in
- * lane c of a register by a lane-select; the masks of up to 64 chunks of a
- * row leave with one coalesced 64-bit store per level.  NLEV is the number of
- * levels, a compile-time constant so that nothing is guarded per level. */
+/* Classification of node row `row` of a job (the single-volume pass). */
 template <typename T, int NLEV>
 __device__ __forceinline__ void classify_row(const JobArgs& a, uint64_t row)
 {
@@ -941,6 +936,59 @@ __device__ __forceinline__ void classify_row(const JobArgs& a, uint64_t row)
     }
 }
 
+/* Classification of chunks [q0, q1) of a job, q = row * NC + chunk: the
+ * chunks of consecutive rows follow each other, so short rows share a wave's
+ * batches of loads and its windows of 64 parked masks, which leave with one
+ * coalesced store per level (the masks of consecutive rows are contiguous). */
+template <typename T, int NLEV>
+__device__ __forceinline__ void classify_chunks(const JobArgs& a, uint64_t q0, uint64_t q1)
+{
+    const uint32_t lane = threadIdx.x & 63u;
+    const T* vals = (const T*)a.values;
+    const uint64_t level_words = (uint64_t)a.nz * a.ny * a.NC;
+    constexpr uint32_t kBatch = 16;
+    for (uint64_t w0 = q0; w0 < q1; w0 += 64) {
+        const uint32_t wn = (q1 - w0) < 64 ? (uint32_t)(q1 - w0) : 64u;
+        uint32_t lo[NLEV], hi[NLEV];
+#pragma unroll
+        for (int l = 0; l < NLEV; ++l) { lo[l] = 0; hi[l] = 0; }
+        uint64_t row = w0 / a.NC;
+        uint32_t c = (uint32_t)(w0 - row * a.NC);
+        for (uint32_t cb = 0; cb < wn; cb += kBatch) {
+            T val[kBatch];
+            uint32_t cs[kBatch];
+#pragma unroll
+            for (uint32_t i = 0; i < kBatch; ++i) {
+                cs[i] = c;
+                const uint32_t x = c * 64u + lane;
+                val[i] = (cb + i < wn && x < a.nx) ? vals[row * a.nx + x] : T(0);
+                if (++c == a.NC) { c = 0; ++row; }
+            }
+#pragma unroll
+            for (uint32_t i = 0; i < kBatch; ++i) {
+                if (cb + i < wn) {
+                    const bool in = cs[i] * 64u + lane < a.nx;
+                    const bool mine = lane == cb + i;
+#pragma unroll
+                    for (int l = 0; l < NLEV; ++l) {
+                        const uint64_t m = __ballot(in && above<T>(val[i], a, l));
+                        lo[l] = mine ? (uint32_t)m : lo[l];
+                        hi[l] = mine ? (uint32_t)(m >> 32) : hi[l];
+                    }
+                }
+            }
+        }
+        /* the 8-level instance also serves passes of 5..7 levels: store only
+         * the pass's own levels */
+        if (lane < wn) {
+#pragma unroll
+            for (int l = 0; l < NLEV; ++l)
+                if (NLEV < 8 || l < (int)a.nlev)
+                    a.bitvol[(uint64_t)l * level_words + w0 + lane] = ((uint64_t)hi[l] << 32) | lo[l];
+        }
+    }
+}
+
 /* Streams the volume once.  Each wave owns whole rows: its lanes load 64
  * consecutive nodes per chunk (one coalesced access), the comparison against
  * level l is a ballot (v_cmp into a lane mask), and the mask is parked in
@@ -969,7 +1017,8 @@ __device__ __forceinline__ uint32_t find_job(const uint64_t* start, uint32_t n, 
 
 /* Classify pass of a batch of jobs sharing the value type T: the node rows of
  * all jobs form one sequence (row_start[j] = first row of job j), cut into one
- * contiguous range per wave so that the job changes rarely along a range. */
+ * contiguous range per wave so that the job changes rarely along a range; a
+ * wave classifies its rows of a job as one run of chunks. */
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_classify_batch(const JobArgs* jobs, const uint64_t* row_start, uint32_t njobs)
 {
@@ -981,17 +1030,19 @@ __global__ __launch_bounds__(kBlock) void k_classify_batch(const JobArgs* jobs, 
     const uint64_t end = g + per < total ? g + per : total;
     if (g >= end) return;
     uint32_t j = __builtin_amdgcn_readfirstlane(find_job(row_start, njobs, g));
-    for (; g < end; ++g) {
+    while (g < end) {
         while (g >= row_start[j + 1]) ++j;
         const JobArgs& a = jobs[j];
-        const uint64_t row = g - row_start[j];
+        const uint64_t je = row_start[j + 1] < end ? row_start[j + 1] : end;
+        const uint64_t q0 = (g - row_start[j]) * a.NC, q1 = (je - row_start[j]) * a.NC;
         switch (a.nlev) {
-        case 1: classify_row<T, 1>(a, row); break;
-        case 2: classify_row<T, 2>(a, row); break;
-        case 3: classify_row<T, 3>(a, row); break;
-        case 4: classify_row<T, 4>(a, row); break;
-        default: classify_row<T, 8>(a, row); break;
+        case 1: classify_chunks<T, 1>(a, q0, q1); break;
+        case 2: classify_chunks<T, 2>(a, q0, q1); break;
+        case 3: classify_chunks<T, 3>(a, q0, q1); break;
+        case 4: classify_chunks<T, 4>(a, q0, q1); break;
+        default: classify_chunks<T, 8>(a, q0, q1); break;
         }
+        g = je;
     }
 }
 
