@@ -188,3 +188,20 @@ def test_batch_on_tile_team_emit(monkeypatch):
         assert len(surfs) == len(levels)
         for lv, got in zip(levels, surfs):
             assert_same_surface(got, plain(v, lv, normal, dup, mm, co), f"team job {j} level {lv}")
+
+
+def test_batch_wide_and_short_rows():
+    # rows of more than 64 chunks (4100 nodes) next to very short rows in one
+    # batch: the classify runs chunks across row ends and 64-chunk windows
+    rng = np.random.default_rng(41)
+    specs = [
+        (rng.standard_normal((3, 5, 4100)).astype(np.float32), [0.1, -0.2], kmc.VERTEX_NORMAL, False),
+        (rng.standard_normal((9, 7, 3)).astype(np.float32), [0.0], kmc.POLYGON_NORMAL, True),
+        (rng.standard_normal((4, 6, 130)).astype(np.float32), [0.3], kmc.POLYGON_NORMAL, False),
+        (rng.standard_normal((2, 3, 4097)).astype(np.float32), [0.05], kmc.POLYGON_NORMAL, True),
+    ]
+    vols = [kmc.make_volume(v) for (v, _, _, _) in specs]
+    res = kmc.batch_extract([(vol, lv, n, d) for vol, (v, lv, n, d) in zip(vols, specs)])
+    for j, ((v, levels, normal, dup), surfs) in enumerate(zip(specs, res)):
+        for lv, got in zip(levels, surfs):
+            assert_same_surface(got, oracle.extract(v, lv, normal, dup), f"wide job {j} level {lv}")
