@@ -236,8 +236,7 @@ def kernel_rooflines(prof, n, nlev, counts):
                       (8 B per vertex, 8 + 1 B per triangle; the 32-byte crossing
                       records of chunks with a vertex are not counted);
       vertices        reads each vertex key and its edge's two values, writes its
-                      coordinates and clears its four cell slots: 8 + 8 + 12 + 16 B
-                      per vertex;
+                      coordinates: 8 + 8 + 12 B per vertex;
       triangles       reads each record, cube index and its three vertices'
                       coordinates, writes connections and face normal:
                       8 + 1 + 36 + 12 + 12 B per triangle (the cell-slot writes,
@@ -257,7 +256,7 @@ def kernel_rooflines(prof, n, nlev, counts):
         "count": masks + 2 * nlev * rows * 4 + 8 * chunks,
         "scan": 2 * nlev * rows * (4 + 8),
         "compact": 8 * chunks + masks + 8 * nv + 9 * nt,
-        "vertices": 44 * nv,
+        "vertices": 28 * nv,
         "triangles": 69 * nt,
         "vertex_normals": 40 * nv + 36 * nt,
     }

@@ -104,8 +104,9 @@ struct JobArgs {
     // per chunk (bit volume layout), written by the compaction for chunks with a vertex
     ChunkCross* cross;
     // VertexNormal: per vertex, for each of the four cells around its edge (in
-    // cell scan order) the item index of that cell's first triangle record
-    // (~0u: no such cell); written by the triangles that use the edge
+    // cell scan order) the item index of that cell's first triangle record;
+    // written by the triangles that use the edge (cells outside the volume:
+    // not written, not read)
     uint32_t* vslot;
     uint64_t voff[kMaxLevels + 1];
     uint64_t toff[kMaxLevels + 1];

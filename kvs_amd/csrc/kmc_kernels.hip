@@ -1539,7 +1539,6 @@ __device__ __forceinline__ void vertex_item(const JobArgs& a, uint64_t g)
     co[3 * id + 0] = p[0];
     co[3 * id + 1] = p[1];
     co[3 * id + 2] = p[2];
-    if (a.vslot) *(uint4*)(a.vslot + 4 * g) = make_uint4(~0u, ~0u, ~0u, ~0u);
 }
 
 template <typename T>
@@ -1661,8 +1660,9 @@ __global__ __launch_bounds__(kBlock) void k_triangles(JobArgs a)
 /* VertexNormal of every vertex: the float sum, in triangle order, of the face
  * normals of the triangles that use it (calculate_normals_on_vertex,
  * :569-605).  The four cells around the edge are visited in cell scan order
- * (their first records come from the vertex's slots, written by the
- * triangles) and each cell's triangles that use the edge in table order. */
+ * (the first records of those in the volume come from the vertex's slots,
+ * written by the triangles) and each cell's triangles that use the edge in
+ * table order. */
 __device__ __forceinline__ void vertex_normal_item(const JobArgs& a, const uint64_t* tri, uint64_t g)
 {
     const uint32_t l = level_at(a.voff, a.nlev, g);
@@ -1670,16 +1670,28 @@ __device__ __forceinline__ void vertex_normal_item(const JobArgs& a, const uint6
 #pragma unroll
     for (int k = 0; k < kMaxLevels; ++k) if (k == (int)l) normals = a.normals[k];
     if (!normals) return;   // a level without outputs (not admitted)
-    const uint32_t axis = (uint32_t)(a.vkey[g] >> 60);
+    const uint64_t key = a.vkey[g];
+    const int x = (int)(key & 0xFFFFFu), y = (int)((key >> 20) & 0xFFFFFu);
+    const int zv = (int)((key >> 40) & 0xFFFFFu);
+    const uint32_t axis = (uint32_t)(key >> 60);
     const uint32_t around = around_nibbles(axis);
+    /* the cells around the edge that lie in the volume: each is active, and the
+     * first of its triangles that uses the edge wrote the slot */
+    bool in[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        int cx, cy, cz;
+        around_cell(axis, q, x, y, zv, cx, cy, cz);
+        in[q] = cx >= 0 && cy >= 0 && cz >= 0 && cx + 1 < (int)a.nx && cy + 1 < (int)a.ny && cz + 1 < (int)a.nz;
+    }
     const uint4 sl = *(const uint4*)(a.vslot + 4 * g);
     const uint32_t first[4] = { sl.x, sl.y, sl.z, sl.w };
     uint32_t ci[4];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) ci[q] = first[q] != ~0u ? a.tci[first[q]] : 0u;
+    for (int q = 0; q < 4; ++q) ci[q] = in[q] ? a.tci[first[q]] : 0u;
     float nv[3] = { 0.0f, 0.0f, 0.0f };
     for (int q = 0; q < 4; ++q) {
-        if (first[q] == ~0u) continue;
+        if (!in[q]) continue;
         const uint64_t w = tri[ci[q]];
         const int nt = (int)(w >> 60);
         const uint32_t me = (around >> (4 * q)) & 0xFu;
