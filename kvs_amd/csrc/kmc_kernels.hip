@@ -1647,8 +1647,9 @@ __device__ __forceinline__ void triangle_item(const JobArgs& a, const uint64_t* 
     }
 }
 
+/* shared-vertex modes fit 64 VGPRs without spills: eight waves per SIMD */
 template <typename T, int M>
-__global__ __launch_bounds__(kBlock) void k_triangles(JobArgs a)
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(M == kModeDup ? 1 : 8))) void k_triangles(JobArgs a)
 {
     __shared__ uint64_t tri[256];
     stage_table_block(tri);
