@@ -1120,13 +1120,30 @@ __device__ __forceinline__ uint32_t corner_pairs(const uint64_t k[8], uint32_t i
     return p0 | p3 << 2 | p4 << 4 | p7 << 6;
 }
 
-/* Exclusive scan of x over the CS lanes of each row group (CS a power of two). */
+/* x of the lane d below within its DPP row of 16 lanes (0 for the row's
+ * first d lanes): row_shr:d. */
+template <int D>
+__device__ __forceinline__ uint32_t row_shr(uint32_t x)
+{
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x110 + D, 0xf, 0xf, true);
+}
+
+/* Exclusive scan of x over the CS lanes of each row group (CS a power of two;
+ * group cl = lane mod CS).  Groups of up to 16 lanes lie inside one DPP row:
+ * the steps are row shifts, with no LDS round trip. */
 __device__ __forceinline__ uint32_t group_excl_scan(uint32_t x, uint32_t cl, uint32_t CS, uint32_t& total)
 {
     uint32_t inc = x;
-    for (uint32_t d = 1; d < CS; d <<= 1) {
-        const uint32_t t = __shfl_up(inc, d, (int)CS);
-        if (cl >= d) inc += t;
+    if (CS <= 16) {
+        if (CS > 1) { const uint32_t t = row_shr<1>(inc); inc += cl >= 1 ? t : 0u; }
+        if (CS > 2) { const uint32_t t = row_shr<2>(inc); inc += cl >= 2 ? t : 0u; }
+        if (CS > 4) { const uint32_t t = row_shr<4>(inc); inc += cl >= 4 ? t : 0u; }
+        if (CS > 8) { const uint32_t t = row_shr<8>(inc); inc += cl >= 8 ? t : 0u; }
+    } else {
+        for (uint32_t d = 1; d < CS; d <<= 1) {
+            const uint32_t t = __shfl_up(inc, d, (int)CS);
+            if (cl >= d) inc += t;
+        }
     }
     total = __shfl(inc, (int)CS - 1, (int)CS);
     return inc - x;
@@ -1171,16 +1188,18 @@ __device__ __forceinline__ void count_rows_t(const JobArgs& a, const uint8_t* nt
                     }
                 }
                 if (chunk_prefixes) {
-                    uint32_t sv, st;
-                    const uint32_t ev = group_excl_scan(cv, cl, CS, sv);
-                    const uint32_t et = group_excl_scan(ct, cl, CS, st);
+                    /* both counts in one scan: a chunk holds at most 192
+                     * crossing edges and 320 triangles, a group of at most 64
+                     * chunks sums below 2^16 in each half */
+                    uint32_t s;
+                    const uint32_t e = group_excl_scan(cv | ct << 16, cl, CS, s);
                     if (ok && c < a.NC) {
                         const uint64_t w = (uint64_t)l * level_words + r * a.NC + c;
-                        a.vpc[w] = vc + ev;
-                        a.tpc[w] = tc + et;
+                        a.vpc[w] = vc + (e & 0xFFFFu);
+                        a.tpc[w] = tc + (e >> 16);
                     }
-                    vc += sv;
-                    tc += st;
+                    vc += s & 0xFFFFu;
+                    tc += s >> 16;
                 } else {
                     vc += cv;
                     tc += ct;
@@ -2047,9 +2066,11 @@ hipError_t launch_emit_t(const JobArgs& a, uint32_t lds_bytes, hipStream_t st)
 template <typename T>
 hipError_t launch_classify_t(const JobArgs& a, hipStream_t st)
 {
+    /* one node row per wave: consecutive workgroups sweep the volume in
+     * address order (measured faster than waves that take several rows) */
     const uint64_t rows = (uint64_t)a.ny * a.nz;
     uint64_t blocks = (rows + kWaves - 1) / kWaves;
-    if (blocks > 65536) blocks = 65536;
+    if (blocks > (1u << 20)) blocks = 1u << 20;
     const dim3 g((uint32_t)blocks), b(kBlock);
     switch (a.nlev) {
     case 1: hipLaunchKernelGGL((k_classify<T, 1>), g, b, 0, st, a); break;
