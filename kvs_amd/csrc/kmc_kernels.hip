@@ -1437,12 +1437,21 @@ __device__ __forceinline__ void compact_chunk(const JobArgs& a, const uint8_t* n
  * and triangles in cell scan order x table order (calculate_isopoints
  * :365-437, connect_isopoints :439-521, extract_surfaces_with_duplication
  * :155-243); chunks with a vertex also leave their crossing record.  Lanes
- * first scan (row, chunk) pairs as the count pass does and queue the
- * non-empty chunks (the chunk prefixes tell) in the wave's LDS queue q (128
- * entries); the work on them runs 64 chunks at a time, one per lane. */
-__device__ __forceinline__ void compact_rows(const JobArgs& a, const uint8_t* ntri, uint64_t* q, uint64_t r0, uint64_t r1,
-                                             uint64_t wave_first, uint64_t wave_step)
+ * first scan (row, chunk) pairs as the count pass does, NLEV levels at a time
+ * (their loads in flight together; NLEV = 0: one level at a time, any level
+ * count), and queue the non-empty chunks (the chunk prefixes tell) in the
+ * wave's LDS queue q (kCompactQueue entries); the work on them runs 64 chunks
+ * at a time, one per lane, taken from the queue's end.  Output positions come
+ * from the prefixes alone, so the queue's order does not matter. */
+constexpr uint32_t kCompactLevels = 4;
+constexpr uint32_t kCompactQueue = 64 * (kCompactLevels + 1);
+
+template <int NLEV>
+__device__ __forceinline__ void compact_rows_t(const JobArgs& a, const uint8_t* ntri, uint64_t* q, uint64_t r0, uint64_t r1,
+                                               uint64_t wave_first, uint64_t wave_step)
 {
+    constexpr uint32_t NL = NLEV ? (uint32_t)NLEV : 1u;
+    static_assert(NL <= kCompactLevels, "queue holds 63 + 64 * NL entries");
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t sub = lane >> a.cshift, cl = lane & ((1u << a.cshift) - 1u);
     const uint32_t G = 64u >> a.cshift, CS = 1u << a.cshift;
@@ -1455,32 +1464,36 @@ __device__ __forceinline__ void compact_rows(const JobArgs& a, const uint8_t* nt
         const bool ok = r < r1;
         const uint32_t z = ok ? (uint32_t)(r / a.ny) : 0u;
         const uint32_t y = ok ? (uint32_t)(r - (uint64_t)z * a.ny) : 0u;
-        for (uint32_t l = 0; l < a.nlev; ++l) {
+        for (uint32_t l0 = 0; l0 < a.nlev; l0 += NL) {
             for (uint32_t c0 = 0; c0 < a.NC; c0 += CS) {
                 const uint32_t c = c0 + cl;
-                bool busy = false;
-                if (ok && c < a.NC) {
-                    /* chunks without crossing edges or active cells (most of them)
-                     * are skipped on the count pass's prefixes alone */
-                    const uint64_t w = (uint64_t)l * level_words + r * a.NC + c;
-                    const uint32_t v0 = a.vpc[w], t0 = a.tpc[w];
-                    const uint32_t v1 = c + 1 < a.NC ? a.vpc[w + 1] : a.counts[(uint64_t)(l * 2 + 0) * a.nrows + r];
-                    const uint32_t t1 = c + 1 < a.NC ? a.tpc[w + 1] : a.counts[(uint64_t)(l * 2 + 1) * a.nrows + r];
-                    busy = (!a.dup && v1 != v0) || t1 != t0;
+                bool busy[NL];
+#pragma unroll
+                for (uint32_t k = 0; k < NL; ++k) {
+                    busy[k] = false;
+                    if (ok && c < a.NC) {
+                        /* chunks without crossing edges or active cells (most of
+                         * them) are skipped on the count pass's prefixes alone */
+                        const uint32_t l = l0 + k;
+                        const uint64_t w = (uint64_t)l * level_words + r * a.NC + c;
+                        const uint32_t v0 = a.vpc[w], t0 = a.tpc[w];
+                        const uint32_t v1 = c + 1 < a.NC ? a.vpc[w + 1] : a.counts[(uint64_t)(l * 2 + 0) * a.nrows + r];
+                        const uint32_t t1 = c + 1 < a.NC ? a.tpc[w + 1] : a.counts[(uint64_t)(l * 2 + 1) * a.nrows + r];
+                        busy[k] = (!a.dup && v1 != v0) || t1 != t0;
+                    }
                 }
-                const uint64_t bm = __ballot(busy);
-                if (busy)
-                    q[qn + (uint32_t)__popcll(bm & below)] =
-                        (uint64_t)y | (uint64_t)z << 20 | (uint64_t)l << 40 | (uint64_t)c << 44;
-                qn += (uint32_t)__popcll(bm);
-                if (qn >= 64) {
+#pragma unroll
+                for (uint32_t k = 0; k < NL; ++k) {
+                    const uint64_t bm = __ballot(busy[k]);
+                    if (busy[k])
+                        q[qn + (uint32_t)__popcll(bm & below)] =
+                            (uint64_t)y | (uint64_t)z << 20 | (uint64_t)(l0 + k) << 40 | (uint64_t)c << 44;
+                    qn += (uint32_t)__popcll(bm);
+                }
+                while (qn >= 64) {
                     wave_sync();
-                    compact_chunk(a, ntri, q[lane]);
-                    const uint32_t rest = qn - 64;
-                    const uint64_t mv = lane < rest ? q[64 + lane] : 0ull;
-                    wave_sync();
-                    if (lane < rest) q[lane] = mv;
-                    qn = rest;
+                    qn -= 64;
+                    compact_chunk(a, ntri, q[qn + lane]);
                     wave_sync();
                 }
             }
@@ -1491,10 +1504,22 @@ __device__ __forceinline__ void compact_rows(const JobArgs& a, const uint8_t* nt
     wave_sync();   // the queue is reused by the next call
 }
 
+__device__ __forceinline__ void compact_rows(const JobArgs& a, const uint8_t* ntri, uint64_t* q, uint64_t r0, uint64_t r1,
+                                             uint64_t wave_first, uint64_t wave_step)
+{
+    switch (a.nlev) {
+    case 1: compact_rows_t<1>(a, ntri, q, r0, r1, wave_first, wave_step); break;
+    case 2: compact_rows_t<2>(a, ntri, q, r0, r1, wave_first, wave_step); break;
+    case 3: compact_rows_t<3>(a, ntri, q, r0, r1, wave_first, wave_step); break;
+    case 4: compact_rows_t<4>(a, ntri, q, r0, r1, wave_first, wave_step); break;
+    default: compact_rows_t<0>(a, ntri, q, r0, r1, wave_first, wave_step); break;
+    }
+}
+
 __global__ __launch_bounds__(kBlock) void k_compact(JobArgs a)
 {
     __shared__ uint8_t ntri[256];
-    __shared__ uint64_t queue[kWaves][128];
+    __shared__ uint64_t queue[kWaves][kCompactQueue];
     stage_ntri(ntri);
     const uint64_t wave_id = ((uint64_t)blockIdx.x * kBlock + threadIdx.x) >> 6;
     const uint64_t nwaves = ((uint64_t)gridDim.x * kBlock) >> 6;
@@ -1506,7 +1531,7 @@ __global__ __launch_bounds__(kBlock) void k_compact(JobArgs a)
 __global__ __launch_bounds__(kBlock) void k_compact_batch(const JobArgs* jobs, const uint64_t* row_start, uint32_t njobs)
 {
     __shared__ uint8_t ntri[256];
-    __shared__ uint64_t queue[kWaves][128];
+    __shared__ uint64_t queue[kWaves][kCompactQueue];
     stage_ntri(ntri);
     const uint64_t total = row_start[njobs];
     const uint64_t nwaves = ((uint64_t)gridDim.x * kBlock) >> 6;
