@@ -1548,15 +1548,11 @@ __global__ __launch_bounds__(kBlock) void k_compact_batch(const JobArgs* jobs, c
     }
 }
 
-/* Vertex id of the shared vertex on the edge of node (x, y, z) along axis:
- * the first vertex of its chunk (row prefix + chunk prefix within the row)
- * + the crossing edges of the chunk's nodes before x (and of x's lower
- * axes), from the chunk's compaction record. */
-__device__ __forceinline__ uint64_t vertex_id(const JobArgs& a, const ChunkCross* cross_l, uint32_t x, uint32_t y,
-                                              uint32_t z, uint32_t axis)
+/* Rank of the vertex on the edge along `axis` of node i of a chunk among the
+ * chunk's vertices, plus the chunk's first vertex id: the crossing edges of the
+ * chunk's nodes before i and of node i's lower axes. */
+__device__ __forceinline__ uint64_t cross_rank(const ChunkCross& o, uint32_t i, uint32_t axis)
 {
-    const uint32_t c = x >> 6, i = x & 63u;
-    const ChunkCross o = cross_l[((uint64_t)z * a.ny + y) * a.NC + c];
     const uint64_t lo = (1ull << i) - 1ull;
     uint64_t r = o.base;
     r += (uint64_t)(__popcll(o.x & lo) + __popcll(o.y & lo) + __popcll(o.z & lo));
@@ -1565,12 +1561,22 @@ __device__ __forceinline__ uint64_t vertex_id(const JobArgs& a, const ChunkCross
     return r;
 }
 
+/* Vertex id of the shared vertex on the edge of node (x, y, z) along axis:
+ * the first vertex of its chunk (row prefix + chunk prefix within the row)
+ * + the crossing edges of the chunk's nodes before x (and of x's lower
+ * axes), from the chunk's compaction record. */
+__device__ __forceinline__ uint64_t vertex_id(const JobArgs& a, const ChunkCross* cross_l, uint32_t x, uint32_t y,
+                                              uint32_t z, uint32_t axis)
+{
+    const uint32_t c = x >> 6, i = x & 63u;
+    return cross_rank(cross_l[((uint64_t)z * a.ny + y) * a.NC + c], i, axis);
+}
+
 /* Coordinates of vertex g of the job's items (calculate_isopoints, :365-437). */
 template <typename T>
-__device__ __forceinline__ void vertex_item(const JobArgs& a, uint64_t g)
+__device__ __forceinline__ void vertex_item(const JobArgs& a, uint64_t g, uint64_t key)
 {
     const uint32_t l = level_at(a.voff, a.nlev, g);
-    const uint64_t key = a.vkey[g];
     const uint32_t x = (uint32_t)(key & 0xFFFFFu), y = (uint32_t)((key >> 20) & 0xFFFFFu);
     const uint32_t z = (uint32_t)((key >> 40) & 0xFFFFFu), axis = (uint32_t)(key >> 60);
     float p[3];
@@ -1588,9 +1594,15 @@ __device__ __forceinline__ void vertex_item(const JobArgs& a, uint64_t g)
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_vertices(JobArgs a)
 {
-    const uint64_t n = a.voff[a.nlev];
-    for (uint64_t g = (uint64_t)blockIdx.x * kBlock + threadIdx.x; g < n; g += (uint64_t)gridDim.x * kBlock)
-        vertex_item<T>(a, g);
+    const uint64_t n = a.voff[a.nlev], step = (uint64_t)gridDim.x * kBlock;
+    uint64_t g = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+    /* the next item's key is loaded while this one is worked on */
+    uint64_t key = g < n ? a.vkey[g] : 0ull;
+    for (; g < n; g += step) {
+        const uint64_t cur = key;
+        if (g + step < n) key = a.vkey[g + step];
+        vertex_item<T>(a, g, cur);
+    }
 }
 
 __device__ __forceinline__ void stage_table_block(uint64_t* tri)
@@ -1606,15 +1618,14 @@ __device__ __forceinline__ void stage_table_block(uint64_t* tri)
  * (calculate_normals_on_polygon, :531-559) or, with VertexNormal, kept in
  * a.fn for the per-vertex sums. */
 template <typename T, int M>
-__device__ __forceinline__ void triangle_item(const JobArgs& a, const uint64_t* tri, uint64_t g)
+__device__ __forceinline__ void triangle_item(const JobArgs& a, const uint64_t* tri, uint64_t g, uint64_t rec, uint32_t ci)
 {
     const T* vol = (const T*)a.values;
     const uint64_t level_words = (uint64_t)a.nz * a.ny * a.NC;
     const uint32_t l = level_at(a.toff, a.nlev, g);
-    const uint64_t rec = a.trec[g];
     const uint32_t x = (uint32_t)(rec & 0xFFFFFu), y = (uint32_t)((rec >> 20) & 0xFFFFFu);
     const uint32_t zc = (uint32_t)((rec >> 40) & 0xFFFFFu), t = (uint32_t)(rec >> 60);
-    const uint64_t w = tri[a.tci[g]];
+    const uint64_t w = tri[ci];
     const uint32_t eds[3] = { (uint32_t)((w >> (12 * t)) & 0xF), (uint32_t)((w >> (12 * t + 8)) & 0xF),
                               (uint32_t)((w >> (12 * t + 4)) & 0xF) };
     float* coords = nullptr;
@@ -1647,12 +1658,21 @@ __device__ __forceinline__ void triangle_item(const JobArgs& a, const uint64_t* 
         normals[3 * id + 2] = nn[2];
     } else {
         const ChunkCross* cross_l = a.cross + (uint64_t)l * level_words;
-        uint64_t vid[3];
+        /* the owner chunks' crossing records of all three edges are loaded
+         * before any is used, so the three loads are in flight together */
+        ChunkCross cc[3];
+        uint32_t bit[3], axs[3];
 #pragma unroll
         for (int jj = 0; jj < 3; ++jj) {
             const uint32_t o = edge_owner(eds[jj]);
-            vid[jj] = vertex_id(a, cross_l, x + (o & 1u), y + ((o >> 1) & 1u), zc + ((o >> 2) & 1u), o >> 3);
+            const uint32_t vx = x + (o & 1u), vy = y + ((o >> 1) & 1u), vz = zc + ((o >> 2) & 1u);
+            bit[jj] = vx & 63u;
+            axs[jj] = o >> 3;
+            cc[jj] = cross_l[((uint64_t)vz * a.ny + vy) * a.NC + (vx >> 6)];
         }
+        uint64_t vid[3];
+#pragma unroll
+        for (int jj = 0; jj < 3; ++jj) vid[jj] = cross_rank(cc[jj], bit[jj], axs[jj]);
         conn[3 * id + 0] = (uint32_t)vid[0];
         conn[3 * id + 1] = (uint32_t)vid[1];
         conn[3 * id + 2] = (uint32_t)vid[2];
@@ -1691,15 +1711,25 @@ __device__ __forceinline__ void triangle_item(const JobArgs& a, const uint64_t* 
     }
 }
 
-/* shared-vertex modes fit 64 VGPRs without spills: eight waves per SIMD */
 template <typename T, int M>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(M == kModeDup ? 1 : 8))) void k_triangles(JobArgs a)
+__global__ __launch_bounds__(kBlock) void k_triangles(JobArgs a)
 {
     __shared__ uint64_t tri[256];
     stage_table_block(tri);
-    const uint64_t n = a.toff[a.nlev];
-    for (uint64_t g = (uint64_t)blockIdx.x * kBlock + threadIdx.x; g < n; g += (uint64_t)gridDim.x * kBlock)
-        triangle_item<T, M>(a, tri, g);
+    const uint64_t n = a.toff[a.nlev], step = (uint64_t)gridDim.x * kBlock;
+    uint64_t g = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+    /* the next item's record is loaded while this one is worked on */
+    uint64_t rec = g < n ? a.trec[g] : 0ull;
+    uint32_t ci = g < n ? a.tci[g] : 0u;
+    for (; g < n; g += step) {
+        const uint64_t cur = rec;
+        const uint32_t cur_ci = ci;
+        if (g + step < n) {
+            rec = a.trec[g + step];
+            ci = a.tci[g + step];
+        }
+        triangle_item<T, M>(a, tri, g, cur, cur_ci);
+    }
 }
 
 /* VertexNormal of every vertex: the float sum, in triangle order, of the face
@@ -1708,14 +1738,14 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(M == kMo
  * (the first records of those in the volume come from the vertex's slots,
  * written by the triangles) and each cell's triangles that use the edge in
  * table order. */
-__device__ __forceinline__ void vertex_normal_item(const JobArgs& a, const uint64_t* tri, uint64_t g)
+__device__ __forceinline__ void vertex_normal_item(const JobArgs& a, const uint64_t* tri, uint64_t g, uint64_t key,
+                                                   uint4 sl)
 {
     const uint32_t l = level_at(a.voff, a.nlev, g);
     float* normals = nullptr;
 #pragma unroll
     for (int k = 0; k < kMaxLevels; ++k) if (k == (int)l) normals = a.normals[k];
     if (!normals) return;   // a level without outputs (not admitted)
-    const uint64_t key = a.vkey[g];
     const int x = (int)(key & 0xFFFFFu), y = (int)((key >> 20) & 0xFFFFFu);
     const int zv = (int)((key >> 40) & 0xFFFFFu);
     const uint32_t axis = (uint32_t)(key >> 60);
@@ -1729,7 +1759,6 @@ __device__ __forceinline__ void vertex_normal_item(const JobArgs& a, const uint6
         around_cell(axis, q, x, y, zv, cx, cy, cz);
         in[q] = cx >= 0 && cy >= 0 && cz >= 0 && cx + 1 < (int)a.nx && cy + 1 < (int)a.ny && cz + 1 < (int)a.nz;
     }
-    const uint4 sl = *(const uint4*)(a.vslot + 4 * g);
     const uint32_t first[4] = { sl.x, sl.y, sl.z, sl.w };
     uint32_t ci[4];
 #pragma unroll
@@ -1761,9 +1790,24 @@ __global__ __launch_bounds__(kBlock) void k_vertex_normals(JobArgs a)
 {
     __shared__ uint64_t tri[256];
     stage_table_block(tri);
-    const uint64_t n = a.voff[a.nlev];
-    for (uint64_t g = (uint64_t)blockIdx.x * kBlock + threadIdx.x; g < n; g += (uint64_t)gridDim.x * kBlock)
-        vertex_normal_item(a, tri, g);
+    const uint64_t n = a.voff[a.nlev], step = (uint64_t)gridDim.x * kBlock;
+    uint64_t g = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+    /* the next item's key and slots are loaded while this one is worked on */
+    uint64_t key = 0ull;
+    uint4 sl = make_uint4(0u, 0u, 0u, 0u);
+    if (g < n) {
+        key = a.vkey[g];
+        sl = *(const uint4*)(a.vslot + 4 * g);
+    }
+    for (; g < n; g += step) {
+        const uint64_t cur = key;
+        const uint4 cur_sl = sl;
+        if (g + step < n) {
+            key = a.vkey[g + step];
+            sl = *(const uint4*)(a.vslot + 4 * (g + step));
+        }
+        vertex_normal_item(a, tri, g, cur, cur_sl);
+    }
 }
 
 /* Items of a batch of jobs: start[j] is job j's first item in one sequence
@@ -1774,7 +1818,8 @@ __global__ __launch_bounds__(kBlock) void k_vertices_batch(const JobArgs* jobs, 
     const uint64_t base = start[0], n = start[njobs] - base;
     for (uint64_t g = (uint64_t)blockIdx.x * kBlock + threadIdx.x; g < n; g += (uint64_t)gridDim.x * kBlock) {
         const uint32_t j = find_job(start, njobs, g + base);
-        vertex_item<T>(jobs[j], g + base - start[j]);
+        const uint64_t i = g + base - start[j];
+        vertex_item<T>(jobs[j], i, jobs[j].vkey[i]);
     }
 }
 
@@ -1786,7 +1831,8 @@ __global__ __launch_bounds__(kBlock) void k_triangles_batch(const JobArgs* jobs,
     const uint64_t base = start[0], n = start[njobs] - base;
     for (uint64_t g = (uint64_t)blockIdx.x * kBlock + threadIdx.x; g < n; g += (uint64_t)gridDim.x * kBlock) {
         const uint32_t j = find_job(start, njobs, g + base);
-        triangle_item<T, M>(jobs[j], tri, g + base - start[j]);
+        const uint64_t i = g + base - start[j];
+        triangle_item<T, M>(jobs[j], tri, i, jobs[j].trec[i], jobs[j].tci[i]);
     }
 }
 
@@ -1797,7 +1843,8 @@ __global__ __launch_bounds__(kBlock) void k_vertex_normals_batch(const JobArgs* 
     const uint64_t base = start[0], n = start[njobs] - base;
     for (uint64_t g = (uint64_t)blockIdx.x * kBlock + threadIdx.x; g < n; g += (uint64_t)gridDim.x * kBlock) {
         const uint32_t j = find_job(start, njobs, g + base);
-        vertex_normal_item(jobs[j], tri, g + base - start[j]);
+        const uint64_t i = g + base - start[j];
+        vertex_normal_item(jobs[j], tri, i, jobs[j].vkey[i], *(const uint4*)(jobs[j].vslot + 4 * i));
     }
 }
 
