@@ -1660,6 +1660,7 @@ __device__ __forceinline__ void triangle_item(const JobArgs& a, const uint64_t* 
         const ChunkCross* cross_l = a.cross + (uint64_t)l * level_words;
         /* the owner chunks' crossing records of all three edges are loaded
          * before any is used, so the three loads are in flight together */
+        const uint64_t vbase = M == kModeVertex ? a.voff[l] : 0ull;
         ChunkCross cc[3];
         uint32_t bit[3], axs[3];
 #pragma unroll
@@ -1673,6 +1674,14 @@ __device__ __forceinline__ void triangle_item(const JobArgs& a, const uint64_t* 
         uint64_t vid[3];
 #pragma unroll
         for (int jj = 0; jj < 3; ++jj) vid[jj] = cross_rank(cc[jj], bit[jj], axs[jj]);
+        /* the vertices' coordinates are loaded before this item's stores
+         * (vector memory operations complete in issue order) */
+#pragma unroll
+        for (int jj = 0; jj < 3; ++jj) {
+            p[jj][0] = coords[3 * vid[jj] + 0];
+            p[jj][1] = coords[3 * vid[jj] + 1];
+            p[jj][2] = coords[3 * vid[jj] + 2];
+        }
         conn[3 * id + 0] = (uint32_t)vid[0];
         conn[3 * id + 1] = (uint32_t)vid[1];
         conn[3 * id + 2] = (uint32_t)vid[2];
@@ -1680,6 +1689,7 @@ __device__ __forceinline__ void triangle_item(const JobArgs& a, const uint64_t* 
             /* the first triangle of this cell that uses an edge tells the edge's
              * vertex where the cell's records start; the cell sits at position q
              * around the edge (around_cell) */
+            uint32_t* slots = a.vslot + 4 * vbase;
 #pragma unroll
             for (int jj = 0; jj < 3; ++jj) {
                 bool first = true;
@@ -1694,14 +1704,8 @@ __device__ __forceinline__ void triangle_item(const JobArgs& a, const uint64_t* 
                 const uint32_t q = ax == 0 ? ((dy ? 0u : 1u) | (dz ? 0u : 2u))
                                  : ax == 1 ? ((dx ? 0u : 1u) | (dz ? 0u : 2u))
                                            : ((dx ? 0u : 1u) | (dy ? 0u : 2u));
-                a.vslot[4 * (a.voff[l] + vid[jj]) + q] = (uint32_t)(g - t);
+                slots[4 * vid[jj] + q] = (uint32_t)(g - t);
             }
-        }
-#pragma unroll
-        for (int jj = 0; jj < 3; ++jj) {
-            p[jj][0] = coords[3 * vid[jj] + 0];
-            p[jj][1] = coords[3 * vid[jj] + 1];
-            p[jj][2] = coords[3 * vid[jj] + 2];
         }
         face_normal(p[0], p[1], p[2], nn);
         float* dst = M == kModeVertex ? a.fn + 3 * g : normals + 3 * id;
@@ -1763,8 +1767,11 @@ __device__ __forceinline__ void vertex_normal_item(const JobArgs& a, const uint6
     uint32_t ci[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) ci[q] = in[q] ? a.tci[first[q]] : 0u;
-    float nv[3] = { 0.0f, 0.0f, 0.0f };
+    /* per cell, the mask of its triangles (table order) that use the edge */
+    uint32_t use[4];
+#pragma unroll
     for (int q = 0; q < 4; ++q) {
+        use[q] = 0u;
         if (!in[q]) continue;
         const uint64_t w = tri[ci[q]];
         const int nt = (int)(w >> 60);
@@ -1773,11 +1780,55 @@ __device__ __forceinline__ void vertex_normal_item(const JobArgs& a, const uint6
             const uint32_t e0 = (uint32_t)((w >> (12 * t)) & 0xF);
             const uint32_t e1 = (uint32_t)((w >> (12 * t + 8)) & 0xF);
             const uint32_t e2 = (uint32_t)((w >> (12 * t + 4)) & 0xF);
-            if (e0 != me && e1 != me && e2 != me) continue;
-            const float* f = a.fn + 3 * ((uint64_t)first[q] + (uint64_t)t);
-            nv[0] = nv[0] + f[0];
-            nv[1] = nv[1] + f[1];
-            nv[2] = nv[2] + f[2];
+            if (e0 == me || e1 == me || e2 == me) use[q] |= 1u << t;
+        }
+    }
+    /* the face normals of the first two such triangles of every cell are
+     * loaded together (most cells have one or two); the sum then runs in
+     * cell order and table order, loading any further ones as it goes */
+    float f[4][2][3];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        uint32_t m = use[q];
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            f[q][k][0] = f[q][k][1] = f[q][k][2] = 0.0f;
+            if (m) {
+                const uint32_t t = (uint32_t)__ffs(m) - 1u;
+                m &= m - 1u;
+                const float* p = a.fn + 3 * ((uint64_t)first[q] + t);
+                f[q][k][0] = p[0];
+                f[q][k][1] = p[1];
+                f[q][k][2] = p[2];
+            }
+        }
+    }
+    /* a slot without a triangle holds +0, and adding +0 changes no partial
+     * sum: the sum starts at +0 and a float sum is -0 only when both terms
+     * are, so no partial sum is -0.  The starting +0 comes from an opaque
+     * move, so that the compiler does not fold the first addition into the
+     * first load's branch (which would wait for that load alone). */
+    float zero;
+    asm volatile("v_mov_b32 %0, 0" : "=v"(zero));
+    float nv[3] = { zero, zero, zero };
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            nv[0] = nv[0] + f[q][k][0];
+            nv[1] = nv[1] + f[q][k][1];
+            nv[2] = nv[2] + f[q][k][2];
+        }
+        uint32_t m = use[q];
+        m &= m - 1u;
+        m &= m - 1u;
+        while (m) {
+            const uint32_t t = (uint32_t)__ffs(m) - 1u;
+            m &= m - 1u;
+            const float* p = a.fn + 3 * ((uint64_t)first[q] + t);
+            nv[0] = nv[0] + p[0];
+            nv[1] = nv[1] + p[1];
+            nv[2] = nv[2] + p[2];
         }
     }
     const uint64_t id = g - a.voff[l];
