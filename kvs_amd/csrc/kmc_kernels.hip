@@ -91,6 +91,11 @@ __device__ __forceinline__ double iso_of(const JobArgs& a, int l)
 /* interpolation and normals                                                   */
 /* ------------------------------------------------------------------------- */
 
+template <typename T>
+__device__ __forceinline__ void interpolate_values(const JobArgs& a, double iso, T value_a, T value_b,
+                                                   uint32_t ax, uint32_t ay, uint32_t az,
+                                                   uint32_t bx, uint32_t by, uint32_t bz, float out[3]);
+
 /* interpolate_vertex<T> + (p + min_coord) * scale_factor for the edge from
  * node a to node b (integer node coordinates). */
 template <typename T>
@@ -101,8 +106,17 @@ __device__ __forceinline__ void interpolate(const JobArgs& a, const T* vol, doub
     const uint64_t line = a.nx, slice = (uint64_t)a.nx * a.ny;
     const uint64_t ia = ax + ay * line + az * slice;
     const uint64_t ib = bx + by * line + bz * slice;
-    double va = (double)vol[ia];
-    double vb = (double)vol[ib];
+    interpolate_values<T>(a, iso, vol[ia], vol[ib], ax, ay, az, bx, by, bz, out);
+}
+
+/* The interpolation of interpolate() from the two node values. */
+template <typename T>
+__device__ __forceinline__ void interpolate_values(const JobArgs& a, double iso, T value_a, T value_b,
+                                                   uint32_t ax, uint32_t ay, uint32_t az,
+                                                   uint32_t bx, uint32_t by, uint32_t bz, float out[3])
+{
+    double va = (double)value_a;
+    double vb = (double)value_b;
     va = va <= a.minv ? a.minv : (va <= a.maxv ? va : a.maxv);
     vb = vb <= a.minv ? a.minv : (vb <= a.maxv ? vb : a.maxv);
     double r = (iso - va) / (vb - va);
@@ -1572,19 +1586,29 @@ __device__ __forceinline__ uint64_t vertex_id(const JobArgs& a, const ChunkCross
     return cross_rank(cross_l[((uint64_t)z * a.ny + y) * a.NC + c], i, axis);
 }
 
-/* Coordinates of vertex g of the job's items (calculate_isopoints, :365-437). */
+/* Node values at the two ends of the edge of vertex key `key`. */
 template <typename T>
-__device__ __forceinline__ void vertex_item(const JobArgs& a, uint64_t g, uint64_t key)
+__device__ __forceinline__ void edge_values(const JobArgs& a, uint64_t key, T& va, T& vb)
 {
-    const uint32_t l = level_at(a.voff, a.nlev, g);
+    const T* vol = (const T*)a.values;
+    const uint64_t x = key & 0xFFFFFu, y = (key >> 20) & 0xFFFFFu, z = (key >> 40) & 0xFFFFFu;
+    const uint32_t axis = (uint32_t)(key >> 60);
+    const uint64_t line = a.nx, slice = (uint64_t)a.nx * a.ny;
+    const uint64_t ia = x + y * line + z * slice;
+    va = vol[ia];
+    vb = vol[ia + (axis == 0 ? 1ull : axis == 1 ? line : slice)];
+}
+
+/* Coordinates of vertex g (of level l, which has outputs) from its key and
+ * its edge's node values (calculate_isopoints, :365-437). */
+template <typename T>
+__device__ __forceinline__ void vertex_item(const JobArgs& a, uint32_t l, uint64_t g, uint64_t key, T va, T vb)
+{
     const uint32_t x = (uint32_t)(key & 0xFFFFFu), y = (uint32_t)((key >> 20) & 0xFFFFFu);
     const uint32_t z = (uint32_t)((key >> 40) & 0xFFFFFu), axis = (uint32_t)(key >> 60);
     float p[3];
-    node_edge_coords<T>(a, (const T*)a.values, iso_of(a, (int)l), x, y, z, axis, p);
-    float* co = nullptr;
-#pragma unroll
-    for (int k = 0; k < kMaxLevels; ++k) if (k == (int)l) co = a.coords[k];
-    if (!co) return;   // a level without outputs (not admitted)
+    interpolate_values<T>(a, a.iso[l], va, vb, x, y, z, x + (axis == 0), y + (axis == 1), z + (axis == 2), p);
+    float* co = a.coords[l];
     const uint64_t id = g - a.voff[l];
     co[3 * id + 0] = p[0];
     co[3 * id + 1] = p[1];
@@ -1594,14 +1618,39 @@ __device__ __forceinline__ void vertex_item(const JobArgs& a, uint64_t g, uint64
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_vertices(JobArgs a)
 {
-    const uint64_t n = a.voff[a.nlev], step = (uint64_t)gridDim.x * kBlock;
-    uint64_t g = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
-    /* the next item's key is loaded while this one is worked on */
-    uint64_t key = g < n ? a.vkey[g] : 0ull;
-    for (; g < n; g += step) {
-        const uint64_t cur = key;
-        if (g + step < n) key = a.vkey[g + step];
-        vertex_item<T>(a, g, cur);
+    const uint64_t step = (uint64_t)gridDim.x * kBlock;
+    /* level by level, so that the level's isolevel, first item and outputs
+     * are uniform (scalar loads, which leave the vector memory operations in
+     * flight alone).  Software pipeline: while item g is finished, the node
+     * values of the next item and the key of the one after it are in flight
+     * (vector memory operations complete in issue order, so they are issued
+     * before item g's stores; indices past the level's end are clamped to its
+     * last item, so that every iteration issues the same loads). */
+    for (uint32_t l = 0; l < a.nlev; ++l) {
+        const uint64_t n = a.voff[l + 1];
+        uint64_t g = a.voff[l] + (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+        if (g >= n || !a.coords[l]) continue;   // no items, or a level without outputs (not admitted)
+        const uint64_t last = n - 1;
+        /* two register sets take turns (unrolled by two), so that no loaded
+         * value is copied -- a copy would wait for its load */
+        uint64_t k0 = a.vkey[g], k1;
+        T a0, b0, a1, b1;
+        edge_values<T>(a, k0, a0, b0);
+        uint64_t kn = a.vkey[g + step < last ? g + step : last];
+        for (;;) {
+            k1 = kn;
+            edge_values<T>(a, k1, a1, b1);
+            kn = a.vkey[g + 2 * step < last ? g + 2 * step : last];
+            vertex_item<T>(a, l, g, k0, a0, b0);
+            g += step;
+            if (g >= n) break;
+            k0 = kn;
+            edge_values<T>(a, k0, a0, b0);
+            kn = a.vkey[g + 2 * step < last ? g + 2 * step : last];
+            vertex_item<T>(a, l, g, k1, a1, b1);
+            g += step;
+            if (g >= n) break;
+        }
     }
 }
 
@@ -1742,13 +1791,10 @@ __global__ __launch_bounds__(kBlock) void k_triangles(JobArgs a)
  * (the first records of those in the volume come from the vertex's slots,
  * written by the triangles) and each cell's triangles that use the edge in
  * table order. */
-__device__ __forceinline__ void vertex_normal_item(const JobArgs& a, const uint64_t* tri, uint64_t g, uint64_t key,
-                                                   uint4 sl)
+__device__ __forceinline__ void vertex_normal_item(const JobArgs& a, const uint64_t* tri, uint32_t l, uint64_t g,
+                                                   uint64_t key, uint4 sl)
 {
-    const uint32_t l = level_at(a.voff, a.nlev, g);
-    float* normals = nullptr;
-#pragma unroll
-    for (int k = 0; k < kMaxLevels; ++k) if (k == (int)l) normals = a.normals[k];
+    float* normals = a.normals[l];
     if (!normals) return;   // a level without outputs (not admitted)
     const int x = (int)(key & 0xFFFFFu), y = (int)((key >> 20) & 0xFFFFFu);
     const int zv = (int)((key >> 40) & 0xFFFFFu);
@@ -1841,23 +1887,24 @@ __global__ __launch_bounds__(kBlock) void k_vertex_normals(JobArgs a)
 {
     __shared__ uint64_t tri[256];
     stage_table_block(tri);
-    const uint64_t n = a.voff[a.nlev], step = (uint64_t)gridDim.x * kBlock;
-    uint64_t g = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
-    /* the next item's key and slots are loaded while this one is worked on */
-    uint64_t key = 0ull;
-    uint4 sl = make_uint4(0u, 0u, 0u, 0u);
-    if (g < n) {
-        key = a.vkey[g];
-        sl = *(const uint4*)(a.vslot + 4 * g);
-    }
-    for (; g < n; g += step) {
-        const uint64_t cur = key;
-        const uint4 cur_sl = sl;
-        if (g + step < n) {
-            key = a.vkey[g + step];
-            sl = *(const uint4*)(a.vslot + 4 * (g + step));
+    const uint64_t step = (uint64_t)gridDim.x * kBlock;
+    /* level by level (uniform per-level values); the next item's key and
+     * slots are loaded while this one is worked on */
+    for (uint32_t l = 0; l < a.nlev; ++l) {
+        const uint64_t n = a.voff[l + 1];
+        uint64_t g = a.voff[l] + (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+        if (g >= n) continue;
+        const uint64_t last = n - 1;
+        uint64_t key = a.vkey[g];
+        uint4 sl = *(const uint4*)(a.vslot + 4 * g);
+        for (; g < n; g += step) {
+            const uint64_t cur = key;
+            const uint4 cur_sl = sl;
+            const uint64_t gn = g + step < last ? g + step : last;
+            key = a.vkey[gn];
+            sl = *(const uint4*)(a.vslot + 4 * gn);
+            vertex_normal_item(a, tri, l, g, cur, cur_sl);
         }
-        vertex_normal_item(a, tri, g, cur, cur_sl);
     }
 }
 
@@ -1870,7 +1917,12 @@ __global__ __launch_bounds__(kBlock) void k_vertices_batch(const JobArgs* jobs, 
     for (uint64_t g = (uint64_t)blockIdx.x * kBlock + threadIdx.x; g < n; g += (uint64_t)gridDim.x * kBlock) {
         const uint32_t j = find_job(start, njobs, g + base);
         const uint64_t i = g + base - start[j];
-        vertex_item<T>(jobs[j], i, jobs[j].vkey[i]);
+        const uint32_t l = level_at(jobs[j].voff, jobs[j].nlev, i);
+        if (!jobs[j].coords[l]) continue;   // a level without outputs (not admitted)
+        const uint64_t key = jobs[j].vkey[i];
+        T va, vb;
+        edge_values<T>(jobs[j], key, va, vb);
+        vertex_item<T>(jobs[j], l, i, key, va, vb);
     }
 }
 
@@ -1895,7 +1947,8 @@ __global__ __launch_bounds__(kBlock) void k_vertex_normals_batch(const JobArgs* 
     for (uint64_t g = (uint64_t)blockIdx.x * kBlock + threadIdx.x; g < n; g += (uint64_t)gridDim.x * kBlock) {
         const uint32_t j = find_job(start, njobs, g + base);
         const uint64_t i = g + base - start[j];
-        vertex_normal_item(jobs[j], tri, i, jobs[j].vkey[i], *(const uint4*)(jobs[j].vslot + 4 * i));
+        vertex_normal_item(jobs[j], tri, level_at(jobs[j].voff, jobs[j].nlev, i), i, jobs[j].vkey[i],
+                           *(const uint4*)(jobs[j].vslot + 4 * i));
     }
 }
 
