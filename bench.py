@@ -375,7 +375,8 @@ def batched_leg(lib, kmc, datasets, torch, args):
         "batched_mtriangles_per_s": round(tris_plain / (batch_ms / 1e3) / 1e6, 2),
         "plain_gvoxels_per_s": round(classified / (plain_ms / 1e3) / 1e9, 2),
         "plan_s": round(plan_s, 2),
-        "autotuned": {"tile_rows": info["tile_rows"], "queue_capacity": info["queue_capacity"]},
+        "autotuned": {"emit": "flat" if info["flat_emit"] else "tile teams", "tile_rows": info["tile_rows"],
+                      "queue_capacity": info["queue_capacity"]},
         "graph_nodes": info["graph_nodes"], "passes": info["n_passes"],
     }
 

@@ -209,6 +209,7 @@ typedef struct kmc_batch_info {
     uint64_t arena_bytes;        /* device bytes of the output arena */
     uint64_t work_bytes;         /* device bytes of counts, prefixes and classification masks */
     double last_run_ms;          /* device time of the last run's graph (HIP events) */
+    uint32_t flat_emit;          /* 1: flat emit (compaction records, one thread per item); 0: tile teams */
 } kmc_batch_info;
 
 /* Plan a batch: validate every job (first failing job decides the status),

@@ -940,6 +940,7 @@ kmc_status kmc_batch_info_get(const kmc_batch* b, kmc_batch_info* info)
     info->arena_bytes = 4 * (b->cap_c + b->cap_n + b->cap_k);
     info->work_bytes = b->n_counts * 12 + b->bitvol_words * 8;
     info->last_run_ms = b->last_ms;
+    info->flat_emit = b->flat ? 1u : 0u;
     return KMC_OK;
 }
 

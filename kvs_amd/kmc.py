@@ -96,6 +96,7 @@ class BatchInfo(ctypes.Structure):
         ("arena_bytes", ctypes.c_uint64),
         ("work_bytes", ctypes.c_uint64),
         ("last_run_ms", ctypes.c_double),
+        ("flat_emit", ctypes.c_uint32),
     ]
 
 

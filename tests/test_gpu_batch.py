@@ -116,6 +116,7 @@ def test_batch_object_reruns_follow_changed_values():
                     k += 1
         info = b.info()
         assert info["n_surfaces"] == 18 and info["n_jobs"] == 9
+        assert info["flat_emit"] == 1           # the default emit form
         assert info["graph_nodes"] > 0
         assert info["recaptures"] >= 2          # the sizing capture plus at least one growth
     finally:
@@ -153,6 +154,7 @@ def test_autotuned_batch_matches_plain():
         info = b.info()
         print("autotuned", info, flush=True)
         assert info["tile_rows"] in (2, 4, 8, 16) and info["queue_capacity"] in (64, 128, 256)
+        assert info["flat_emit"] in (0, 1)
         b.run()
         views = b.surfaces()
         for j, a in enumerate(arrays):
