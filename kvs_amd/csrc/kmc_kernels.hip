@@ -2040,12 +2040,28 @@ __global__ __launch_bounds__(kBlock) void k_scan_apply(const uint32_t* counts, u
     const uint64_t b = (uint64_t)seg * kScanTile + (uint64_t)threadIdx.x * kScanPer;
     uint32_t vals[kScanPer];
     uint64_t s = 0;
+    /* a thread's 16 counts are contiguous: 16-byte loads where they lie
+     * inside the array (the array bases are 16-byte aligned when n is a
+     * multiple of 4) */
+    const bool whole = b + kScanPer <= n && (n & 3u) == 0;
+    if (whole) {
 #pragma unroll
-    for (uint32_t i = 0; i < kScanPer; ++i) {
-        const uint64_t g = b + i;
-        vals[i] = g < n ? src[g] : 0u;
-        s += vals[i];
+        for (uint32_t i = 0; i < kScanPer; i += 4) {
+            const uint4 v = *(const uint4*)(src + b + i);
+            vals[i] = v.x;
+            vals[i + 1] = v.y;
+            vals[i + 2] = v.z;
+            vals[i + 3] = v.w;
+        }
+    } else {
+#pragma unroll
+        for (uint32_t i = 0; i < kScanPer; ++i) {
+            const uint64_t g = b + i;
+            vals[i] = g < n ? src[g] : 0u;
+        }
     }
+#pragma unroll
+    for (uint32_t i = 0; i < kScanPer; ++i) s += vals[i];
     /* exclusive scan of s across the block */
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     uint64_t incl = s;
@@ -2059,11 +2075,23 @@ __global__ __launch_bounds__(kBlock) void k_scan_apply(const uint32_t* counts, u
     uint64_t run = base;
     for (int w = 0; w < wave; ++w) run += sh[w];
     run += incl - s;
+    if (whole) {
 #pragma unroll
-    for (uint32_t i = 0; i < kScanPer; ++i) {
-        const uint64_t g = b + i;
-        if (g < n) dst[g] = run;
-        run += vals[i];
+        for (uint32_t i = 0; i < kScanPer; i += 2) {
+            ulonglong2 o;
+            o.x = run;
+            run += vals[i];
+            o.y = run;
+            run += vals[i + 1];
+            *(ulonglong2*)(dst + b + i) = o;
+        }
+    } else {
+#pragma unroll
+        for (uint32_t i = 0; i < kScanPer; ++i) {
+            const uint64_t g = b + i;
+            if (g < n) dst[g] = run;
+            run += vals[i];
+        }
     }
 }
 
