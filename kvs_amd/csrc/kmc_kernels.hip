@@ -940,12 +940,16 @@ __device__ __forceinline__ void classify_row(const JobArgs& a, uint64_t row)
             }
         }
         /* the 8-level instance also serves passes of 5..7 levels: store only
-         * the pass's own levels */
+         * the pass's own levels.  Nontemporal stores: the masks' writes,
+         * mixed into the volume's read stream, cost more than their bytes
+         * (measured: 384 MiB of mask stores add ~180 us to a 4 GiB read, 25 us
+         * less with the nontemporal hint) */
         if (lane < cend) {
 #pragma unroll
             for (int l = 0; l < NLEV; ++l)
                 if (NLEV < 8 || l < (int)a.nlev)
-                    a.bitvol[(((uint64_t)l * a.nz + z) * a.ny + y) * a.NC + c0 + lane] = ((uint64_t)hi[l] << 32) | lo[l];
+                    __builtin_nontemporal_store(((uint64_t)hi[l] << 32) | lo[l],
+                                                &a.bitvol[(((uint64_t)l * a.nz + z) * a.ny + y) * a.NC + c0 + lane]);
         }
     }
 }
@@ -1209,8 +1213,9 @@ __device__ __forceinline__ void count_rows_t(const JobArgs& a, const uint8_t* nt
                     const uint32_t e = group_excl_scan(cv | ct << 16, cl, CS, s);
                     if (ok && c < a.NC) {
                         const uint64_t w = (uint64_t)l * level_words + r * a.NC + c;
-                        a.vpc[w] = vc + (e & 0xFFFFu);
-                        a.tpc[w] = tc + (e >> 16);
+                        /* coalesced stream of prefixes: nontemporal (measured faster) */
+                        __builtin_nontemporal_store(vc + (e & 0xFFFFu), &a.vpc[w]);
+                        __builtin_nontemporal_store(tc + (e >> 16), &a.tpc[w]);
                     }
                     vc += s & 0xFFFFu;
                     tc += s >> 16;

@@ -78,6 +78,20 @@ def test_levels_fused_equal_separate():
             assert_same_surface(got, want, f"fused level {lv} mode {normal},{dup}")
 
 
+def test_fused_levels_with_empty_levels_between():
+    # levels outside the value range give no items; the per-item kernels walk
+    # the levels one after another and must skip them
+    v = datasets.blobs((33, 20, 70), seed=12, nblobs=5)
+    lo, hi = float(v.min()), float(v.max())
+    levels = [hi + 1.0, 0.3, lo - 1.0, hi + 2.0, 0.6, 0.45, lo - 0.5]
+    for normal, dup in MODES:
+        outs = kmc.extract_levels(v, levels, normal, dup)
+        for lv, got in zip(levels, outs):
+            want = oracle.extract(v, lv, normal, dup)
+            assert_same_surface(got, want, f"level {lv} mode {normal},{dup}")
+        assert outs[0]["n_triangles"] == 0 and outs[1]["n_triangles"] > 0
+
+
 def test_more_levels_than_one_pass():
     v = datasets.blobs(24, seed=11, nblobs=4)
     levels = [0.05 * (i + 1) for i in range(11)]
