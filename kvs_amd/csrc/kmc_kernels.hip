@@ -1213,9 +1213,16 @@ __device__ __forceinline__ void count_rows_t(const JobArgs& a, const uint8_t* nt
                     const uint32_t e = group_excl_scan(cv | ct << 16, cl, CS, s);
                     if (ok && c < a.NC) {
                         const uint64_t w = (uint64_t)l * level_words + r * a.NC + c;
-                        /* coalesced stream of prefixes: nontemporal (measured faster) */
-                        __builtin_nontemporal_store(vc + (e & 0xFFFFu), &a.vpc[w]);
-                        __builtin_nontemporal_store(tc + (e >> 16), &a.tpc[w]);
+                        /* coalesced stream of prefixes: nontemporal (measured
+                         * faster).  Rows of at most 64 chunks are one group,
+                         * so the packed scan result is the chunk's in-row
+                         * prefix pair (chunk_prefix) */
+                        if (a.NC <= 64) {
+                            __builtin_nontemporal_store(e, &a.vpc[w]);
+                        } else {
+                            __builtin_nontemporal_store(vc + (e & 0xFFFFu), &a.vpc[w]);
+                            __builtin_nontemporal_store(tc + (e >> 16), &a.tpc[w]);
+                        }
                     }
                     vc += s & 0xFFFFu;
                     tc += s >> 16;
@@ -1396,6 +1403,22 @@ __device__ __forceinline__ uint32_t level_at(const uint64_t* off, uint32_t nlev,
     return l;
 }
 
+/* In-row prefixes (vertices, triangles) of chunk word w.  Rows of at most 64
+ * chunks keep both in one word of vpc (vertices in the low, triangles in the
+ * high 16 bits: at most 63 chunks of 192 and 320 come before a chunk);
+ * longer rows keep them in vpc and tpc. */
+__device__ __forceinline__ void chunk_prefix(const JobArgs& a, uint64_t w, uint32_t& v, uint32_t& t)
+{
+    if (a.NC <= 64) {
+        const uint32_t p = a.vpc[w];
+        v = p & 0xFFFFu;
+        t = p >> 16;
+    } else {
+        v = a.vpc[w];
+        t = a.tpc[w];
+    }
+}
+
 /* One non-empty chunk of the compaction, queued as y | z << 20 | level << 40
  * | chunk << 44. */
 __device__ __forceinline__ void compact_chunk(const JobArgs& a, const uint8_t* ntri, uint64_t e)
@@ -1409,11 +1432,13 @@ __device__ __forceinline__ void compact_chunk(const JobArgs& a, const uint8_t* n
     const uint64_t* pre_t = a.prefix + (uint64_t)(l * 2 + 1) * a.nrows;
     RowChunk o;
     row_chunk(a, a.bitvol + (uint64_t)l * level_words, y, z, c, true, o);
+    uint32_t pv, pt;
+    chunk_prefix(a, w, pv, pt);
     if (!a.dup) {
         uint64_t m = o.x | o.y | o.z;
         if (m) {
             ChunkCross cc;
-            cc.base = (pre_v[r] - pre_v[0]) + a.vpc[w];
+            cc.base = (pre_v[r] - pre_v[0]) + pv;
             cc.x = o.x;
             cc.y = o.y;
             cc.z = o.z;
@@ -1432,7 +1457,7 @@ __device__ __forceinline__ void compact_chunk(const JobArgs& a, const uint8_t* n
     }
     uint64_t m = o.act;
     if (m) {
-        uint64_t id = a.toff[l] + (pre_t[r] - pre_t[0]) + a.tpc[w];
+        uint64_t id = a.toff[l] + (pre_t[r] - pre_t[0]) + pt;
         const uint64_t cellkey = (uint64_t)y << 20 | (uint64_t)(z - 1) << 40;
         while (m) {
             const uint32_t i = (uint32_t)__ffsll((unsigned long long)m) - 1u;
@@ -1495,9 +1520,14 @@ __device__ __forceinline__ void compact_rows_t(const JobArgs& a, const uint8_t* 
                          * them) are skipped on the count pass's prefixes alone */
                         const uint32_t l = l0 + k;
                         const uint64_t w = (uint64_t)l * level_words + r * a.NC + c;
-                        const uint32_t v0 = a.vpc[w], t0 = a.tpc[w];
-                        const uint32_t v1 = c + 1 < a.NC ? a.vpc[w + 1] : a.counts[(uint64_t)(l * 2 + 0) * a.nrows + r];
-                        const uint32_t t1 = c + 1 < a.NC ? a.tpc[w + 1] : a.counts[(uint64_t)(l * 2 + 1) * a.nrows + r];
+                        uint32_t v0, t0, v1, t1;
+                        chunk_prefix(a, w, v0, t0);
+                        if (c + 1 < a.NC) {
+                            chunk_prefix(a, w + 1, v1, t1);
+                        } else {
+                            v1 = a.counts[(uint64_t)(l * 2 + 0) * a.nrows + r];
+                            t1 = a.counts[(uint64_t)(l * 2 + 1) * a.nrows + r];
+                        }
                         busy[k] = (!a.dup && v1 != v0) || t1 != t0;
                     }
                 }
