@@ -1820,39 +1820,63 @@ __global__ __launch_bounds__(kBlock) void k_triangles(JobArgs a)
     }
 }
 
-/* VertexNormal of every vertex: the float sum, in triangle order, of the face
- * normals of the triangles that use it (calculate_normals_on_vertex,
- * :569-605).  The four cells around the edge are visited in cell scan order
- * (the first records of those in the volume come from the vertex's slots,
- * written by the triangles) and each cell's triangles that use the edge in
- * table order. */
-__device__ __forceinline__ void vertex_normal_item(const JobArgs& a, const uint64_t* tri, uint32_t l, uint64_t g,
-                                                   uint64_t key, uint4 sl)
+/* The cells around vertex key `key`'s edge that lie in the volume: each is
+ * active, and the first of its triangles that uses the edge wrote the slot. */
+__device__ __forceinline__ void vn_cells_in(const JobArgs& a, uint64_t key, bool in[4])
 {
-    float* normals = a.normals[l];
-    if (!normals) return;   // a level without outputs (not admitted)
     const int x = (int)(key & 0xFFFFFu), y = (int)((key >> 20) & 0xFFFFFu);
     const int zv = (int)((key >> 40) & 0xFFFFFu);
     const uint32_t axis = (uint32_t)(key >> 60);
-    const uint32_t around = around_nibbles(axis);
-    /* the cells around the edge that lie in the volume: each is active, and the
-     * first of its triangles that uses the edge wrote the slot */
-    bool in[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
         int cx, cy, cz;
         around_cell(axis, q, x, y, zv, cx, cy, cz);
         in[q] = cx >= 0 && cy >= 0 && cz >= 0 && cx + 1 < (int)a.nx && cy + 1 < (int)a.ny && cz + 1 < (int)a.nz;
     }
+}
+
+/* Cube indices of the cells around the edge (0 for cells outside the volume).
+ * With `always`, cells outside load record 0 instead of nothing (the job has
+ * triangles), so that every call issues the same loads. */
+template <bool always>
+__device__ __forceinline__ void vn_cube_indices(const JobArgs& a, uint64_t key, uint4 sl, uint32_t ci[4])
+{
+    bool in[4];
+    vn_cells_in(a, key, in);
     const uint32_t first[4] = { sl.x, sl.y, sl.z, sl.w };
-    uint32_t ci[4];
-#pragma unroll
-    for (int q = 0; q < 4; ++q) ci[q] = in[q] ? a.tci[first[q]] : 0u;
-    /* per cell, the mask of its triangles (table order) that use the edge */
-    uint32_t use[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-        use[q] = 0u;
+        if constexpr (always) {
+            const uint32_t c = a.tci[in[q] ? first[q] : 0u];
+            ci[q] = in[q] ? c : 0u;
+        } else {
+            ci[q] = in[q] ? a.tci[first[q]] : 0u;
+        }
+    }
+}
+
+/* Per cell around the edge, the mask of its triangles (table order) that use
+ * the edge, and the face normals of the first two such triangles of every
+ * cell, loaded together (most cells have one or two). */
+struct VnGather {
+    uint32_t use[4];
+    uint32_t first[4];
+    float f[4][2][3];
+};
+
+__device__ __forceinline__ void vn_gather(const JobArgs& a, const uint64_t* tri, uint64_t key, uint4 sl,
+                                          const uint32_t ci[4], VnGather& o)
+{
+    bool in[4];
+    vn_cells_in(a, key, in);
+    const uint32_t around = around_nibbles((uint32_t)(key >> 60));
+    o.first[0] = sl.x;
+    o.first[1] = sl.y;
+    o.first[2] = sl.z;
+    o.first[3] = sl.w;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        o.use[q] = 0u;
         if (!in[q]) continue;
         const uint64_t w = tri[ci[q]];
         const int nt = (int)(w >> 60);
@@ -1861,34 +1885,38 @@ __device__ __forceinline__ void vertex_normal_item(const JobArgs& a, const uint6
             const uint32_t e0 = (uint32_t)((w >> (12 * t)) & 0xF);
             const uint32_t e1 = (uint32_t)((w >> (12 * t + 8)) & 0xF);
             const uint32_t e2 = (uint32_t)((w >> (12 * t + 4)) & 0xF);
-            if (e0 == me || e1 == me || e2 == me) use[q] |= 1u << t;
+            if (e0 == me || e1 == me || e2 == me) o.use[q] |= 1u << t;
         }
     }
-    /* the face normals of the first two such triangles of every cell are
-     * loaded together (most cells have one or two); the sum then runs in
-     * cell order and table order, loading any further ones as it goes */
-    float f[4][2][3];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-        uint32_t m = use[q];
+        uint32_t m = o.use[q];
 #pragma unroll
         for (int k = 0; k < 2; ++k) {
-            f[q][k][0] = f[q][k][1] = f[q][k][2] = 0.0f;
+            o.f[q][k][0] = o.f[q][k][1] = o.f[q][k][2] = 0.0f;
             if (m) {
                 const uint32_t t = (uint32_t)__ffs(m) - 1u;
                 m &= m - 1u;
-                const float* p = a.fn + 3 * ((uint64_t)first[q] + t);
-                f[q][k][0] = p[0];
-                f[q][k][1] = p[1];
-                f[q][k][2] = p[2];
+                const float* p = a.fn + 3 * ((uint64_t)o.first[q] + t);
+                o.f[q][k][0] = p[0];
+                o.f[q][k][1] = p[1];
+                o.f[q][k][2] = p[2];
             }
         }
     }
-    /* a slot without a triangle holds +0, and adding +0 changes no partial
-     * sum: the sum starts at +0 and a float sum is -0 only when both terms
-     * are, so no partial sum is -0.  The starting +0 comes from an opaque
-     * move, so that the compiler does not fold the first addition into the
-     * first load's branch (which would wait for that load alone). */
+}
+
+/* VertexNormal of vertex g (level l, which has outputs): the float sum, in
+ * triangle order, of the face normals of the triangles that use it
+ * (calculate_normals_on_vertex, :569-605) -- the four cells around the edge in
+ * cell scan order and each cell's triangles that use the edge in table order.
+ * A slot without a triangle holds +0, and adding +0 changes no partial sum:
+ * the sum starts at +0 and a float sum is -0 only when both terms are, so no
+ * partial sum is -0.  The starting +0 comes from an opaque move, so that the
+ * compiler does not fold the first addition into the first load's branch
+ * (which would wait for that load alone). */
+__device__ __forceinline__ void vn_finish(const JobArgs& a, uint32_t l, uint64_t g, const VnGather& o)
+{
     float zero;
     asm volatile("v_mov_b32 %0, 0" : "=v"(zero));
     float nv[3] = { zero, zero, zero };
@@ -1896,26 +1924,38 @@ __device__ __forceinline__ void vertex_normal_item(const JobArgs& a, const uint6
     for (int q = 0; q < 4; ++q) {
 #pragma unroll
         for (int k = 0; k < 2; ++k) {
-            nv[0] = nv[0] + f[q][k][0];
-            nv[1] = nv[1] + f[q][k][1];
-            nv[2] = nv[2] + f[q][k][2];
+            nv[0] = nv[0] + o.f[q][k][0];
+            nv[1] = nv[1] + o.f[q][k][1];
+            nv[2] = nv[2] + o.f[q][k][2];
         }
-        uint32_t m = use[q];
+        uint32_t m = o.use[q];
         m &= m - 1u;
         m &= m - 1u;
-        while (m) {
+        while (m) {   // further triangles of the cell that use the edge (rare)
             const uint32_t t = (uint32_t)__ffs(m) - 1u;
             m &= m - 1u;
-            const float* p = a.fn + 3 * ((uint64_t)first[q] + t);
+            const float* p = a.fn + 3 * ((uint64_t)o.first[q] + t);
             nv[0] = nv[0] + p[0];
             nv[1] = nv[1] + p[1];
             nv[2] = nv[2] + p[2];
         }
     }
+    float* normals = a.normals[l];
     const uint64_t id = g - a.voff[l];
     normals[3 * id + 0] = nv[0];
     normals[3 * id + 1] = nv[1];
     normals[3 * id + 2] = nv[2];
+}
+
+__device__ __forceinline__ void vertex_normal_item(const JobArgs& a, const uint64_t* tri, uint32_t l, uint64_t g,
+                                                   uint64_t key, uint4 sl)
+{
+    if (!a.normals[l]) return;   // a level without outputs (not admitted)
+    uint32_t ci[4];
+    vn_cube_indices<false>(a, key, sl, ci);
+    VnGather o;
+    vn_gather(a, tri, key, sl, ci, o);
+    vn_finish(a, l, g, o);
 }
 
 __global__ __launch_bounds__(kBlock) void k_vertex_normals(JobArgs a)
@@ -1923,22 +1963,55 @@ __global__ __launch_bounds__(kBlock) void k_vertex_normals(JobArgs a)
     __shared__ uint64_t tri[256];
     stage_table_block(tri);
     const uint64_t step = (uint64_t)gridDim.x * kBlock;
-    /* level by level (uniform per-level values); the next item's key and
-     * slots are loaded while this one is worked on */
+    const bool triangles = a.toff[a.nlev] > 0;
+    /* level by level (uniform per-level values) */
     for (uint32_t l = 0; l < a.nlev; ++l) {
         const uint64_t n = a.voff[l + 1];
         uint64_t g = a.voff[l] + (uint64_t)blockIdx.x * kBlock + threadIdx.x;
-        if (g >= n) continue;
+        if (g >= n || !a.normals[l]) continue;   // no items, or a level without outputs (not admitted)
         const uint64_t last = n - 1;
-        uint64_t key = a.vkey[g];
-        uint4 sl = *(const uint4*)(a.vslot + 4 * g);
-        for (; g < n; g += step) {
-            const uint64_t cur = key;
-            const uint4 cur_sl = sl;
-            const uint64_t gn = g + step < last ? g + step : last;
-            key = a.vkey[gn];
-            sl = *(const uint4*)(a.vslot + 4 * gn);
-            vertex_normal_item(a, tri, l, g, cur, cur_sl);
+        if (!triangles) {
+            /* no triangle at all: no vertex has a cell in the volume around it */
+            for (; g < n; g += step) vertex_normal_item(a, tri, l, g, a.vkey[g], *(const uint4*)(a.vslot + 4 * g));
+            continue;
+        }
+        /* software pipeline over two register sets that take turns (a copy
+         * would wait for its load): while vertex g is summed, the next
+         * vertex's cube indices are in flight, and the key and slots of the one
+         * after it are loaded before this vertex's store (vector memory
+         * operations complete in issue order); indices past the level's end are
+         * clamped to its last item */
+        uint64_t k0 = a.vkey[g], k1;
+        uint4 s0 = *(const uint4*)(a.vslot + 4 * g), s1;
+        uint32_t c0[4], c1[4];
+        vn_cube_indices<true>(a, k0, s0, c0);
+        {
+            const uint64_t g1 = g + step < last ? g + step : last;
+            k1 = a.vkey[g1];
+            s1 = *(const uint4*)(a.vslot + 4 * g1);
+        }
+        VnGather o;
+        for (;;) {
+            vn_cube_indices<true>(a, k1, s1, c1);
+            vn_gather(a, tri, k0, s0, c0, o);
+            {
+                const uint64_t g2 = g + 2 * step < last ? g + 2 * step : last;
+                k0 = a.vkey[g2];
+                s0 = *(const uint4*)(a.vslot + 4 * g2);
+            }
+            vn_finish(a, l, g, o);
+            g += step;
+            if (g >= n) break;
+            vn_cube_indices<true>(a, k0, s0, c0);
+            vn_gather(a, tri, k1, s1, c1, o);
+            {
+                const uint64_t g2 = g + 2 * step < last ? g + 2 * step : last;
+                k1 = a.vkey[g2];
+                s1 = *(const uint4*)(a.vslot + 4 * g2);
+            }
+            vn_finish(a, l, g, o);
+            g += step;
+            if (g >= n) break;
         }
     }
 }
