@@ -1597,8 +1597,9 @@ __global__ __launch_bounds__(kBlock) void k_compact_batch(const JobArgs* jobs, c
     }
 }
 
-/* Rank of the vertex on the edge along `axis` of node i of a chunk among the
- * chunk's vertices, plus the chunk's first vertex id: the crossing edges of the
+/* Vertex id (within its level) of the vertex on the edge along `axis` of node
+ * i of a chunk, from the chunk's crossing record o: the chunk's first vertex id
+ * (row prefix + chunk prefix within the row) + the crossing edges of the
  * chunk's nodes before i and of node i's lower axes. */
 __device__ __forceinline__ uint64_t cross_rank(const ChunkCross& o, uint32_t i, uint32_t axis)
 {
@@ -1608,17 +1609,6 @@ __device__ __forceinline__ uint64_t cross_rank(const ChunkCross& o, uint32_t i, 
     if (axis >= 1) r += (o.x >> i) & 1ull;
     if (axis >= 2) r += (o.y >> i) & 1ull;
     return r;
-}
-
-/* Vertex id of the shared vertex on the edge of node (x, y, z) along axis:
- * the first vertex of its chunk (row prefix + chunk prefix within the row)
- * + the crossing edges of the chunk's nodes before x (and of x's lower
- * axes), from the chunk's compaction record. */
-__device__ __forceinline__ uint64_t vertex_id(const JobArgs& a, const ChunkCross* cross_l, uint32_t x, uint32_t y,
-                                              uint32_t z, uint32_t axis)
-{
-    const uint32_t c = x >> 6, i = x & 63u;
-    return cross_rank(cross_l[((uint64_t)z * a.ny + y) * a.NC + c], i, axis);
 }
 
 /* Node values at the two ends of the edge of vertex key `key`. */
