@@ -230,9 +230,11 @@ def kernel_rooflines(prof, n, nlev, counts):
       classify        reads the volume (4 B per node), writes the classification
                       masks (nlev bits per node, in 64-node words);
       count           reads the masks, writes 2 x 4 B per node row and level and
-                      the in-row chunk prefixes (2 x 4 B per 64-node chunk and level);
+                      the in-row chunk prefixes (4 B per 64-node chunk and level
+                      for rows of at most 64 chunks, else 2 x 4 B);
       scan            reads those row counts, writes 8 B prefixes;
-      compact         reads the chunk prefixes and the masks, writes the records
+      compact         reads the chunk prefixes (as written by count) and the
+                      masks, writes the records
                       (8 B per vertex, 8 + 1 B per triangle; the 32-byte crossing
                       records of chunks with a vertex are not counted);
       vertices        reads each vertex key and its edge's two values, writes its
@@ -248,14 +250,15 @@ def kernel_rooflines(prof, n, nlev, counts):
     nc = (n + 63) // 64
     masks = nlev * n * n * nc * 8
     chunks = nlev * n * n * nc
+    prefix_bytes = (4 if nc <= 64 else 8) * chunks
     rows = n * n
     nv = sum(v for v, _ in counts)
     nt = sum(t for _, t in counts)
     algo = {
         "classify": n ** 3 * 4 + masks,
-        "count": masks + 2 * nlev * rows * 4 + 8 * chunks,
+        "count": masks + 2 * nlev * rows * 4 + prefix_bytes,
         "scan": 2 * nlev * rows * (4 + 8),
-        "compact": 8 * chunks + masks + 8 * nv + 9 * nt,
+        "compact": prefix_bytes + masks + 8 * nv + 9 * nt,
         "vertices": 28 * nv,
         "triangles": 69 * nt,
         "vertex_normals": 40 * nv + 36 * nt,
