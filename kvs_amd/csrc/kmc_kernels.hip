@@ -171,6 +171,12 @@ __device__ __forceinline__ uint64_t readlane64(uint64_t x, int k)
     return (uint64_t)hi << 32 | lo;
 }
 
+/* Set bits of the wave mask m on the lanes below this one (v_mbcnt). */
+__device__ __forceinline__ uint32_t lanes_below(uint64_t m)
+{
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
 /* LDS arrays of one team (layout of lds_layout) and the shared case table. */
 struct Team {
     const uint64_t* tri;  // [256] case table
@@ -923,7 +929,7 @@ __device__ __forceinline__ void classify_row(const JobArgs& a, uint64_t row)
 #pragma unroll
             for (uint32_t i = 0; i < kBatch; ++i) {
                 const uint32_t x = (c0 + cb + i) * 64u + lane;
-                val[i] = (cb + i < cend && x < a.nx) ? src[x] : T(0);
+                val[i] = (cb + i < cend && x < a.nx) ? __builtin_nontemporal_load(src + x) : T(0);   // read once
             }
 #pragma unroll
             for (uint32_t i = 0; i < kBatch; ++i) {
@@ -1500,7 +1506,6 @@ __device__ __forceinline__ void compact_rows_t(const JobArgs& a, const uint8_t* 
     const uint32_t sub = lane >> a.cshift, cl = lane & ((1u << a.cshift) - 1u);
     const uint32_t G = 64u >> a.cshift, CS = 1u << a.cshift;
     const uint64_t level_words = (uint64_t)a.nz * a.ny * a.NC;
-    const uint64_t below = (1ull << lane) - 1ull;
     if (!a.trec) return;   // the records did not fit (batched form): nothing to write
     uint32_t qn = 0;
     for (uint64_t g = r0 + wave_first * G; g < r1; g += wave_step * G) {
@@ -1535,7 +1540,7 @@ __device__ __forceinline__ void compact_rows_t(const JobArgs& a, const uint8_t* 
                 for (uint32_t k = 0; k < NL; ++k) {
                     const uint64_t bm = __ballot(busy[k]);
                     if (busy[k])
-                        q[qn + (uint32_t)__popcll(bm & below)] =
+                        q[qn + lanes_below(bm)] =
                             (uint64_t)y | (uint64_t)z << 20 | (uint64_t)(l0 + k) << 40 | (uint64_t)c << 44;
                     qn += (uint32_t)__popcll(bm);
                 }
