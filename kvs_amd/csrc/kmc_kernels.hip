@@ -1273,7 +1273,9 @@ __device__ __forceinline__ void stage_ntri(uint8_t* ntri)
     __syncthreads();
 }
 
-__global__ __launch_bounds__(kBlock) void k_count(JobArgs a)
+/* eight waves per SIMD (64 VGPRs, a few bytes of spill) beat seven without:
+ * the pass is bound by VALU issue and load latency */
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) void k_count(JobArgs a)
 {
     __shared__ uint8_t ntri[256];
     stage_ntri(ntri);
@@ -1953,7 +1955,8 @@ __device__ __forceinline__ void vertex_normal_item(const JobArgs& a, const uint6
     vn_finish(a, l, g, o);
 }
 
-__global__ __launch_bounds__(kBlock) void k_vertex_normals(JobArgs a)
+/* eight waves per SIMD (64 VGPRs, a few bytes of spill) beat six without */
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) void k_vertex_normals(JobArgs a)
 {
     __shared__ uint64_t tri[256];
     stage_table_block(tri);
