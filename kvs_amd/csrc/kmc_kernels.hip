@@ -1077,11 +1077,14 @@ __global__ __launch_bounds__(kBlock) void k_classify_batch(const JobArgs* jobs, 
 /* Crossing edges and active cells of chunk c of node row (y, z) of one level,
  * read straight from the bit volume `lev` (coalesced across the lanes of a
  * row, neighbours from L1/L2).  x/y/z: crossing edges owned by the row's
- * nodes; act: active cells of cell row (y, z-1) with their eight corner
- * masks (bit i of k[q] is corner q of cell x = 64c + i). */
+ * nodes; act: active cells of cell row (y, z-1); k0/k3/k4/k7: the masks of
+ * the cells' four node rows (bit i of kq is corner q of cell x = 64c + i; the
+ * x+1 corners 1, 2, 5, 6 are the next bits of the same masks); next: bit 0 of
+ * the next chunk's word of those four rows (corners 1, 2, 5, 6 of cell 63). */
 struct RowChunk {
     uint64_t x, y, z, act;
-    uint64_t k[8];
+    uint64_t k0, k3, k4, k7;
+    uint32_t next;
 };
 
 __device__ __forceinline__ void row_chunk(const JobArgs& a, const uint64_t* lev, uint32_t y, uint32_t z, uint32_t c,
@@ -1104,27 +1107,24 @@ __device__ __forceinline__ void row_chunk(const JobArgs& a, const uint64_t* lev,
     o.y = has_up ? ((b ^ u) & valid_mask(c, a.nx)) : 0ull;
     o.z = has_next ? ((b ^ n0) & valid_mask(c, a.nx)) : 0ull;
     o.act = 0ull;
+    o.k0 = o.k3 = o.k4 = o.k7 = 0ull;
+    o.next = 0u;
     if (has_cells) {
         const uint64_t* pr = row - slice_words;
         const uint64_t* pu = pr + a.NC;
         const uint64_t p = pr[c], p1 = nxt ? pr[c + 1] : 0ull;
         const uint64_t q = pu[c], q1 = nxt ? pu[c + 1] : 0ull;
-        o.k[0] = p; o.k[1] = (p >> 1) | (p1 << 63);
-        o.k[3] = q; o.k[2] = (q >> 1) | (q1 << 63);
-        o.k[4] = c4; o.k[5] = c5; o.k[6] = c6; o.k[7] = c7;
-        uint64_t or_ = 0, an = ~0ull;
-#pragma unroll
-        for (int s = 0; s < 8; ++s) { or_ |= o.k[s]; an &= o.k[s]; }
+        const uint64_t c1 = (p >> 1) | (p1 << 63), c2 = (q >> 1) | (q1 << 63);
+        const uint64_t or_ = p | c1 | c2 | q | c4 | c5 | c6 | c7;
+        const uint64_t an = p & c1 & c2 & q & c4 & c5 & c6 & c7;
         o.act = (or_ & ~an) & valid_mask(c, a.nx - 1);
+        o.k0 = p;
+        o.k3 = q;
+        o.k4 = c4;
+        o.k7 = c7;
+        o.next = (uint32_t)(p1 & 1ull) | (uint32_t)(q1 & 1ull) << 1 | (uint32_t)(b1 & 1ull) << 2 |
+                 (uint32_t)(u1 & 1ull) << 3;
     }
-}
-
-__device__ __forceinline__ uint32_t corner_bits(const uint64_t k[8], uint32_t i)
-{
-    uint32_t ci = 0;
-#pragma unroll
-    for (int q = 0; q < 8; ++q) ci |= (uint32_t)((k[q] >> i) & 1ull) << q;
-    return ci;
 }
 
 /* The cube index with bits 2, 3 and bits 6, 7 swapped (an involution). */
@@ -1133,14 +1133,19 @@ __device__ __forceinline__ uint32_t swap_pairs(uint32_t c)
     return (c & 0x33u) | ((c & 0x44u) << 1) | ((c & 0x88u) >> 1);
 }
 
-/* swap_pairs(cube index) of cell i of a chunk: for i < 63 the corners (x, x+1)
- * of each of the cell's four node rows are two adjacent bits of that row's
- * mask (k[0], k[3], k[4], k[7]), taken as one 2-bit window. */
-__device__ __forceinline__ uint32_t corner_pairs(const uint64_t k[8], uint32_t i)
+/* swap_pairs(cube index) of cell i of a chunk: the corners (x, x+1) of each of
+ * the cell's four node rows are two adjacent bits of that row's mask (k0, k3,
+ * k4, k7), taken as one 2-bit window; for cell 63 the x+1 bits come from the
+ * next chunk (o.next). */
+__device__ __forceinline__ uint32_t corner_pairs(const RowChunk& o, uint32_t i)
 {
-    if (i == 63u) return swap_pairs(corner_bits(k, i));
-    const uint32_t p0 = (uint32_t)(k[0] >> i) & 3u, p3 = (uint32_t)(k[3] >> i) & 3u;
-    const uint32_t p4 = (uint32_t)(k[4] >> i) & 3u, p7 = (uint32_t)(k[7] >> i) & 3u;
+    if (i == 63u) {
+        return (uint32_t)(o.k0 >> 63) | (o.next & 1u) << 1 | (uint32_t)(o.k3 >> 63) << 2 |
+               ((o.next >> 1) & 1u) << 3 | (uint32_t)(o.k4 >> 63) << 4 | ((o.next >> 2) & 1u) << 5 |
+               (uint32_t)(o.k7 >> 63) << 6 | ((o.next >> 3) & 1u) << 7;
+    }
+    const uint32_t p0 = (uint32_t)(o.k0 >> i) & 3u, p3 = (uint32_t)(o.k3 >> i) & 3u;
+    const uint32_t p4 = (uint32_t)(o.k4 >> i) & 3u, p7 = (uint32_t)(o.k7 >> i) & 3u;
     return p0 | p3 << 2 | p4 << 4 | p7 << 6;
 }
 
@@ -1208,7 +1213,7 @@ __device__ __forceinline__ void count_rows_t(const JobArgs& a, const uint8_t* nt
                     while (m) {
                         const uint32_t i = (uint32_t)__ffsll((unsigned long long)m) - 1u;
                         m &= m - 1ull;
-                        ct += ntri[corner_pairs(o.k, i)];
+                        ct += ntri[corner_pairs(o, i)];
                     }
                 }
                 if (chunk_prefixes) {
@@ -1470,7 +1475,7 @@ __device__ __forceinline__ void compact_chunk(const JobArgs& a, const uint8_t* n
         while (m) {
             const uint32_t i = (uint32_t)__ffsll((unsigned long long)m) - 1u;
             m &= m - 1ull;
-            const uint32_t pi = corner_pairs(o.k, i);
+            const uint32_t pi = corner_pairs(o, i);
             const uint32_t ci = swap_pairs(pi);
             const uint32_t nt = ntri[pi];
             const uint64_t key = cellkey | (c * 64u + i);
