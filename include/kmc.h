@@ -268,6 +268,7 @@ typedef struct kmc_server_stats {
     double p50_ms, p99_ms, mean_ms, max_ms;  /* latency: submission to result ready */
     double span_s;               /* first submission to last completion */
     double triangles_per_s;      /* triangles / span_s */
+    uint64_t peak_batch_device_bytes; /* most device bytes one batch allocated (work arrays, outputs, records) */
 } kmc_server_stats;
 
 kmc_status kmc_server_create(const kmc_server_config* config, kmc_server** out);

@@ -292,6 +292,22 @@ kmc_status min_max_on_device(const kmc_volume* v, const void* dvals, hipStream_t
     return s;
 }
 
+/* Row geometry every emit needs: 64-node chunks per row, the lane split of
+ * the count and compaction passes, node rows. */
+void plan_rows(JobArgs& a)
+{
+    a.NC = (a.nx + 63) / 64;
+    a.cshift = 0;
+    while ((1u << a.cshift) < a.NC && a.cshift < 6) ++a.cshift;
+    a.nrows = (uint64_t)a.nz * a.ny;
+}
+
+bool flat_emit_fits(uint32_t ny, uint32_t nz)
+{
+    const char* env_emit = std::getenv("KMC_EMIT");
+    return ny <= (1u << 20) && nz <= (1u << 20) && !(env_emit && std::strcmp(env_emit, "team") == 0);
+}
+
 bool plan_geometry(JobArgs& a, uint32_t* lds_bytes, uint32_t r_max, uint32_t queue_cap)
 {
     /* tuning overrides: KMC_TILE_ROWS sets the largest R, KMC_QUEUE_CAP the work-queue
@@ -306,12 +322,9 @@ bool plan_geometry(JobArgs& a, uint32_t* lds_bytes, uint32_t r_max, uint32_t que
     r_max = std::min<uint32_t>(r_max, 31);                 // cell rows are 5-bit queue fields
     if (queue_cap == 0) queue_cap = a.vnormal ? 112u : 128u;
     if (env_q) queue_cap = (uint32_t)std::max(64, std::atoi(env_q));
-    a.NC = (a.nx + 63) / 64;
-    a.cshift = 0;
-    while ((1u << a.cshift) < a.NC && a.cshift < 6) ++a.cshift;
+    plan_rows(a);
     a.queue_cap = queue_cap;
     a.fn_cap = a.vnormal ? (env_f ? (uint32_t)std::max(64, std::atoi(env_f)) : 112u) : 0u;
-    a.nrows = (uint64_t)a.nz * a.ny;
     /* the tallest band whose team fits the budget (VertexNormal: four 4-team
      * workgroups per CU, else five); for very long rows R = 1 with as many
      * teams as a 64 KiB workgroup holds */
@@ -350,6 +363,8 @@ bool plan_geometry(JobArgs& a, uint32_t* lds_bytes, uint32_t r_max, uint32_t que
 
 uint32_t levels_per_pass(const kmc_volume* v, int vnormal)
 {
+    /* the flat emit keeps no per-team LDS state: every row length fits */
+    if (flat_emit_fits(v->resolution[1], v->resolution[2])) return kMaxLevels;
     for (uint32_t nl = kMaxLevels; nl >= 1; --nl) {
         JobArgs a;
         std::memset(&a, 0, sizeof(a));
@@ -471,8 +486,14 @@ kmc_status run_pass(const kmc_volume* v, const void* dvals, const double* iso, u
 {
     JobArgs a;
     init_job(a, v, dvals, iso, nlev, normal_type, dup, mn, mx, minc, maxc);
+    /* flat emit (compaction records + one thread per item) unless the records'
+     * 20-bit y / z fields do not fit or KMC_EMIT=team asks for the tile teams
+     * (read per call); only the tile teams need their LDS state to fit */
+    const bool flat = flat_emit_fits(a.ny, a.nz);
     uint32_t lds = 0;
-    if (!plan_geometry(a, &lds)) return KMC_ERR_INVALID_ARGUMENT;
+    if (!plan_geometry(a, &lds) && !flat) return KMC_ERR_INVALID_ARGUMENT;
+    plan_rows(a);
+    a.rec_levels = (1u << nlev) - 1u;
 
     const uint32_t narr = 2 * nlev;
     const uint64_t n = a.nrows;
@@ -484,11 +505,6 @@ kmc_status run_pass(const kmc_volume* v, const void* dvals, const double* iso, u
     uint64_t* aux = (uint64_t*)device_alloc((size_t)(nparts + narr) * sizeof(uint64_t));
     const size_t chunks = (size_t)nlev * a.nz * a.ny * a.NC;
     uint64_t* bitvol = (uint64_t*)device_alloc(chunks * sizeof(uint64_t));
-    /* flat emit (compaction records + one thread per item) unless the records'
-     * 20-bit y / z fields do not fit or KMC_EMIT=team asks for the tile teams
-     * (read per call) */
-    const char* env_emit = std::getenv("KMC_EMIT");
-    const bool flat = a.ny <= (1u << 20) && a.nz <= (1u << 20) && !(env_emit && std::strcmp(env_emit, "team") == 0);
     uint32_t* cpre = flat ? (uint32_t*)device_alloc(2 * chunks * sizeof(uint32_t)) : nullptr;
     ChunkCross* cross = (flat && !dup) ? (ChunkCross*)device_alloc(chunks * sizeof(ChunkCross)) : nullptr;
     void* recs = nullptr;
@@ -675,7 +691,7 @@ kmc_status extract_levels_impl(const kmc_volume* v, const double* levels, uint32
     }
 
     const uint32_t per = levels_per_pass(v, !dup && normal_type == KMC_VERTEX_NORMAL);
-    if (per == 0 && n_levels > 0) s = KMC_ERR_INVALID_ARGUMENT;   // rows too long for the emit tile
+    if (per == 0 && n_levels > 0) s = KMC_ERR_INVALID_ARGUMENT;   // rows too long for the tile teams' LDS
     for (uint32_t l0 = 0; l0 < n_levels && s == KMC_OK; l0 += per) {
         const uint32_t nl = (n_levels - l0) < per ? (n_levels - l0) : per;
         s = run_pass(v, dvals, levels + l0, nl, normal_type, dup, mn, mx, minc, maxc, st, outs + l0);

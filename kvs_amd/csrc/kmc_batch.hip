@@ -43,13 +43,16 @@ __host__ __device__ inline uint64_t record_bytes(uint64_t nv, uint64_t nt, bool 
 }
 
 /* Point pass a's records at base (or at nothing when base is null) and set its
- * per-level item offsets from the level totals nv[l] / nt[l]. */
+ * per-level item offsets from the level totals nv[l] / nt[l]; a level given
+ * zero totals gets no records, and the compaction skips it (rec_levels). */
 __host__ __device__ inline void place_records(JobArgs& a, const uint64_t* nv, const uint64_t* nt, uint8_t* base)
 {
     a.voff[0] = a.toff[0] = 0;
+    a.rec_levels = 0;
     for (uint32_t l = 0; l < a.nlev; ++l) {
         a.voff[l + 1] = a.voff[l] + (base && !a.dup ? nv[l] : 0);
         a.toff[l + 1] = a.toff[l] + (base ? nt[l] : 0);
+        if (base && ((!a.dup && nv[l]) || nt[l])) a.rec_levels |= 1u << l;
     }
     if (!base) {
         a.vkey = nullptr; a.trec = nullptr; a.tci = nullptr; a.fn = nullptr; a.vslot = nullptr;
@@ -381,8 +384,12 @@ kmc_status plan_tiles(kmc_batch* b, uint32_t r_max, uint32_t queue_cap)
     free_plan_buffers(b);
     b->r_max = r_max;
     b->queue_cap = queue_cap;
-    for (Pass& p : b->passes)
-        if (!plan_geometry(p.a, &p.lds, r_max, queue_cap)) return KMC_ERR_INVALID_ARGUMENT;
+    /* the flat emit keeps no per-team LDS state: a pass whose team does not
+     * fit only rules out the tile teams */
+    for (Pass& p : b->passes) {
+        if (!plan_geometry(p.a, &p.lds, r_max, queue_cap) && !b->flat) return KMC_ERR_INVALID_ARGUMENT;
+        plan_rows(p.a);
+    }
     for (Group& g : b->groups) {
         std::vector<uint64_t> rs, ts;
         uint64_t r = 0, t = 0;
@@ -712,9 +719,8 @@ kmc_status create_impl(const kmc_job* jobs, uint32_t n_jobs, uint32_t flags, hip
         b->prefix_d = (uint64_t*)device_alloc(coff * 8);
         b->scan_aux_d = (uint64_t*)device_alloc((b->scan_parts + 1) * 8);
         b->bitvol_d = (uint64_t*)device_alloc(boff * 8);
-        const char* env_emit = std::getenv("KMC_EMIT");
-        b->flat_ok = !(env_emit && std::strcmp(env_emit, "team") == 0);
-        for (const Pass& p : b->passes) b->flat_ok = b->flat_ok && p.a.ny <= (1u << 20) && p.a.nz <= (1u << 20);
+        b->flat_ok = true;
+        for (const Pass& p : b->passes) b->flat_ok = b->flat_ok && flat_emit_fits(p.a.ny, p.a.nz);
         b->flat = b->flat_ok;
         if (b->flat_ok) {
             b->cpre_d = (uint32_t*)device_alloc(2 * boff * 4);
@@ -831,12 +837,20 @@ hipError_t batch_launch_emit(kmc_batch* b, hipStream_t st)
         uint64_t bytes = 0;
         b->istart_h.assign(2 * np + 2, 0);
         uint64_t vsum = 0, tsum = 0;
+        /* only surfaces with outputs (admitted by the caller) get records */
+        auto level_items = [&](const Pass& p, uint32_t l, uint64_t& nv, uint64_t& nt) {
+            const bool on = p.a.coords[l] != nullptr;
+            nv = on ? b->segout_h[p.seg0 + l].nv : 0;
+            nt = on ? b->segout_h[p.seg0 + l].nt : 0;
+        };
         for (uint32_t i = 0; i < np; ++i) {
             const Pass& p = b->passes[i];
             uint64_t V = 0, T = 0;
             for (uint32_t l = 0; l < p.nlev; ++l) {
-                V += p.a.dup ? 0 : b->segout_h[p.seg0 + l].nv;
-                T += b->segout_h[p.seg0 + l].nt;
+                uint64_t nv, nt;
+                level_items(p, l, nv, nt);
+                V += p.a.dup ? 0 : nv;
+                T += nt;
             }
             off[i] = bytes;
             bytes += record_bytes(V, T, p.a.vnormal != 0);
@@ -856,10 +870,7 @@ hipError_t batch_launch_emit(kmc_batch* b, hipStream_t st)
         for (uint32_t i = 0; i < np; ++i) {
             Pass& p = b->passes[i];
             uint64_t nv[kMaxLevels], nt[kMaxLevels];
-            for (uint32_t l = 0; l < p.nlev; ++l) {
-                nv[l] = b->segout_h[p.seg0 + l].nv;
-                nt[l] = b->segout_h[p.seg0 + l].nt;
-            }
+            for (uint32_t l = 0; l < p.nlev; ++l) level_items(p, l, nv[l], nt[l]);
             place_records(p.a, nv, nt, b->arena_r + off[i]);
         }
         const hipError_t e = hipMemcpyAsync(b->istart_d, b->istart_h.data(), b->istart_h.size() * 8,
@@ -881,6 +892,8 @@ uint64_t batch_work_bytes(const kmc_batch* b)
     return b->n_counts * 12 + b->bitvol_words * (b->flat_ok ? 48 : 8) + b->passes.size() * (sizeof(JobArgs) + 16) +
            b->seg_pass_level.size() * (4 + sizeof(SegOut)) + (uint64_t)(b->scan_parts + 1) * 8;
 }
+
+uint64_t batch_record_arena_bytes(const kmc_batch* b) { return b->flat ? b->cap_r : 0; }
 
 /* Flat-emit record bytes surface i adds once its sizes are known. */
 uint64_t batch_record_bytes(const kmc_batch* b, uint32_t i)

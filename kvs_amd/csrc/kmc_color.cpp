@@ -238,7 +238,11 @@ kmc_status kmc_surface_color(int32_t colormap, uint32_t resolution, int32_t type
     const float value = (float)isolevel;
     const float v0 = clampf(value, lo, hi);
     const float r = (float)(resolution - 1);
-    const float v = (v0 - lo) / (hi - lo) * r;
+    float v = (v0 - lo) / (hi - lo) * r;
+    /* when float(min) == float(max) and the +1 of setRange is absorbed
+     * (|min| >= 2^24), hi - lo is 0 and v is NaN: the reference converts NaN to
+     * size_t and reads outside its table; here the table's first colour */
+    if (!(v >= 0.0f)) v = 0.0f;
     const size_t s0 = (size_t)v;
     const size_t s1 = (s0 + 1 < (size_t)(resolution - 1)) ? s0 + 1 : (size_t)(resolution - 1);
     const Rgb c0 = { t[3 * s0], t[3 * s0 + 1], t[3 * s0 + 2] };

@@ -44,14 +44,21 @@ void init_surface(kmc_surface* s, double iso, int normal_type, int dup, double m
 void init_job(JobArgs& a, const kmc_volume* v, const void* dvals, const double* iso, uint32_t nlev, int normal_type,
               int dup, double mn, double mx, const float minc[3], const float maxc[3]);
 
+/* Row geometry of every emit (a.nx, a.ny, a.nz set): a.NC, a.cshift, a.nrows. */
+void plan_rows(JobArgs& a);
+
+/* True when the flat emit serves a volume of these y / z sizes (its records
+ * pack y and z in 20 bits) and KMC_EMIT does not ask for the tile teams. */
+bool flat_emit_fits(uint32_t ny, uint32_t nz);
+
 /* Tile geometry and emit LDS size (a.nlev set): the tallest band R <= r_max
  * whose team LDS state fits the per-team budget (queue_cap 0: the default).
  * False if not even R = 1 fits a workgroup. */
 bool plan_geometry(JobArgs& a, uint32_t* lds_bytes, uint32_t r_max = 16, uint32_t queue_cap = 0);
 
-/* Isolevels per pass over this volume (emit teams work on one level each, so
- * this is kMaxLevels whenever a team's LDS state fits at all; 0 if it does
- * not, for VertexNormal when vnormal). */
+/* Isolevels per pass over this volume: kMaxLevels with the flat emit; with
+ * the tile teams (one level per team) kMaxLevels whenever a team's LDS state
+ * fits at all, 0 if it does not (for VertexNormal when vnormal). */
 uint32_t levels_per_pass(const kmc_volume* v, int vnormal);
 
 /* Kernel launchers (kmc_kernels.hip). */
@@ -95,6 +102,8 @@ void batch_set_outputs(kmc_batch* b, uint32_t i, float* coords, float* normals, 
 uint64_t batch_record_bytes(const kmc_batch* b, uint32_t i);
 hipError_t batch_launch_emit(kmc_batch* b, hipStream_t st);
 uint64_t batch_work_bytes(const kmc_batch* b);
+/* Device bytes of the flat-emit record arena the last batch_launch_emit used. */
+uint64_t batch_record_arena_bytes(const kmc_batch* b);
 void batch_destroy(kmc_batch* b);
 
 }  // namespace kmc

@@ -73,6 +73,7 @@ struct JobArgs {
     uint32_t fn_cap;   // face normals held per cell slice (VertexNormal)
     uint32_t team_stride;  // bytes of LDS per team (>= the team layout's size)
     uint32_t teams;        // teams per emit workgroup (<= kEmitTeams; fewer when a team's LDS is large)
+    uint32_t rec_levels;   // flat emit: bit l set when level l has records (levels without outputs get none)
     uint64_t nunits;   // nlev * nb * nslabs
     uint64_t nrows;    // nz * ny: counts and prefixes are kept per node row
     double minv, maxv; // clamp range of the interpolation (volume min/max)

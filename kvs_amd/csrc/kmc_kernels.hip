@@ -1531,6 +1531,7 @@ __device__ __forceinline__ void compact_rows_t(const JobArgs& a, const uint8_t* 
                         /* chunks without crossing edges or active cells (most of
                          * them) are skipped on the count pass's prefixes alone */
                         const uint32_t l = l0 + k;
+                        if (!((a.rec_levels >> l) & 1u)) continue;   // a level without records
                         const uint64_t w = (uint64_t)l * level_words + r * a.NC + c;
                         uint32_t v0, t0, v1, t1;
                         chunk_prefix(a, w, v0, t0);

@@ -249,6 +249,14 @@ void run_batch(kmc_server* S, hipStream_t st, const std::vector<uint64_t>& ids)
             (void)hipGetLastError();
             s = KMC_ERR_DEVICE;
         }
+        if (s == KMC_OK) {
+            /* what this batch holds on the device: its work arrays, the outputs
+             * of its admitted requests and the records of their emit */
+            uint64_t held = batch_work_bytes(b) + batch_record_arena_bytes(b);
+            for (size_t i = 0; i < n; ++i) held += admitted[i] ? output_bytes(out[i]) : 0;
+            std::lock_guard<std::mutex> lk(S->mu);
+            S->st.peak_batch_device_bytes = std::max<uint64_t>(S->st.peak_batch_device_bytes, held);
+        }
         if (s == KMC_OK && S->cfg.out_memory == KMC_MEMORY_HOST)
             for (size_t i = 0; i < n && s == KMC_OK; ++i)
                 if (admitted[i]) s = surface_to_host(&out[i], st);

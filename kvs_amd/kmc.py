@@ -130,6 +130,7 @@ class ServerStats(ctypes.Structure):
         ("max_ms", ctypes.c_double),
         ("span_s", ctypes.c_double),
         ("triangles_per_s", ctypes.c_double),
+        ("peak_batch_device_bytes", ctypes.c_uint64),
     ]
 
 # Every symbol include/kmc.h declares (checked by the CPU tests).

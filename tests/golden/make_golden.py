@@ -194,6 +194,49 @@ def main():
                    "min": repr(lo), "max": repr(hi)})
     meta["minmax"] = mm
 
+    # 8. 64-bit integers beyond 2^53: double(v) rounds (several values share one
+    #    double), isolevels equal to or one double away from data values, and
+    #    uint64 values above 2^63 (MarchingCubes.cpp:299-316, :326-356)
+    rng = np.random.default_rng(53)
+    pk = Packer()
+    cases = []
+    shape = (6, 5, 7)
+    n = int(np.prod(shape))
+    i62, u63 = np.int64(1) << np.int64(62), np.uint64(1) << np.uint64(63)
+    near = (i62 + rng.integers(-3000, 3000, n) * 512).astype(np.int64)
+    # two far values keep float(min) != float(max): with a float range of one
+    # value the reference's colour lookup divides 0 by 0 (MapperBase.cpp:69-78,
+    # ColorMap.cpp:1045-1047) and indexes its table with size_t(NaN)
+    near[0], near[-1] = i62 - (1 << 45), i62 + (1 << 45)
+    big = [
+        ("int64_near_2p62", near, None),
+        ("int64_wide", rng.integers(-(1 << 62), 1 << 62, n, dtype=np.int64), None),
+        ("int64_near_min", (np.int64(-(1 << 63)) + rng.integers(0, 1 << 41, n)).astype(np.int64), None),
+        ("uint64_above_2p63", u63 + rng.integers(0, 1 << 62, n, dtype=np.int64).astype(np.uint64), None),
+        ("uint64_near_max", np.uint64(0xFFFFFFFFFFFFFFFF) - rng.integers(0, 1 << 40, n, dtype=np.int64).astype(np.uint64),
+         None),
+        ("uint64_clamped", u63 + rng.integers(0, 1 << 62, n, dtype=np.int64).astype(np.uint64), 0.3),
+    ]
+    for name, flat, clamp in big:
+        v = flat.reshape(shape)
+        d = np.sort(v.astype(np.float64).ravel())
+        mid = float(d[n // 2])
+        mm = None
+        if clamp is not None:
+            lo, hi = float(d[0]), float(d[-1])
+            mm = (lo + clamp * (hi - lo), hi - clamp * (hi - lo))
+        for iso in (mid, float(np.nextafter(mid, np.inf)), float(np.nextafter(mid, -np.inf))):
+            pk.add("in_bytes", np.frombuffer(v.tobytes(), dtype=np.uint8))
+            for normal, dup in MODES:
+                r = oracle.ref_marching_cubes(v, iso, normal, dup, min_max_values=mm)
+                mk = f"n{normal}_d{int(dup)}"
+                pk.add(mk + "_coords", r["coords"])
+                pk.add(mk + "_conn", r["connections"])
+                pk.add(mk + "_normals", r["normals"])
+            cases.append({"name": name, "iso": iso, "dtype": v.dtype.name, "shape": list(shape), "min_max": mm})
+    pk.save(os.path.join(HERE, "bigint.npz"))
+    meta["bigint"] = cases
+
     with open(os.path.join(HERE, "golden.json"), "w") as f:
         json.dump(meta, f, indent=1, sort_keys=True)
     print("wrote", sorted(os.listdir(HERE)))

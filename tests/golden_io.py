@@ -64,6 +64,24 @@ def random_cases():
         yield v, c["iso"], mm, co, outs
 
 
+def bigint_cases():
+    """Yield (name, values, isolevel, min_max, {mode: (coords, conn, normals)}) for
+    the 64-bit integer volumes whose values exceed 2^53."""
+    m = meta()["bigint"]
+    p = Packed("bigint.npz")
+    for i, c in enumerate(m):
+        dt = np.dtype(c["dtype"])
+        v = np.frombuffer(p.get("in_bytes", i).tobytes(), dtype=dt).reshape(c["shape"])
+        outs = {}
+        for normal, dup in MODES:
+            mk = f"n{normal}_d{int(dup)}"
+            outs[(normal, dup)] = (p.get(mk + "_coords", i).astype(np.float32),
+                                   p.get(mk + "_conn", i).astype(np.uint32),
+                                   p.get(mk + "_normals", i).astype(np.float32))
+        mm = tuple(c["min_max"]) if c["min_max"] is not None else None
+        yield c["name"], v, c["iso"], mm, outs
+
+
 def hydrogen():
     return np.load(os.path.join(GOLDEN, "hydrogen64.npz"))["values"]
 

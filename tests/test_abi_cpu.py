@@ -94,3 +94,15 @@ def test_random_case_colors_match_reference():
         lo, hi = (mm if mm is not None else (float(v.min()), float(v.max())))
         got = kmc.surface_color(kmc.COLORMAP_DEFAULT, 256, kmc.NUMPY_TYPES[v.dtype], lo, hi, iso)
         assert list(got) == list(colors)
+
+
+def test_surface_color_with_a_one_value_float_range():
+    # int64 data near 2^62: float(min) == float(max) and the +1.0f of
+    # MapperBase::setRange is absorbed, so ColorMap::at divides 0 by 0; the
+    # reference then indexes its table with size_t(NaN), kmc_surface_color
+    # returns the table's first colour
+    lo, hi = float(2 ** 62 - 3000 * 512), float(2 ** 62 + 3000 * 512)
+    assert np.float32(lo) == np.float32(hi)
+    got = kmc.surface_color(kmc.COLORMAP_DEFAULT, 256, kmc.INT64, lo, hi, float(2 ** 62))
+    first = kmc.colormap_table(kmc.COLORMAP_DEFAULT, 256)[:3]
+    assert list(got) == list(first)

@@ -29,6 +29,15 @@ def test_random_volumes():
             assert bits_equal(r["normals"], no), (i, normal, dup)
 
 
+def test_large_64bit_integers():
+    for name, v, iso, mm, outs in golden_io.bigint_cases():
+        for (normal, dup), (co, cn, no) in outs.items():
+            r = kmc.extract(v, iso, normal, dup, min_max=mm)
+            assert bits_equal(r["coords"], co), (name, iso, normal, dup)
+            assert bits_equal(r["connections"], cn), (name, iso, normal, dup)
+            assert bits_equal(r["normals"], no), (name, iso, normal, dup)
+
+
 def test_hydrogen64_digests():
     h = golden_io.hydrogen()
     meta = golden_io.meta()["hydrogen64"]

@@ -7,6 +7,7 @@ import time
 import numpy as np
 import pytest
 
+import oracle
 from kvs_amd import datasets, kmc
 from tests.conftest import assert_same_surface
 
@@ -63,7 +64,9 @@ def check_mix(server, vols, reqs, concurrent_submitters=1):
         got = server.wait(ids[k])
         want = kmc.extract(vols[j], lv, normal, dup)
         label = f"request {k}: volume {j} {vols[j].dtype} {vols[j].shape} level {lv} mode {normal},{dup}"
+        # the same alone (a plain call) and in the mix, and equal to the oracle
         assert_same_surface(got, want, label)
+        assert_same_surface(got, oracle.extract(vols[j], lv, normal, dup), label + " (oracle)")
         assert got["empty_range"] == want["empty_range"], label
         assert got["min_value"] == want["min_value"] and got["max_value"] == want["max_value"], label
 
@@ -137,5 +140,35 @@ def test_outputs_larger_than_budget_fail_with_budget():
         with pytest.raises(kmc.KmcError) as e:
             s.wait(rid)
         assert e.value.status == kmc.ERR_BUDGET
+    finally:
+        s.close()
+
+
+def test_refused_request_in_a_batch_gets_no_device_memory():
+    """A request whose outputs can never fit the budget fails with
+    KMC_ERR_BUDGET; the batch it came in allocates nothing for it (neither
+    outputs nor emit records), so no batch holds more device memory than the
+    budget, and the other requests come out exact."""
+    small = [datasets.blobs(24, seed=s, nblobs=3) for s in (1, 2, 3)]
+    big = datasets.turbulence(64, seed=8)
+    budget = 1536 << 10
+    s = kmc.Server(budget_bytes=budget, workers=1, max_batch=8)
+    try:
+        vids = [s.register(kmc.make_volume(v)) for v in small]
+        vbig = s.register(kmc.make_volume(big))
+        order = [(vids[0], 0.5), (vbig, 0.0), (vids[1], 0.45), (vids[2], 0.55), (vbig, 0.1), (vids[0], 0.4)]
+        ids = [s.submit(vid, lv, kmc.VERTEX_NORMAL, False) for vid, lv in order]
+        for (vid, lv), rid in zip(order, ids):
+            if vid == vbig:
+                with pytest.raises(kmc.KmcError) as e:
+                    s.wait(rid)
+                assert e.value.status == kmc.ERR_BUDGET
+            else:
+                v = small[vids.index(vid)]
+                assert_same_surface(s.wait(rid), oracle.extract(v, lv, kmc.VERTEX_NORMAL, False), f"small {vid} {lv}")
+        st = s.stats()
+        print("stats", st, flush=True)
+        assert st["failed"] == 2 and st["completed"] == 4
+        assert 0 < st["peak_batch_device_bytes"] <= budget
     finally:
         s.close()

@@ -28,6 +28,28 @@ def test_tables_match_reference_hash():
     assert golden_io.sha(vert) == m["VertexID_sha256"]
 
 
+def test_device_table_words_match_reference_hash():
+    """KMC_TABLE_WORDS of kvs_amd/csrc/mc_tables.h -- the case table the kernels
+    use, in its packed form (triangle count in bits 60..63, nibble j = j-th
+    edge id in table order) -- decoded back into TriangleID[256][16] and
+    pinned by the reference table's SHA-256 and per-case triangle counts."""
+    import os
+    import re
+    path = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "kvs_amd", "csrc", "mc_tables.h")
+    text = open(path).read()
+    body = text[text.index("#define KMC_TABLE_WORDS"):text.index("static const uint64_t kTriangles")]
+    words = [int(w, 16) for w in re.findall(r"0x([0-9a-fA-F]{16})ull", body)]
+    assert len(words) == 256
+    tri = np.full((256, 16), -1, dtype="<i4")
+    for ci, w in enumerate(words):
+        n = w >> 60
+        assert (w >> (12 * n)) & ((1 << (60 - 12 * n)) - 1) == 0, ci    # nothing past the last triangle
+        tri[ci, :3 * n] = [(w >> (4 * j)) & 0xF for j in range(3 * n)]
+    assert golden_io.sha(tri) == golden_io.meta()["tables"]["TriangleID_sha256"]
+    counts = np.load(os.path.join(golden_io.GOLDEN, "tables.npz"))["triangle_counts"]
+    assert np.array_equal(counts, [w >> 60 for w in words])
+
+
 def test_table_topology_invariants():
     # every triangle edge of case ci is an edge whose corner bits differ, and
     # every such edge is used: a plausible transcription slip breaks one of these
@@ -56,6 +78,18 @@ def test_random_volumes_against_golden():
             assert bits_equal(r["coords"], co), (i, normal, dup)
             assert bits_equal(r["connections"], cn), (i, normal, dup)
             assert bits_equal(r["normals"], no), (i, normal, dup)
+
+
+def test_large_64bit_integers_against_golden():
+    # int64 / uint64 values beyond 2^53 (several share one double), isolevels
+    # equal to or one double away from a data value, a clamped range
+    for name, v, iso, mm, outs in golden_io.bigint_cases():
+        for (normal, dup), (co, cn, no) in outs.items():
+            r = oracle.extract(v, iso, normal, dup, min_max_values=mm)
+            assert bits_equal(r["coords"], co), (name, iso, normal, dup)
+            assert bits_equal(r["connections"], cn), (name, iso, normal, dup)
+            assert bits_equal(r["normals"], no), (name, iso, normal, dup)
+            assert r["n_triangles"] > 0, name
 
 
 def test_hydrogen_against_golden_digests():
