@@ -56,14 +56,16 @@ def main():
     ap.add_argument("--serve-volumes", type=int, default=32, help="resident volumes of the serving leg (0: skip it)")
     ap.add_argument("--serve-requests", type=int, default=300)
     ap.add_argument("--no-ct", action="store_true", help="skip the 512^3 uint8 CT-like leg")
+    ap.add_argument("--no-rows", action="store_true",
+                    help="skip the SlicePlane / unstructured / KVSML / STL leg")
     ap.add_argument("--only-main", action="store_true",
-                    help="only the 1024^3 step (no CPU baseline, batched, serving, CT or FP64 legs): for profiling")
+                    help="only the 1024^3 step (no CPU baseline, rows, batched, serving, CT or FP64 legs): for profiling")
     args = ap.parse_args()
     if args.only_main:
-        args.no_cpu_baseline, args.no_ct, args.batch_jobs, args.serve_volumes = True, True, 0, 0
+        args.no_cpu_baseline, args.no_ct, args.no_rows, args.batch_jobs, args.serve_volumes = True, True, True, 0, 0
     if args.gpus != 1:
-        print("bench.py measures one GPU only: every MI355X box here has a single GPU and the "
-              "path shards by z-slab (see DESIGN.md); run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU only (across GPUs the path would shard by z-slab); "
+              "run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
@@ -144,9 +146,12 @@ def main():
     lib.kmc_set_timing(0)
     kernels = kernel_rooflines(prof, n, nlev, counts)
 
-    cpu = None
+    cpu = fp64 = None
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline(vol, args.cpu_slices, mn.value, mx.value)
+        cpu, fp64 = cpu_baseline(vol, args.cpu_slices, mn.value, mx.value, kmc)
+    rows = None
+    if not args.no_rows:
+        rows = rows_leg(lib, kmc, datasets, torch, vol, v, mn.value, mx.value, sptr, not args.no_cpu_baseline)
     del vol
     torch.cuda.empty_cache()
 
@@ -163,7 +168,6 @@ def main():
     ct = None
     if not args.no_ct:
         ct = ct_leg(lib, kmc, datasets, torch, args)
-    fp64 = None if args.only_main else fp64_reevaluation(kmc, datasets)
 
     out = {
         "metric": METRIC,
@@ -188,7 +192,7 @@ def main():
         "triangles_per_step": ntri,
         "vertices_per_step": nvert,
         "wall_ms_per_step": round(wall / args.steps * 1e3, 3),
-        "roofline": dominant_roofline(kernels),
+        "roofline": dominant_roofline(kernels, n, 4),
         "kernels": kernels,
         "job_roofline": {
             "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -202,6 +206,7 @@ def main():
         "serving": serving,
         "ct_512": ct,
         "fp64_reevaluation": fp64,
+        "rows": rows,
     }
     print(json.dumps(out), flush=True)
 
@@ -277,12 +282,19 @@ def kernel_rooflines(prof, n, nlev, counts):
     return out
 
 
-def dominant_roofline(kernels):
+def dominant_roofline(kernels, n, itemsize):
+    """The longest kernel of the step (classify) against HBM peak.  Its share
+    of the job's algorithmic bytes is the volume read once (N * sizeof(T));
+    the classification masks it writes are intermediate data, reported apart
+    and left out of `achieved`."""
     k = max((x for x in kernels if isinstance(kernels[x], dict)), key=lambda x: kernels[x]["ms"])
     e = kernels[k]
-    return {"bound": "hbm", "kernel": e["kernel"], "achieved": e["achieved_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": e["frac"], "traffic": e["traffic"], "algorithmic_bytes": e["algorithmic_bytes"],
-            "launch_ms": e["ms"]}
+    alg = n ** 3 * itemsize if k == "classify" else e["algorithmic_bytes"]
+    gbs = alg / (e["ms"] / 1e3) / 1e9 if e["ms"] > 0 else 0.0
+    return {"bound": "hbm", "kernel": e["kernel"], "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": e["traffic"], "algorithmic_bytes": alg,
+            "intermediate_bytes": e["algorithmic_bytes"] - alg, "launch_ms": e["ms"],
+            "scope": "this kernel alone; the whole call's fraction of HBM peak is job_roofline.frac"}
 
 
 def copy_ceiling(torch, dev, nbytes=1 << 30, reps=5):
@@ -556,66 +568,270 @@ def ct_leg(lib, kmc, datasets, torch, args):
     }
 
 
-def fp64_reevaluation(kmc, datasets):
-    """Largest deviation of the surface coordinates from an FP64 re-evaluation
-    of the interpolation (64^3 blob field, isolevel 0.5, shared vertices): the
-    crossing edges are enumerated in node scan order x (x, y, z) edges, the
-    order of the returned vertices, and interpolated in float64 (the same
-    clamp and ratio, no rounding to float until the end).  The coordinates are
-    the reference's bit for bit; this measures how far that float rounding is
-    from exact arithmetic."""
+def fp64_reevaluation(d32, iso, coords, lo, hi):
+    """Largest deviation of shared-vertex surface coordinates from an FP64
+    re-evaluation of the interpolation on the volume d32 ([z][y][x], unit
+    spacing, clamp range [lo, hi]): the crossing edges are enumerated in node
+    scan order x (x, y, z) edges, the order of the returned vertices, and
+    interpolated in float64 (the same clamp and ratio, no rounding to float
+    until the end).  The coordinates are the reference's bit for bit; this
+    measures how far that float rounding is from exact arithmetic.  Returns
+    (vertices, max_abs, max_rel)."""
     import numpy as np
 
-    v = datasets.blobs(64, seed=1000)
-    iso = 0.5
-    s = kmc.extract(v, iso, kmc.POLYGON_NORMAL, False)
-    coords = s["coords"].reshape(-1, 3).astype(np.float64)
-    d = v.astype(np.float64)
-    lo, hi = float(v.min()), float(v.max())
-    nz, ny, nx = d.shape
-    above = d > iso
-    pts = []
+    coords = coords.reshape(-1, 3)
+    nz, ny, nx = d32.shape
+    above = d32.astype(np.float64) > iso
+    keys, pts = [], []
     for axis, (dz, dy, dx) in enumerate(((0, 0, 1), (0, 1, 0), (1, 0, 0))):
-        a = d[:nz - dz, :ny - dy, :nx - dx]
-        b = d[dz:, dy:, dx:]
         cross = above[:nz - dz, :ny - dy, :nx - dx] != above[dz:, dy:, dx:]
         zz, yy, xx = np.nonzero(cross)
-        va = np.clip(a[zz, yy, xx], lo, hi)
-        vb = np.clip(b[zz, yy, xx], lo, hi)
+        del cross
+        va = np.clip(d32[zz, yy, xx].astype(np.float64), lo, hi)
+        vb = np.clip(d32[zz + dz, yy + dy, xx + dx].astype(np.float64), lo, hi)
         r = np.abs((iso - va) / (vb - va))
-        p = np.stack([xx + r * dx, yy + r * dy, zz + r * dz], axis=1).astype(np.float64)
-        key = (zz * ny + yy) * nx + xx
-        pts.append((key * 3 + axis, p))
-    keys = np.concatenate([k for k, _ in pts])
-    exact = np.concatenate([p for _, p in pts])[np.argsort(keys, kind="stable")]
+        pts.append(np.stack([xx + r * dx, yy + r * dy, zz + r * dz], axis=1))
+        keys.append(((zz.astype(np.int64) * ny + yy) * nx + xx) * 3 + axis)
+    del above
+    order = np.argsort(np.concatenate(keys), kind="stable")
+    exact = np.concatenate(pts)[order]
     assert exact.shape == coords.shape, (exact.shape, coords.shape)
-    err = np.abs(coords - exact)
+    err = np.abs(coords.astype(np.float64) - exact)
     rel = err / np.maximum(np.abs(exact), 1e-30)
-    return {"workload": "64^3 blobs, isolevel 0.5, shared vertices", "vertices": int(coords.shape[0]),
-            "max_abs": float(err.max()), "max_rel": float(rel[np.abs(exact) > 1e-6].max())}
+    big = np.abs(exact) > 1e-6
+    return int(coords.shape[0]), float(err.max(initial=0.0)), float(rel[big].max(initial=0.0))
 
 
-def cpu_baseline(vol, slices, vmin, vmax):
+def rows_leg(lib, kmc, datasets, torch, vol, v, mn, mx, sptr, with_cpu):
+    """The paths beside the isosurface, each on the GPU with the reference's
+    own code timed on the host beside it (1 thread) and the outputs compared:
+
+      slice_plane    kvs::SlicePlane through the centre of the resident 1024^3
+                     volume (oblique plane, normal (1, 2, 3)); reference on a
+                     32-slice z-slab cut by the same plane;
+      unstructured   kvs::Isosurface over a tetrahedral UnstructuredVolumeObject
+                     (6 tetrahedra per cube of a jittered 81^3 node grid, 3.07 M
+                     cells, blob field, isolevel 0.5, PolygonNormal +
+                     duplication), device-resident mesh and outputs;
+      kvsml_ingest   a 1024 x 1024 x 256 float32 slab of the volume written as
+                     KVSML with an external binary array (1 GiB, no min/max in
+                     the header): read file -> HBM with the device min/max, vs
+                     the reference's StructuredVolumeObject::read into host
+                     memory; the file is in the page cache (written just before);
+      stl_export     the slab's isolevel-0 VertexNormal surface written as a
+                     binary STL: kmc_export_stl (device expansion) vs
+                     PolygonExporter<Stl> + Stl::write; the files are compared
+                     byte for byte; plus the device-to-device expansion alone."""
+    import shutil
+    import tempfile
+
+    import numpy as np
+
+    import oracle
+
+    ref = with_cpu and oracle.ref_available()
+    res = {}
+    n = int(vol.shape[0])
+
+    def timed(fn, reps):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            r = fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / reps * 1e3, r
+
+    # SlicePlane
+    print("[bench] rows leg: SlicePlane", file=sys.stderr, flush=True)
+    table = kmc.colormap_table(0)
+    tb = table.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))
+    nrm = np.array([1.0, 2.0, 3.0]) / np.sqrt(14.0)
+    centre = (n - 1) / 2.0
+    coef = [float(x) for x in np.float32(list(nrm) + [-nrm.sum() * centre])]
+    cf = (ctypes.c_float * 4)(*coef)
+    out = kmc.Surface()
+
+    def slice_once():
+        st = lib.kmc_slice_plane(ctypes.byref(v), cf, tb, 256, None, kmc.MEMORY_DEVICE, sptr, ctypes.byref(out))
+        if st != 0:
+            raise kmc.KmcError(st)
+        t = int(out.n_triangles)
+        lib.kmc_surface_release(ctypes.byref(out))
+        return t
+
+    ms, ntri = timed(slice_once, 10)
+    sp = {"workload": f"{n}^3 float32, plane through the centre with normal (1,2,3)/sqrt(14), device outputs",
+          "ms": round(ms, 3), "triangles": ntri, "gcells_per_s": round((n - 1) ** 3 / (ms / 1e3) / 1e9, 1),
+          "mtriangles_per_s": round(ntri / (ms / 1e3) / 1e6, 1)}
+    if ref:
+        z0, k = n // 2 - 16, 32
+        slab = vol[z0:z0 + k].cpu().numpy()
+        c2 = [coef[0], coef[1], coef[2], float(np.float32(coef[3] + coef[2] * z0))]
+        r = oracle.ref_slice_plane(slab, c2, tf_kind=0, min_max_values=(mn, mx))
+        cells = (n - 1) * (n - 1) * (k - 1)
+        sp["cpu_reference"] = {"sample": f"z-slab {n}x{n}x{k}, the same plane", "s": round(r["seconds"], 3),
+                               "gcells_per_s": round(cells / r["seconds"] / 1e9, 4), "cores": 1}
+        del slab, r
+    res["slice_plane"] = sp
+
+    # unstructured tetrahedra
+    print("[bench] rows leg: unstructured tetrahedra (generating the mesh)", file=sys.stderr, flush=True)
+    uv, uc, uk = datasets.unstructured_mesh((81, 81, 81), datasets.TETRAHEDRA, seed=11)
+    tv, tc, tk = (torch.from_numpy(np.ascontiguousarray(a)).cuda() for a in (uv, uc, uk))
+    torch.cuda.synchronize()
+    u = kmc.UnstructuredVolume()
+    u.values, u.coords, u.connections = tv.data_ptr(), tc.data_ptr(), tk.data_ptr()
+    u.memory, u.type = kmc.MEMORY_DEVICE, kmc.NUMPY_TYPES[np.dtype(np.float32)]
+    u.n_nodes, u.n_cells, u.cell_type, u.veclen = uv.size, uk.size // 4, datasets.TETRAHEDRA, 1
+
+    def tet_once(keep=False):
+        st = lib.kmc_extract_unstructured(ctypes.byref(u), 0.5, kmc.POLYGON_NORMAL, 1, kmc.MEMORY_DEVICE, sptr,
+                                          ctypes.byref(out))
+        if st != 0:
+            raise kmc.KmcError(st)
+        t = int(out.n_triangles)
+        r = kmc.device_surface_to_numpy(out) if keep else None
+        lib.kmc_surface_release(ctypes.byref(out))
+        return t, r
+
+    ms, (ntri, _) = timed(tet_once, 10)
+    cells = uk.size // 4
+    un = {"workload": f"{cells} tetrahedra (81^3 jittered node grid, 6 per cube), float32 blob field, isolevel 0.5, "
+                      f"PolygonNormal + duplication, device-resident mesh and outputs",
+          "ms": round(ms, 3), "triangles": ntri, "mcells_per_s": round(cells / (ms / 1e3) / 1e6, 1),
+          "mtriangles_per_s": round(ntri / (ms / 1e3) / 1e6, 1)}
+    if ref:
+        _, mine = tet_once(keep=True)
+        r = oracle.ref_unstructured_isosurface(uv, uc, uk, datasets.TETRAHEDRA, 0.5)
+        same = all(np.asarray(mine[key]).tobytes() == np.asarray(r[key]).tobytes() for key in ("coords", "normals"))
+        un["cpu_reference"] = {"s": round(r["seconds"], 4), "mcells_per_s": round(cells / r["seconds"] / 1e6, 2),
+                               "cores": 1, "gpu_output_identical": bool(same)}
+        del r, mine
+    res["unstructured"] = un
+    del tv, tc, tk, uv, uc, uk
+
+    # KVSML ingestion and STL export
+    tmp = tempfile.mkdtemp(prefix="kmc_bench_")
+    try:
+        print("[bench] rows leg: writing a 1 GiB KVSML file", file=sys.stderr, flush=True)
+        k = 256
+        slab = vol[:k].cpu().numpy()
+        dat = os.path.join(tmp, "slab_value.dat")
+        slab.tofile(dat)
+        path = os.path.join(tmp, "slab.kvsml")
+        with open(path, "w") as f:
+            f.write('<?xml version="1.0" ?>\n<KVSML>\n    <Object type="StructuredVolumeObject">\n'
+                    f'        <StructuredVolumeObject grid_type="uniform" resolution="{n} {n} {k}">\n'
+                    '            <Node>\n                <Value veclen="1">\n'
+                    '                    <DataArray type="float" format="binary" file="slab_value.dat" '
+                    'endian="little" />\n                </Value>\n            </Node>\n'
+                    '        </StructuredVolumeObject>\n    </Object>\n</KVSML>\n')
+        print("[bench] rows leg: KVSML to HBM", file=sys.stderr, flush=True)
+        kv = kmc.KvsmlVolume(path, kmc.MEMORY_DEVICE)
+        kv.close()
+        t0 = time.perf_counter()
+        reps = 3
+        for _ in range(reps):
+            kv = kmc.KvsmlVolume(path, kmc.MEMORY_DEVICE)
+            kv.close()
+        ms = (time.perf_counter() - t0) / reps * 1e3
+        kv = kmc.KvsmlVolume(path, kmc.MEMORY_DEVICE)
+        same = bool(np.array_equal(kv.values(), slab)) and (kv.volume.min_value, kv.volume.max_value) == \
+            (float(slab.min()), float(slab.max()))
+        ki = {"workload": f"{n}x{n}x{k} float32 KVSML, external binary array ({slab.nbytes / 2**30:.2f} GiB), "
+                          f"min/max computed at load; file in the page cache",
+              "ms": round(ms, 2), "gbs": round(slab.nbytes / (ms / 1e3) / 1e9, 2), "values_identical": same}
+        if ref:
+            r = None
+            t0 = time.perf_counter()
+            r = oracle.ref_read_kvsml_volume(path)
+            rs = time.perf_counter() - t0
+            ki["cpu_reference"] = {"s": round(rs, 3), "gbs": round(slab.nbytes / rs / 1e9, 2), "cores": 1,
+                                   "note": "StructuredVolumeObject::read into host memory, then a copy out "
+                                           "of the reference's arrays (bound in)",
+                                   "values_identical": bool(np.array_equal(r["values"], slab))}
+            del r
+        res["kvsml_ingest"] = ki
+
+        print("[bench] rows leg: STL export", file=sys.stderr, flush=True)
+        dev = kv.volume
+        st = lib.kmc_extract(ctypes.byref(dev), 0.0, kmc.VERTEX_NORMAL, 0, kmc.MEMORY_DEVICE, sptr, ctypes.byref(out))
+        if st != 0:
+            raise kmc.KmcError(st)
+        soup = kmc.Surface()
+
+        def expand_once():
+            e = lib.kmc_surface_expand(ctypes.byref(out), kmc.MEMORY_DEVICE, sptr, ctypes.byref(soup))
+            if e != 0:
+                raise kmc.KmcError(e)
+            lib.kmc_surface_release(ctypes.byref(soup))
+
+        ems, _ = timed(expand_once, 10)
+        nt = int(out.n_triangles)
+        host = kmc.device_surface_to_numpy(out)
+        lib.kmc_surface_release(ctypes.byref(out))
+        kv.close()
+        mine = os.path.join(tmp, "mine.stl")
+        t0 = time.perf_counter()
+        kmc.export_stl(host, kmc.VERTEX_NORMAL, mine)
+        gs = time.perf_counter() - t0
+        ex = {"workload": f"isolevel 0 of the slab, VertexNormal: {nt} triangles, binary STL "
+                          f"({os.path.getsize(mine) / 2**20:.0f} MiB)",
+              "export_s": round(gs, 3),
+              "expand_device_ms": round(ems, 3),
+              "expand_gbs": round(nt * (12 + 72 + 36 + 12) / (ems / 1e3) / 1e9, 1),
+              "expand_bytes_note": "per triangle: connections 12 B, 3 corners' coordinates and normals 72 B "
+                                   "gathered, 36 + 12 B written"}
+        if ref:
+            theirs = os.path.join(tmp, "ref.stl")
+            t0 = time.perf_counter()
+            oracle.ref_export_stl(host, oracle.VERTEX_NORMAL, theirs)
+            rs = time.perf_counter() - t0
+            with open(mine, "rb") as a, open(theirs, "rb") as b:
+                same = a.read() == b.read()
+            ex["cpu_reference"] = {"s": round(rs, 3), "cores": 1, "file_identical": bool(same)}
+        res["stl_export"] = ex
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+    return res
+
+
+def cpu_baseline(vol, slices, vmin, vmax, kmc):
     """The reference's own kvs::MarchingCubes (oracle/_ref, 1 thread) on a
-    z-slab sample of the same volume bytes, min/max pre-cached."""
+    z-slab sample of the same volume bytes, min/max pre-cached; the GPU's
+    surfaces of the same slab (one kmc_extract_levels call, host in / host
+    out) are compared with it bit for bit and re-evaluated in FP64.
+    Returns (cpu_baseline, fp64_reevaluation)."""
     import numpy as np
 
     import oracle
 
     if not oracle.ref_available():
-        return None
+        return None, None
     n = vol.shape[2]
     z0 = (vol.shape[0] - slices) // 2
     sample = vol[z0:z0 + slices].cpu().numpy()
     nodes = sample.size
+    gpu = kmc.extract_levels(sample, LEVELS, kmc.VERTEX_NORMAL, False, min_max=(vmin, vmax))
     total = 0.0
     tris = 0
-    for lv in LEVELS:
+    identical = True
+    worst = {"vertices": 0, "max_abs": 0.0, "max_rel": 0.0}
+    for lv, g in zip(LEVELS, gpu):
         print(f"[bench] cpu baseline level {lv}", file=sys.stderr, flush=True)
         r = oracle.ref_marching_cubes(sample, lv, oracle.VERTEX_NORMAL, False, min_max_values=(vmin, vmax))
         total += r["seconds"]
         tris += r["connections"].size // 3
-    return {
+        for k in ("coords", "connections", "normals"):
+            a, b = np.asarray(g[k]), np.asarray(r[k])
+            identical &= a.shape == b.shape and a.dtype == b.dtype and a.tobytes() == b.tobytes()
+        del r
+        nv, ea, er = fp64_reevaluation(sample, lv, g["coords"], vmin, vmax)
+        worst["vertices"] += nv
+        worst["max_abs"] = max(worst["max_abs"], ea)
+        worst["max_rel"] = max(worst["max_rel"], er)
+    cpu = {
         "value": round(nodes * len(LEVELS) / total / 1e9, 4),
         "unit": "Gvoxels/s",
         "cores": 1,
@@ -623,9 +839,13 @@ def cpu_baseline(vol, slices, vmin, vmax):
         "sample": f"z-slab {n}x{n}x{slices} of the same volume ({nodes} nodes), {len(LEVELS)} levels, "
                   f"VertexNormal, kvs::MarchingCubes constructor wall time {total:.2f}s, {tris} triangles",
         "mtriangles_per_s": round(tris / total / 1e6, 3),
+        "gpu_output_identical": bool(identical),
         "cpu_model": cpu_model(),
         "nproc": os.cpu_count(),
     }
+    fp64 = dict(worst, workload=f"the same z-slab, {len(LEVELS)} levels, shared vertices: coordinates against "
+                                f"an FP64 evaluation of the same interpolation")
+    return cpu, fp64
 
 
 if __name__ == "__main__":

@@ -36,7 +36,10 @@ typedef enum kmc_status {
     KMC_ERR_DEVICE = 7,           /* a HIP runtime call failed */
     KMC_ERR_BUDGET = 8,           /* serving: request does not fit the HBM budget */
     KMC_ERR_NOT_READY = 9,        /* serving: result not finished yet */
-    KMC_ERR_NO_DEVICE = 10        /* no HIP device visible */
+    KMC_ERR_NO_DEVICE = 10,       /* no HIP device visible */
+    KMC_ERR_CELL_TYPE = 13,       /* unstructured: not tetrahedra / hexahedra / pyramid / prism cells */
+    KMC_ERR_FILE = 14,            /* a file cannot be read or written */
+    KMC_ERR_NOT_TRIANGLES = 15    /* "Input object is not triangle polygon." (PolygonExporter.cpp:80-85) */
 } kmc_status;
 
 /* Value types: the order of kvs::Type::TypeID (Source/Core/Utility/Type.h:40-53). */
@@ -92,6 +95,8 @@ typedef struct kmc_surface {
     double min_value, max_value; /* the volume min/max used (clamp range and transfer-function range) */
     float min_coord[3], max_coord[3]; /* min/max object coordinates copied to the polygon (MapperBase.cpp:116-121) */
     void* internal;              /* ownership handle; release with kmc_surface_release */
+    uint8_t* colors;             /* per-vertex RGB (kmc_slice_plane: color_type 0 = VertexColor); else NULL */
+    uint64_t n_colors;           /* RGB triples */
 } kmc_surface;
 
 /* Select the HIP device used by this thread's calls (default 0). */
@@ -121,6 +126,116 @@ kmc_status kmc_extract(const kmc_volume* volume, double isolevel, int32_t normal
 kmc_status kmc_extract_levels(const kmc_volume* volume, const double* isolevels, uint32_t n_levels,
                               int32_t normal_type, int32_t duplication, int32_t out_memory,
                               void* stream, kmc_surface* outs);
+
+/*
+ * kvs::SlicePlane over a structured volume: the drop-in for
+ * SlicePlane(volume, coefficients, transfer_function) (SlicePlane.cpp:26-36,
+ * exec :65-85, mapping :93-150, extract_plane<T> :165-280).  coefficients:
+ * plane(x,y,z) = c[0]*x + c[1]*y + c[2]*z + c[3] over node indices (the
+ * point / normal form stores { normal, -point.dot(normal) }).  The surface
+ * holds, like the reference's polygon: coords per triangle corner
+ * (n_vertices = 3 * n_triangles), one normal per triangle
+ * (-(v2 - v0) x (v1 - v0)), one colour per corner (colors, n_colors) from
+ * the colour map color_table (3 bytes per entry, color_resolution entries:
+ * ColorMap::table()) over color_range ([min, max]; NULL: the range
+ * MapperBase::setRange gives a transfer function without one, MapperBase.cpp:54-83);
+ * polygon_type 3, color_type 0 (VertexColor), normal_type PolygonNormal.
+ * Unstructured volumes: kmc_slice_plane_unstructured.
+ */
+kmc_status kmc_slice_plane(const kmc_volume* volume, const float coefficients[4], const uint8_t* color_table,
+                           uint32_t color_resolution, const float* color_range, int32_t out_memory, void* stream,
+                           kmc_surface* out);
+
+/* ------------------------------------------------------------------------- */
+/* Unstructured volumes: the unstructured branch of kvs::Isosurface          */
+/* (Isosurface.cpp:192-257) and kvs::SlicePlane over unstructured cells.      */
+/* ------------------------------------------------------------------------- */
+
+/* kvs::UnstructuredVolumeObject::CellType (UnstructuredVolumeObject.h:31-41)
+ * of the cell types with a marching table. */
+typedef enum kmc_cell_type {
+    KMC_CELL_TETRAHEDRA = 1, KMC_CELL_HEXAHEDRA = 2, KMC_CELL_PYRAMID = 5, KMC_CELL_PRISM = 7
+} kmc_cell_type;
+
+/* The fields of kvs::UnstructuredVolumeObject the path reads
+ * (UnstructuredVolumeObject.h:28-99): node values, node coordinates (3 floats
+ * per node) and the cells' node indices (4 / 8 / 5 / 6 per cell), all in the
+ * same memory. */
+typedef struct kmc_unstructured_volume {
+    const void* values;          /* n_nodes values */
+    const float* coords;         /* 3 * n_nodes */
+    const uint32_t* connections; /* nodes per cell * n_cells */
+    int32_t memory;              /* kmc_memory of the three arrays */
+    int32_t type;                /* kmc_type of values */
+    uint64_t n_nodes, n_cells;
+    int32_t cell_type;           /* kmc_cell_type */
+    uint32_t veclen;             /* must be 1 */
+    int32_t has_min_max_values;  /* else GetMinMaxValues in connection order (UnstructuredVolumeObject.cpp:123-141) */
+    double min_value, max_value;
+    int32_t has_min_max_coords;  /* else updateMinMaxCoords in connection order (UnstructuredVolumeObject.cpp:425-455) */
+    float min_coord[3], max_coord[3];
+} kmc_unstructured_volume;
+
+/*
+ * kvs::Isosurface over an unstructured volume: MarchingTetrahedra /
+ * MarchingHexahedra / MarchingPyramid / MarchingPrism by cell type
+ * (MarchingTetrahedra.cpp:25-286 and the same in the others).  Only their
+ * duplicated extraction exists: VertexNormal forces duplication off and a
+ * surface without duplication is empty, with the types the mapper leaves.
+ * Triangles in cell order and table order; per triangle 3 corners and its
+ * normal (v1 - v0) x (v2 - v0).  The colour is kmc_surface_color at isolevel.
+ */
+kmc_status kmc_extract_unstructured(const kmc_unstructured_volume* volume, double isolevel, int32_t normal_type,
+                                    int32_t duplication, int32_t out_memory, void* stream, kmc_surface* out);
+
+/* kvs::SlicePlane over an unstructured volume (SlicePlane.cpp:288-800): as
+ * kmc_slice_plane, with node coordinates in place of the grid (no scaling). */
+kmc_status kmc_slice_plane_unstructured(const kmc_unstructured_volume* volume, const float coefficients[4],
+                                        const uint8_t* color_table, uint32_t color_resolution,
+                                        const float* color_range, int32_t out_memory, void* stream,
+                                        kmc_surface* out);
+
+/* ------------------------------------------------------------------------- */
+/* Volume ingestion: KVSML straight to HBM                                    */
+/* ------------------------------------------------------------------------- */
+
+/*
+ * kvs::StructuredVolumeObject::read(filename) for KVSML files
+ * (StructuredVolumeObject.cpp:171-229, KVSMLStructuredVolumeObject.cpp:159-300,
+ * DataArrayTag.cpp:217-420): internal text, external ascii and external binary
+ * arrays (either endianness) of char .. double values, uniform / rectilinear /
+ * curvilinear grids.  With memory = KMC_MEMORY_DEVICE an external binary array
+ * streams from the file into device memory through two pinned staging buffers
+ * (disk reads overlap the copies; no host copy of the volume).  *out is a
+ * volume ready for kmc_extract: has_min_max_values set (from the file's
+ * min_value / max_value, else GetMinMaxValues on the device) and
+ * has_min_max_coords set (object_coord, else updateMinMaxCoords of the grid
+ * type; non-uniform coordinates themselves are not kept, as the marching-cubes
+ * path only reads their extent).  Release with kmc_kvsml_volume_release.
+ */
+kmc_status kmc_kvsml_read_volume(const char* filename, int32_t memory, void* stream, kmc_volume* out);
+void kmc_kvsml_volume_release(kmc_volume* volume);
+
+/* ------------------------------------------------------------------------- */
+/* Output side                                                                 */
+/* ------------------------------------------------------------------------- */
+
+/*
+ * The surface as a triangle soup, built on the device: 3 corners per triangle
+ * gathered through the connections and one normal per triangle (VertexNormal
+ * surfaces: (n0 + n1 + n2) / 3.0f of the corners' normals) -- the arrays
+ * PolygonExporter<kvs::Stl> builds (Exporter/PolygonExporter.cpp:91-160), and
+ * the per-face expansion a renderer of polygon colours / normals needs
+ * (Renderer/PolygonRendererGLSL.cpp:105-186).  A duplicated surface is copied.
+ * *out: n_vertices = 3 * n_triangles, n_normals = n_triangles, no connections,
+ * PolygonNormal, in out_memory (release with kmc_surface_release).
+ */
+kmc_status kmc_surface_expand(const kmc_surface* surface, int32_t out_memory, void* stream, kmc_surface* out);
+
+/* PolygonExporter<kvs::Stl>(polygon) then Stl::write(filename) with the file
+ * type Ascii (ascii != 0) or Binary (FileFormat/STL/Stl.cpp:447-540): the same
+ * bytes as the reference. */
+kmc_status kmc_export_stl(const kmc_surface* surface, const char* filename, int32_t ascii, void* stream);
 
 /* Release the arrays of a surface returned by any kmc_* extraction call. */
 void kmc_surface_release(kmc_surface* surface);

@@ -185,10 +185,12 @@ void free_storage(SurfaceStorage* s)
         device_free(s->coords);
         device_free(s->connections);
         device_free(s->normals);
+        device_free(s->colors);
     } else {
         if (s->coords) (void)hipHostFree(s->coords);
         if (s->connections) (void)hipHostFree(s->connections);
         if (s->normals) (void)hipHostFree(s->normals);
+        if (s->colors) (void)hipHostFree(s->colors);
     }
     delete s;
 }
@@ -629,13 +631,16 @@ kmc_status to_host(kmc_surface* s, hipStream_t st)
     const size_t bc = s->n_vertices * 3 * sizeof(float);
     const size_t bn = s->n_normals * 3 * sizeof(float);
     const size_t bk = s->n_connections * sizeof(uint32_t);
+    const size_t bl = s->colors ? s->n_colors * 3 : 0;
     bool ok = true;
     if (bc) ok = ok && hipHostMalloc(&hs->coords, bc, hipHostMallocDefault) == hipSuccess;
     if (bn) ok = ok && hipHostMalloc(&hs->normals, bn, hipHostMallocDefault) == hipSuccess;
     if (bk) ok = ok && hipHostMalloc(&hs->connections, bk, hipHostMallocDefault) == hipSuccess;
+    if (bl) ok = ok && hipHostMalloc(&hs->colors, bl, hipHostMallocDefault) == hipSuccess;
     if (ok && bc) ok = hipMemcpyAsync(hs->coords, s->coords, bc, hipMemcpyDeviceToHost, st) == hipSuccess;
     if (ok && bn) ok = hipMemcpyAsync(hs->normals, s->normals, bn, hipMemcpyDeviceToHost, st) == hipSuccess;
     if (ok && bk) ok = hipMemcpyAsync(hs->connections, s->connections, bk, hipMemcpyDeviceToHost, st) == hipSuccess;
+    if (ok && bl) ok = hipMemcpyAsync(hs->colors, s->colors, bl, hipMemcpyDeviceToHost, st) == hipSuccess;
     if (ok) ok = hipStreamSynchronize(st) == hipSuccess;
     if (!ok) {
         (void)hipGetLastError();
@@ -648,6 +653,7 @@ kmc_status to_host(kmc_surface* s, hipStream_t st)
     s->coords = (float*)hs->coords;
     s->normals = (float*)hs->normals;
     s->connections = (uint32_t*)hs->connections;
+    s->colors = (uint8_t*)hs->colors;
     return KMC_OK;
 }
 
@@ -738,6 +744,9 @@ const char* kmc_status_string(kmc_status s)
     case KMC_ERR_BUDGET: return "Request exceeds the memory budget.";
     case KMC_ERR_NOT_READY: return "Result not ready.";
     case KMC_ERR_NO_DEVICE: return "No HIP device available.";
+    case KMC_ERR_CELL_TYPE: return "Input volume is not tetrahedra, hexahedra, pyramid or prism-cell data.";
+    case KMC_ERR_FILE: return "Cannot read or write the file.";
+    case KMC_ERR_NOT_TRIANGLES: return "Input object is not triangle polygon.";
     }
     return "Unknown status.";
 }
@@ -768,7 +777,8 @@ void kmc_surface_release(kmc_surface* s)
     s->coords = nullptr;
     s->connections = nullptr;
     s->normals = nullptr;
-    s->n_vertices = s->n_triangles = s->n_connections = s->n_normals = 0;
+    s->colors = nullptr;
+    s->n_vertices = s->n_triangles = s->n_connections = s->n_normals = s->n_colors = 0;
 }
 
 void kmc_set_timing(int32_t enable) { t_timing = enable != 0; }

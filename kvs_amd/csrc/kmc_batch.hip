@@ -466,16 +466,24 @@ hipError_t record_front(kmc_batch* b, hipStream_t st, bool copy_sizes)
     return hipMemcpyAsync(b->segout_h, b->segout_d, (nseg + 1) * sizeof(SegOut), hipMemcpyDeviceToHost, st);
 }
 
-hipError_t record_emit(kmc_batch* b, hipStream_t st)
+/* Launches of the emit.  host_sizes: the passes' record offsets are set on
+ * the host (the serving form's two phases), so the per-item work of each pass
+ * runs as the plain call's kernels with its arguments in the launch; in a
+ * captured graph they live on the device only. */
+hipError_t record_emit(kmc_batch* b, hipStream_t st, bool host_sizes)
 {
     hipError_t e = hipSuccess;
     if (b->flat) {
         const uint32_t np = (uint32_t)b->passes.size();
         for (const Group& g : b->groups) {
             e = launch_compact_batch(b->jobs_d + g.p0, g.row_start_d, g.p1 - g.p0, g.rows, st);
-            if (e == hipSuccess)
+            if (e != hipSuccess) return e;
+            if (host_sizes) {
+                for (uint32_t i = g.p0; i < g.p1 && e == hipSuccess; ++i) e = launch_flat_items(g.dtype, b->passes[i].a, st);
+            } else {
                 e = launch_flat_batch(g.dtype, g.mode, b->jobs_d + g.p0, b->istart_d + g.p0, b->istart_d + np + 1 + g.p0,
                                       g.p1 - g.p0, st);
+            }
             if (e != hipSuccess) return e;
         }
         return e;
@@ -494,7 +502,7 @@ hipError_t record(kmc_batch* b, hipStream_t st)
     const uint32_t nseg = (uint32_t)b->seg_pass_level.size();
     if (nseg == 0) return hipSuccess;
     hipError_t e = record_front(b, st, false);
-    if (e == hipSuccess) e = record_emit(b, st);
+    if (e == hipSuccess) e = record_emit(b, st, false);
     if (e != hipSuccess) return e;
     return hipMemcpyAsync(b->segout_h, b->segout_d, (nseg + 1) * sizeof(SegOut), hipMemcpyDeviceToHost, st);
 }
@@ -882,7 +890,7 @@ hipError_t batch_launch_emit(kmc_batch* b, hipStream_t st)
     const hipError_t e = hipMemcpyAsync(b->jobs_d, b->jobs_h.data(), b->jobs_h.size() * sizeof(JobArgs),
                                         hipMemcpyHostToDevice, st);
     if (e != hipSuccess) return e;
-    return record_emit(b, st);
+    return record_emit(b, st, true);
 }
 
 /* Device bytes the plan holds besides the volumes (counts, prefixes, masks,

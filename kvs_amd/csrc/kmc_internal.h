@@ -26,8 +26,11 @@ struct SurfaceStorage {
     void* coords;
     void* connections;
     void* normals;
+    void* colors;
 };
 void free_storage(SurfaceStorage* s);
+/* Move a device surface (arrays owned by its SurfaceStorage) to pinned host memory. */
+kmc_status to_host(kmc_surface* s, hipStream_t st);
 
 size_t type_size(int t);
 float float_threshold(double iso);
@@ -73,15 +76,24 @@ hipError_t launch_flat_emit(int dtype, const JobArgs& a, hipStream_t st, hipEven
  * per-item kernels over the items start[j] .. start[j + 1] of each job. */
 hipError_t launch_compact_batch(const JobArgs* jobs, const uint64_t* row_start, uint32_t njobs, uint64_t rows,
                                 hipStream_t st);
+/* the per-item kernels of one pass whose record offsets are set (after its compaction) */
+hipError_t launch_flat_items(int dtype, const JobArgs& a, hipStream_t st);
 hipError_t launch_flat_batch(int dtype, int mode, const JobArgs* jobs, const uint64_t* vstart, const uint64_t* tstart,
                              uint32_t njobs, hipStream_t st);
 hipError_t launch_classify(int dtype, const JobArgs& a, hipStream_t st);
+/* SlicePlane: masks from the plane's sign (a.plane), and the duplicated emit
+ * (compaction, then one thread per triangle with per-vertex colours). */
+hipError_t launch_classify_plane(const JobArgs& a, hipStream_t st);
+hipError_t launch_slice_emit(int dtype, const JobArgs& a, hipStream_t st);
 hipError_t launch_count(const JobArgs& a, hipStream_t st);
 hipError_t launch_scan(const uint32_t* counts, uint64_t n, uint32_t narr, uint64_t* partials,
                        uint64_t* prefix, uint64_t* totals, hipStream_t st);
 uint32_t scan_partials_needed(uint64_t n, uint32_t narr);
 hipError_t launch_minmax(int dtype, const void* vol, uint64_t n, void* scratch, double* out, hipStream_t st);
 size_t minmax_scratch_bytes();
+/* The unstructured form: vol[0], then vol[conn[i]] for every connection entry in order. */
+hipError_t launch_minmax_gather(int dtype, const void* vol, const uint32_t* conn, uint64_t nconn, void* scratch,
+                                double* out, hipStream_t st);
 hipError_t launch_classify_batch(int dtype, const JobArgs* jobs, const uint64_t* row_start, uint32_t njobs,
                                  uint64_t rows, hipStream_t st);
 hipError_t launch_count_batch(const JobArgs* jobs, const uint64_t* row_start, uint32_t njobs, uint64_t rows,

@@ -117,6 +117,13 @@ struct JobArgs {
     uint32_t* conn[kMaxLevels];
     uint64_t cap_vertices[kMaxLevels];   // capacity in coordinate triples
     uint64_t cap_triangles[kMaxLevels];
+    // SlicePlane: nodes classify by the plane's sign instead of their values;
+    // each vertex gets the colour of the value interpolated along its edge
+    float plane[4];            // coefficients: plane(x,y,z) = plane[0]*x + plane[1]*y + plane[2]*z + plane[3]
+    const uint8_t* ctable;     // colour map table, 3 bytes per entry (device)
+    uint32_t cres;             // entries of ctable
+    float crange[2];           // colour map range
+    uint8_t* colors;           // one RGB per vertex (9 bytes per triangle)
 };
 
 /* LDS layout of one team (one level); sizes in bytes, offsets 16-byte

@@ -346,3 +346,59 @@ def serving_volumes(n=32, seed=4000, sizes=tuple(range(64, 513, 32))):
     rng = SplitMix64(seed)
     kinds = ("float32", "uint16", "uint8")
     return [(sizes[rng.integer(0, len(sizes) - 1)], kinds[i % 3], seed + 1 + i) for i in range(n)]
+
+
+# kvs::UnstructuredVolumeObject::CellType
+TETRAHEDRA, HEXAHEDRA, PYRAMID, PRISM = 1, 2, 5, 7
+
+
+def unstructured_mesh(dims, cell_type, seed=0, jitter=0.2, nblobs=4, dtype=np.float32):
+    """An unstructured volume cut from a (nx, ny, nz) node grid with seeded
+    jitter: per grid cube 6 tetrahedra around its main diagonal, 1 hexahedron,
+    2 prisms, or 6 pyramids with an extra apex node at the cube centre.  Node
+    values are the Gaussian-blob field of blobs() at the node positions.
+    Returns (values [nnodes], coords float32 [nnodes, 3], connections uint32
+    [ncells * nodes per cell]) in kvs::UnstructuredVolumeObject's layout."""
+    if isinstance(dims, int):
+        dims = (dims, dims, dims)
+    nx, ny, nz = dims
+    rng = SplitMix64(seed)
+    z, y, x = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    pos = np.stack([x, y, z], axis=-1).reshape(-1, 3).astype(np.float64)
+    pos += (rng.bulk_uniform(pos.size).reshape(pos.shape) - 0.5) * jitter
+
+    def node(i, j, k):
+        return (k * ny + j) * nx + i
+
+    cz, cy, cx = np.meshgrid(np.arange(nz - 1), np.arange(ny - 1), np.arange(nx - 1), indexing="ij")
+    cx, cy, cz = cx.ravel(), cy.ravel(), cz.ravel()
+    # cube corners in marching-cubes order: 0 (0,0,0) 1 (1,0,0) 2 (1,1,0) 3 (0,1,0), 4..7 the same at z+1
+    off = [(0, 0, 0), (1, 0, 0), (1, 1, 0), (0, 1, 0), (0, 0, 1), (1, 0, 1), (1, 1, 1), (0, 1, 1)]
+    corner = np.stack([node(cx + a, cy + b, cz + c) for a, b, c in off], axis=1)
+    if cell_type == TETRAHEDRA:
+        tets = [(0, 1, 2, 6), (0, 2, 3, 6), (0, 3, 7, 6), (0, 7, 4, 6), (0, 4, 5, 6), (0, 5, 1, 6)]
+        conn = np.concatenate([corner[:, list(t)] for t in tets], axis=1)
+    elif cell_type == HEXAHEDRA:
+        # connections 4..7 become the mapper's local nodes 0..3 (MarchingHexahedra.cpp:163-170)
+        conn = corner[:, [4, 5, 6, 7, 0, 1, 2, 3]]
+    elif cell_type == PRISM:
+        conn = np.concatenate([corner[:, [0, 1, 2, 4, 5, 6]], corner[:, [0, 2, 3, 4, 6, 7]]], axis=1)
+    elif cell_type == PYRAMID:
+        centre = pos.shape[0] + np.arange(corner.shape[0])
+        cpos = np.mean(pos[corner], axis=1)
+        pos = np.concatenate([pos, cpos])
+        faces = [(0, 3, 2, 1), (4, 5, 6, 7), (0, 1, 5, 4), (1, 2, 6, 5), (2, 3, 7, 6), (3, 0, 4, 7)]
+        conn = np.concatenate([np.concatenate([centre[:, None], corner[:, list(f)]], axis=1) for f in faces], axis=1)
+    else:
+        raise ValueError(f"cell type {cell_type}")
+    _, params = blob_params(seed + 7, nblobs)
+    p = pos / np.array([max(nx - 1, 1), max(ny - 1, 1), max(nz - 1, 1)], dtype=np.float64)
+    f = np.zeros(pos.shape[0])
+    for c, s in params:
+        f += np.exp(-np.sum((p - np.asarray(c)) ** 2, axis=1) / (2 * s * s))
+    if np.dtype(dtype).kind in "ui":
+        top = float(np.iinfo(dtype).max if np.dtype(dtype).itemsize < 4 else 1000)
+        values = np.rint(f / f.max() * top).astype(dtype)
+    else:
+        values = f.astype(dtype)
+    return values, pos.astype(np.float32), conn.reshape(-1).astype(np.uint32)

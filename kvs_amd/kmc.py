@@ -29,6 +29,8 @@ COLORMAP_DEFAULT, COLORMAP_GREYS = 0, 1
 OK = 0
 ERR_NULL_INPUT, ERR_NOT_STRUCTURED, ERR_NOT_SCALAR, ERR_UNSUPPORTED_TYPE = 1, 2, 3, 4
 ERR_INVALID_ARGUMENT, ERR_OUT_OF_MEMORY, ERR_DEVICE, ERR_BUDGET, ERR_NOT_READY, ERR_NO_DEVICE = 5, 6, 7, 8, 9, 10
+ERR_CELL_TYPE, ERR_FILE, ERR_NOT_TRIANGLES = 13, 14, 15
+TETRAHEDRA, HEXAHEDRA, PYRAMID, PRISM = 1, 2, 5, 7
 
 
 class Volume(ctypes.Structure):
@@ -68,6 +70,28 @@ class Surface(ctypes.Structure):
         ("min_coord", ctypes.c_float * 3),
         ("max_coord", ctypes.c_float * 3),
         ("internal", ctypes.c_void_p),
+        ("colors", ctypes.POINTER(ctypes.c_uint8)),
+        ("n_colors", ctypes.c_uint64),
+    ]
+
+
+class UnstructuredVolume(ctypes.Structure):
+    _fields_ = [
+        ("values", ctypes.c_void_p),
+        ("coords", ctypes.c_void_p),
+        ("connections", ctypes.c_void_p),
+        ("memory", ctypes.c_int32),
+        ("type", ctypes.c_int32),
+        ("n_nodes", ctypes.c_uint64),
+        ("n_cells", ctypes.c_uint64),
+        ("cell_type", ctypes.c_int32),
+        ("veclen", ctypes.c_uint32),
+        ("has_min_max_values", ctypes.c_int32),
+        ("min_value", ctypes.c_double),
+        ("max_value", ctypes.c_double),
+        ("has_min_max_coords", ctypes.c_int32),
+        ("min_coord", ctypes.c_float * 3),
+        ("max_coord", ctypes.c_float * 3),
     ]
 
 
@@ -141,7 +165,9 @@ EXPORTED = [
     "kmc_batch_extract",
     "kmc_server_create", "kmc_server_register_volume", "kmc_server_submit", "kmc_server_poll", "kmc_server_wait",
     "kmc_server_get_stats", "kmc_server_reset_stats", "kmc_server_destroy",
-    "kmc_set_timing", "kmc_last_timing",
+    "kmc_set_timing", "kmc_last_timing", "kmc_slice_plane",
+    "kmc_extract_unstructured", "kmc_slice_plane_unstructured", "kmc_kvsml_read_volume", "kmc_kvsml_volume_release",
+    "kmc_surface_expand", "kmc_export_stl",
 ]
 
 
@@ -180,6 +206,27 @@ def load():
                                        ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_void_p,
                                        ctypes.POINTER(Surface)]
     lib.kmc_extract_levels.restype = ctypes.c_int
+    lib.kmc_slice_plane.argtypes = [ctypes.POINTER(Volume), ctypes.POINTER(ctypes.c_float),
+                                    ctypes.POINTER(ctypes.c_uint8), ctypes.c_uint32, ctypes.POINTER(ctypes.c_float),
+                                    ctypes.c_int32, ctypes.c_void_p, ctypes.POINTER(Surface)]
+    lib.kmc_slice_plane.restype = ctypes.c_int
+    lib.kmc_extract_unstructured.argtypes = [ctypes.POINTER(UnstructuredVolume), ctypes.c_double, ctypes.c_int32,
+                                             ctypes.c_int32, ctypes.c_int32, ctypes.c_void_p, ctypes.POINTER(Surface)]
+    lib.kmc_extract_unstructured.restype = ctypes.c_int
+    lib.kmc_slice_plane_unstructured.argtypes = [ctypes.POINTER(UnstructuredVolume), ctypes.POINTER(ctypes.c_float),
+                                                 ctypes.POINTER(ctypes.c_uint8), ctypes.c_uint32,
+                                                 ctypes.POINTER(ctypes.c_float), ctypes.c_int32, ctypes.c_void_p,
+                                                 ctypes.POINTER(Surface)]
+    lib.kmc_slice_plane_unstructured.restype = ctypes.c_int
+    lib.kmc_kvsml_read_volume.argtypes = [ctypes.c_char_p, ctypes.c_int32, ctypes.c_void_p, ctypes.POINTER(Volume)]
+    lib.kmc_kvsml_read_volume.restype = ctypes.c_int
+    lib.kmc_kvsml_volume_release.argtypes = [ctypes.POINTER(Volume)]
+    lib.kmc_kvsml_volume_release.restype = None
+    lib.kmc_surface_expand.argtypes = [ctypes.POINTER(Surface), ctypes.c_int32, ctypes.c_void_p,
+                                       ctypes.POINTER(Surface)]
+    lib.kmc_surface_expand.restype = ctypes.c_int
+    lib.kmc_export_stl.argtypes = [ctypes.POINTER(Surface), ctypes.c_char_p, ctypes.c_int32, ctypes.c_void_p]
+    lib.kmc_export_stl.restype = ctypes.c_int
     lib.kmc_surface_release.argtypes = [ctypes.POINTER(Surface)]
     lib.kmc_surface_release.restype = None
     lib.kmc_volume_min_max.argtypes = [ctypes.POINTER(Volume), ctypes.c_void_p,
@@ -273,8 +320,10 @@ def surface_to_numpy(s):
     coords = np.ctypeslib.as_array(s.coords, shape=(int(s.n_vertices) * 3,)).copy() if s.n_vertices else np.zeros(0, np.float32)
     conn = np.ctypeslib.as_array(s.connections, shape=(int(s.n_connections),)).copy() if s.n_connections else np.zeros(0, np.uint32)
     normals = np.ctypeslib.as_array(s.normals, shape=(int(s.n_normals) * 3,)).copy() if s.n_normals else np.zeros(0, np.float32)
+    colors = (np.ctypeslib.as_array(s.colors, shape=(int(s.n_colors) * 3,)).copy() if s.n_colors and s.colors
+              else np.zeros(0, np.uint8))
     return {
-        "coords": coords, "connections": conn, "normals": normals,
+        "coords": coords, "connections": conn, "normals": normals, "colors": colors,
         "n_triangles": int(s.n_triangles), "polygon_type": s.polygon_type, "normal_type": s.normal_type,
         "color_type": s.color_type, "empty_range": s.empty_range,
         "min_value": s.min_value, "max_value": s.max_value,
@@ -313,6 +362,170 @@ def extract_levels(values, isolevels, normal_type=POLYGON_NORMAL, duplication=Tr
     finally:
         for i in range(n):
             lib.kmc_surface_release(ctypes.byref(outs[i]))
+
+
+def slice_plane(values, coefficients, color_table, color_range=None, min_max=None, coords=None,
+                out_memory=MEMORY_HOST):
+    """kmc_slice_plane: kvs::SlicePlane over a host numpy volume [z][y][x];
+    color_table: the colour map's table (3 bytes per entry)."""
+    lib = load()
+    v = make_volume(values, min_max=min_max, coords=coords)
+    c = (ctypes.c_float * 4)(*[float(x) for x in coefficients])
+    t = np.ascontiguousarray(color_table, dtype=np.uint8)
+    rng = (ctypes.c_float * 2)(*[float(x) for x in color_range]) if color_range is not None else None
+    s = Surface()
+    st = lib.kmc_slice_plane(ctypes.byref(v), c, t.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), t.size // 3,
+                             rng, out_memory, None, ctypes.byref(s))
+    if st != OK:
+        raise KmcError(st)
+    try:
+        return surface_to_numpy(s) if s.memory == MEMORY_HOST else device_surface_to_numpy(s)
+    finally:
+        lib.kmc_surface_release(ctypes.byref(s))
+
+
+CELL_NODES = {TETRAHEDRA: 4, HEXAHEDRA: 8, PYRAMID: 5, PRISM: 6}
+
+
+def make_unstructured_volume(values, coords, connections, cell_type, min_max=None, min_max_coords=None, veclen=1):
+    """A host kmc_unstructured_volume over numpy arrays (kept alive by the struct)."""
+    v = UnstructuredVolume()
+    vals = np.ascontiguousarray(values).ravel()
+    co = np.ascontiguousarray(coords, dtype=np.float32).ravel()
+    cn = np.ascontiguousarray(connections, dtype=np.uint32).ravel()
+    v.values, v.coords, v.connections = vals.ctypes.data, co.ctypes.data, cn.ctypes.data
+    v._keep = (vals, co, cn)
+    v.memory = MEMORY_HOST
+    v.type = NUMPY_TYPES[vals.dtype]
+    v.n_nodes = vals.size // max(1, veclen)
+    v.n_cells = cn.size // CELL_NODES.get(int(cell_type), 4)
+    v.cell_type = int(cell_type)
+    v.veclen = veclen
+    if min_max is not None:
+        v.has_min_max_values = 1
+        v.min_value, v.max_value = float(min_max[0]), float(min_max[1])
+    if min_max_coords is not None:
+        v.has_min_max_coords = 1
+        v.min_coord[:] = [float(c) for c in min_max_coords[0]]
+        v.max_coord[:] = [float(c) for c in min_max_coords[1]]
+    return v
+
+
+def extract_unstructured(values, coords, connections, cell_type, isolevel, normal_type=POLYGON_NORMAL,
+                         duplication=True, min_max=None, min_max_coords=None, out_memory=MEMORY_HOST):
+    """kmc_extract_unstructured on host numpy arrays."""
+    lib = load()
+    v = make_unstructured_volume(values, coords, connections, cell_type, min_max, min_max_coords)
+    s = Surface()
+    st = lib.kmc_extract_unstructured(ctypes.byref(v), float(isolevel), int(normal_type), int(bool(duplication)),
+                                      out_memory, None, ctypes.byref(s))
+    if st != OK:
+        raise KmcError(st)
+    try:
+        return surface_to_numpy(s) if s.memory == MEMORY_HOST else device_surface_to_numpy(s)
+    finally:
+        lib.kmc_surface_release(ctypes.byref(s))
+
+
+def slice_plane_unstructured(values, coords, connections, cell_type, coefficients, color_table, color_range=None,
+                             min_max=None, out_memory=MEMORY_HOST):
+    """kmc_slice_plane_unstructured on host numpy arrays."""
+    lib = load()
+    v = make_unstructured_volume(values, coords, connections, cell_type, min_max)
+    c = (ctypes.c_float * 4)(*[float(x) for x in coefficients])
+    t = np.ascontiguousarray(color_table, dtype=np.uint8)
+    rng = (ctypes.c_float * 2)(*[float(x) for x in color_range]) if color_range is not None else None
+    s = Surface()
+    st = lib.kmc_slice_plane_unstructured(ctypes.byref(v), c, t.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
+                                          t.size // 3, rng, out_memory, None, ctypes.byref(s))
+    if st != OK:
+        raise KmcError(st)
+    try:
+        return surface_to_numpy(s) if s.memory == MEMORY_HOST else device_surface_to_numpy(s)
+    finally:
+        lib.kmc_surface_release(ctypes.byref(s))
+
+
+class KvsmlVolume:
+    """A volume read by kmc_kvsml_read_volume; .volume is the kmc_volume (values
+    in host or device memory, owned until close())."""
+
+    def __init__(self, filename, memory=MEMORY_DEVICE):
+        self.lib = load()
+        self.volume = Volume()
+        st = self.lib.kmc_kvsml_read_volume(str(filename).encode(), memory, None, ctypes.byref(self.volume))
+        if st != OK:
+            raise KmcError(st)
+
+    def values(self):
+        """The value array as numpy [z][y][x] (copied to the host)."""
+        v = self.volume
+        dtype = [k for k, t in NUMPY_TYPES.items() if t == v.type][0]
+        n = int(v.resolution[0]) * int(v.resolution[1]) * int(v.resolution[2]) * int(v.veclen)
+        out = np.zeros(n, dtype=dtype)
+        if v.memory == MEMORY_HOST:
+            ctypes.memmove(out.ctypes.data, v.values, out.nbytes)
+        elif n:
+            rc = _hip().hipMemcpy(out.ctypes.data, ctypes.c_void_p(v.values), out.nbytes, 2)
+            if rc != 0:
+                raise RuntimeError(f"hipMemcpy failed: {rc}")
+        shape = (int(v.resolution[2]), int(v.resolution[1]), int(v.resolution[0])) + ((int(v.veclen),) if v.veclen > 1 else ())
+        return out.reshape(shape)
+
+    def close(self):
+        if self.volume.values:
+            self.lib.kmc_kvsml_volume_release(ctypes.byref(self.volume))
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def host_surface(d, normal_type, polygon_type=3):
+    """A host kmc_surface over numpy arrays (coords, connections, normals of a
+    surface dict as extract returns); kept alive by the struct."""
+    s = Surface()
+    co = np.ascontiguousarray(d["coords"], dtype=np.float32)
+    cn = np.ascontiguousarray(d["connections"], dtype=np.uint32)
+    no = np.ascontiguousarray(d["normals"], dtype=np.float32)
+    s._keep = (co, cn, no)
+    s.coords = co.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+    s.connections = cn.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))
+    s.normals = no.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+    s.n_vertices = co.size // 3
+    s.n_connections = cn.size
+    s.n_normals = no.size // 3
+    s.n_triangles = cn.size // 3 if cn.size else co.size // 9
+    s.memory = MEMORY_HOST
+    s.polygon_type = polygon_type
+    s.normal_type = normal_type
+    s.color_type = 1
+    return s
+
+
+def surface_expand(d, normal_type, out_memory=MEMORY_HOST):
+    """kmc_surface_expand of a host surface dict."""
+    lib = load()
+    s = host_surface(d, normal_type)
+    o = Surface()
+    st = lib.kmc_surface_expand(ctypes.byref(s), out_memory, None, ctypes.byref(o))
+    if st != OK:
+        raise KmcError(st)
+    try:
+        return surface_to_numpy(o) if o.memory == MEMORY_HOST else device_surface_to_numpy(o)
+    finally:
+        lib.kmc_surface_release(ctypes.byref(o))
+
+
+def export_stl(d, normal_type, filename, ascii=False):
+    """kmc_export_stl of a host surface dict."""
+    lib = load()
+    s = host_surface(d, normal_type)
+    st = lib.kmc_export_stl(ctypes.byref(s), str(filename).encode(), int(bool(ascii)), None)
+    if st != OK:
+        raise KmcError(st)
 
 
 def min_max(values):
@@ -508,6 +721,7 @@ def device_surface_to_numpy(s):
         "coords": fetch(s.coords, int(s.n_vertices) * 3, np.float32),
         "connections": fetch(s.connections, int(s.n_connections), np.uint32),
         "normals": fetch(s.normals, int(s.n_normals) * 3, np.float32),
+        "colors": fetch(s.colors, int(s.n_colors) * 3 if s.colors else 0, np.uint8),
         "n_triangles": int(s.n_triangles), "polygon_type": s.polygon_type, "normal_type": s.normal_type,
         "color_type": s.color_type, "empty_range": s.empty_range,
         "min_value": s.min_value, "max_value": s.max_value,

@@ -18,6 +18,14 @@
 #include <kvs/AnyValueArray>
 #include <kvs/Type>
 #include <kvs/HydrogenVolumeData>
+#include <kvs/SlicePlane>
+#include <kvs/PolygonExporter>
+#include <kvs/Stl>
+#include <kvs/UnstructuredVolumeObject>
+#include <kvs/MarchingTetrahedraTable>
+#include <kvs/MarchingHexahedraTable>
+#include <kvs/MarchingPyramidTable>
+#include <kvs/MarchingPrismTable>
 #include <cstdint>
 #include <cstdlib>
 #include <cstring>
@@ -221,6 +229,221 @@ int ref_hydrogen( const uint32_t dims[3], uint8_t* out )
     const auto& v = h.values();
     std::memcpy( out, v.data(), v.byteSize() );
     return int( v.byteSize() );
+}
+
+// kvs::SlicePlane( volume, coefficients, tf ) over a structured volume.
+int ref_slice_plane( const ref_volume* v, const float coef[4], int tf_kind, int tf_resolution, ref_mesh* out )
+{
+    auto* vol = MakeVolume( v );
+    const auto t0 = std::chrono::steady_clock::now();
+    auto* sp = new kvs::SlicePlane( vol, kvs::Vec4( coef[0], coef[1], coef[2], coef[3] ), MakeTF( tf_kind, tf_resolution ) );
+    const auto t1 = std::chrono::steady_clock::now();
+    Harvest( sp, vol, sp->isSuccess() ? 1 : 0, out );
+    out->seconds = std::chrono::duration<double>( t1 - t0 ).count();
+    delete sp;
+    delete vol;
+    return 0;
+}
+
+// kvs::SlicePlane( volume, point, normal, tf ): the coefficients the
+// point-normal form stores (SlicePlane.h:51: { normal, -point.dot( normal ) }).
+void ref_slice_plane_coefficients( const float point[3], const float normal[3], float coef[4] )
+{
+    kvs::SlicePlane sp;
+    sp.setPlane( kvs::Vec3( point[0], point[1], point[2] ), kvs::Vec3( normal[0], normal[1], normal[2] ) );
+    const kvs::Vec3 p( point[0], point[1], point[2] ), n( normal[0], normal[1], normal[2] );
+    coef[0] = n[0]; coef[1] = n[1]; coef[2] = n[2]; coef[3] = -p.dot( n );
+}
+
+struct ref_uvolume
+{
+    const void* values;
+    int32_t dtype;
+    uint64_t nnodes, ncells;
+    int32_t cell_type;                 // kvs::UnstructuredVolumeObject::CellType
+    const float* coords;               // 3 * nnodes
+    const uint32_t* connections;       // nodes per cell * ncells
+    int32_t has_min_max_values;
+    double min_value, max_value;
+};
+
+static kvs::UnstructuredVolumeObject* MakeUVolume( const ref_uvolume* v )
+{
+    auto* vol = new kvs::UnstructuredVolumeObject();
+    vol->setCellType( kvs::UnstructuredVolumeObject::CellType( v->cell_type ) );
+    vol->setVeclen( 1 );
+    vol->setNumberOfNodes( v->nnodes );
+    vol->setNumberOfCells( v->ncells );
+    vol->setValues( MakeAny( v->values, v->dtype, v->nnodes ) );
+    kvs::ValueArray<kvs::Real32> coords( 3 * v->nnodes );
+    std::memcpy( coords.data(), v->coords, 3 * v->nnodes * sizeof( float ) );
+    vol->setCoords( coords );
+    const size_t per = vol->numberOfCellNodes();
+    kvs::ValueArray<kvs::UInt32> conn( per * v->ncells );
+    std::memcpy( conn.data(), v->connections, per * v->ncells * sizeof( uint32_t ) );
+    vol->setConnections( conn );
+    if ( v->has_min_max_values ) vol->setMinMaxValues( v->min_value, v->max_value );
+    vol->updateMinMaxCoords();
+    return vol;
+}
+
+static void HarvestU( const kvs::PolygonObject* p, const kvs::UnstructuredVolumeObject* vol, int success, ref_mesh* out )
+{
+    std::memset( out, 0, sizeof( *out ) );
+    out->coords = static_cast<float*>( CopyOut( p->coords(), &out->n_coords ) );
+    out->connections = static_cast<uint32_t*>( CopyOut( p->connections(), &out->n_connections ) );
+    out->normals = static_cast<float*>( CopyOut( p->normals(), &out->n_normals ) );
+    out->colors = static_cast<uint8_t*>( CopyOut( p->colors(), &out->n_colors ) );
+    out->opacities = static_cast<uint8_t*>( CopyOut( p->opacities(), &out->n_opacities ) );
+    out->polygon_type = p->polygonType();
+    out->normal_type = p->normalType();
+    out->color_type = p->colorType();
+    out->success = success;
+    out->volume_min_value = vol->minValue();
+    out->volume_max_value = vol->maxValue();
+    for ( int k = 0; k < 3; ++k )
+    {
+        out->min_obj[k] = p->minObjectCoord()[k];
+        out->max_obj[k] = p->maxObjectCoord()[k];
+    }
+}
+
+// kvs::Isosurface( volume, isolevel, normal_type, duplication, tf ) over an
+// unstructured volume: MarchingTetrahedra / Hexahedra / Pyramid / Prism by
+// cell type (Isosurface.cpp:192-257).
+int ref_isosurface_unstructured( const ref_uvolume* v, double isolevel, int normal_type, int duplication,
+                                 int tf_kind, int tf_resolution, ref_mesh* out )
+{
+    auto* vol = MakeUVolume( v );
+    const auto t0 = std::chrono::steady_clock::now();
+    auto* s = new kvs::Isosurface( vol, isolevel, kvs::PolygonObject::NormalType( normal_type ),
+                                   duplication != 0, MakeTF( tf_kind, tf_resolution ) );
+    const auto t1 = std::chrono::steady_clock::now();
+    HarvestU( s, vol, s->isSuccess() ? 1 : 0, out );
+    out->seconds = std::chrono::duration<double>( t1 - t0 ).count();
+    delete s;
+    delete vol;
+    return 0;
+}
+
+// kvs::SlicePlane( volume, coefficients, tf ) over an unstructured volume.
+int ref_slice_plane_unstructured( const ref_uvolume* v, const float coef[4], int tf_kind, int tf_resolution,
+                                  ref_mesh* out )
+{
+    auto* vol = MakeUVolume( v );
+    auto* sp = new kvs::SlicePlane( vol, kvs::Vec4( coef[0], coef[1], coef[2], coef[3] ), MakeTF( tf_kind, tf_resolution ) );
+    HarvestU( sp, vol, sp->isSuccess() ? 1 : 0, out );
+    delete sp;
+    delete vol;
+    return 0;
+}
+
+// The unstructured marching tables (triangle edge lists and edge endpoints)
+// for pinning transcriptions: cell 0 tetrahedra [16][7] / [6][2], 1
+// hexahedra [256][16] / [12][2], 2 pyramid [36][13] / [8][2], 3 prism
+// [64][10] / [9][2].  Returns the number of int32 written to tri.
+int ref_unstructured_tables( int cell, int32_t* tri, int32_t* vert )
+{
+    switch ( cell )
+    {
+    case 0:
+        std::memcpy( tri, kvs::MarchingTetrahedraTable::TriangleID, sizeof( kvs::MarchingTetrahedraTable::TriangleID ) );
+        std::memcpy( vert, kvs::MarchingTetrahedraTable::VertexID, sizeof( kvs::MarchingTetrahedraTable::VertexID ) );
+        return int( sizeof( kvs::MarchingTetrahedraTable::TriangleID ) / 4 );
+    case 1:
+        std::memcpy( tri, kvs::MarchingHexahedraTable::TriangleID, sizeof( kvs::MarchingHexahedraTable::TriangleID ) );
+        std::memcpy( vert, kvs::MarchingHexahedraTable::VertexID, sizeof( kvs::MarchingHexahedraTable::VertexID ) );
+        return int( sizeof( kvs::MarchingHexahedraTable::TriangleID ) / 4 );
+    case 2:
+        std::memcpy( tri, kvs::MarchingPyramidTable::TriangleID, sizeof( kvs::MarchingPyramidTable::TriangleID ) );
+        std::memcpy( vert, kvs::MarchingPyramidTable::VertexID, sizeof( kvs::MarchingPyramidTable::VertexID ) );
+        return int( sizeof( kvs::MarchingPyramidTable::TriangleID ) / 4 );
+    default:
+        std::memcpy( tri, kvs::MarchingPrismTable::TriangleID, sizeof( kvs::MarchingPrismTable::TriangleID ) );
+        std::memcpy( vert, kvs::MarchingPrismTable::VertexID, sizeof( kvs::MarchingPrismTable::VertexID ) );
+        return int( sizeof( kvs::MarchingPrismTable::TriangleID ) / 4 );
+    }
+}
+
+// StructuredVolumeObject::write( filename, ascii, external ) (KVSML) of a
+// uniform volume, or of a rectilinear one when rect_coords is given
+// (nx + ny + nz floats).
+int ref_write_kvsml_volume( const ref_volume* v, const float* rect_coords, const char* filename, int ascii,
+                            int external )
+{
+    auto* vol = MakeVolume( v );
+    if ( rect_coords )
+    {
+        vol->setGridTypeToRectilinear();
+        const size_t n = size_t( v->dims[0] ) + v->dims[1] + v->dims[2];
+        kvs::ValueArray<kvs::Real32> c( n );
+        std::memcpy( c.data(), rect_coords, n * sizeof( float ) );
+        vol->setCoords( c );
+    }
+    if ( !v->has_min_max_values ) vol->updateMinMaxValues();
+    const bool ok = vol->write( filename, ascii != 0, external != 0 );
+    delete vol;
+    return ok ? 0 : 1;
+}
+
+struct ref_read_volume
+{
+    void* values;                 // malloc'ed copy of the value array
+    uint64_t nbytes;
+    int32_t dtype;                // kvs::Type::TypeID order
+    uint32_t dims[3];
+    uint32_t veclen;
+    int32_t grid_type;            // kvs::StructuredVolumeObject::GridType
+    double min_value, max_value;
+    float min_obj[3], max_obj[3];
+};
+
+// StructuredVolumeObject::read( filename ) (KVSML): what the reference's
+// reader makes of a file.
+int ref_read_kvsml_volume( const char* filename, ref_read_volume* out )
+{
+    std::memset( out, 0, sizeof( *out ) );
+    kvs::StructuredVolumeObject vol;
+    if ( !vol.read( filename ) ) return 1;
+    const auto& a = vol.values();
+    out->nbytes = a.byteSize();
+    out->values = std::malloc( out->nbytes ? out->nbytes : 1 );
+    std::memcpy( out->values, a.data(), out->nbytes );
+    out->dtype = int32_t( a.typeID() ) - int32_t( kvs::Type::TypeInt8 );
+    for ( int k = 0; k < 3; ++k ) out->dims[k] = vol.resolution()[k];
+    out->veclen = uint32_t( vol.veclen() );
+    out->grid_type = int32_t( vol.gridType() );
+    out->min_value = vol.minValue();
+    out->max_value = vol.maxValue();
+    for ( int k = 0; k < 3; ++k )
+    {
+        out->min_obj[k] = vol.minObjectCoord()[k];
+        out->max_obj[k] = vol.maxObjectCoord()[k];
+    }
+    return 0;
+}
+
+void ref_read_volume_free( ref_read_volume* r ) { std::free( r->values ); std::memset( r, 0, sizeof( *r ) ); }
+
+// PolygonExporter<kvs::Stl>( polygon ) and Stl::write( filename ) for a
+// triangle polygon with the given arrays and normal type.
+int ref_export_stl( const float* coords, uint64_t ncoords, const uint32_t* conn, uint64_t nconn,
+                    const float* normals, uint64_t nnormals, int normal_type, int ascii, const char* filename )
+{
+    kvs::PolygonObject p;
+    kvs::ValueArray<kvs::Real32> c( ncoords ); std::memcpy( c.data(), coords, ncoords * 4 );
+    kvs::ValueArray<kvs::UInt32> k( nconn ); if ( nconn ) std::memcpy( k.data(), conn, nconn * 4 );
+    kvs::ValueArray<kvs::Real32> n( nnormals ); if ( nnormals ) std::memcpy( n.data(), normals, nnormals * 4 );
+    p.setCoords( c );
+    if ( nconn ) p.setConnections( k );
+    p.setNormals( n );
+    p.setPolygonType( kvs::PolygonObject::Triangle );
+    p.setNormalType( kvs::PolygonObject::NormalType( normal_type ) );
+    p.setColorType( kvs::PolygonObject::PolygonColor );
+    p.setColor( kvs::RGBColor( 255, 255, 255 ) );
+    kvs::PolygonExporter<kvs::Stl> e( &p );
+    e.setFileType( ascii ? kvs::Stl::Ascii : kvs::Stl::Binary );
+    return e.write( filename ) ? 0 : 1;
 }
 
 // MarchingCubesTable::TriangleID and VertexID, for pinning transcriptions.

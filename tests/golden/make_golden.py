@@ -8,8 +8,8 @@ from its sources by oracle/Makefile -- exists:
 
 Every expected output below comes from that library; inputs are either the
 reference's own HydrogenVolumeData or seeded synthetic volumes.  The files are
-data only (inputs and expected outputs) and are what the GPU box checks
-against, since the reference does not exist there.
+data only (inputs and expected outputs) and are what the GPU tests check
+against, since the reference is not available where they run.
 """
 import hashlib
 import json
@@ -151,7 +151,7 @@ def main():
     r = oracle.ref_isosurface(h, None, 1, True)
     meta["hydrogen64"]["isosurface_default_level"] = digest(r)
 
-    # 5. the north star's CPU configuration: 64^3 float32, 3 Gaussian blobs + noise,
+    # 5. the 64^3 CPU configuration: 64^3 float32, 3 Gaussian blobs + noise,
     #    isolevel 0.5, PolygonNormal, duplication, default transfer function
     c0 = datasets.blobs(64, seed=1000)
     meta["c0"] = {"sha256": sha(c0), "levels": {}}
@@ -236,6 +236,166 @@ def main():
             cases.append({"name": name, "iso": iso, "dtype": v.dtype.name, "shape": list(shape), "min_max": mm})
     pk.save(os.path.join(HERE, "bigint.npz"))
     meta["bigint"] = cases
+
+    # 9. kvs::SlicePlane over structured volumes: every dtype (the colour value
+    #    follows C++'s promotions, unsigned differences wrap), planes given by
+    #    coefficients and by point / normal, two colour maps, user min/max and
+    #    object coordinates
+    rng = np.random.default_rng(91)
+    pk = Packer()
+    cases = []
+    for t in range(24):
+        dtype = DTYPES[t % len(DTYPES)]
+        shape = tuple(int(s) for s in rng.integers(3, 16, 3))
+        if np.dtype(dtype).kind == "f":
+            v = (rng.standard_normal(shape) * 40).astype(dtype)
+        else:
+            info = np.iinfo(dtype)
+            v = rng.integers(max(int(info.min), -120), min(int(info.max), 250) + 1, shape).astype(dtype)
+        nz, ny, nx = shape
+        if t % 3 == 0:
+            point = [float(np.float32(rng.uniform(0, nx - 1))), float(np.float32(rng.uniform(0, ny - 1))),
+                     float(np.float32(rng.uniform(0, nz - 1)))]
+            normal = [float(np.float32(x)) for x in rng.standard_normal(3)]
+            coef = oracle.ref_slice_plane_coefficients(point, normal)
+            plane = {"point": point, "normal": normal}
+        elif t % 3 == 1:
+            # an axis-aligned plane through nodes: ties (plane == 0) count as below
+            coef = [0.0, 0.0, 1.0, -float(nz // 2)]
+            plane = {"coefficients": coef}
+        else:
+            coef = [float(np.float32(x)) for x in rng.standard_normal(3)] + [float(np.float32(rng.uniform(-5, 5)))]
+            plane = {"coefficients": coef}
+        mm = None
+        coords = None
+        if t % 4 == 1:
+            lo, hi = float(v.min()), float(v.max())
+            mm = (lo + 0.1 * (hi - lo), hi - 0.1 * (hi - lo))
+        if t % 4 == 2:
+            coords = ([float(rng.uniform(-3, 3)) for _ in range(3)], [float(rng.uniform(4, 9)) for _ in range(3)])
+        kind = t % 2
+        r = oracle.ref_slice_plane(v, coef, tf_kind=kind, min_max_values=mm, coords=coords)
+        pk.add("in_bytes", np.frombuffer(v.tobytes(), dtype=np.uint8))
+        pk.add("coords", r["coords"])
+        pk.add("normals", r["normals"])
+        pk.add("colors", r["colors"])
+        cases.append({"dtype": np.dtype(dtype).name, "shape": list(shape), "coefficients": [float(c) for c in coef],
+                      "plane": plane, "tf_kind": kind, "min_max": mm, "coords": coords,
+                      "color_type": r["color_type"], "normal_type": r["normal_type"],
+                      "polygon_type": r["polygon_type"], "opacities": [int(o) for o in r["opacities"]],
+                      "volume_min_max": [r["volume_min_value"], r["volume_max_value"]]})
+    pk.save(os.path.join(HERE, "slice.npz"))
+    meta["slice"] = cases
+
+    # 10. the unstructured branch of kvs::Isosurface (MarchingTetrahedra /
+    #     Hexahedra / Pyramid / Prism) and kvs::SlicePlane over unstructured
+    #     cells; their case tables pinned by hash
+    meta["cell_tables"] = {}
+    for cell, name in enumerate(("tetrahedra", "hexahedra", "pyramid", "prism")):
+        tri, vert = oracle.ref_unstructured_tables(cell)
+        meta["cell_tables"][name] = {"TriangleID_sha256": sha(tri.astype("<i4")),
+                                     "VertexID_sha256": sha(vert.astype("<i4")),
+                                     "TriangleID_shape": list(tri.shape), "VertexID_shape": list(vert.shape)}
+    pk = Packer()
+    cases = []
+    for ci, cell_type in enumerate((datasets.TETRAHEDRA, datasets.HEXAHEDRA, datasets.PYRAMID, datasets.PRISM)):
+        for t, dtype in enumerate((np.float32, np.float64, np.uint8, np.int16, np.uint32)):
+            dims = (5 + t, 4 + (t + ci) % 3, 6 - t % 2)
+            v, c, k = datasets.unstructured_mesh(dims, cell_type, seed=100 * ci + t, dtype=dtype)
+            hi = float(v.max())
+            iso = 0.45 * hi if np.dtype(dtype).kind == "f" else float(int(0.45 * hi)) + 0.5
+            mm = None
+            if t == 3:
+                lo = float(v.min())
+                mm = (lo + 0.1 * (hi - lo), hi - 0.1 * (hi - lo))
+            entry = {"cell_type": cell_type, "dtype": np.dtype(dtype).name, "dims": list(dims), "iso": iso,
+                     "min_max": mm, "n_nodes": int(v.size), "n_conn": int(k.size)}
+            pk.add("values", np.frombuffer(v.tobytes(), dtype=np.uint8))
+            pk.add("coords", c)
+            pk.add("conn", k)
+            for normal, dup in MODES:
+                r = oracle.ref_unstructured_isosurface(v, c, k, cell_type, iso, normal, dup, min_max_values=mm)
+                mk = f"n{normal}_d{int(dup)}"
+                pk.add(mk + "_coords", r["coords"])
+                pk.add(mk + "_normals", r["normals"])
+                entry[mk] = {"polygon_type": r["polygon_type"], "normal_type": r["normal_type"],
+                             "color_type": r["color_type"], "colors": [int(x) for x in r["colors"]],
+                             "opacities": [int(x) for x in r["opacities"]]}
+            entry["volume_min_max"] = [r["volume_min_value"], r["volume_max_value"]]
+            entry["min_obj"], entry["max_obj"] = r["min_obj"], r["max_obj"]
+            coef = [0.3 - 0.1 * t, -0.2, 0.5 + 0.05 * ci, -0.4 * dims[2]]
+            coef = [float(np.float32(x)) for x in coef]
+            r = oracle.ref_unstructured_slice_plane(v, c, k, cell_type, coef, tf_kind=t % 2, min_max_values=mm)
+            pk.add("slice_coords", r["coords"])
+            pk.add("slice_normals", r["normals"])
+            pk.add("slice_colors", r["colors"])
+            entry["slice"] = {"coefficients": coef, "tf_kind": t % 2, "color_type": r["color_type"]}
+            cases.append(entry)
+    pk.save(os.path.join(HERE, "unstructured.npz"))
+    meta["unstructured"] = cases
+
+    # 11. KVSML volume files written by the reference's own writer
+    #     (StructuredVolumeObject::write), and what its reader makes of them;
+    #     two hand-edited variants: a big-endian copy and an object_coord attribute
+    kdir = os.path.join(HERE, "kvsml")
+    os.makedirs(kdir, exist_ok=True)
+    rng = np.random.default_rng(111)
+    files = []
+    blob = datasets.blobs((23, 17, 11), seed=5)
+    specs = [
+        ("f32_binary", blob, False, True, None),
+        ("u8_ascii_internal", rng.integers(0, 256, (6, 7, 9)).astype(np.uint8), True, False, None),
+        ("i16_ascii_external", rng.integers(-300, 300, (5, 8, 7)).astype(np.int16), True, True, None),
+        ("f64_binary", (rng.standard_normal((9, 6, 8)) * 3).astype(np.float64), False, True, None),
+        ("f32_ascii_internal", (rng.standard_normal((4, 5, 6)) * 7).astype(np.float32), True, False, None),
+        ("u16_binary_rectilinear", rng.integers(0, 4000, (7, 6, 5)).astype(np.uint16), False, True,
+         np.concatenate([np.cumsum(rng.uniform(0.5, 2.0, n)) for n in (5, 6, 7)]).astype(np.float32)),
+    ]
+    for name, vol, ascii_, external, rect in specs:
+        path = os.path.join(kdir, name + ".kvsml")
+        oracle.ref_write_kvsml_volume(vol, path, ascii=ascii_, external=external, rect_coords=rect)
+        files.append(name)
+    # big-endian copy of f32_binary: byte-swapped data, endian="big"
+    with open(os.path.join(kdir, "f32_binary.kvsml")) as f:
+        text = f.read()
+    dat = [t.split('"')[0] for t in text.split('file="')[1:]][0]
+    raw = np.fromfile(os.path.join(kdir, dat), dtype="<f4")
+    raw.astype(">f4").tofile(os.path.join(kdir, "f32_big_value.dat"))
+    with open(os.path.join(kdir, "f32_big.kvsml"), "w") as f:
+        f.write(text.replace(dat, "f32_big_value.dat").replace('endian="little"', 'endian="big"'))
+    files.append("f32_big")
+    # object_coord on the Object tag
+    with open(os.path.join(kdir, "f32_ascii_internal.kvsml")) as f:
+        text = f.read()
+    with open(os.path.join(kdir, "f32_object_coord.kvsml"), "w") as f:
+        f.write(text.replace('type="StructuredVolumeObject"',
+                             'type="StructuredVolumeObject" object_coord="-1.5 0.25 2 4.5 8.75 9.125"', 1))
+    files.append("f32_object_coord")
+    pk = Packer()
+    cases = []
+    for name in files:
+        r = oracle.ref_read_kvsml_volume(os.path.join(kdir, name + ".kvsml"))
+        pk.add("values", np.frombuffer(r["values"].tobytes(), dtype=np.uint8))
+        cases.append({"name": name, "dtype": r["values"].dtype.name, "shape": list(r["values"].shape),
+                      "min_value": r["min_value"], "max_value": r["max_value"], "grid_type": r["grid_type"],
+                      "min_obj": r["min_obj"], "max_obj": r["max_obj"]})
+    pk.save(os.path.join(HERE, "kvsml.npz"))
+    meta["kvsml"] = cases
+
+    # 12. STL files the reference's PolygonExporter<kvs::Stl> + Stl::write make
+    #     of marching-cubes surfaces in the three modes, binary and ascii
+    sdir = os.path.join(HERE, "stl")
+    os.makedirs(sdir, exist_ok=True)
+    stl = []
+    for vol_name, vol, iso, ascii_ in (("blobs20", datasets.blobs(20, seed=6), 0.45, False),
+                                        ("blobs10", datasets.blobs(10, seed=7, nblobs=2), 0.4, True)):
+        for normal, dup in MODES:
+            r = oracle.ref_marching_cubes(vol, iso, normal, dup)
+            name = f"{vol_name}_n{normal}_d{int(dup)}_{'ascii' if ascii_ else 'binary'}.stl"
+            oracle.ref_export_stl(r, r["normal_type"], os.path.join(sdir, name), ascii=ascii_)
+            stl.append({"file": name, "volume": vol_name, "iso": iso, "normal": normal, "dup": dup, "ascii": ascii_,
+                        "sha256": hashlib.sha256(open(os.path.join(sdir, name), "rb").read()).hexdigest()})
+    meta["stl"] = stl
 
     with open(os.path.join(HERE, "golden.json"), "w") as f:
         json.dump(meta, f, indent=1, sort_keys=True)

@@ -82,6 +82,35 @@ def bigint_cases():
         yield c["name"], v, c["iso"], mm, outs
 
 
+def slice_cases():
+    """Yield (case, values, (coords, normals, colors)) for the kvs::SlicePlane fixtures."""
+    m = meta()["slice"]
+    p = Packed("slice.npz")
+    for i, c in enumerate(m):
+        dt = np.dtype(c["dtype"])
+        v = np.frombuffer(p.get("in_bytes", i).tobytes(), dtype=dt).reshape(c["shape"])
+        yield c, v, (p.get("coords", i).astype(np.float32), p.get("normals", i).astype(np.float32),
+                     p.get("colors", i).astype(np.uint8))
+
+
+def unstructured_cases():
+    """Yield (case, values, coords, connections, {mode: (coords, normals)},
+    (slice coords, slice normals, slice colors)) for the unstructured fixtures."""
+    m = meta()["unstructured"]
+    p = Packed("unstructured.npz")
+    for i, c in enumerate(m):
+        v = np.frombuffer(p.get("values", i).tobytes(), dtype=np.dtype(c["dtype"]))
+        co = p.get("coords", i).astype(np.float32).reshape(-1, 3)
+        k = p.get("conn", i).astype(np.uint32)
+        outs = {}
+        for normal, dup in MODES:
+            mk = f"n{normal}_d{int(dup)}"
+            outs[(normal, dup)] = (p.get(mk + "_coords", i).astype(np.float32), p.get(mk + "_normals", i).astype(np.float32))
+        sl = (p.get("slice_coords", i).astype(np.float32), p.get("slice_normals", i).astype(np.float32),
+              p.get("slice_colors", i).astype(np.uint8))
+        yield c, v, co, k, outs, sl
+
+
 def hydrogen():
     return np.load(os.path.join(GOLDEN, "hydrogen64.npz"))["values"]
 

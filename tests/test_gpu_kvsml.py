@@ -1,0 +1,77 @@
+"""KVSML volume ingestion (kmc_kvsml_read_volume) against what the reference's
+own reader (StructuredVolumeObject::read) makes of files its writer produced
+(tests/golden/kvsml/): values bit for bit, min/max values (the file's rounded
+attributes when present) and min/max object coordinates, into device and
+host memory; and the isosurface of the device-resident volume against the
+oracle."""
+import ctypes
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from kvs_amd import kmc
+from tests import golden_io
+from tests.conftest import assert_same_surface, bits_equal
+
+pytestmark = pytest.mark.gpu
+
+KDIR = os.path.join(golden_io.GOLDEN, "kvsml")
+
+
+def cases():
+    p = golden_io.Packed("kvsml.npz")
+    for i, c in enumerate(golden_io.meta()["kvsml"]):
+        v = np.frombuffer(p.get("values", i).tobytes(), dtype=np.dtype(c["dtype"])).reshape(c["shape"])
+        yield c, v
+
+
+@pytest.mark.parametrize("memory", [kmc.MEMORY_DEVICE, kmc.MEMORY_HOST])
+def test_kvsml_read_matches_reference_reader(memory):
+    for c, want in cases():
+        print("case", c["name"], memory, flush=True)
+        kv = kmc.KvsmlVolume(os.path.join(KDIR, c["name"] + ".kvsml"), memory=memory)
+        try:
+            v = kv.volume
+            assert v.memory == memory and v.kind == kmc.STRUCTURED and v.veclen == 1
+            assert list(v.resolution) == c["shape"][::-1]
+            assert bits_equal(kv.values(), want), c["name"]
+            assert (v.min_value, v.max_value) == (c["min_value"], c["max_value"]), c["name"]
+            assert list(v.min_coord) == c["min_obj"] and list(v.max_coord) == c["max_obj"], c["name"]
+        finally:
+            kv.close()
+
+
+def test_isosurface_of_a_kvsml_volume_in_hbm():
+    lib = kmc.load()
+    for c, want in cases():
+        kv = kmc.KvsmlVolume(os.path.join(KDIR, c["name"] + ".kvsml"))
+        try:
+            v = kv.volume
+            lo, hi = c["min_value"], c["max_value"]
+            iso = 0.5 * (lo + hi)
+            if np.dtype(c["dtype"]).kind in "ui":
+                iso = float(np.floor(iso)) + 0.5
+            for normal, dup in [(kmc.POLYGON_NORMAL, True), (kmc.VERTEX_NORMAL, False)]:
+                s = kmc.Surface()
+                st = lib.kmc_extract(ctypes.byref(v), iso, normal, int(dup), kmc.MEMORY_HOST, None, ctypes.byref(s))
+                assert st == kmc.OK
+                try:
+                    got = kmc.surface_to_numpy(s)
+                finally:
+                    lib.kmc_surface_release(ctypes.byref(s))
+                exp = oracle.extract(want, iso, normal, dup, min_max_values=(lo, hi),
+                                     coords=(c["min_obj"], c["max_obj"]))
+                assert_same_surface(got, exp, f"{c['name']} mode {normal},{dup}")
+        finally:
+            kv.close()
+
+
+def test_kvsml_errors():
+    with pytest.raises(kmc.KmcError) as e:
+        kmc.KvsmlVolume(os.path.join(KDIR, "missing.kvsml"))
+    assert e.value.status == kmc.ERR_FILE
+    with pytest.raises(kmc.KmcError) as e:
+        kmc.KvsmlVolume(os.path.join(KDIR, "f32_binary_value.dat"))   # not a KVSML extension
+    assert e.value.status == kmc.ERR_FILE

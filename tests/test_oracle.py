@@ -198,3 +198,176 @@ def test_fp64_reevaluation_error_is_small():
     err = np.abs(r["coords"].astype(np.float64) - r["coords_f64"])
     scale = np.maximum(np.abs(r["coords_f64"]), 1.0)
     assert (err / scale).max() < 1e-6
+
+
+def _slice_color_inputs(case, v):
+    """The colour map the fixture's transfer function used and its range."""
+    table = np.array(golden_io.meta()["colormap_tables"][("BrewerRdBu", "BrewerSingleHueGreys")[case["tf_kind"]] + "_256"],
+                     dtype=np.uint8)
+    mm = tuple(case["min_max"]) if case["min_max"] is not None else None
+    return table, oracle.tf_range(v, mm)
+
+
+def test_slice_plane_against_golden():
+    # kvs::SlicePlane restated (oracle/slice_oracle.c) against the reference's
+    # own output: coordinates, negated face normals and per-vertex colours
+    for case, v, (co, no, cl) in golden_io.slice_cases():
+        table, rng = _slice_color_inputs(case, v)
+        r = oracle.slice_plane(v, case["coefficients"], table, color_range=rng, coords=case["coords"])
+        label = (case["dtype"], case["plane"])
+        assert bits_equal(r["coords"], co), label
+        assert bits_equal(r["normals"], no), label
+        assert np.array_equal(r["colors"], cl), label
+        assert case["color_type"] == 0 and case["normal_type"] == 1 and case["polygon_type"] == 3
+
+
+def test_slice_plane_axis_aligned_closed_form():
+    # the plane z = k + 0.5 through an n^3 grid: one quad (two triangles) per
+    # cell of the slab, every vertex at z = k + 0.5, every normal along -z or +z
+    n = 7
+    v = datasets.blobs(n, seed=2)
+    table = np.arange(3 * 256, dtype=np.uint8)
+    for k in range(n - 1):
+        r = oracle.slice_plane(v, [0.0, 0.0, 1.0, -(k + 0.5)], table)
+        c = r["coords"].reshape(-1, 3)
+        assert r["n_triangles"] == 2 * (n - 1) * (n - 1)
+        assert np.all(c[:, 2] == np.float32(k + 0.5))
+        nrm = r["normals"].reshape(-1, 3)
+        assert np.all(nrm[:, 0] == 0) and np.all(nrm[:, 1] == 0) and np.all(nrm[:, 2] != 0)
+
+
+@pytest.mark.skipif(not oracle.ref_available(), reason="reference library not built here")
+def test_slice_plane_equals_reference_fresh():
+    rng = np.random.default_rng(17)
+    for t in range(12):
+        v = datasets.turbulence((int(rng.integers(5, 30)), int(rng.integers(5, 30)), int(rng.integers(5, 30))), seed=t)
+        coef = [float(np.float32(x)) for x in rng.standard_normal(4)]
+        coef[3] *= 10
+        for kind in (0, 1):
+            a = oracle.slice_plane(v, coef, oracle.ref_tf_table(kind))
+            b = oracle.ref_slice_plane(v, coef, tf_kind=kind)
+            assert bits_equal(a["coords"], b["coords"]) and bits_equal(a["normals"], b["normals"]), (t, kind)
+            assert np.array_equal(a["colors"], b["colors"]), (t, kind)
+
+
+CELL_NAMES = ("tetrahedra", "hexahedra", "pyramid", "prism")
+CELL_SHORT = ("Tet", "Hex", "Pyr", "Prism")
+
+
+def _decode_device_cell_tables():
+    import os
+    import re
+    path = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "kvs_amd", "csrc", "cell_tables.h")
+    text = open(path).read()
+    out = {}
+    for short in CELL_SHORT:
+        body = text[text.index(f"k{short}Tri["):text.index(f"k{short}Edge[")]
+        words = [int(w, 16) for w in re.findall(r"0x([0-9a-f]{16})ull", body)]
+        edge_line = text[text.index(f"k{short}Edge["):].split("\n")[0]
+        edges = [int(b, 16) for b in re.findall(r"0x([0-9a-f]{2})", edge_line)]
+        out[short] = (words, edges)
+    return out
+
+
+def _decode_oracle_cell_tables():
+    import os
+    import re
+    path = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle", "cell_tables_oracle.h")
+    text = open(path).read()
+    out = {}
+    for short in CELL_SHORT:
+        body = text[text.index(f"kOracle{short}Cases["):text.index(f"kOracle{short}Edges")]
+        cases = re.findall(r'"([0-9a-f]*)"', body)
+        edges = re.search(rf'kOracle{short}Edges = "([0-9]*)"', text).group(1)
+        out[short] = (cases, edges)
+    return out
+
+
+def test_unstructured_tables_match_reference_hash():
+    """Both transcriptions of the unstructured case tables -- the device words of
+    kvs_amd/csrc/cell_tables.h and the oracle's strings -- decode to the
+    reference's TriangleID / VertexID arrays (pinned by SHA-256)."""
+    meta = golden_io.meta()["cell_tables"]
+    dev, orc = _decode_device_cell_tables(), _decode_oracle_cell_tables()
+    for name, short in zip(CELL_NAMES, CELL_SHORT):
+        m = meta[name]
+        rows, width = m["TriangleID_shape"]
+        words, edges = dev[short]
+        tri = np.full((rows, width), -1, dtype="<i4")
+        for i, w in enumerate(words):
+            n = w >> 60
+            assert (w >> (12 * n)) & ((1 << (60 - 12 * n)) - 1) == 0, (name, i)
+            tri[i, :3 * n] = [(w >> (4 * j)) & 0xF for j in range(3 * n)]
+        assert len(words) == rows
+        assert golden_io.sha(tri) == m["TriangleID_sha256"], name
+        vert = np.array([[e & 0xF, e >> 4] for e in edges], dtype="<i4")
+        assert golden_io.sha(vert) == m["VertexID_sha256"], name
+        cases, estr = orc[short]
+        tri2 = np.full((rows, width), -1, dtype="<i4")
+        for i, c in enumerate(cases):
+            tri2[i, :len(c)] = [int(ch, 16) for ch in c]
+        assert golden_io.sha(tri2) == m["TriangleID_sha256"], name
+        vert2 = np.array([[int(estr[2 * e]), int(estr[2 * e + 1])] for e in range(len(estr) // 2)], dtype="<i4")
+        assert golden_io.sha(vert2) == m["VertexID_sha256"], name
+
+
+def test_unstructured_against_golden():
+    # MarchingTetrahedra / Hexahedra / Pyramid / Prism and the unstructured
+    # SlicePlane restated (oracle/unstructured_oracle.c) against the reference
+    for case, v, co, k, outs, (sc, sn, scl) in golden_io.unstructured_cases():
+        ct = case["cell_type"]
+        mm = tuple(case["min_max"]) if case["min_max"] is not None else None
+        label = (ct, case["dtype"], case["dims"])
+        for (normal, dup), (want_c, want_n) in outs.items():
+            r = oracle.unstructured_isosurface(v, co, k, ct, case["iso"], normal, dup, min_max_values=mm)
+            assert bits_equal(r["coords"], want_c), label + (normal, dup)
+            assert bits_equal(r["normals"], want_n), label + (normal, dup)
+        assert outs[(1, True)][0].size > 0, label
+        table = np.array(golden_io.meta()["colormap_tables"][("BrewerRdBu", "BrewerSingleHueGreys")[case["slice"]["tf_kind"]] + "_256"],
+                         dtype=np.uint8)
+        r = oracle.unstructured_slice_plane(v, co, k, ct, case["slice"]["coefficients"], table, min_max_values=mm)
+        assert bits_equal(r["coords"], sc) and bits_equal(r["normals"], sn), label
+        assert np.array_equal(r["colors"], scl), label
+        assert list(oracle.unstructured_min_max(v, k)) == case["volume_min_max"] or mm is not None, label
+        lo, hi = oracle.unstructured_min_max_coords(co, k)
+        assert lo == case["min_obj"] and hi == case["max_obj"], label
+
+
+def test_unstructured_min_max_connection_order():
+    # seeded with values[0] even when node 0 is in no cell; NaN handling of the
+    # serial Math::Min / Math::Max scan; coordinates restart after a NaN
+    v = np.array([5.0, 1.0, 2.0, 3.0, 4.0], np.float32)
+    k = np.array([1, 2, 3, 4], np.uint32)
+    assert oracle.unstructured_min_max(v, k) == (1.0, 5.0)
+    c = np.array([[0, 0, 0], [1, 2, 3], [np.nan, 5, 5], [0.5, 7, 1], [2, 1, 9]], np.float32)
+    lo, hi = oracle.unstructured_min_max_coords(c, np.array([1, 2, 3, 4], np.uint32))
+    assert lo[0] == 0.5 and hi[0] == 2.0          # x restarts after the NaN
+    assert lo[1] == 1.0 and hi[1] == 7.0 and lo[2] == 1.0 and hi[2] == 9.0
+
+
+@pytest.mark.skipif(not oracle.ref_available(), reason="reference library not built here")
+def test_unstructured_equals_reference_fresh():
+    for ct in (datasets.TETRAHEDRA, datasets.HEXAHEDRA, datasets.PYRAMID, datasets.PRISM):
+        for seed in range(3):
+            v, c, k = datasets.unstructured_mesh((7 + seed, 6, 5), ct, seed=seed + 50)
+            for iso in (0.25, 0.5):
+                a = oracle.unstructured_isosurface(v, c, k, ct, iso)
+                b = oracle.ref_unstructured_isosurface(v, c, k, ct, iso)
+                assert bits_equal(a["coords"], b["coords"]) and bits_equal(a["normals"], b["normals"]), (ct, seed, iso)
+
+
+def test_stl_fixtures_hash_and_layout():
+    # the committed STL files are the ones make_golden.py recorded; the binary
+    # ones hold 80 + 4 + 50 * T bytes for the triangles of the oracle's surface
+    import hashlib
+    import os
+    for case in golden_io.meta()["stl"]:
+        data = open(os.path.join(golden_io.GOLDEN, "stl", case["file"]), "rb").read()
+        assert hashlib.sha256(data).hexdigest() == case["sha256"], case["file"]
+        vol = datasets.blobs(20, seed=6) if case["volume"] == "blobs20" else datasets.blobs(10, seed=7, nblobs=2)
+        r = oracle.extract(vol, case["iso"], case["normal"], case["dup"])
+        if not case["ascii"]:
+            assert data[:16] == b"Generated by KVS" and len(data) == 84 + 50 * r["n_triangles"]
+            assert int.from_bytes(data[80:84], "little") == r["n_triangles"]
+        else:
+            assert data.count(b"facet normal") == r["n_triangles"]
