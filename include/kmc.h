@@ -37,6 +37,8 @@ typedef enum kmc_status {
     KMC_ERR_BUDGET = 8,           /* serving: request does not fit the HBM budget */
     KMC_ERR_NOT_READY = 9,        /* serving: result not finished yet */
     KMC_ERR_NO_DEVICE = 10,       /* no HIP device visible */
+    KMC_ERR_SCRATCH = 11,         /* kmc_extract_noalloc: work exceeds KMC_NOALLOC_SCRATCH_BYTES */
+    KMC_ERR_CAPACITY = 12,        /* kmc_extract_noalloc: output arrays too small (sizes reported) */
     KMC_ERR_CELL_TYPE = 13,       /* unstructured: not tetrahedra / hexahedra / pyramid / prism cells */
     KMC_ERR_FILE = 14,            /* a file cannot be read or written */
     KMC_ERR_NOT_TRIANGLES = 15    /* "Input object is not triangle polygon." (PolygonExporter.cpp:80-85) */
@@ -126,6 +128,34 @@ kmc_status kmc_extract(const kmc_volume* volume, double isolevel, int32_t normal
 kmc_status kmc_extract_levels(const kmc_volume* volume, const double* isolevels, uint32_t n_levels,
                               int32_t normal_type, int32_t duplication, int32_t out_memory,
                               void* stream, kmc_surface* outs);
+
+/*
+ * kmc_extract without any device allocation.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).
+ *
+ * The volume must be in device memory; the caller passes device output arrays
+ * with their capacities in elements (floats for coords and normals, uint32
+ * for connections; connections may be NULL with duplication).  Every work
+ * array lives in one fixed device array of KMC_NOALLOC_SCRATCH_BYTES that the
+ * library holds on each GPU it runs on until the process ends:
+ *   - KMC_ERR_SCRATCH when the volume's work (about 48 bytes per 64-node chunk
+ *     and 24 per node row) or the emit's records (about 8 bytes per vertex and
+ *     9 per triangle, 28 more per vertex with VertexNormal) do not fit;
+ *   - KMC_ERR_CAPACITY when the outputs do not fit: *out then holds the sizes
+ *     (n_vertices, n_triangles, n_connections, n_normals) and nothing was written.
+ * On KMC_OK *out describes the caller's arrays (release is not needed).
+ * Calls on one GPU share its scratch array: the library runs them one at a
+ * time (a per-device lock); kernels of calls made from other processes on the
+ * same GPU use their own process's array.  The call returns when the surface
+ * is complete.
+ */
+#define KMC_NOALLOC_SCRATCH_BYTES (256ull << 20)
+
+kmc_status kmc_extract_noalloc(const kmc_volume* volume, double isolevel, int32_t normal_type,
+                               int32_t duplication, void* stream, float* coords, uint64_t coords_capacity,
+                               uint32_t* connections, uint64_t connections_capacity, float* normals,
+                               uint64_t normals_capacity, kmc_surface* out);
 
 /*
  * kvs::SlicePlane over a structured volume: the drop-in for

@@ -28,7 +28,7 @@ HIP_FLAGS = [
 ]
 CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", "-I" + INCLUDE, "-I" + CSRC]
 
-HIP_SOURCES = ["kmc_kernels.hip", "kmc_api.hip", "kmc_batch.hip", "kmc_server.hip",
+HIP_SOURCES = ["kmc_kernels.hip", "kmc_api.hip", "kmc_batch.hip", "kmc_server.hip", "kmc_noalloc.hip",
                "kmc_slice.hip", "kmc_unstructured.hip",
                "kmc_kvsml.hip", "kmc_export.hip"]
 CXX_SOURCES = ["kmc_color.cpp"]

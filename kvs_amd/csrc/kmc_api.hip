@@ -744,6 +744,8 @@ const char* kmc_status_string(kmc_status s)
     case KMC_ERR_BUDGET: return "Request exceeds the memory budget.";
     case KMC_ERR_NOT_READY: return "Result not ready.";
     case KMC_ERR_NO_DEVICE: return "No HIP device available.";
+    case KMC_ERR_SCRATCH: return "Work arrays exceed the fixed scratch of kmc_extract_noalloc.";
+    case KMC_ERR_CAPACITY: return "Output arrays are too small.";
     case KMC_ERR_CELL_TYPE: return "Input volume is not tetrahedra, hexahedra, pyramid or prism-cell data.";
     case KMC_ERR_FILE: return "Cannot read or write the file.";
     case KMC_ERR_NOT_TRIANGLES: return "Input object is not triangle polygon.";

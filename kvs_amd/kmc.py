@@ -29,7 +29,7 @@ COLORMAP_DEFAULT, COLORMAP_GREYS = 0, 1
 OK = 0
 ERR_NULL_INPUT, ERR_NOT_STRUCTURED, ERR_NOT_SCALAR, ERR_UNSUPPORTED_TYPE = 1, 2, 3, 4
 ERR_INVALID_ARGUMENT, ERR_OUT_OF_MEMORY, ERR_DEVICE, ERR_BUDGET, ERR_NOT_READY, ERR_NO_DEVICE = 5, 6, 7, 8, 9, 10
-ERR_CELL_TYPE, ERR_FILE, ERR_NOT_TRIANGLES = 13, 14, 15
+ERR_SCRATCH, ERR_CAPACITY, ERR_CELL_TYPE, ERR_FILE, ERR_NOT_TRIANGLES = 11, 12, 13, 14, 15
 TETRAHEDRA, HEXAHEDRA, PYRAMID, PRISM = 1, 2, 5, 7
 
 
@@ -165,7 +165,7 @@ EXPORTED = [
     "kmc_batch_extract",
     "kmc_server_create", "kmc_server_register_volume", "kmc_server_submit", "kmc_server_poll", "kmc_server_wait",
     "kmc_server_get_stats", "kmc_server_reset_stats", "kmc_server_destroy",
-    "kmc_set_timing", "kmc_last_timing", "kmc_slice_plane",
+    "kmc_set_timing", "kmc_last_timing", "kmc_extract_noalloc", "kmc_slice_plane",
     "kmc_extract_unstructured", "kmc_slice_plane_unstructured", "kmc_kvsml_read_volume", "kmc_kvsml_volume_release",
     "kmc_surface_expand", "kmc_export_stl",
 ]
@@ -206,6 +206,10 @@ def load():
                                        ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_void_p,
                                        ctypes.POINTER(Surface)]
     lib.kmc_extract_levels.restype = ctypes.c_int
+    lib.kmc_extract_noalloc.argtypes = [ctypes.POINTER(Volume), ctypes.c_double, ctypes.c_int32, ctypes.c_int32,
+                                        ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p,
+                                        ctypes.c_uint64, ctypes.c_void_p, ctypes.c_uint64, ctypes.POINTER(Surface)]
+    lib.kmc_extract_noalloc.restype = ctypes.c_int
     lib.kmc_slice_plane.argtypes = [ctypes.POINTER(Volume), ctypes.POINTER(ctypes.c_float),
                                     ctypes.POINTER(ctypes.c_uint8), ctypes.c_uint32, ctypes.POINTER(ctypes.c_float),
                                     ctypes.c_int32, ctypes.c_void_p, ctypes.POINTER(Surface)]

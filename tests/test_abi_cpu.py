@@ -106,3 +106,20 @@ def test_surface_color_with_a_one_value_float_range():
     got = kmc.surface_color(kmc.COLORMAP_DEFAULT, 256, kmc.INT64, lo, hi, float(2 ** 62))
     first = kmc.colormap_table(kmc.COLORMAP_DEFAULT, 256)[:3]
     assert list(got) == list(first)
+
+
+def test_noalloc_refuses_work_beyond_its_scratch():
+    # the work a 2048 x 2048 x 512 volume needs (48 bytes per 64-node chunk)
+    # exceeds the fixed scratch: refused before any device call, so this holds
+    # without a GPU; the values pointer is never dereferenced
+    lib = kmc.load()
+    v = kmc.make_volume(1, dims=(2048, 2048, 512), memory=kmc.MEMORY_DEVICE, dtype=np.float32, min_max=(0.0, 1.0))
+    s = kmc.Surface()
+    st = lib.kmc_extract_noalloc(ctypes.byref(v), 0.5, kmc.POLYGON_NORMAL, 1, None, None, 0, None, 0, None, 0,
+                                 ctypes.byref(s))
+    assert st == kmc.ERR_SCRATCH
+    # host memory is refused: the call never copies (that would allocate)
+    a = np.zeros((4, 4, 4), np.float32)
+    st = lib.kmc_extract_noalloc(ctypes.byref(kmc.make_volume(a)), 0.5, kmc.POLYGON_NORMAL, 1, None, None, 0, None,
+                                 0, None, 0, ctypes.byref(s))
+    assert st == kmc.ERR_INVALID_ARGUMENT
