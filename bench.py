@@ -58,6 +58,8 @@ def main():
     ap.add_argument("--no-ct", action="store_true", help="skip the 512^3 uint8 CT-like leg")
     ap.add_argument("--no-rows", action="store_true",
                     help="skip the SlicePlane / unstructured / KVSML / STL leg")
+    ap.add_argument("--only-serving", action="store_true",
+                    help="only the serving leg (its plain calls, then the server): for profiling")
     ap.add_argument("--only-main", action="store_true",
                     help="only the 1024^3 step (no CPU baseline, rows, batched, serving, CT or FP64 legs): for profiling")
     args = ap.parse_args()
@@ -75,6 +77,10 @@ def main():
 
     lib = kmc.load()
     dev = torch.device("cuda:0")
+    if args.only_serving:
+        torch.cuda.init()
+        print(json.dumps({"serving": serving_leg(lib, kmc, datasets, torch, args)}), flush=True)
+        return
     n = args.size
     print(f"[bench] generating {n}^3 float32 turbulence field on the GPU", file=sys.stderr, flush=True)
     t0 = time.time()

@@ -11,6 +11,7 @@
  * surface is already a triangle soup and is copied as it is.
  */
 #include <hip/hip_runtime.h>
+#include <algorithm>
 #include <cstdio>
 #include <cstring>
 #include <vector>
@@ -168,15 +169,26 @@ kmc_status write_stl(const kmc_surface* soup, const char* filename, int ascii)
         }
         std::fprintf(f, "endsolid\n");
     } else {
+        /* the same bytes, assembled in blocks of facets and written with one
+         * call per block: 80-byte header, 32-bit count, per facet its normal,
+         * three corners and two zero bytes (50 bytes) */
         char header[80];
         std::memset(header, 0, 80);
         std::snprintf(header, 80, "Generated by KVS");
         const uint32_t count = (uint32_t)nt;
         ok = std::fwrite(header, 1, 80, f) == 80 && std::fwrite(&count, 4, 1, f) == 1;
-        const char unused[2] = { 0, 0 };
-        for (uint64_t i = 0; i < nt && ok; ++i)
-            ok = std::fwrite(n + 3 * i, 4, 3, f) == 3 && std::fwrite(c + 9 * i, 4, 9, f) == 9 &&
-                 std::fwrite(unused, 1, 2, f) == 2;
+        constexpr uint64_t kFacets = 1u << 16;
+        std::vector<char> buf(std::min<uint64_t>(nt, kFacets) * 50);
+        for (uint64_t i0 = 0; i0 < nt && ok; i0 += kFacets) {
+            const uint64_t i1 = std::min<uint64_t>(nt, i0 + kFacets);
+            char* o = buf.data();
+            for (uint64_t i = i0; i < i1; ++i, o += 50) {
+                std::memcpy(o, n + 3 * i, 12);
+                std::memcpy(o + 12, c + 9 * i, 36);
+                o[48] = o[49] = 0;
+            }
+            ok = std::fwrite(buf.data(), 50, i1 - i0, f) == i1 - i0;
+        }
     }
     ok = std::fclose(f) == 0 && ok;
     return ok ? KMC_OK : KMC_ERR_FILE;

@@ -1306,7 +1306,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) voi
 {
     __shared__ uint8_t ntri[256];
     stage_ntri(ntri);
-    /* row groups of 64 >> cshift rows as the units, one contiguous part per XCD */
+    /* row groups of 64 >> cshift rows as the units */
     const uint64_t G = 64u >> a.cshift, groups = (a.nrows + G - 1) / G;
     const Span sp = grid_span(0, groups, kWaves, threadIdx.x >> 6);
     const uint64_t r1 = sp.end * G < a.nrows ? sp.end * G : a.nrows;

@@ -302,6 +302,22 @@ __device__ __forceinline__ void span_merge(CoordSpan& f, const CoordSpan& g)
 
 constexpr uint32_t kSpanParts = 1024;
 
+/* Merge of the spans sh[0 .. kBlock) in order into sh[0]: a tree whose step
+ * s merges sh[i] with sh[i + s] for i a multiple of 2s (span_merge is the
+ * composition of the ranges' scans, so any bracketing of the ordered
+ * sequence gives the same result). */
+__device__ __forceinline__ void block_span_merge(CoordSpan* sh)
+{
+    for (uint32_t s = 1; s < kBlock; s <<= 1) {
+        __syncthreads();
+        if ((threadIdx.x & (2 * s - 1)) == 0) span_merge(sh[threadIdx.x], sh[threadIdx.x + s]);
+    }
+    __syncthreads();
+}
+
+/* Part b of kSpanParts: the span of connection entries [b * per, (b+1) * per),
+ * each thread a contiguous run of them (its loads unrolled so that several
+ * coordinate gathers are in flight), the runs merged in order. */
 __global__ __launch_bounds__(kBlock) void k_coord_span_partial(const float* coords, const uint32_t* conn, uint64_t nconn,
                                                              CoordSpan* parts)
 {
@@ -309,38 +325,59 @@ __global__ __launch_bounds__(kBlock) void k_coord_span_partial(const float* coor
     const uint64_t nth = (uint64_t)gridDim.x * kBlock;
     const uint64_t tid = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
     const uint64_t per = (nconn + nth - 1) / nth;
-    const uint64_t b = tid * per, e = b + per < nconn ? b + per : nconn;
+    const uint64_t b = tid * per < nconn ? tid * per : nconn, e = b + per < nconn ? b + per : nconn;
     CoordSpan s;
     s.valid = 0;
     s.nan = 0;
-    for (uint64_t i = b; i < e; ++i) {
-        const float* c = coords + 3 * (uint64_t)conn[i];
-        CoordSpan x;
-        x.valid = 1;
-        x.nan = 0;
+    constexpr uint32_t kU = 4;
+    for (uint64_t i0 = b; i0 < e; i0 += kU) {
+        float c[kU][3];
 #pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            x.mn[k] = x.mx[k] = c[k];
-            x.nan |= (c[k] != c[k] ? 1u : 0u) << k;
+        for (uint32_t u = 0; u < kU; ++u) {
+            const uint64_t i = i0 + u < e ? i0 + u : e - 1;
+            const float* p = coords + 3 * (uint64_t)conn[i];
+            c[u][0] = p[0];
+            c[u][1] = p[1];
+            c[u][2] = p[2];
         }
-        span_merge(s, x);
+#pragma unroll
+        for (uint32_t u = 0; u < kU; ++u) {
+            if (i0 + u >= e) break;
+            CoordSpan x;
+            x.valid = 1;
+            x.nan = 0;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                x.mn[k] = x.mx[k] = c[u][k];
+                x.nan |= (c[u][k] != c[u][k] ? 1u : 0u) << k;
+            }
+            span_merge(s, x);
+        }
     }
     sh[threadIdx.x] = s;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        CoordSpan r = sh[0];
-        for (uint32_t i = 1; i < kBlock; ++i) span_merge(r, sh[i]);
-        parts[blockIdx.x] = r;
-    }
+    block_span_merge(sh);
+    if (threadIdx.x == 0) parts[blockIdx.x] = sh[0];
 }
 
-__global__ void k_coord_span_final(const CoordSpan* parts, uint32_t n, float* out)
+/* The parts merged in order: each thread merges kSpanParts / kBlock
+ * consecutive parts, then the tree. */
+__global__ __launch_bounds__(kBlock) void k_coord_span_final(const CoordSpan* parts, float* out)
 {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    CoordSpan r = parts[0];
-    for (uint32_t i = 1; i < n; ++i) span_merge(r, parts[i]);
-    for (int k = 0; k < 3; ++k) { out[k] = r.mn[k]; out[3 + k] = r.mx[k]; }
+    __shared__ CoordSpan sh[kBlock];
+    constexpr uint32_t kPer = kSpanParts / kBlock;
+    CoordSpan loaded[kPer];
+#pragma unroll
+    for (uint32_t i = 0; i < kPer; ++i) loaded[i] = parts[threadIdx.x * kPer + i];
+    CoordSpan r = loaded[0];
+#pragma unroll
+    for (uint32_t i = 1; i < kPer; ++i) span_merge(r, loaded[i]);
+    sh[threadIdx.x] = r;
+    block_span_merge(sh);
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 3; ++k) { out[k] = sh[0].mn[k]; out[3 + k] = sh[0].mx[k]; }
 }
+
+static_assert(kSpanParts % kBlock == 0, "the final merge gives every thread the same number of parts");
 
 template <typename T, int C, bool SLICE>
 hipError_t launch_cells_t(const UArgs& a, bool emit, hipStream_t st)
@@ -465,7 +502,7 @@ kmc_status run_cells(const kmc_unstructured_volume* v, bool slice, double iso, i
         if (ok && need_coords) {
             hipLaunchKernelGGL(k_coord_span_partial, dim3(kSpanParts), dim3(kBlock), 0, st, dcoords, dconn, nconn,
                                spans);
-            hipLaunchKernelGGL(k_coord_span_final, dim3(1), dim3(64), 0, st, spans, kSpanParts, dcc);
+            hipLaunchKernelGGL(k_coord_span_final, dim3(1), dim3(kBlock), 0, st, spans, dcc);
             ok = hipGetLastError() == hipSuccess &&
                  hipMemcpyAsync(hcc, dcc, 24, hipMemcpyDeviceToHost, st) == hipSuccess;
         }
