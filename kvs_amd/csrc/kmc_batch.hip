@@ -446,13 +446,26 @@ hipError_t record_front(kmc_batch* b, hipStream_t st, bool copy_sizes)
     const uint32_t nseg = (uint32_t)b->seg_pass_level.size();
     if (nseg == 0) return hipSuccess;
     hipError_t e = hipSuccess;
-    for (const Group& g : b->groups) {
-        e = launch_classify_batch(g.dtype, b->jobs_d + g.p0, g.row_start_d, g.p1 - g.p0, g.rows, st);
+    if (copy_sizes) {
+        /* launched from the host (the serving form's two phases): each pass
+         * as the plain call's classify and count, its arguments in the
+         * launch -- for the large volumes served, faster than the batch
+         * kernels' shared row ranges */
+        for (const Group& g : b->groups)
+            for (uint32_t i = g.p0; i < g.p1 && e == hipSuccess; ++i) {
+                e = launch_classify(g.dtype, b->passes[i].a, st);
+                if (e == hipSuccess) e = launch_count(b->passes[i].a, st);
+            }
         if (e != hipSuccess) return e;
-    }
-    for (const Group& g : b->groups) {
-        e = launch_count_batch(b->jobs_d + g.p0, g.row_start_d, g.p1 - g.p0, g.rows, st);
-        if (e != hipSuccess) return e;
+    } else {
+        for (const Group& g : b->groups) {
+            e = launch_classify_batch(g.dtype, b->jobs_d + g.p0, g.row_start_d, g.p1 - g.p0, g.rows, st);
+            if (e != hipSuccess) return e;
+        }
+        for (const Group& g : b->groups) {
+            e = launch_count_batch(b->jobs_d + g.p0, g.row_start_d, g.p1 - g.p0, g.rows, st);
+            if (e != hipSuccess) return e;
+        }
     }
     uint64_t* partials = b->scan_aux_d;
     uint64_t* totals = b->scan_aux_d + b->scan_parts;
@@ -467,22 +480,22 @@ hipError_t record_front(kmc_batch* b, hipStream_t st, bool copy_sizes)
 }
 
 /* Launches of the emit.  host_sizes: the passes' record offsets are set on
- * the host (the serving form's two phases), so the per-item work of each pass
- * runs as the plain call's kernels with its arguments in the launch; in a
- * captured graph they live on the device only. */
+ * the host (the serving form's two phases), so each pass runs as the plain
+ * call's compaction and per-item kernels with its arguments in the launch; in
+ * a captured graph they live on the device only. */
 hipError_t record_emit(kmc_batch* b, hipStream_t st, bool host_sizes)
 {
     hipError_t e = hipSuccess;
     if (b->flat) {
         const uint32_t np = (uint32_t)b->passes.size();
         for (const Group& g : b->groups) {
-            e = launch_compact_batch(b->jobs_d + g.p0, g.row_start_d, g.p1 - g.p0, g.rows, st);
-            if (e != hipSuccess) return e;
             if (host_sizes) {
-                for (uint32_t i = g.p0; i < g.p1 && e == hipSuccess; ++i) e = launch_flat_items(g.dtype, b->passes[i].a, st);
+                for (uint32_t i = g.p0; i < g.p1 && e == hipSuccess; ++i) e = launch_flat_emit(g.dtype, b->passes[i].a, st);
             } else {
-                e = launch_flat_batch(g.dtype, g.mode, b->jobs_d + g.p0, b->istart_d + g.p0, b->istart_d + np + 1 + g.p0,
-                                      g.p1 - g.p0, st);
+                e = launch_compact_batch(b->jobs_d + g.p0, g.row_start_d, g.p1 - g.p0, g.rows, st);
+                if (e == hipSuccess)
+                    e = launch_flat_batch(g.dtype, g.mode, b->jobs_d + g.p0, b->istart_d + g.p0,
+                                          b->istart_d + np + 1 + g.p0, g.p1 - g.p0, st);
             }
             if (e != hipSuccess) return e;
         }
