@@ -244,6 +244,22 @@ kmc_status kmc_slice_plane_unstructured(const kmc_unstructured_volume* volume, c
  * path only reads their extent).  Release with kmc_kvsml_volume_release.
  */
 kmc_status kmc_kvsml_read_volume(const char* filename, int32_t memory, void* stream, kmc_volume* out);
+
+/*
+ * kvs::StructuredVolumeImporter(filename) for an AVS field file (.fld)
+ * (Importer/StructuredVolumeImporter.cpp:37-60, 150-187; FileFormat/AVSField/
+ * AVSField.cpp:174-419): the "tag = value" header up to the form feed, the
+ * byte / short / integer / float / double values right after the file's first
+ * "\f\f", uniform fields only (others: KMC_ERR_FILE, as the importer refuses
+ * them).  With memory = KMC_MEMORY_DEVICE the values stream from the file into
+ * device memory as kmc_kvsml_read_volume's do.  *out: object coordinates
+ * 0 .. dim - 1, min / max values by GetMinMaxValues (computed on the device).
+ * Release with kmc_kvsml_volume_release.
+ */
+kmc_status kmc_avs_read_volume(const char* filename, int32_t memory, void* stream, kmc_volume* out);
+
+/* Release the values of a volume read by kmc_kvsml_read_volume or
+ * kmc_avs_read_volume. */
 void kmc_kvsml_volume_release(kmc_volume* volume);
 
 /* ------------------------------------------------------------------------- */

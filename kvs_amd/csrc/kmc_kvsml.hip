@@ -1,5 +1,5 @@
 /*
- * kmc_kvsml.hip -- volume ingestion from KVSML straight to HBM:
+ * kmc_kvsml.hip -- volume ingestion straight to HBM, from KVSML and AVS field files:
  * kvs::StructuredVolumeObject::read (StructuredVolumeObject.cpp:171-229) over
  * KVSMLStructuredVolumeObject::read (KVSMLStructuredVolumeObject.cpp:159-300)
  * and DataArrayTag::read_data (DataArrayTag.cpp:217-420).
@@ -231,10 +231,12 @@ constexpr size_t kStage = size_t(64) << 20;   // bytes per staging buffer
 /* External binary file -> sink.  To the device: chunks read into two pinned
  * buffers in turn, each copied asynchronously; a buffer is refilled only after
  * its previous copy has completed (its event). */
-kmc_status read_binary(const std::string& path, size_t bytes, size_t esz, bool swap, const Sink& sink)
+kmc_status read_binary(const std::string& path, size_t bytes, size_t esz, bool swap, const Sink& sink,
+                       long offset = 0)
 {
     FILE* f = std::fopen(path.c_str(), "rb");
     if (!f) return KMC_ERR_FILE;
+    if (offset && std::fseek(f, offset, SEEK_SET) != 0) { std::fclose(f); return KMC_ERR_FILE; }
     kmc_status s = KMC_OK;
     if (sink.memory == KMC_MEMORY_HOST) {
         if (std::fread(sink.dst, 1, bytes, f) != bytes) s = KMC_ERR_FILE;
@@ -459,11 +461,124 @@ kmc_status read_volume(const char* filename, int32_t memory, hipStream_t st, kmc
     return KMC_OK;
 }
 
+/* ------------------------------------------------------------------------- */
+/* AVS field                                                                   */
+/* ------------------------------------------------------------------------- */
+
+/* StructuredVolumeImporter( filename ) for an AVS field file
+ * (Importer/StructuredVolumeImporter.cpp:37-60, 150-187) over
+ * AVSField::read (FileFormat/AVSField/AVSField.cpp:174-419): the header's
+ * "tag = value" lines up to the form feed, the values right after the first
+ * "\f\f" of the file in the host's byte order, uniform fields only; object
+ * coordinates 0 .. dim - 1; min / max by GetMinMaxValues on the device. */
+kmc_status read_avs(const char* filename, int32_t memory, hipStream_t st, kmc_volume* out)
+{
+    if (!filename || !out) return KMC_ERR_INVALID_ARGUMENT;
+    if (memory != KMC_MEMORY_HOST && memory != KMC_MEMORY_DEVICE) return KMC_ERR_INVALID_ARGUMENT;
+    std::memset(out, 0, sizeof(*out));
+    /* AVSField::CheckExtension (:92-101) */
+    const std::string fn(filename);
+    const size_t dot = fn.find_last_of('.');
+    const std::string ext = dot == std::string::npos ? "" : fn.substr(dot + 1);
+    if (ext != "fld" && ext != "FLD") return KMC_ERR_FILE;
+    FILE* f = std::fopen(filename, "rb");
+    if (!f) return KMC_ERR_FILE;
+    /* CheckHeader and read_header (:40-56, 216-300): lines of at most 255
+     * characters; '#' lines skipped; a line starting with a form feed ends
+     * the header; tag, '=', value split at " ,\t\n" */
+    char buf[256];
+    int veclen = 1, field = 1, dtype = -1;   // AVSField's defaults: uniform, no data type (AVSField.h:54-62)
+    uint32_t dim[3] = { 1, 1, 1 };
+    bool ok = std::fgets(buf, sizeof(buf), f) && std::strncmp(buf, "# AVS", 5) == 0;
+    while (ok && std::fgets(buf, sizeof(buf), f)) {
+        if (buf[0] == '#') continue;
+        if (buf[0] == '\f') break;
+        char* save = nullptr;
+        const char* seps = " ,\t\n";
+        char* tag = strtok_r(buf, seps, &save);
+        char* equal = tag ? strtok_r(nullptr, seps, &save) : nullptr;
+        char* value = equal ? strtok_r(nullptr, seps, &save) : nullptr;
+        if (!equal || std::strcmp(equal, "=") || !value) continue;
+        if (!std::strcmp(tag, "veclen")) veclen = std::atoi(value);
+        if (!std::strcmp(tag, "dim1")) dim[0] = (uint32_t)std::atoi(value);
+        if (!std::strcmp(tag, "dim2")) dim[1] = (uint32_t)std::atoi(value);
+        if (!std::strcmp(tag, "dim3")) dim[2] = (uint32_t)std::atoi(value);
+        if (!std::strcmp(tag, "field"))
+            field = !std::strcmp(value, "uniform") ? 1 : !std::strcmp(value, "rectilinear") ? 2
+                  : !std::strcmp(value, "irregular") ? 3 : 0;
+        if (!std::strcmp(tag, "data"))
+            dtype = !std::strcmp(value, "byte") ? KMC_UINT8 : !std::strcmp(value, "short") ? KMC_INT16
+                  : !std::strcmp(value, "integer") ? KMC_INT32 : !std::strcmp(value, "float") ? KMC_REAL32
+                  : !std::strcmp(value, "double") ? KMC_REAL64 : -1;
+    }
+    /* SkipHeader (:63-79): the values follow the first "\f\f" */
+    long offset = -1;
+    if (ok) {
+        std::fseek(f, 0, SEEK_SET);
+        int c;
+        while ((c = std::fgetc(f)) != EOF)
+            if (c == '\f' && std::fgetc(f) == '\f') { offset = std::ftell(f); break; }
+    }
+    std::fclose(f);
+    if (!ok || offset < 0 || dtype < 0 || veclen < 1) return KMC_ERR_FILE;
+    if (field != 1) return KMC_ERR_FILE;   // "'Uniform' type in kvs::AVSField format is only supported."
+
+    kmc_status s = ensure_device();
+    if (s != KMC_OK) return s;
+    const size_t nnodes = (size_t)dim[0] * dim[1] * dim[2], n = nnodes * (size_t)veclen;
+    const size_t bytes = n * type_size(dtype);
+    void* dst = nullptr;
+    if (memory == KMC_MEMORY_DEVICE) dst = device_alloc(bytes ? bytes : 1);
+    else if (hipHostMalloc(&dst, bytes ? bytes : 1, hipHostMallocDefault) != hipSuccess) dst = nullptr;
+    if (!dst) return KMC_ERR_OUT_OF_MEMORY;
+    auto fail = [&](kmc_status e) {
+        if (memory == KMC_MEMORY_DEVICE) device_free(dst);
+        else (void)hipHostFree(dst);
+        std::memset(out, 0, sizeof(*out));
+        return e;
+    };
+    s = read_binary(fn, bytes, type_size(dtype), false, Sink{ memory, dst, st }, offset);
+    if (s != KMC_OK) return fail(s);
+    out->values = dst;
+    out->memory = memory;
+    out->type = dtype;
+    out->resolution[0] = dim[0]; out->resolution[1] = dim[1]; out->resolution[2] = dim[2];
+    out->veclen = (uint32_t)veclen;
+    out->kind = KMC_STRUCTURED;
+    out->has_min_max_coords = 1;
+    for (int k = 0; k < 3; ++k) { out->min_coord[k] = 0.0f; out->max_coord[k] = (float)dim[k] - 1.0f; }
+    /* updateMinMaxValues */
+    if (veclen != 1) return fail(KMC_ERR_NOT_SCALAR);   // the magnitude scan of vector data is not supported
+    double mn = 0, mx = 0;
+    if (memory == KMC_MEMORY_DEVICE) {
+        s = min_max_on_device(out, dst, st, &mn, &mx);
+    } else {
+        void* tmp = device_alloc(bytes ? bytes : 1);
+        if (!tmp) return fail(KMC_ERR_OUT_OF_MEMORY);
+        s = hipMemcpyAsync(tmp, dst, bytes, hipMemcpyHostToDevice, st) == hipSuccess ? KMC_OK : KMC_ERR_DEVICE;
+        if (s == KMC_OK) s = min_max_on_device(out, tmp, st, &mn, &mx);
+        (void)hipStreamSynchronize(st);
+        device_free(tmp);
+    }
+    if (s != KMC_OK) return fail(s);
+    out->has_min_max_values = 1;
+    out->min_value = mn;
+    out->max_value = mx;
+    return KMC_OK;
+}
+
 }  // namespace
 
 }  // namespace kmc
 
 extern "C" {
+
+kmc_status kmc_avs_read_volume(const char* filename, int32_t memory, void* stream, kmc_volume* out)
+{
+    const kmc_status s = kmc::read_avs(filename, memory, (hipStream_t)stream, out);
+    kmc::report(s);
+    return s;
+}
 
 kmc_status kmc_kvsml_read_volume(const char* filename, int32_t memory, void* stream, kmc_volume* out)
 {

@@ -166,7 +166,8 @@ EXPORTED = [
     "kmc_server_create", "kmc_server_register_volume", "kmc_server_submit", "kmc_server_poll", "kmc_server_wait",
     "kmc_server_get_stats", "kmc_server_reset_stats", "kmc_server_destroy",
     "kmc_set_timing", "kmc_last_timing", "kmc_extract_noalloc", "kmc_slice_plane",
-    "kmc_extract_unstructured", "kmc_slice_plane_unstructured", "kmc_kvsml_read_volume", "kmc_kvsml_volume_release",
+    "kmc_extract_unstructured", "kmc_slice_plane_unstructured",
+    "kmc_kvsml_read_volume", "kmc_avs_read_volume", "kmc_kvsml_volume_release",
     "kmc_surface_expand", "kmc_export_stl",
 ]
 
@@ -224,6 +225,8 @@ def load():
     lib.kmc_slice_plane_unstructured.restype = ctypes.c_int
     lib.kmc_kvsml_read_volume.argtypes = [ctypes.c_char_p, ctypes.c_int32, ctypes.c_void_p, ctypes.POINTER(Volume)]
     lib.kmc_kvsml_read_volume.restype = ctypes.c_int
+    lib.kmc_avs_read_volume.argtypes = [ctypes.c_char_p, ctypes.c_int32, ctypes.c_void_p, ctypes.POINTER(Volume)]
+    lib.kmc_avs_read_volume.restype = ctypes.c_int
     lib.kmc_kvsml_volume_release.argtypes = [ctypes.POINTER(Volume)]
     lib.kmc_kvsml_volume_release.restype = None
     lib.kmc_surface_expand.argtypes = [ctypes.POINTER(Surface), ctypes.c_int32, ctypes.c_void_p,
@@ -451,13 +454,15 @@ def slice_plane_unstructured(values, coords, connections, cell_type, coefficient
 
 
 class KvsmlVolume:
-    """A volume read by kmc_kvsml_read_volume; .volume is the kmc_volume (values
-    in host or device memory, owned until close())."""
+    """A volume read by kmc_kvsml_read_volume (or, for an AVS field file .fld,
+    kmc_avs_read_volume); .volume is the kmc_volume (values in host or device
+    memory, owned until close())."""
 
     def __init__(self, filename, memory=MEMORY_DEVICE):
         self.lib = load()
         self.volume = Volume()
-        st = self.lib.kmc_kvsml_read_volume(str(filename).encode(), memory, None, ctypes.byref(self.volume))
+        read = self.lib.kmc_avs_read_volume if str(filename).lower().endswith(".fld") else self.lib.kmc_kvsml_read_volume
+        st = read(str(filename).encode(), memory, None, ctypes.byref(self.volume))
         if st != OK:
             raise KmcError(st)
 

@@ -176,6 +176,13 @@ def ref_lib():
         lib.ref_read_kvsml_volume.argtypes = [ctypes.c_char_p, ctypes.POINTER(_RefReadVolume)]
         lib.ref_read_kvsml_volume.restype = ctypes.c_int
         lib.ref_read_volume_free.argtypes = [ctypes.POINTER(_RefReadVolume)]
+        lib.ref_write_avs_volume.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(ctypes.c_uint32),
+                                             ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(ctypes.c_float),
+                                             ctypes.c_char_p]
+        lib.ref_write_avs_volume.restype = ctypes.c_int
+        lib.ref_read_avs_volume.argtypes = [ctypes.c_char_p, ctypes.POINTER(_RefReadVolume),
+                                            ctypes.POINTER(ctypes.c_float)]
+        lib.ref_read_avs_volume.restype = ctypes.c_int
         lib.ref_export_stl.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p, ctypes.c_uint64,
                                        ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int, ctypes.c_int, ctypes.c_char_p]
         lib.ref_export_stl.restype = ctypes.c_int
@@ -562,6 +569,37 @@ def ref_read_kvsml_volume(filename):
         vals = np.frombuffer(buf, dtype=dtype).reshape(r.dims[2], r.dims[1], r.dims[0]).copy()
         return {"values": vals, "min_value": r.min_value, "max_value": r.max_value, "grid_type": r.grid_type,
                 "min_obj": list(r.min_obj), "max_obj": list(r.max_obj)}
+    finally:
+        ref_lib().ref_read_volume_free(ctypes.byref(r))
+
+
+def ref_write_avs_volume(values, filename, ext=None):
+    """AVSField::write of a uniform field by the reference (oracle/_ref);
+    values [z][y][x] of uint8 / int16 / int32 / float32 / float64; ext: the
+    optional (min_ext, max_ext)."""
+    a = np.ascontiguousarray(values)
+    nz, ny, nx = a.shape
+    dims = (ctypes.c_uint32 * 3)(nx, ny, nz)
+    e = (ctypes.c_float * 6)(*(list(ext[0]) + list(ext[1]))) if ext is not None else (ctypes.c_float * 6)()
+    if ref_lib().ref_write_avs_volume(a.ctypes.data, NUMPY_TYPES[a.dtype], dims, 1, int(ext is not None), e,
+                                      str(filename).encode()) != 0:
+        raise RuntimeError("reference AVS write failed")
+
+
+def ref_read_avs_volume(filename):
+    """StructuredVolumeImporter( filename ) of an AVS field by the reference:
+    values [z][y][x], min/max values, min/max object and external coordinates."""
+    r = _RefReadVolume()
+    ext = (ctypes.c_float * 6)()
+    if ref_lib().ref_read_avs_volume(str(filename).encode(), ctypes.byref(r), ext) != 0:
+        raise RuntimeError("reference AVS read failed")
+    try:
+        dtype = [k for k, t in NUMPY_TYPES.items() if t == r.dtype][0]
+        buf = ctypes.string_at(r.values, r.nbytes)
+        vals = np.frombuffer(buf, dtype=dtype).reshape(r.dims[2], r.dims[1], r.dims[0]).copy()
+        return {"values": vals, "min_value": r.min_value, "max_value": r.max_value, "grid_type": r.grid_type,
+                "min_obj": list(r.min_obj), "max_obj": list(r.max_obj), "min_ext": list(ext[:3]),
+                "max_ext": list(ext[3:])}
     finally:
         ref_lib().ref_read_volume_free(ctypes.byref(r))
 

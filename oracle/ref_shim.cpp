@@ -13,6 +13,8 @@
 #include <kvs/Isosurface>
 #include <kvs/StructuredVolumeObject>
 #include <kvs/TransferFunction>
+#include <kvs/AVSField>
+#include <kvs/StructuredVolumeImporter>
 #include <kvs/ColorMap>
 #include <kvs/ValueArray>
 #include <kvs/AnyValueArray>
@@ -424,6 +426,76 @@ int ref_read_kvsml_volume( const char* filename, ref_read_volume* out )
 }
 
 void ref_read_volume_free( ref_read_volume* r ) { std::free( r->values ); std::memset( r, 0, sizeof( *r ) ); }
+
+// AVSField::write of a uniform 3-D field (byte / short / integer / float /
+// double values, kvs::Type::TypeID order as dtype), with min_ext / max_ext
+// when has_ext.
+int ref_write_avs_volume( const void* values, int32_t dtype, const uint32_t dims[3], int32_t veclen, int32_t has_ext,
+                          const float ext[6], const char* filename )
+{
+    kvs::AVSField f;
+    const size_t n = size_t( dims[0] ) * dims[1] * dims[2] * size_t( veclen );
+    auto set = [&] ( auto tag, kvs::AVSField::DataType type, int bits, bool sign )
+    {
+        using T = decltype( tag );
+        kvs::ValueArray<T> a( n );
+        std::memcpy( a.data(), values, n * sizeof( T ) );
+        f.setValues( kvs::AnyValueArray( a ) );
+        f.setDataType( type );
+        f.setBits( bits );
+        f.setSigned( sign );
+    };
+    switch ( dtype )
+    {
+    case 4: set( kvs::UInt8(), kvs::AVSField::Byte, 8, false ); break;
+    case 1: set( kvs::Int16(), kvs::AVSField::Short, 16, true ); break;
+    case 2: set( kvs::Int32(), kvs::AVSField::Integer, 32, true ); break;
+    case 8: set( kvs::Real32(), kvs::AVSField::Float, 32, true ); break;
+    case 9: set( kvs::Real64(), kvs::AVSField::Double, 64, true ); break;
+    default: return 2;
+    }
+    f.setVeclen( veclen );
+    f.setNSpace( 3 );
+    f.setNDim( 3 );
+    f.setDim( kvs::Vec3ui( dims[0], dims[1], dims[2] ) );
+    f.setFieldType( kvs::AVSField::Uniform );
+    if ( has_ext )
+    {
+        f.setMinExt( kvs::Vec3( ext[0], ext[1], ext[2] ) );
+        f.setMaxExt( kvs::Vec3( ext[3], ext[4], ext[5] ) );
+    }
+    return f.write( filename ) ? 0 : 1;
+}
+
+// StructuredVolumeImporter( filename ) for an AVS field: what the reference's
+// importer makes of the file (ext: its min / max external coordinates).  Its
+// success flag is not set on the AVS path (ImporterBase.h:26 leaves it
+// uninitialised and import( const AVSField* ) only ever clears it), so a
+// refused or unreadable file is told by the values it did not set.
+int ref_read_avs_volume( const char* filename, ref_read_volume* out, float ext[6] )
+{
+    std::memset( out, 0, sizeof( *out ) );
+    kvs::StructuredVolumeImporter vol( filename );
+    if ( vol.values().size() == 0 ) return 1;
+    const auto& a = vol.values();
+    out->nbytes = a.byteSize();
+    out->values = std::malloc( out->nbytes ? out->nbytes : 1 );
+    std::memcpy( out->values, a.data(), out->nbytes );
+    out->dtype = int32_t( a.typeID() ) - int32_t( kvs::Type::TypeInt8 );
+    for ( int k = 0; k < 3; ++k ) out->dims[k] = vol.resolution()[k];
+    out->veclen = uint32_t( vol.veclen() );
+    out->grid_type = int32_t( vol.gridType() );
+    out->min_value = vol.minValue();
+    out->max_value = vol.maxValue();
+    for ( int k = 0; k < 3; ++k )
+    {
+        out->min_obj[k] = vol.minObjectCoord()[k];
+        out->max_obj[k] = vol.maxObjectCoord()[k];
+        ext[k] = vol.minExternalCoord()[k];
+        ext[3 + k] = vol.maxExternalCoord()[k];
+    }
+    return 0;
+}
 
 // PolygonExporter<kvs::Stl>( polygon ) and Stl::write( filename ) for a
 // triangle polygon with the given arrays and normal type.

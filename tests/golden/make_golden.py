@@ -397,6 +397,51 @@ def main():
                         "sha256": hashlib.sha256(open(os.path.join(sdir, name), "rb").read()).hexdigest()})
     meta["stl"] = stl
 
+    # 13. AVS field files written by the reference's AVSField::write, and what
+    #     its StructuredVolumeImporter makes of them; hand-edited variants: no
+    #     "field" line (the reader's default, uniform) with extra comments, and
+    #     a rectilinear field (refused by the importer)
+    os.environ.setdefault("USER", "kvs")   # AVSField::write puts getenv("USER") in the header
+    adir = os.path.join(HERE, "avs")
+    os.makedirs(adir, exist_ok=True)
+    rng = np.random.default_rng(131)
+    blob = datasets.blobs((19, 13, 11), seed=9)
+    aspecs = [
+        ("u8_ext", (blob * 255).astype(np.uint8), ((0.5, -1.0, 2.0), (10.0, 12.5, 20.25))),
+        ("i16", rng.integers(-3000, 3000, (6, 9, 7)).astype(np.int16), None),
+        ("i32", rng.integers(-10 ** 6, 10 ** 6, (5, 4, 8)).astype(np.int32), None),
+        ("f32", blob.astype(np.float32), ((0.0, 0.0, 0.0), (1.0, 1.0, 1.0))),
+        ("f64", (rng.standard_normal((7, 5, 6)) * 4).astype(np.float64), None),
+    ]
+    avs = []
+    for name, vol, ext in aspecs:
+        oracle.ref_write_avs_volume(vol, os.path.join(adir, name + ".fld"), ext=ext)
+        avs.append(name)
+    with open(os.path.join(adir, "f32.fld"), "rb") as f:
+        raw = f.read()
+    head, data = raw.split(b"\f\f", 1)
+    with open(os.path.join(adir, "f32_nofield.fld"), "wb") as f:
+        f.write(head.replace(b"field  = uniform\n", b"# a comment line = 7\n") + b"\f\f" + data)
+    avs.append("f32_nofield")
+    with open(os.path.join(adir, "f32_rectilinear.fld"), "wb") as f:
+        f.write(head.replace(b"field  = uniform", b"field  = rectilinear") + b"\f\f" + data)
+    pk = Packer()
+    cases = []
+    for name in avs:
+        r = oracle.ref_read_avs_volume(os.path.join(adir, name + ".fld"))
+        pk.add("values", np.frombuffer(r["values"].tobytes(), dtype=np.uint8))
+        cases.append({"name": name, "dtype": r["values"].dtype.name, "shape": list(r["values"].shape),
+                      "min_value": r["min_value"], "max_value": r["max_value"], "grid_type": r["grid_type"],
+                      "min_obj": r["min_obj"], "max_obj": r["max_obj"], "min_ext": r["min_ext"],
+                      "max_ext": r["max_ext"]})
+    try:
+        oracle.ref_read_avs_volume(os.path.join(adir, "f32_rectilinear.fld"))
+        refused = False
+    except RuntimeError:
+        refused = True
+    pk.save(os.path.join(HERE, "avs.npz"))
+    meta["avs"] = {"cases": cases, "refused": {"f32_rectilinear": refused}}
+
     with open(os.path.join(HERE, "golden.json"), "w") as f:
         json.dump(meta, f, indent=1, sort_keys=True)
     print("wrote", sorted(os.listdir(HERE)))

@@ -1,8 +1,10 @@
-"""KVSML volume ingestion (kmc_kvsml_read_volume) against what the reference's
-own reader (StructuredVolumeObject::read) makes of files its writer produced
-(tests/golden/kvsml/): values bit for bit, min/max values (the file's rounded
-attributes when present) and min/max object coordinates, into device and
-host memory; and the isosurface of the device-resident volume against the
+"""Volume ingestion: KVSML (kmc_kvsml_read_volume) against what the
+reference's own reader (StructuredVolumeObject::read) makes of files its
+writer produced (tests/golden/kvsml/), and AVS fields (kmc_avs_read_volume)
+against its StructuredVolumeImporter on files its AVSField::write produced
+(tests/golden/avs/): values bit for bit, min/max values (a KVSML file's
+rounded attributes when present) and min/max object coordinates, into device
+and host memory; and the isosurface of the device-resident volume against the
 oracle."""
 import ctypes
 import os
@@ -75,3 +77,59 @@ def test_kvsml_errors():
     with pytest.raises(kmc.KmcError) as e:
         kmc.KvsmlVolume(os.path.join(KDIR, "f32_binary_value.dat"))   # not a KVSML extension
     assert e.value.status == kmc.ERR_FILE
+
+
+ADIR = os.path.join(golden_io.GOLDEN, "avs")
+
+
+def avs_cases():
+    p = golden_io.Packed("avs.npz")
+    for i, c in enumerate(golden_io.meta()["avs"]["cases"]):
+        v = np.frombuffer(p.get("values", i).tobytes(), dtype=np.dtype(c["dtype"])).reshape(c["shape"])
+        yield c, v
+
+
+@pytest.mark.parametrize("memory", [kmc.MEMORY_DEVICE, kmc.MEMORY_HOST])
+def test_avs_read_matches_reference_importer(memory):
+    # kmc_avs_read_volume against kvs::StructuredVolumeImporter on files the
+    # reference's AVSField::write produced (and a copy without a field line)
+    for c, want in avs_cases():
+        print("avs case", c["name"], memory, flush=True)
+        kv = kmc.KvsmlVolume(os.path.join(ADIR, c["name"] + ".fld"), memory=memory)
+        try:
+            v = kv.volume
+            assert v.memory == memory and v.kind == kmc.STRUCTURED and v.veclen == 1
+            assert list(v.resolution) == c["shape"][::-1]
+            assert bits_equal(kv.values(), want), c["name"]
+            assert (v.min_value, v.max_value) == (c["min_value"], c["max_value"]), c["name"]
+            assert list(v.min_coord) == c["min_obj"] and list(v.max_coord) == c["max_obj"], c["name"]
+        finally:
+            kv.close()
+
+
+def test_avs_refusals_and_isosurface():
+    # the importer takes uniform fields only
+    assert golden_io.meta()["avs"]["refused"]["f32_rectilinear"]
+    with pytest.raises(kmc.KmcError) as e:
+        kmc.KvsmlVolume(os.path.join(ADIR, "f32_rectilinear.fld"))
+    assert e.value.status == kmc.ERR_FILE
+    with pytest.raises(kmc.KmcError) as e:
+        kmc.KvsmlVolume(os.path.join(ADIR, "missing.fld"))
+    assert e.value.status == kmc.ERR_FILE
+    # an isosurface of the device-resident volume equals the oracle's
+    lib = kmc.load()
+    c, want = next(x for x in avs_cases() if x[0]["name"] == "f32")
+    kv = kmc.KvsmlVolume(os.path.join(ADIR, "f32.fld"))
+    try:
+        s = kmc.Surface()
+        st = lib.kmc_extract(ctypes.byref(kv.volume), 0.5, kmc.VERTEX_NORMAL, 0, kmc.MEMORY_HOST, None, ctypes.byref(s))
+        assert st == kmc.OK
+        try:
+            got = kmc.surface_to_numpy(s)
+        finally:
+            lib.kmc_surface_release(ctypes.byref(s))
+        exp = oracle.extract(want, 0.5, kmc.VERTEX_NORMAL, False, min_max_values=(c["min_value"], c["max_value"]),
+                             coords=(c["min_obj"], c["max_obj"]))
+        assert_same_surface(got, exp, "avs f32")
+    finally:
+        kv.close()
