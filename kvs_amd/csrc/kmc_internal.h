@@ -85,6 +85,12 @@ hipError_t launch_classify(int dtype, const JobArgs& a, hipStream_t st);
  * (compaction, then one thread per triangle with per-vertex colours). */
 hipError_t launch_classify_plane(const JobArgs& a, hipStream_t st);
 hipError_t launch_slice_emit(int dtype, const JobArgs& a, hipStream_t st);
+/* SlicePlane by node-row thresholds: thresholds (ny * nz int32) and per cell
+ * row triangle counts ((ny - 1) * (nz - 1) uint32); then, with their
+ * exclusive prefix, one thread per triangle (a.toff[1] triangles) */
+hipError_t launch_slice_runs_front(const JobArgs& a, int32_t* thr, uint32_t* counts, hipStream_t st);
+hipError_t launch_slice_runs_emit(int dtype, const JobArgs& a, const int32_t* thr, const uint64_t* prefix,
+                                  hipStream_t st);
 hipError_t launch_count(const JobArgs& a, hipStream_t st);
 hipError_t launch_scan(const uint32_t* counts, uint64_t n, uint32_t narr, uint64_t* partials,
                        uint64_t* prefix, uint64_t* totals, hipStream_t st);

@@ -2288,6 +2288,138 @@ __global__ __launch_bounds__(kBlock) void k_slice_triangles(JobArgs a)
     for (uint64_t g = sp.first; g < sp.end; g += sp.step) slice_triangle<T>(a, tri, g, a.trec[g], a.tci[g]);
 }
 
+/* SlicePlane without classifying every node.  Along a node row (y, z) the
+ * plane value plane_at(c, x, y, z) is monotone in x -- each float operation
+ * is monotone in its operand, rounding included -- so the nodes above the
+ * plane are x >= t (c0 > 0; c0 == 0 with t = 0 or nx) or x < t (c0 < 0) for
+ * one threshold t per node row, found by bisection on the same float
+ * expression.  A cell row's active cells are then the one run
+ * [min t - 1, max t - 1] over its four node rows, cut by the points t - 1
+ * and t of each row into at most nine runs of one cube index.  Counts per
+ * cell row, a scan, then one thread per triangle.  The host takes this path
+ * when the plane is finite at the volume's eight corners (and so, being
+ * monotone in each coordinate, everywhere inside). */
+__device__ __forceinline__ bool above_thr(int dir, int64_t t, int64_t x) { return dir < 0 ? x < t : x >= t; }
+
+__global__ __launch_bounds__(kBlock) void k_slice_thresholds(JobArgs a, int32_t* thr)
+{
+    const int dir = a.plane[0] > 0.0f ? 1 : (a.plane[0] < 0.0f ? -1 : 0);
+    const uint64_t nrows = (uint64_t)a.ny * a.nz, step = (uint64_t)gridDim.x * kBlock;
+    for (uint64_t row = (uint64_t)blockIdx.x * kBlock + threadIdx.x; row < nrows; row += step) {
+        const uint32_t z = (uint32_t)(row / a.ny), y = (uint32_t)(row - (uint64_t)z * a.ny);
+        const float fy = (float)y, fz = (float)z;
+        int32_t t;
+        if (dir == 0) {
+            t = plane_at(a.plane, 0.0f, fy, fz) > 0.0f ? 0 : (int32_t)a.nx;
+        } else {
+            /* the first x whose node is above (c0 > 0) or not above (c0 < 0) */
+            uint32_t lo = 0, hi = a.nx;
+            while (lo < hi) {
+                const uint32_t mid = (lo + hi) >> 1;
+                const bool up = plane_at(a.plane, (float)mid, fy, fz) > 0.0f;
+                if (up == (dir > 0)) hi = mid; else lo = mid + 1;
+            }
+            t = (int32_t)lo;
+        }
+        thr[row] = t;
+    }
+}
+
+/* The active cells of cell row (y, z) as runs of one cube index. */
+struct SliceRuns {
+    uint32_t n;
+    uint32_t start[10];   // run i covers cells [start[i], start[i + 1])
+    uint32_t ci[9];
+};
+
+__device__ __forceinline__ uint32_t slice_cube_index(int dir, const int64_t t[4], int64_t x)
+{
+    /* corners 0..7: (x,y,z) (x+1,y,z) (x+1,y+1,z) (x,y+1,z) and the same at z+1;
+     * t: node rows (y,z), (y+1,z), (y,z+1), (y+1,z+1) */
+    return (uint32_t)above_thr(dir, t[0], x) | (uint32_t)above_thr(dir, t[0], x + 1) << 1 |
+           (uint32_t)above_thr(dir, t[1], x + 1) << 2 | (uint32_t)above_thr(dir, t[1], x) << 3 |
+           (uint32_t)above_thr(dir, t[2], x) << 4 | (uint32_t)above_thr(dir, t[2], x + 1) << 5 |
+           (uint32_t)above_thr(dir, t[3], x + 1) << 6 | (uint32_t)above_thr(dir, t[3], x) << 7;
+}
+
+__device__ __forceinline__ void slice_runs(const JobArgs& a, const int32_t* thr, uint32_t y, uint32_t z, SliceRuns& r)
+{
+    const int dir = a.plane[0] < 0.0f ? -1 : 1;
+    const uint64_t r0 = (uint64_t)z * a.ny + y;
+    const int64_t t[4] = { thr[r0], thr[r0 + 1], thr[r0 + a.ny], thr[r0 + a.ny + 1] };
+    int64_t tmin = t[0], tmax = t[0];
+    for (int k = 1; k < 4; ++k) { tmin = t[k] < tmin ? t[k] : tmin; tmax = t[k] > tmax ? t[k] : tmax; }
+    const int64_t lo = tmin - 1 > 0 ? tmin - 1 : 0;
+    const int64_t hi = tmax - 1 < (int64_t)a.nx - 2 ? tmax - 1 : (int64_t)a.nx - 2;
+    r.n = 0;
+    if (lo > hi) return;
+    /* breakpoints strictly inside (lo, hi + 1): t - 1 and t of every row */
+    int64_t b[10];
+    uint32_t nb = 0;
+    b[nb++] = lo;
+    for (int k = 0; k < 4; ++k)
+        for (int64_t c = t[k] - 1; c <= t[k]; ++c)
+            if (c > lo && c <= hi) b[nb++] = c;
+    for (uint32_t i = 1; i < nb; ++i)   // insertion sort (at most nine points)
+        for (uint32_t j = i; j > 0 && b[j - 1] > b[j]; --j) { const int64_t u = b[j]; b[j] = b[j - 1]; b[j - 1] = u; }
+    for (uint32_t i = 0; i < nb; ++i) {
+        if (r.n && (int64_t)r.start[r.n - 1] == b[i]) continue;   // duplicate point
+        r.start[r.n] = (uint32_t)b[i];
+        r.ci[r.n] = slice_cube_index(dir, t, b[i]);
+        ++r.n;
+    }
+    r.start[r.n] = (uint32_t)(hi + 1);
+}
+
+__global__ __launch_bounds__(kBlock) void k_slice_row_counts(JobArgs a, const int32_t* thr, uint32_t* counts)
+{
+    const uint64_t ncr = (uint64_t)(a.ny - 1) * (a.nz - 1), step = (uint64_t)gridDim.x * kBlock;
+    for (uint64_t cr = (uint64_t)blockIdx.x * kBlock + threadIdx.x; cr < ncr; cr += step) {
+        const uint32_t z = (uint32_t)(cr / (a.ny - 1)), y = (uint32_t)(cr - (uint64_t)z * (a.ny - 1));
+        SliceRuns r;
+        slice_runs(a, thr, y, z, r);
+        uint32_t c = 0;
+        for (uint32_t i = 0; i < r.n; ++i) c += (r.start[i + 1] - r.start[i]) * (uint32_t)(d_tri[r.ci[i]] >> 60);
+        counts[cr] = c;
+    }
+}
+
+/* One thread per triangle: its cell row (the last with prefix <= g), then its
+ * cell and table index within the row's runs. */
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_slice_triangles_runs(JobArgs a, const int32_t* thr, const uint64_t* prefix)
+{
+    __shared__ uint64_t tri[256];
+    stage_table_block(tri);
+    const uint64_t ncr = (uint64_t)(a.ny - 1) * (a.nz - 1), n = a.toff[1];
+    const Span sp = grid_span(0, n, kBlock, threadIdx.x);
+    for (uint64_t g = sp.first; g < sp.end; g += sp.step) {
+        uint64_t lo = 0, hi = ncr - 1;
+        while (lo < hi) {
+            const uint64_t mid = (lo + hi + 1) >> 1;
+            if (prefix[mid] <= g) lo = mid; else hi = mid - 1;
+        }
+        const uint32_t z = (uint32_t)(lo / (a.ny - 1)), y = (uint32_t)(lo - (uint64_t)z * (a.ny - 1));
+        SliceRuns r;
+        slice_runs(a, thr, y, z, r);
+        uint64_t k = g - prefix[lo];
+        uint32_t x = 0, t = 0, ci = 0;
+        for (uint32_t i = 0; i < r.n; ++i) {
+            const uint32_t nt = (uint32_t)(tri[r.ci[i]] >> 60);
+            const uint64_t m = (uint64_t)(r.start[i + 1] - r.start[i]) * nt;
+            if (k < m) {
+                x = r.start[i] + (uint32_t)(k / nt);
+                t = (uint32_t)(k % nt);
+                ci = r.ci[i];
+                break;
+            }
+            k -= m;
+        }
+        const uint64_t rec = (uint64_t)x | (uint64_t)y << 20 | (uint64_t)z << 40 | (uint64_t)t << 60;
+        slice_triangle<T>(a, tri, g, rec, ci);
+    }
+}
+
 /* ------------------------------------------------------------------------- */
 /* row-count scan                                                              */
 /* ------------------------------------------------------------------------- */
@@ -2788,6 +2920,39 @@ hipError_t launch_slice_emit_t(const JobArgs& a, hipStream_t st)
     const uint64_t nt = a.toff[1];
     if (nt) hipLaunchKernelGGL((k_slice_triangles<T>), dim3(flat_blocks(nt)), dim3(kBlock), 0, st, a);
     return hipGetLastError();
+}
+
+hipError_t launch_slice_runs_front(const JobArgs& a, int32_t* thr, uint32_t* counts, hipStream_t st)
+{
+    const uint64_t rows = (uint64_t)a.ny * a.nz, ncr = (uint64_t)(a.ny - 1) * (a.nz - 1);
+    hipLaunchKernelGGL(k_slice_thresholds, dim3(flat_blocks(rows)), dim3(kBlock), 0, st, a, thr);
+    if (ncr) hipLaunchKernelGGL(k_slice_row_counts, dim3(flat_blocks(ncr)), dim3(kBlock), 0, st, a, thr, counts);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_slice_runs_emit_t(const JobArgs& a, const int32_t* thr, const uint64_t* prefix, hipStream_t st)
+{
+    const uint64_t nt = a.toff[1];
+    if (nt) hipLaunchKernelGGL((k_slice_triangles_runs<T>), dim3(flat_blocks(nt)), dim3(kBlock), 0, st, a, thr, prefix);
+    return hipGetLastError();
+}
+
+hipError_t launch_slice_runs_emit(int dtype, const JobArgs& a, const int32_t* thr, const uint64_t* prefix,
+                                  hipStream_t st)
+{
+    switch (dtype) {
+    case 0: return launch_slice_runs_emit_t<int8_t>(a, thr, prefix, st);
+    case 1: return launch_slice_runs_emit_t<int16_t>(a, thr, prefix, st);
+    case 2: return launch_slice_runs_emit_t<int32_t>(a, thr, prefix, st);
+    case 3: return launch_slice_runs_emit_t<int64_t>(a, thr, prefix, st);
+    case 4: return launch_slice_runs_emit_t<uint8_t>(a, thr, prefix, st);
+    case 5: return launch_slice_runs_emit_t<uint16_t>(a, thr, prefix, st);
+    case 6: return launch_slice_runs_emit_t<uint32_t>(a, thr, prefix, st);
+    case 7: return launch_slice_runs_emit_t<uint64_t>(a, thr, prefix, st);
+    case 8: return launch_slice_runs_emit_t<float>(a, thr, prefix, st);
+    default: return launch_slice_runs_emit_t<double>(a, thr, prefix, st);
+    }
 }
 
 hipError_t launch_slice_emit(int dtype, const JobArgs& a, hipStream_t st)

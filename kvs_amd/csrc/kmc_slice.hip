@@ -12,6 +12,7 @@
  */
 #include <hip/hip_runtime.h>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -35,6 +36,93 @@ void default_color_range(int type, double mn, double mx, float range[2])
     range[1] = (float)mx;
     const float d = range[0] - range[1];
     if ((double)(d < 0 ? -d : d) < 1.e-6) range[1] = range[1] + 1.0f;   // Math::Equal -> max + 1
+}
+
+/* SlicePlane::substitute_plane_equation in float, left to right (:936-971). */
+float plane_value(const float c[4], float x, float y, float z)
+{
+    float s = c[0] * x;
+    s = s + c[1] * y;
+    s = s + c[2] * z;
+    return s + c[3];
+}
+
+/* The threshold path (one threshold per node row, the plane being monotone
+ * along x) when the plane value is finite at the eight corners of the node
+ * box -- monotone in each coordinate, it is then finite everywhere inside --
+ * unless KMC_SLICE=classify asks for the classification of every node. */
+bool threshold_path(const kmc_volume* v, const float coef[4])
+{
+    const char* env = std::getenv("KMC_SLICE");
+    if (env && std::strcmp(env, "classify") == 0) return false;
+    for (int k = 0; k < 8; ++k) {
+        const float x = (k & 1) ? (float)(v->resolution[0] - 1) : 0.0f;
+        const float y = (k & 2) ? (float)(v->resolution[1] - 1) : 0.0f;
+        const float z = (k & 4) ? (float)(v->resolution[2] - 1) : 0.0f;
+        if (!std::isfinite(plane_value(coef, x, y, z))) return false;
+    }
+    return true;
+}
+
+/* Thresholds, per cell row triangle counts, their scan, then one thread per
+ * triangle (no classification masks, no records). */
+template <typename Alloc>
+kmc_status slice_by_thresholds(const kmc_volume* v, JobArgs& a, hipStream_t st, Alloc& alloc, kmc_surface* out,
+                               double mn, double mx, const float minc[3], const float maxc[3], const uint8_t* table,
+                               uint32_t res)
+{
+    const uint64_t rows = (uint64_t)a.ny * a.nz, ncr = (uint64_t)(a.ny - 1) * (a.nz - 1);
+    const uint32_t nparts = scan_partials_needed(ncr ? ncr : 1, 1);
+    int32_t* thr = (int32_t*)alloc(rows * 4);
+    uint32_t* counts = (uint32_t*)alloc((ncr ? ncr : 1) * 4);
+    uint64_t* prefix = (uint64_t*)alloc((ncr ? ncr : 1) * 8);
+    uint64_t* aux = (uint64_t*)alloc((nparts + 1) * 8);
+    uint8_t* dtable = (uint8_t*)alloc(3 * (size_t)res);
+    if (!thr || !counts || !prefix || !aux || !dtable) return KMC_ERR_OUT_OF_MEMORY;
+    a.ctable = dtable;
+    uint64_t nt = 0;
+    if (hipMemcpyAsync(dtable, table, 3 * (size_t)res, hipMemcpyHostToDevice, st) != hipSuccess ||
+        (ncr && (launch_slice_runs_front(a, thr, counts, st) != hipSuccess ||
+                 launch_scan(counts, ncr, 1, aux, prefix, aux + nparts, st) != hipSuccess ||
+                 hipMemcpyAsync(&nt, aux + nparts, 8, hipMemcpyDeviceToHost, st) != hipSuccess)) ||
+        hipStreamSynchronize(st) != hipSuccess) {
+        (void)hipGetLastError();
+        return KMC_ERR_DEVICE;
+    }
+    init_surface(out, 0.0, KMC_POLYGON_NORMAL, 1, mn, mx, minc, maxc);
+    out->color_type = 0;   // VertexColor
+    SurfaceStorage* store = new SurfaceStorage();
+    store->memory = KMC_MEMORY_DEVICE;
+    out->internal = store;
+    out->memory = KMC_MEMORY_DEVICE;
+    if (nt == 0) return KMC_OK;
+    out->n_triangles = nt;
+    out->n_vertices = 3 * nt;
+    out->n_normals = nt;
+    out->n_colors = 3 * nt;
+    store->coords = device_alloc(9 * nt * sizeof(float));
+    store->normals = device_alloc(3 * nt * sizeof(float));
+    store->colors = device_alloc(9 * nt);
+    if (!store->coords || !store->normals || !store->colors) {
+        kmc_surface_release(out);
+        return KMC_ERR_OUT_OF_MEMORY;
+    }
+    out->coords = (float*)store->coords;
+    out->normals = (float*)store->normals;
+    out->colors = (uint8_t*)store->colors;
+    a.toff[0] = 0;
+    a.toff[1] = nt;
+    a.coords[0] = out->coords;
+    a.normals[0] = out->normals;
+    a.colors = out->colors;
+    a.cap_vertices[0] = 3 * nt;
+    a.cap_triangles[0] = nt;
+    if (launch_slice_runs_emit(v->type, a, thr, prefix, st) != hipSuccess || hipStreamSynchronize(st) != hipSuccess) {
+        (void)hipGetLastError();
+        kmc_surface_release(out);
+        return KMC_ERR_DEVICE;
+    }
+    return KMC_OK;
 }
 
 kmc_status slice_plane_impl(const kmc_volume* v, const float coef[4], const uint8_t* table, uint32_t res,
@@ -86,6 +174,15 @@ kmc_status slice_plane_impl(const kmc_volume* v, const float coef[4], const uint
     if (color_range) { a.crange[0] = color_range[0]; a.crange[1] = color_range[1]; }
     else default_color_range(v->type, mn, mx, a.crange);
     a.cres = res;
+
+    if (threshold_path(v, coef)) {
+        s = slice_by_thresholds(v, a, st, alloc, out, mn, mx, minc, maxc, table, res);
+        if (s == KMC_OK && out_memory == KMC_MEMORY_HOST) {
+            s = to_host(out, st);
+            if (s != KMC_OK) kmc_surface_release(out);
+        }
+        return finish(s);
+    }
 
     const uint64_t R = a.nrows, C = a.nrows * a.NC;
     const uint32_t nparts = scan_partials_needed(R, 2);

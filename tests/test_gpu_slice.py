@@ -70,3 +70,25 @@ def test_slice_plane_missing_the_volume_is_empty():
     v = datasets.blobs(20, seed=1)
     got = kmc.slice_plane(v, [1.0, 0.0, 0.0, 100.0], table_of(0))
     assert got["n_triangles"] == 0 and got["coords"].size == 0 and got["colors"].size == 0
+
+
+def test_slice_plane_both_device_paths(monkeypatch):
+    # the threshold path (a bisection per node row, runs of cube indices per
+    # cell row) and the classification of every node give the same surface;
+    # axis-aligned, tiny-c0, negative-c0 and oblique planes, and planes whose
+    # values overflow to infinity at the volume's corners (classification path)
+    v = datasets.turbulence((70, 33, 29), seed=8, period=64)
+    table = table_of(0)
+    planes = [[0.0, 0.0, 1.0, -14.5], [0.0, 1.0, 0.0, -10.0], [1.0, 0.0, 0.0, -35.0], [1e-7, 1.0, 0.5, -20.0],
+              [-1.3, 0.2, 0.7, 40.0], [0.3, -0.5, 0.8, -3.0], [2.0, 3.0, -1.0, -60.0], [3e38, 3e38, 0.0, -1.0]]
+    for coef in planes:
+        coef = [float(np.float32(c)) for c in coef]
+        want = oracle.slice_plane(v, coef, table)
+        monkeypatch.delenv("KMC_SLICE", raising=False)
+        fast = kmc.slice_plane(v, coef, table)
+        monkeypatch.setenv("KMC_SLICE", "classify")
+        full = kmc.slice_plane(v, coef, table)
+        print("case plane", coef, want["n_triangles"], flush=True)
+        same(fast, want, f"threshold path {coef}")
+        same(full, want, f"classification path {coef}")
+    assert want["n_triangles"] >= 0
