@@ -89,8 +89,8 @@ hipError_t launch_slice_emit(int dtype, const JobArgs& a, hipStream_t st);
  * row triangle counts ((ny - 1) * (nz - 1) uint32); then, with their
  * exclusive prefix, one thread per triangle (a.toff[1] triangles) */
 hipError_t launch_slice_runs_front(const JobArgs& a, int32_t* thr, uint32_t* counts, hipStream_t st);
-hipError_t launch_slice_runs_emit(int dtype, const JobArgs& a, const int32_t* thr, const uint64_t* prefix,
-                                  hipStream_t st);
+hipError_t launch_slice_runs_emit(int dtype, const JobArgs& a, const int32_t* thr, const uint32_t* counts,
+                                  const uint64_t* prefix, uint64_t* units, hipStream_t st);
 hipError_t launch_count(const JobArgs& a, hipStream_t st);
 hipError_t launch_scan(const uint32_t* counts, uint64_t n, uint32_t narr, uint64_t* partials,
                        uint64_t* prefix, uint64_t* totals, hipStream_t st);

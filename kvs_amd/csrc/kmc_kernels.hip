@@ -2189,11 +2189,26 @@ __device__ __forceinline__ int round_half_away(float a) { return a >= 0 ? (int)(
 
 __device__ __forceinline__ float clampf(float a, float lo, float hi) { return a <= lo ? lo : (a <= hi ? a : hi); }
 
+/* Colour-map entry s as r | g << 8 | b << 16: from the device table (3 bytes
+ * an entry) or from a packed copy in LDS. */
+struct GlobalColors {
+    const uint8_t* t;
+    __device__ __forceinline__ uint32_t operator()(uint32_t s) const
+    {
+        return (uint32_t)t[3 * s] | (uint32_t)t[3 * s + 1] << 8 | (uint32_t)t[3 * s + 2] << 16;
+    }
+};
+struct LdsColors {
+    const uint32_t* t;
+    __device__ __forceinline__ uint32_t operator()(uint32_t s) const { return t[s]; }
+};
+
 /* ColorMap::at (ColorMap.cpp:1043-1054): the two table entries around the
  * value, mixed by RGBColor::Mix (RGBColor.cpp:23-31) in [0,1] floats,
  * x*(1-t) + y*t, and rounded back to bytes.  A NaN position (an empty range)
  * takes the first entry. */
-__device__ __forceinline__ void color_at(const JobArgs& a, float value, uint8_t rgb[3])
+template <typename C>
+__device__ __forceinline__ void color_at(const JobArgs& a, const C& tab, float value, uint8_t rgb[3])
 {
     const float lo = a.crange[0], hi = a.crange[1];
     const float v0 = clampf(value, lo, hi);
@@ -2203,10 +2218,11 @@ __device__ __forceinline__ void color_at(const JobArgs& a, float value, uint8_t 
     const uint32_t s0 = (uint32_t)v;
     const uint32_t s1 = (s0 + 1 < a.cres - 1) ? s0 + 1 : a.cres - 1;
     const float t = v - (float)s0;
+    const uint32_t e0 = tab(s0), e1 = tab(s1);
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-        const float c0 = (float)a.ctable[3 * s0 + k] / 255.0f;
-        const float c1 = (float)a.ctable[3 * s1 + k] / 255.0f;
+        const float c0 = (float)((e0 >> (8 * k)) & 0xFFu) / 255.0f;
+        const float c1 = (float)((e1 >> (8 * k)) & 0xFFu) / 255.0f;
         const float p = c0 * (1 - t);
         const float q = c1 * t;
         rgb[k] = (uint8_t)round_half_away(clampf(p + q, 0.0f, 1.0f) * 255.0f);
@@ -2218,9 +2234,9 @@ __device__ __forceinline__ void color_at(const JobArgs& a, float value, uint8_t 
  * (interpolate_vertex :981-990, float), (p + min) * scale, and the colour of
  * the value interpolated at the same ratio; the normal is
  * -(v2 - v0) x (v1 - v0). */
-template <typename T>
-__device__ __forceinline__ void slice_triangle(const JobArgs& a, const uint64_t* tri, uint64_t id, uint64_t rec,
-                                               uint32_t ci)
+template <typename T, typename C, typename Sink>
+__device__ __forceinline__ void slice_triangle_values(const JobArgs& a, const uint64_t* tri, uint64_t rec, uint32_t ci,
+                                                      const C& colors, Sink& out)
 {
     const T* vol = (const T*)a.values;
     const uint32_t x = (uint32_t)(rec & 0xFFFFFu), y = (uint32_t)((rec >> 20) & 0xFFFFFu);
@@ -2230,7 +2246,6 @@ __device__ __forceinline__ void slice_triangle(const JobArgs& a, const uint64_t*
                               (uint32_t)((w >> (12 * t + 4)) & 0xF) };
     const uint64_t line = a.nx, slice = (uint64_t)a.nx * a.ny;
     float p[3][3];
-    uint8_t rgb[3][3];
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
         const uint32_t qq = edge_ends(eds[j]);
@@ -2252,31 +2267,47 @@ __device__ __forceinline__ void slice_triangle(const JobArgs& a, const uint64_t*
         }
         const T va = vol[ax + ay * line + az * slice];
         const T vb = vol[bx + by * line + bz * slice];
-        color_at(a, (float)slice_value<T>(va, vb, r), rgb[j]);
+        uint8_t rgb[3];
+        color_at(a, colors, (float)slice_value<T>(va, vb, r), rgb);
+        out.vertex(j, p[j], rgb);
     }
     const float d1x = p[2][0] - p[0][0], d1y = p[2][1] - p[0][1], d1z = p[2][2] - p[0][2];
     const float d2x = p[1][0] - p[0][0], d2y = p[1][1] - p[0][1], d2z = p[1][2] - p[0][2];
-    const float nx = d1y * d2z - d1z * d2y;
-    const float ny = d1z * d2x - d1x * d2z;
-    const float nz = d1x * d2y - d1y * d2x;
-    float* co = a.coords[0] + 9 * id;
-#pragma unroll
-    for (int j = 0; j < 3; ++j) {
-        co[3 * j + 0] = p[j][0];
-        co[3 * j + 1] = p[j][1];
-        co[3 * j + 2] = p[j][2];
+    const float nn[3] = { -(d1y * d2z - d1z * d2y), -(d1z * d2x - d1x * d2z), -(d1x * d2y - d1y * d2x) };
+    out.normal(nn);
+}
+
+/* Where a triangle's corners (coordinates, colour) and normal go: arrays of
+ * 9, 9 and 3 elements a triangle, from element 0 of triangle 0. */
+template <typename F, typename B>
+struct SliceSink {
+    F* co;
+    F* no;
+    B* cl;
+    __device__ __forceinline__ void vertex(int j, const float p[3], const uint8_t rgb[3])
+    {
+        co[3 * j + 0] = p[0];
+        co[3 * j + 1] = p[1];
+        co[3 * j + 2] = p[2];
+        cl[3 * j + 0] = rgb[0];
+        cl[3 * j + 1] = rgb[1];
+        cl[3 * j + 2] = rgb[2];
     }
-    float* no = a.normals[0] + 3 * id;
-    no[0] = -nx;
-    no[1] = -ny;
-    no[2] = -nz;
-    uint8_t* cl = a.colors + 9 * id;
-#pragma unroll
-    for (int j = 0; j < 3; ++j) {
-        cl[3 * j + 0] = rgb[j][0];
-        cl[3 * j + 1] = rgb[j][1];
-        cl[3 * j + 2] = rgb[j][2];
+    __device__ __forceinline__ void normal(const float n[3])
+    {
+        no[0] = n[0];
+        no[1] = n[1];
+        no[2] = n[2];
     }
+};
+
+/* Triangle `id` of the output, written by its own thread. */
+template <typename T>
+__device__ __forceinline__ void slice_triangle(const JobArgs& a, const uint64_t* tri, uint64_t id, uint64_t rec,
+                                               uint32_t ci)
+{
+    SliceSink<float, uint8_t> out{ a.coords[0] + 9 * id, a.normals[0] + 3 * id, a.colors + 9 * id };
+    slice_triangle_values<T>(a, tri, rec, ci, GlobalColors{ a.ctable }, out);
 }
 
 template <typename T>
@@ -2299,7 +2330,7 @@ __global__ __launch_bounds__(kBlock) void k_slice_triangles(JobArgs a)
  * cell row, a scan, then one thread per triangle.  The host takes this path
  * when the plane is finite at the volume's eight corners (and so, being
  * monotone in each coordinate, everywhere inside). */
-__device__ __forceinline__ bool above_thr(int dir, int64_t t, int64_t x) { return dir < 0 ? x < t : x >= t; }
+__device__ __forceinline__ bool above_thr(int dir, int32_t t, int32_t x) { return dir < 0 ? x < t : x >= t; }
 
 __global__ __launch_bounds__(kBlock) void k_slice_thresholds(JobArgs a, int32_t* thr)
 {
@@ -2325,14 +2356,18 @@ __global__ __launch_bounds__(kBlock) void k_slice_thresholds(JobArgs a, int32_t*
     }
 }
 
-/* The active cells of cell row (y, z) as runs of one cube index. */
+/* The active cells of cell row (y, z) as nine runs [b[i], b[i + 1]) of one
+ * cube index each (ci(i)): the breakpoints t - 1 and t of the four node rows,
+ * clamped to [lo, hi + 1] and sorted, so that coinciding or outside points
+ * give empty runs.  Every index is static, so the runs stay in registers. */
 struct SliceRuns {
-    uint32_t n;
-    uint32_t start[10];   // run i covers cells [start[i], start[i + 1])
-    uint32_t ci[9];
+    int32_t b[10];
+    int32_t t[4];   // thresholds of node rows (y,z), (y+1,z), (y,z+1), (y+1,z+1)
+    int dir;
+    __device__ __forceinline__ uint32_t ci(int i) const;
 };
 
-__device__ __forceinline__ uint32_t slice_cube_index(int dir, const int64_t t[4], int64_t x)
+__device__ __forceinline__ uint32_t slice_cube_index(int dir, const int32_t t[4], int32_t x)
 {
     /* corners 0..7: (x,y,z) (x+1,y,z) (x+1,y+1,z) (x,y+1,z) and the same at z+1;
      * t: node rows (y,z), (y+1,z), (y,z+1), (y+1,z+1) */
@@ -2342,81 +2377,164 @@ __device__ __forceinline__ uint32_t slice_cube_index(int dir, const int64_t t[4]
            (uint32_t)above_thr(dir, t[3], x + 1) << 6 | (uint32_t)above_thr(dir, t[3], x) << 7;
 }
 
-__device__ __forceinline__ void slice_runs(const JobArgs& a, const int32_t* thr, uint32_t y, uint32_t z, SliceRuns& r)
+__device__ __forceinline__ uint32_t SliceRuns::ci(int i) const { return slice_cube_index(dir, t, b[i]); }
+
+__device__ __forceinline__ void order2(int32_t& p, int32_t& q)
 {
-    const int dir = a.plane[0] < 0.0f ? -1 : 1;
+    const int32_t lo = p < q ? p : q, hi = p < q ? q : p;
+    p = lo;
+    q = hi;
+}
+
+/* Returns false when the cell row has no active cell. */
+__device__ __forceinline__ bool slice_runs(const JobArgs& a, const int32_t* thr, uint32_t y, uint32_t z, SliceRuns& r)
+{
+    r.dir = a.plane[0] < 0.0f ? -1 : 1;
     const uint64_t r0 = (uint64_t)z * a.ny + y;
-    const int64_t t[4] = { thr[r0], thr[r0 + 1], thr[r0 + a.ny], thr[r0 + a.ny + 1] };
-    int64_t tmin = t[0], tmax = t[0];
-    for (int k = 1; k < 4; ++k) { tmin = t[k] < tmin ? t[k] : tmin; tmax = t[k] > tmax ? t[k] : tmax; }
-    const int64_t lo = tmin - 1 > 0 ? tmin - 1 : 0;
-    const int64_t hi = tmax - 1 < (int64_t)a.nx - 2 ? tmax - 1 : (int64_t)a.nx - 2;
-    r.n = 0;
-    if (lo > hi) return;
-    /* breakpoints strictly inside (lo, hi + 1): t - 1 and t of every row */
-    int64_t b[10];
-    uint32_t nb = 0;
-    b[nb++] = lo;
-    for (int k = 0; k < 4; ++k)
-        for (int64_t c = t[k] - 1; c <= t[k]; ++c)
-            if (c > lo && c <= hi) b[nb++] = c;
-    for (uint32_t i = 1; i < nb; ++i)   // insertion sort (at most nine points)
-        for (uint32_t j = i; j > 0 && b[j - 1] > b[j]; --j) { const int64_t u = b[j]; b[j] = b[j - 1]; b[j - 1] = u; }
-    for (uint32_t i = 0; i < nb; ++i) {
-        if (r.n && (int64_t)r.start[r.n - 1] == b[i]) continue;   // duplicate point
-        r.start[r.n] = (uint32_t)b[i];
-        r.ci[r.n] = slice_cube_index(dir, t, b[i]);
-        ++r.n;
+    int32_t* t = r.t;
+    t[0] = thr[r0];
+    t[1] = thr[r0 + 1];
+    t[2] = thr[r0 + a.ny];
+    t[3] = thr[r0 + a.ny + 1];
+    const int32_t tmin = min(min(t[0], t[1]), min(t[2], t[3]));
+    const int32_t tmax = max(max(t[0], t[1]), max(t[2], t[3]));
+    const int32_t lo = max(tmin - 1, 0), hi = min(tmax - 1, (int32_t)a.nx - 2);
+    if (lo > hi) return false;
+    int32_t* b = r.b;
+    b[0] = lo;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        b[1 + 2 * k] = min(max(t[k] - 1, lo), hi + 1);
+        b[2 + 2 * k] = min(max(t[k], lo), hi + 1);
     }
-    r.start[r.n] = (uint32_t)(hi + 1);
+    b[9] = hi + 1;
+    /* sorting network of b[1..8] (19 comparators) */
+    order2(b[1], b[3]); order2(b[2], b[4]); order2(b[5], b[7]); order2(b[6], b[8]);
+    order2(b[1], b[5]); order2(b[2], b[6]); order2(b[3], b[7]); order2(b[4], b[8]);
+    order2(b[1], b[2]); order2(b[3], b[4]); order2(b[5], b[6]); order2(b[7], b[8]);
+    order2(b[3], b[5]); order2(b[4], b[6]);
+    order2(b[2], b[5]); order2(b[4], b[7]);
+    order2(b[2], b[3]); order2(b[4], b[5]); order2(b[6], b[7]);
+    return true;
 }
 
 __global__ __launch_bounds__(kBlock) void k_slice_row_counts(JobArgs a, const int32_t* thr, uint32_t* counts)
 {
+    __shared__ uint8_t ntri[256];
+    for (uint32_t i = threadIdx.x; i < 256; i += kBlock) ntri[i] = (uint8_t)(d_tri[i] >> 60);
+    __syncthreads();
     const uint64_t ncr = (uint64_t)(a.ny - 1) * (a.nz - 1), step = (uint64_t)gridDim.x * kBlock;
     for (uint64_t cr = (uint64_t)blockIdx.x * kBlock + threadIdx.x; cr < ncr; cr += step) {
         const uint32_t z = (uint32_t)(cr / (a.ny - 1)), y = (uint32_t)(cr - (uint64_t)z * (a.ny - 1));
         SliceRuns r;
-        slice_runs(a, thr, y, z, r);
         uint32_t c = 0;
-        for (uint32_t i = 0; i < r.n; ++i) c += (r.start[i + 1] - r.start[i]) * (uint32_t)(d_tri[r.ci[i]] >> 60);
+        if (slice_runs(a, thr, y, z, r)) {
+#pragma unroll
+            for (int i = 0; i < 9; ++i) c += (uint32_t)(r.b[i + 1] - r.b[i]) * ntri[r.ci(i)];
+        }
         counts[cr] = c;
     }
 }
 
-/* One thread per triangle: its cell row (the last with prefix <= g), then its
- * cell and table index within the row's runs. */
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_slice_triangles_runs(JobArgs a, const int32_t* thr, const uint64_t* prefix)
+/* The cell rows of the first and the last triangle of every unit of 64
+ * triangles (unit u: triangles 64u .. min(64u + 63, n - 1)), written by the
+ * rows that hold them: one writer each. */
+__global__ __launch_bounds__(kBlock) void k_slice_unit_rows(uint64_t ncr, const uint32_t* counts, const uint64_t* prefix,
+                                                           uint64_t n, uint64_t* first, uint64_t* last)
+{
+    const uint64_t step = (uint64_t)gridDim.x * kBlock;
+    for (uint64_t cr = (uint64_t)blockIdx.x * kBlock + threadIdx.x; cr < ncr; cr += step) {
+        const uint32_t c = counts[cr];
+        if (!c) continue;
+        const uint64_t p = prefix[cr], e = p + c;   // the row's triangles [p, e)
+        for (uint64_t u = p >> 6; (u << 6) < e; ++u) {
+            const uint64_t f = u << 6, l = f + 63 < n ? f + 63 : n - 1;
+            if (f >= p) first[u] = cr;
+            if (l >= p && l < e) last[u] = cr;
+        }
+    }
+}
+
+/* Colour maps of up to this many entries are staged in LDS by the run emit. */
+constexpr uint32_t kSliceLdsColors = 1024;
+
+/* One thread per triangle.  A wave's 64 triangles lie in the cell rows
+ * between those of its first and last (k_slice_unit_rows); each lane bisects
+ * that short range, then walks its row's runs to its cell and table index.
+ * The wave's 64 triangles are consecutive in the output: their coordinates,
+ * normals and colours are assembled in LDS and written as whole 16-byte
+ * words. */
+template <typename T, bool kLdsColors>
+__global__ __launch_bounds__(kBlock) void k_slice_triangles_runs(JobArgs a, const int32_t* thr, const uint64_t* prefix,
+                                                                const uint64_t* unit_first, const uint64_t* unit_last)
 {
     __shared__ uint64_t tri[256];
+    __shared__ __align__(16) float stage[kWaves][9 * 64 + 3 * 64 + 9 * 64 / 4];   // coords, normals, colours
+    extern __shared__ __align__(16) uint8_t smem[];
+    uint32_t* ctab = (uint32_t*)smem;
+    if (kLdsColors)
+        for (uint32_t i = threadIdx.x; i < a.cres; i += kBlock) ctab[i] = GlobalColors{ a.ctable }(i);
     stage_table_block(tri);
-    const uint64_t ncr = (uint64_t)(a.ny - 1) * (a.nz - 1), n = a.toff[1];
+    const uint64_t n = a.toff[1];
+    const uint32_t lane = threadIdx.x & 63u;
+    float* sc = stage[threadIdx.x >> 6];
+    float* sn = sc + 9 * 64;
+    uint8_t* scl = (uint8_t*)(sn + 3 * 64);
     const Span sp = grid_span(0, n, kBlock, threadIdx.x);
-    for (uint64_t g = sp.first; g < sp.end; g += sp.step) {
-        uint64_t lo = 0, hi = ncr - 1;
-        while (lo < hi) {
-            const uint64_t mid = (lo + hi + 1) >> 1;
-            if (prefix[mid] <= g) lo = mid; else hi = mid - 1;
-        }
-        const uint32_t z = (uint32_t)(lo / (a.ny - 1)), y = (uint32_t)(lo - (uint64_t)z * (a.ny - 1));
-        SliceRuns r;
-        slice_runs(a, thr, y, z, r);
-        uint64_t k = g - prefix[lo];
-        uint32_t x = 0, t = 0, ci = 0;
-        for (uint32_t i = 0; i < r.n; ++i) {
-            const uint32_t nt = (uint32_t)(tri[r.ci[i]] >> 60);
-            const uint64_t m = (uint64_t)(r.start[i + 1] - r.start[i]) * nt;
-            if (k < m) {
-                x = r.start[i] + (uint32_t)(k / nt);
-                t = (uint32_t)(k % nt);
-                ci = r.ci[i];
-                break;
+    for (uint64_t gw = sp.first - lane; gw < n; gw += sp.step) {   // wave-uniform
+        uint64_t lo = unit_first[gw >> 6], hi = unit_last[gw >> 6];
+        const uint64_t g = gw + lane;
+        if (g < n) {
+            while (lo < hi) {
+                const uint64_t mid = (lo + hi + 1) >> 1;
+                if (prefix[mid] <= g) lo = mid; else hi = mid - 1;
             }
-            k -= m;
+            const uint32_t z = (uint32_t)(lo / (a.ny - 1)), y = (uint32_t)(lo - (uint64_t)z * (a.ny - 1));
+            SliceRuns r;
+            slice_runs(a, thr, y, z, r);   // the row has triangles, so active cells
+            uint32_t k = (uint32_t)(g - prefix[lo]);
+            uint32_t x = 0, t = 0, ci = 0;
+            bool found = false;
+#pragma unroll
+            for (int i = 0; i < 9; ++i) {
+                const uint32_t ci_i = r.ci(i);
+                const uint32_t nt = (uint32_t)(tri[ci_i] >> 60);
+                const uint32_t m = (uint32_t)(r.b[i + 1] - r.b[i]) * nt;
+                if (!found) {
+                    if (k < m) {
+                        x = (uint32_t)r.b[i] + k / nt;
+                        t = k % nt;
+                        ci = ci_i;
+                        found = true;
+                    } else {
+                        k -= m;
+                    }
+                }
+            }
+            const uint64_t rec = (uint64_t)x | (uint64_t)y << 20 | (uint64_t)z << 40 | (uint64_t)t << 60;
+            SliceSink<float, uint8_t> out{ sc + 9 * lane, sn + 3 * lane, scl + 9 * lane };
+            if (kLdsColors)
+                slice_triangle_values<T>(a, tri, rec, ci, LdsColors{ ctab }, out);
+            else
+                slice_triangle_values<T>(a, tri, rec, ci, GlobalColors{ a.ctable }, out);
         }
-        const uint64_t rec = (uint64_t)x | (uint64_t)y << 20 | (uint64_t)z << 40 | (uint64_t)t << 60;
-        slice_triangle<T>(a, tri, g, rec, ci);
+        wave_sync();
+        /* the wave's output: triangles gw .. gw + cnt - 1 (9 * gw floats and
+         * bytes from the array starts: 16-byte aligned when cnt is 64) */
+        const uint32_t cnt = n - gw < 64 ? (uint32_t)(n - gw) : 64u;
+        float* co = a.coords[0] + 9 * gw;
+        float* no = a.normals[0] + 3 * gw;
+        uint8_t* cl = a.colors + 9 * gw;
+        if (cnt == 64) {
+            for (uint32_t i = lane; i < 9 * 64 / 4; i += 64) ((float4*)co)[i] = ((const float4*)sc)[i];
+            if (lane < 3 * 64 / 4) ((float4*)no)[lane] = ((const float4*)sn)[lane];
+            if (lane < 9 * 64 / 16) ((uint4*)cl)[lane] = ((const uint4*)scl)[lane];
+        } else {
+            for (uint32_t i = lane; i < 9 * cnt; i += 64) co[i] = sc[i];
+            for (uint32_t i = lane; i < 3 * cnt; i += 64) no[i] = sn[i];
+            for (uint32_t i = lane; i < 9 * cnt; i += 64) cl[i] = scl[i];
+        }
+        wave_sync();
     }
 }
 
@@ -2931,27 +3049,38 @@ hipError_t launch_slice_runs_front(const JobArgs& a, int32_t* thr, uint32_t* cou
 }
 
 template <typename T>
-hipError_t launch_slice_runs_emit_t(const JobArgs& a, const int32_t* thr, const uint64_t* prefix, hipStream_t st)
+hipError_t launch_slice_runs_emit_t(const JobArgs& a, const int32_t* thr, const uint32_t* counts,
+                                    const uint64_t* prefix, uint64_t* units, hipStream_t st)
 {
-    const uint64_t nt = a.toff[1];
-    if (nt) hipLaunchKernelGGL((k_slice_triangles_runs<T>), dim3(flat_blocks(nt)), dim3(kBlock), 0, st, a, thr, prefix);
+    const uint64_t nt = a.toff[1], ncr = (uint64_t)(a.ny - 1) * (a.nz - 1), nunits = (nt + 63) / 64;
+    if (!nt) return hipSuccess;
+    uint64_t* first = units;
+    uint64_t* last = units + nunits;
+    hipLaunchKernelGGL(k_slice_unit_rows, dim3(flat_blocks(ncr)), dim3(kBlock), 0, st, ncr, counts, prefix, nt, first,
+                       last);
+    if (a.cres <= kSliceLdsColors)
+        hipLaunchKernelGGL((k_slice_triangles_runs<T, true>), dim3(flat_blocks(nt)), dim3(kBlock), 4 * a.cres, st, a,
+                           thr, prefix, first, last);
+    else
+        hipLaunchKernelGGL((k_slice_triangles_runs<T, false>), dim3(flat_blocks(nt)), dim3(kBlock), 0, st, a, thr,
+                           prefix, first, last);
     return hipGetLastError();
 }
 
-hipError_t launch_slice_runs_emit(int dtype, const JobArgs& a, const int32_t* thr, const uint64_t* prefix,
-                                  hipStream_t st)
+hipError_t launch_slice_runs_emit(int dtype, const JobArgs& a, const int32_t* thr, const uint32_t* counts,
+                                  const uint64_t* prefix, uint64_t* units, hipStream_t st)
 {
     switch (dtype) {
-    case 0: return launch_slice_runs_emit_t<int8_t>(a, thr, prefix, st);
-    case 1: return launch_slice_runs_emit_t<int16_t>(a, thr, prefix, st);
-    case 2: return launch_slice_runs_emit_t<int32_t>(a, thr, prefix, st);
-    case 3: return launch_slice_runs_emit_t<int64_t>(a, thr, prefix, st);
-    case 4: return launch_slice_runs_emit_t<uint8_t>(a, thr, prefix, st);
-    case 5: return launch_slice_runs_emit_t<uint16_t>(a, thr, prefix, st);
-    case 6: return launch_slice_runs_emit_t<uint32_t>(a, thr, prefix, st);
-    case 7: return launch_slice_runs_emit_t<uint64_t>(a, thr, prefix, st);
-    case 8: return launch_slice_runs_emit_t<float>(a, thr, prefix, st);
-    default: return launch_slice_runs_emit_t<double>(a, thr, prefix, st);
+    case 0: return launch_slice_runs_emit_t<int8_t>(a, thr, counts, prefix, units, st);
+    case 1: return launch_slice_runs_emit_t<int16_t>(a, thr, counts, prefix, units, st);
+    case 2: return launch_slice_runs_emit_t<int32_t>(a, thr, counts, prefix, units, st);
+    case 3: return launch_slice_runs_emit_t<int64_t>(a, thr, counts, prefix, units, st);
+    case 4: return launch_slice_runs_emit_t<uint8_t>(a, thr, counts, prefix, units, st);
+    case 5: return launch_slice_runs_emit_t<uint16_t>(a, thr, counts, prefix, units, st);
+    case 6: return launch_slice_runs_emit_t<uint32_t>(a, thr, counts, prefix, units, st);
+    case 7: return launch_slice_runs_emit_t<uint64_t>(a, thr, counts, prefix, units, st);
+    case 8: return launch_slice_runs_emit_t<float>(a, thr, counts, prefix, units, st);
+    default: return launch_slice_runs_emit_t<double>(a, thr, counts, prefix, units, st);
     }
 }
 

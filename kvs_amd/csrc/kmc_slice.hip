@@ -117,7 +117,13 @@ kmc_status slice_by_thresholds(const kmc_volume* v, JobArgs& a, hipStream_t st, 
     a.colors = out->colors;
     a.cap_vertices[0] = 3 * nt;
     a.cap_triangles[0] = nt;
-    if (launch_slice_runs_emit(v->type, a, thr, prefix, st) != hipSuccess || hipStreamSynchronize(st) != hipSuccess) {
+    uint64_t* units = (uint64_t*)alloc(2 * ((nt + 63) / 64) * 8);   // first and last cell row of each 64 triangles
+    if (!units) {
+        kmc_surface_release(out);
+        return KMC_ERR_OUT_OF_MEMORY;
+    }
+    if (launch_slice_runs_emit(v->type, a, thr, counts, prefix, units, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess) {
         (void)hipGetLastError();
         kmc_surface_release(out);
         return KMC_ERR_DEVICE;

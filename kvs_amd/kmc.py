@@ -10,7 +10,9 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libkmc.so")
+# KMC_LIBRARY names another build of the library (a different install or a
+# build of modified sources); by default the in-tree build.
+LIB_PATH = os.environ.get("KMC_LIBRARY") or os.path.join(HERE, "libkmc.so")
 
 # kvs::Type::TypeID order (Source/Core/Utility/Type.h:40-53)
 INT8, INT16, INT32, INT64, UINT8, UINT16, UINT32, UINT64, REAL32, REAL64 = range(10)

@@ -92,3 +92,20 @@ def test_slice_plane_both_device_paths(monkeypatch):
         same(fast, want, f"threshold path {coef}")
         same(full, want, f"classification path {coef}")
     assert want["n_triangles"] >= 0
+
+
+@pytest.mark.parametrize("res", [1, 2, 1024, 1025, 4000])
+def test_slice_plane_color_map_resolutions(monkeypatch, res):
+    # the threshold path keeps colour maps of up to 1024 entries in LDS and
+    # reads larger ones from device memory; both paths against the reference's
+    # own kvs::SlicePlane with a transfer function of that resolution
+    v = datasets.turbulence((40, 30, 20), seed=3, period=32)
+    coef = [0.3, -0.5, 0.8, -3.0]
+    for kind in (0, 1):
+        table = np.asarray(oracle.ref_tf_table(kind, res), dtype=np.uint8)
+        want = oracle.ref_slice_plane(v, coef, tf_kind=kind, tf_resolution=res)
+        print("case colour map", res, kind, flush=True)
+        monkeypatch.delenv("KMC_SLICE", raising=False)
+        same(kmc.slice_plane(v, coef, table), want, f"threshold path {res} {kind}")
+        monkeypatch.setenv("KMC_SLICE", "classify")
+        same(kmc.slice_plane(v, coef, table), want, f"classification path {res} {kind}")

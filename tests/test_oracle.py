@@ -250,6 +250,20 @@ def test_slice_plane_equals_reference_fresh():
             assert np.array_equal(a["colors"], b["colors"]), (t, kind)
 
 
+@pytest.mark.skipif(not oracle.ref_available(), reason="reference library not built here")
+def test_slice_plane_color_map_resolutions():
+    # ColorMap::at with one entry (both neighbours the same), two, and tables
+    # larger than the default 256
+    v = datasets.turbulence((40, 30, 20), seed=3, period=32)
+    coef = [0.3, -0.5, 0.8, -3.0]
+    for res in (1, 2, 1024, 1025, 4000):
+        for kind in (0, 1):
+            a = oracle.slice_plane(v, coef, oracle.ref_tf_table(kind, res))
+            b = oracle.ref_slice_plane(v, coef, tf_kind=kind, tf_resolution=res)
+            assert a["n_triangles"] > 0
+            assert bits_equal(a["coords"], b["coords"]) and np.array_equal(a["colors"], b["colors"]), (res, kind)
+
+
 CELL_NAMES = ("tetrahedra", "hexahedra", "pyramid", "prism")
 CELL_SHORT = ("Tet", "Hex", "Pyr", "Prism")
 
