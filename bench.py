@@ -496,7 +496,7 @@ def serving_leg(lib, kmc, datasets, torch, args):
         sweep.append({"offered_qps": round(frac * sat_rate, 1), "achieved_qps": round(len(reqs) / st["span_s"], 1),
                       "p50_ms": round(st["p50_ms"], 3), "p99_ms": round(st["p99_ms"], 3),
                       "mtriangles_per_s": round(st["triangles_per_s"] / 1e6, 1),
-                      "peak_inflight": st["peak_inflight"], "batches": st["batches"]})
+                      "peak_inflight": st["peak_inflight"], "batches": st["batches"], "deferred": st["deferred"]})
     server.close()
     return {
         "workload": f"{len(specs)} resident volumes (cubic, edge 64..512; float32 turbulence, uint16/uint8 "

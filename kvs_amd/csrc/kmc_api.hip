@@ -30,7 +30,13 @@ std::mutex g_pool_mutex;
 std::multimap<size_t, void*> g_free;           // size -> block
 std::unordered_map<void*, size_t> g_size;      // block -> size
 size_t g_cached_bytes = 0;
-const size_t kPoolKeepBytes = size_t(64) << 30;  // cached free bytes that trigger a partial release
+/* Cached free bytes past which a free releases blocks: half of the device's
+ * memory (64 GiB until the first allocation has asked the device).  Every
+ * hipFree waits for the whole device, so a release while other streams are
+ * busy stalls the releasing thread behind all of their kernels; below this
+ * mark blocks are only released when an allocation fails. */
+size_t g_keep_bytes = size_t(64) << 30;
+bool g_keep_known = false;
 const bool g_pool_trace = std::getenv("KMC_POOL_TRACE") != nullptr;
 
 /* Size classes: 4 KiB steps up to 1 MiB, then four classes per power of two
@@ -80,6 +86,16 @@ void* device_alloc(size_t n)
     }
     /* a miss: allocate without holding the pool lock */
     if (g_pool_trace) std::fprintf(stderr, "[kmc_pool] hipMalloc %zu bytes\n", sz);
+    if (!g_keep_known) {
+        size_t fr = 0, total = 0;
+        if (hipMemGetInfo(&fr, &total) == hipSuccess && total > 0) {
+            std::lock_guard<std::mutex> lk(g_pool_mutex);
+            g_keep_bytes = std::max(g_keep_bytes, total / 2);
+            g_keep_known = true;
+        } else {
+            (void)hipGetLastError();
+        }
+    }
     void* p = nullptr;
     if (hipMalloc(&p, sz) != hipSuccess) {
         (void)hipGetLastError();
@@ -110,9 +126,9 @@ void device_free(void* p)
         if (it == g_size.end()) return;
         g_free.emplace(it->second, p);
         g_cached_bytes += it->second;
-        if (g_cached_bytes > kPoolKeepBytes) {
-            /* release the largest cached blocks down to half the limit */
-            while (g_cached_bytes > kPoolKeepBytes / 2 && !g_free.empty()) {
+        if (g_cached_bytes > g_keep_bytes) {
+            /* release the largest cached blocks down to half the mark */
+            while (g_cached_bytes > g_keep_bytes / 2 && !g_free.empty()) {
                 auto last = std::prev(g_free.end());
                 release.push_back(last->second);
                 g_cached_bytes -= last->first;

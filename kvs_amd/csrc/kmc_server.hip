@@ -262,6 +262,33 @@ void run_batch(kmc_server* S, hipStream_t st, const std::vector<uint64_t>& ids)
                 if (admitted[i]) s = surface_to_host(&out[i], st);
     }
     t4 = Clock::now();
+
+    /* deliver before the batch's arrays go back to the pool: a free may
+     * release cached blocks, and hipFree waits for the whole device */
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        for (size_t i = 0; i < n; ++i) {
+            /* after a good front phase only admitted requests are still this
+             * batch's (the others failed early or went back to the queue) */
+            if (front_ok && !admitted[i]) continue;
+            Request& r = S->reqs.at(ids[i]);
+            if (s != KMC_OK) {
+                if (admitted[i]) kmc_surface_release(&out[i]);
+                finish_locked(S, ids[i], r, s);
+            } else {
+                r.result = out[i];
+                finish_locked(S, ids[i], r, KMC_OK);
+            }
+            S->used -= r.scratch + (admitted[i] ? obytes[i] : 0);
+            --S->inflight;
+        }
+        --S->running;
+        ++S->st.batches;
+    }
+    S->cv_done.notify_all();
+    S->cv_work.notify_all();
+
+    const Clock::time_point t5 = Clock::now();
     if (b) batch_destroy(b);
     if (S->trace) {
         auto ms = [](Clock::time_point a, Clock::time_point c) {
@@ -271,30 +298,9 @@ void run_batch(kmc_server* S, hipStream_t st, const std::vector<uint64_t>& ids)
         uint64_t ob = 0;
         for (size_t i = 0; i < n; ++i) { na += admitted[i]; ob += admitted[i] ? obytes[i] : 0; }
         std::fprintf(stderr, "[kmc_server] batch %zu req %zu admitted %.1f MB out: plan %.3f front %.3f alloc %.3f "
-                     "emit %.3f destroy %.3f ms\n", n, na, ob / 1e6, ms(t0, t1), ms(t1, t2), ms(t2, t3), ms(t3, t4),
-                     ms(t4, Clock::now()));
+                     "emit %.3f deliver %.3f destroy %.3f ms\n", n, na, ob / 1e6, ms(t0, t1), ms(t1, t2), ms(t2, t3),
+                     ms(t3, t4), ms(t4, t5), ms(t5, Clock::now()));
     }
-
-    std::lock_guard<std::mutex> lk(S->mu);
-    for (size_t i = 0; i < n; ++i) {
-        /* after a good front phase only admitted requests are still this batch's
-         * (the others failed early or went back to the queue) */
-        if (front_ok && !admitted[i]) continue;
-        Request& r = S->reqs.at(ids[i]);
-        if (s != KMC_OK) {
-            if (admitted[i]) kmc_surface_release(&out[i]);
-            finish_locked(S, ids[i], r, s);
-        } else {
-            r.result = out[i];
-            finish_locked(S, ids[i], r, KMC_OK);
-        }
-        S->used -= r.scratch + (admitted[i] ? obytes[i] : 0);
-        --S->inflight;
-    }
-    --S->running;
-    ++S->st.batches;
-    S->cv_done.notify_all();
-    S->cv_work.notify_all();
 }
 
 void worker_main(kmc_server* S)
