@@ -300,7 +300,7 @@ __device__ __forceinline__ void span_merge(CoordSpan& f, const CoordSpan& g)
     }
 }
 
-constexpr uint32_t kSpanParts = 1024;
+constexpr uint32_t kSpanParts = 4096;
 
 /* Merge of the spans sh[0 .. kBlock) in order into sh[0]: a tree whose step
  * s merges sh[i] with sh[i + s] for i a multiple of 2s (span_merge is the
@@ -329,7 +329,7 @@ __global__ __launch_bounds__(kBlock) void k_coord_span_partial(const float* coor
     CoordSpan s;
     s.valid = 0;
     s.nan = 0;
-    constexpr uint32_t kU = 4;
+    constexpr uint32_t kU = 8;
     for (uint64_t i0 = b; i0 < e; i0 += kU) {
         float c[kU][3];
 #pragma unroll
@@ -360,17 +360,22 @@ __global__ __launch_bounds__(kBlock) void k_coord_span_partial(const float* coor
 }
 
 /* The parts merged in order: each thread merges kSpanParts / kBlock
- * consecutive parts, then the tree. */
+ * consecutive parts (loaded four at a time), then the tree. */
 __global__ __launch_bounds__(kBlock) void k_coord_span_final(const CoordSpan* parts, float* out)
 {
     __shared__ CoordSpan sh[kBlock];
-    constexpr uint32_t kPer = kSpanParts / kBlock;
-    CoordSpan loaded[kPer];
+    constexpr uint32_t kPer = kSpanParts / kBlock, kLoad = 4;
+    static_assert(kPer % kLoad == 0, "whole groups of loads");
+    CoordSpan r;
+    r.valid = 0;
+    r.nan = 0;
+    for (uint32_t i0 = 0; i0 < kPer; i0 += kLoad) {
+        CoordSpan loaded[kLoad];
 #pragma unroll
-    for (uint32_t i = 0; i < kPer; ++i) loaded[i] = parts[threadIdx.x * kPer + i];
-    CoordSpan r = loaded[0];
+        for (uint32_t i = 0; i < kLoad; ++i) loaded[i] = parts[threadIdx.x * kPer + i0 + i];
 #pragma unroll
-    for (uint32_t i = 1; i < kPer; ++i) span_merge(r, loaded[i]);
+        for (uint32_t i = 0; i < kLoad; ++i) span_merge(r, loaded[i]);
+    }
     sh[threadIdx.x] = r;
     block_span_merge(sh);
     if (threadIdx.x == 0)
