@@ -14,10 +14,17 @@
  * object coordinates follow updateMinMaxCoords (StructuredVolumeObject.cpp:348-424).
  */
 #include <hip/hip_runtime.h>
+#include <fcntl.h>
+#include <unistd.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cerrno>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "kmc.h"
@@ -227,6 +234,40 @@ struct Sink {
 };
 
 constexpr size_t kStage = size_t(64) << 20;   // bytes per staging buffer
+constexpr size_t kReaders = 8;                  // threads reading one range
+constexpr size_t kMinPart = size_t(4) << 20;    // no thread reads less than this
+
+/* Bytes [offset, offset + n) of file fd into dst, by up to kReaders threads
+ * each reading (and, when asked, byte-swapping) a contiguous part of whole
+ * elements: one thread copying out of the page cache reads about 20 GB/s,
+ * below what the host-to-device copy takes. */
+bool read_range(int fd, uint8_t* dst, size_t n, off_t offset, size_t esz, bool swap)
+{
+    const unsigned hw = std::thread::hardware_concurrency();
+    const size_t threads = std::max<size_t>(1, std::min<size_t>({ kReaders, hw ? (size_t)hw : 1, n / kMinPart + 1 }));
+    const size_t part = ((n + threads - 1) / threads + esz - 1) / esz * esz;
+    std::atomic<bool> ok{ true };
+    auto work = [&](size_t t) {
+        const size_t b = t * part;
+        if (b >= n) return;
+        const size_t e = std::min(n, b + part);
+        for (size_t got = b; got < e;) {
+            const ssize_t r = ::pread(fd, dst + got, e - got, offset + (off_t)got);
+            if (r < 0 && errno == EINTR) continue;
+            if (r <= 0) {
+                ok = false;
+                return;
+            }
+            got += (size_t)r;
+        }
+        if (swap) swap_bytes(dst + b, (e - b) / esz, esz);
+    };
+    std::vector<std::thread> pool;
+    for (size_t t = 1; t < threads; ++t) pool.emplace_back(work, t);
+    work(0);
+    for (std::thread& th : pool) th.join();
+    return ok;
+}
 
 /* External binary file -> sink.  To the device: chunks read into two pinned
  * buffers in turn, each copied asynchronously; a buffer is refilled only after
@@ -234,14 +275,12 @@ constexpr size_t kStage = size_t(64) << 20;   // bytes per staging buffer
 kmc_status read_binary(const std::string& path, size_t bytes, size_t esz, bool swap, const Sink& sink,
                        long offset = 0)
 {
-    FILE* f = std::fopen(path.c_str(), "rb");
-    if (!f) return KMC_ERR_FILE;
-    if (offset && std::fseek(f, offset, SEEK_SET) != 0) { std::fclose(f); return KMC_ERR_FILE; }
+    const int fd = ::open(path.c_str(), O_RDONLY);
+    if (fd < 0) return KMC_ERR_FILE;
     kmc_status s = KMC_OK;
     if (sink.memory == KMC_MEMORY_HOST) {
-        if (std::fread(sink.dst, 1, bytes, f) != bytes) s = KMC_ERR_FILE;
-        else if (swap) swap_bytes((uint8_t*)sink.dst, bytes / esz, esz);
-        std::fclose(f);
+        if (!read_range(fd, (uint8_t*)sink.dst, bytes, offset, esz, swap)) s = KMC_ERR_FILE;
+        ::close(fd);
         return s;
     }
     uint8_t* stage[2] = { (uint8_t*)host_pinned_alloc(kStage), (uint8_t*)host_pinned_alloc(kStage) };
@@ -254,8 +293,7 @@ kmc_status read_binary(const std::string& path, size_t bytes, size_t esz, bool s
     for (size_t off = 0, k = 0; s == KMC_OK && off < bytes; off += chunk, k ^= 1) {
         const size_t n = bytes - off < chunk ? bytes - off : chunk;
         if (used[k] && hipEventSynchronize(done[k]) != hipSuccess) { s = KMC_ERR_DEVICE; break; }
-        if (std::fread(stage[k], 1, n, f) != n) { s = KMC_ERR_FILE; break; }
-        if (swap) swap_bytes(stage[k], n / esz, esz);
+        if (!read_range(fd, stage[k], n, offset + (off_t)off, esz, swap)) { s = KMC_ERR_FILE; break; }
         if (hipMemcpyAsync((uint8_t*)sink.dst + off, stage[k], n, hipMemcpyHostToDevice, sink.st) != hipSuccess ||
             hipEventRecord(done[k], sink.st) != hipSuccess) {
             s = KMC_ERR_DEVICE;
@@ -269,7 +307,7 @@ kmc_status read_binary(const std::string& path, size_t bytes, size_t esz, bool s
         if (done[k]) (void)hipEventDestroy(done[k]);
         host_pinned_free(stage[k]);
     }
-    std::fclose(f);
+    ::close(fd);
     return s;
 }
 

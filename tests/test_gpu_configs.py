@@ -110,3 +110,25 @@ def test_batched_wide_rows():
             for lv, got in zip(levels, job_outs):
                 want = oracle.extract(v, lv, normal, dup)
                 assert_same_surface(got, want, f"batched {v.shape} level {lv} mode {normal},{dup}")
+
+
+@pytest.mark.parametrize("nx", [21000, 33000])
+def test_rows_too_long_for_a_tile_team(nx):
+    # 329 and 516 chunks a row: no tile team's LDS state fits such rows, so
+    # they run on the flat emit only -- plain, fused, batched and served
+    rng = np.random.default_rng(nx)
+    v = rng.standard_normal((3, 4, nx)).astype(np.float32)
+    for normal, dup in MODES:
+        check_levels(v, [0.1], normal, dup, f"long rows nx={nx}")
+        check_levels(v, [-0.2, 0.3], normal, dup, f"long rows nx={nx} fused")
+    specs = [(kmc.make_volume(v), [0.1], normal, dup) for normal, dup in MODES]
+    for (vol, levels, normal, dup), job_outs in zip(specs, kmc.batch_extract(specs)):
+        assert_same_surface(job_outs[0], oracle.extract(v, 0.1, normal, dup), f"batched nx={nx} mode {normal},{dup}")
+    server = kmc.Server(budget_bytes=1 << 30, workers=2)
+    try:
+        vid = server.register(kmc.make_volume(v))
+        ids = [server.submit(vid, 0.1, normal, dup) for normal, dup in MODES]
+        for rid, (normal, dup) in zip(ids, MODES):
+            assert_same_surface(server.wait(rid), oracle.extract(v, 0.1, normal, dup), f"served nx={nx} mode {normal},{dup}")
+    finally:
+        server.close()

@@ -45,6 +45,33 @@ def test_kvsml_read_matches_reference_reader(memory):
             kv.close()
 
 
+@pytest.mark.parametrize("memory", [kmc.MEMORY_DEVICE, kmc.MEMORY_HOST])
+def test_large_external_arrays_in_both_byte_orders(tmp_path, memory):
+    # arrays of several 64 MiB staging buffers, each range read by several
+    # threads that byte-swap their own parts; the small fixtures above pin the
+    # byte-order rule against the reference's reader, this the split reads
+    rng = np.random.default_rng(3)
+    for dtype, name, (nz, ny, nx) in ((np.float64, "double", (97, 301, 331)), (np.int16, "short", (61, 1031, 1201))):
+        v = (rng.standard_normal((nz, ny, nx)) * 1000).astype(dtype)
+        for endian in ("little", "big"):
+            dat = tmp_path / f"v_{endian}.dat"
+            v.astype(v.dtype.newbyteorder("<" if endian == "little" else ">")).tofile(dat)
+            path = tmp_path / f"v_{endian}.kvsml"
+            path.write_text('<?xml version="1.0" ?>\n<KVSML>\n<Object type="StructuredVolumeObject">\n'
+                            f'<StructuredVolumeObject grid_type="uniform" resolution="{nx} {ny} {nz}">\n'
+                            '<Node>\n<Value veclen="1">\n'
+                            f'<DataArray type="{name}" format="binary" file="{dat.name}" endian="{endian}" />\n'
+                            '</Value>\n</Node>\n</StructuredVolumeObject>\n</Object>\n</KVSML>\n')
+            print("case", name, endian, v.nbytes, memory, flush=True)
+            kv = kmc.KvsmlVolume(str(path), memory=memory)
+            try:
+                got = kv.values().reshape(v.shape)
+                assert got.dtype == v.dtype and np.array_equal(got, v), (name, endian)
+                assert (kv.volume.min_value, kv.volume.max_value) == (float(v.min()), float(v.max())), (name, endian)
+            finally:
+                kv.close()
+
+
 def test_isosurface_of_a_kvsml_volume_in_hbm():
     lib = kmc.load()
     for c, want in cases():
