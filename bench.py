@@ -628,7 +628,10 @@ def rows_leg(lib, kmc, datasets, torch, vol, v, mn, mx, sptr, with_cpu):
       stl_export     the slab's isolevel-0 VertexNormal surface written as a
                      binary STL: kmc_export_stl (device expansion) vs
                      PolygonExporter<Stl> + Stl::write; the files are compared
-                     byte for byte; plus the device-to-device expansion alone."""
+                     byte for byte; plus the device-to-device expansion alone;
+      kvsml_export   the same surface as a KVSML polygon object, external
+                     binary and ascii: kmc_export_kvsml vs PolygonExporter<
+                     KVSMLPolygonObject> + write; every file compared."""
     import shutil
     import tempfile
 
@@ -799,6 +802,31 @@ def rows_leg(lib, kmc, datasets, torch, vol, v, mn, mx, sptr, with_cpu):
                 same = a.read() == b.read()
             ex["cpu_reference"] = {"s": round(rs, 3), "cores": 1, "file_identical": bool(same)}
         res["stl_export"] = ex
+
+        print("[bench] rows leg: KVSML polygon export", file=sys.stderr, flush=True)
+        color = (255, 255, 255)
+        kx = {"workload": f"the same surface ({nt} triangles, VertexNormal, one colour and opacity) as a KVSML "
+                          f"polygon object"}
+        mesh = dict(host, colors=np.array(color, np.uint8), opacities=np.array([255], np.uint8), color_type=1,
+                    normal_type=kmc.VERTEX_NORMAL)
+        for writing, label in ((kmc.KVSML_EXTERNAL_BINARY, "external_binary"), (kmc.KVSML_ASCII, "ascii")):
+            d_mine, d_ref = os.path.join(tmp, "mine_" + label), os.path.join(tmp, "ref_" + label)
+            os.makedirs(d_mine)
+            os.makedirs(d_ref)
+            t0 = time.perf_counter()
+            kmc.export_kvsml(host, kmc.VERTEX_NORMAL, os.path.join(d_mine, "s.kvsml"), writing, color=color)
+            leg = {"s": round(time.perf_counter() - t0, 3),
+                   "bytes": sum(os.path.getsize(os.path.join(d_mine, f)) for f in os.listdir(d_mine))}
+            if ref:
+                t0 = time.perf_counter()
+                oracle.ref_export_kvsml_polygon(mesh, os.path.join(d_ref, "s.kvsml"), writing)
+                rs = time.perf_counter() - t0
+                same = sorted(os.listdir(d_mine)) == sorted(os.listdir(d_ref)) and all(
+                    open(os.path.join(d_mine, f), "rb").read() == open(os.path.join(d_ref, f), "rb").read()
+                    for f in os.listdir(d_mine))
+                leg["cpu_reference"] = {"s": round(rs, 3), "cores": 1, "files_identical": bool(same)}
+            kx[label] = leg
+        res["kvsml_export"] = kx
     finally:
         shutil.rmtree(tmp, ignore_errors=True)
     return res

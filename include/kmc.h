@@ -283,6 +283,29 @@ kmc_status kmc_surface_expand(const kmc_surface* surface, int32_t out_memory, vo
  * bytes as the reference. */
 kmc_status kmc_export_stl(const kmc_surface* surface, const char* filename, int32_t ascii, void* stream);
 
+/* KVSMLPolygonObject::WritingDataType (FileFormat/KVSML/KVSMLPolygonObject.h:30-35). */
+typedef enum kmc_kvsml_writing {
+    KMC_KVSML_ASCII = 0,            /* the values inside the XML file */
+    KMC_KVSML_EXTERNAL_ASCII = 1,   /* beside it, one ", "-separated text file per array */
+    KMC_KVSML_EXTERNAL_BINARY = 2   /* beside it, one raw little-endian file per array */
+} kmc_kvsml_writing;
+
+/*
+ * PolygonExporter<kvs::KVSMLPolygonObject>(polygon), setWritingDataType(writing)
+ * and write(filename) (Exporter/PolygonExporter.cpp:22-70,
+ * FileFormat/KVSML/KVSMLPolygonObject.cpp:349-495, KVSML/DataWriter.cpp,
+ * DataArrayTag.cpp:74-172, DataValueTag.h:80-105): the same bytes as the
+ * reference, in the file and, for the external types, in <base>_coord.dat,
+ * _color.dat, _opacity.dat, _normal.dat and _connect.dat in its directory
+ * (base: the file name without its last extension).  The polygon's colours
+ * are the surface's per-vertex colours when it has them, else `color` (3
+ * bytes: the single colour the marching-cubes mappers set; NULL: none);
+ * `opacity` is its single opacity (the mappers set 255; -1: none).  A device
+ * surface's arrays are copied to the host on `stream` first.
+ */
+kmc_status kmc_export_kvsml(const kmc_surface* surface, const uint8_t* color, int32_t opacity, const char* filename,
+                            int32_t writing, void* stream);
+
 /* Release the arrays of a surface returned by any kmc_* extraction call. */
 void kmc_surface_release(kmc_surface* surface);
 

@@ -9,11 +9,22 @@
  * connections and, for VertexNormal surfaces, averages the three vertex
  * normals as the exporter does: (n0 + n1 + n2) / 3.0f in float.  A duplicated
  * surface is already a triangle soup and is copied as it is.
+ *
+ * The KVSML polygon file (FileFormat/KVSML/KVSMLPolygonObject.cpp:349-495) is
+ * written from the surface's arrays as the reference's exporter and writer
+ * lay it out: the same elements in the same order, printed by the rules of
+ * the XML library the reference bundles, numbers formatted as its streams
+ * format them.
  */
 #include <hip/hip_runtime.h>
 #include <algorithm>
+#include <charconv>
 #include <cstdio>
 #include <cstring>
+#include <string>
+#include <thread>
+#include <type_traits>
+#include <utility>
 #include <vector>
 
 #include "kmc.h"
@@ -194,6 +205,303 @@ kmc_status write_stl(const kmc_surface* soup, const char* filename, int ascii)
     return ok ? KMC_OK : KMC_ERR_FILE;
 }
 
+/* ------------------------------------------------------------------------- */
+/* KVSML polygon files                                                         */
+/* ------------------------------------------------------------------------- */
+
+/* An XML element as the reference builds it (tag, attributes in insertion
+ * order, element children or one text child). */
+struct XmlNode {
+    std::string name;
+    std::vector<std::pair<std::string, std::string>> attrs;
+    std::vector<XmlNode> kids;
+    bool has_text = false;
+    std::string text;
+    XmlNode& kid(const std::string& n)
+    {
+        kids.push_back(XmlNode());
+        kids.back().name = n;
+        return kids.back();
+    }
+};
+
+/* The bundled TinyXML's printing (FileFormat/XML/TinyXML.cpp:1044-1080,
+ * 1222-1335): four spaces of indent a level; an element without children
+ * closes itself ("<a x=\"1\" />"); element children start on new lines, a text
+ * child follows the tag directly, and the closing tag comes on a line of its
+ * own; an attribute value holding '"' is quoted with '\''. */
+void xml_print(std::string& out, const XmlNode& n, int depth)
+{
+    for (int i = 0; i < depth; ++i) out += "    ";
+    out += "<" + n.name;
+    for (const auto& a : n.attrs) {
+        const char q = a.second.find('"') != std::string::npos ? '\'' : '"';
+        out += " " + a.first + "=" + q + a.second + q;
+    }
+    if (n.kids.empty() && !n.has_text) {
+        out += " />";
+        return;
+    }
+    out += ">";
+    if (n.has_text) out += n.text;
+    for (const XmlNode& k : n.kids) {
+        out += "\n";
+        xml_print(out, k, depth + 1);
+    }
+    out += "\n";
+    for (int i = 0; i < depth; ++i) out += "    ";
+    out += "</" + n.name + ">";
+}
+
+/* Values as the reference's streams print them, each followed by `delim`:
+ * floats as `ostream << float` (%g, six significant digits), bytes as
+ * integers, uint32 as unsigned.  Large arrays are formatted by several
+ * threads, in order. */
+template <typename T>
+void format_part(std::string& out, const T* v, size_t n, const char* delim)
+{
+    /* std::to_chars in the general format at precision 6 gives printf's
+     * "%.6g" (the stream's conversion of the float's double value) */
+    const size_t dl = std::strlen(delim);
+    char buf[64];
+    out.reserve(out.size() + n * (std::is_same<T, float>::value ? 12 : 8));
+    for (size_t i = 0; i < n; ++i) {
+        std::to_chars_result r;
+        if constexpr (std::is_same<T, float>::value)
+            r = std::to_chars(buf, buf + sizeof(buf), (double)v[i], std::chars_format::general, 6);
+        else
+            r = std::to_chars(buf, buf + sizeof(buf), (unsigned)v[i]);
+        std::memcpy(r.ptr, delim, dl);
+        out.append(buf, (size_t)(r.ptr - buf) + dl);
+    }
+}
+
+template <typename T>
+std::string format_values(const T* v, size_t n, const char* delim)
+{
+    constexpr size_t kPerThread = size_t(1) << 16;
+    const unsigned hw = std::thread::hardware_concurrency();
+    const size_t parts = std::max<size_t>(1, std::min<size_t>({ (n + kPerThread - 1) / kPerThread, hw ? (size_t)hw : 1,
+                                                               16 }));
+    if (parts == 1) {
+        std::string out;
+        format_part(out, v, n, delim);
+        return out;
+    }
+    std::vector<std::string> pieces(parts);
+    const size_t per = (n + parts - 1) / parts;
+    std::vector<std::thread> pool;
+    for (size_t p = 0; p < parts; ++p)
+        pool.emplace_back([&, p] {
+            const size_t b = p * per, e = std::min(n, b + per);
+            if (b < e) format_part(pieces[p], v + b, e - b, delim);
+        });
+    for (std::thread& t : pool) t.join();
+    std::string out;
+    size_t total = 0;
+    for (const std::string& x : pieces) total += x.size();
+    out.reserve(total);
+    for (const std::string& x : pieces) out += x;
+    return out;
+}
+
+/* kvs::File: the directory ("." without one) and the name without its last
+ * extension (Utility/File.cpp:142-157, 244-268). */
+void split_path(const std::string& path, std::string& dir, std::string& base)
+{
+    const size_t sep = path.find_last_of('/');
+    std::string file;
+    if (sep == std::string::npos) { dir = "."; file = path; }
+    else if (sep == 0) { dir = "/"; file = path.substr(1); }
+    else { dir = path.substr(0, sep); file = path.substr(sep + 1); }
+    const size_t dot = file.find_last_of('.');
+    base = dot == std::string::npos ? file : file.substr(0, dot);
+}
+
+bool write_file(const std::string& path, const void* data, size_t bytes)
+{
+    FILE* f = std::fopen(path.c_str(), bytes ? "wb" : "w");
+    if (!f) return false;
+    const bool ok = !bytes || std::fwrite(data, 1, bytes, f) == bytes;
+    return std::fclose(f) == 0 && ok;
+}
+
+struct KvsmlWriter {
+    int type;            // kmc_kvsml_writing
+    std::string dir, base;
+    bool ok = true;
+
+    /* DataArrayTag::write (DataArrayTag.cpp:74-172) with the file names of
+     * DataArray::GetDataFilename (DataArray.h:94-97) */
+    template <typename T>
+    void data_array(XmlNode& parent, const char* kvs_type, const T* v, size_t n, const char* what)
+    {
+        if (n == 0) return;
+        XmlNode& e = parent.kid("DataArray");
+        e.attrs.push_back({ "type", kvs_type });
+        if (type == KMC_KVSML_ASCII) {
+            e.has_text = true;
+            e.text = format_values(v, n, " ");
+            return;
+        }
+        const std::string file = base + "_" + what + ".dat";
+        e.attrs.push_back({ "format", type == KMC_KVSML_EXTERNAL_ASCII ? "ascii" : "binary" });
+        e.attrs.push_back({ "file", file });
+        const std::string path = dir + "/" + file;
+        if (type == KMC_KVSML_EXTERNAL_ASCII) {
+            const std::string text = format_values(v, n, ", ");
+            ok = write_file(path, text.data(), text.size()) && ok;
+        } else {
+            e.attrs.push_back({ "endian", "little" });
+            ok = write_file(path, v, n * sizeof(T)) && ok;
+        }
+    }
+
+    /* DataValueTag::write (DataValueTag.h:80-105) */
+    template <typename T>
+    void data_value(XmlNode& parent, const T* v, size_t n)
+    {
+        XmlNode& e = parent.kid("DataValue");
+        e.has_text = true;
+        e.text = format_values(v, n, " ");
+    }
+
+    /* WriteCoordData / WriteColorData / WriteOpacityData / WriteNormalData /
+     * WriteConnectionData (KVSML/DataWriter.cpp): one value array becomes a
+     * <DataValue> when it holds a single colour (3 bytes), opacity (1) or
+     * normal (3). */
+    void coords(XmlNode& p, const float* v, size_t n)
+    {
+        if (n) data_array(p.kid("Coord"), "float", v, n, "coord");
+    }
+    void colors(XmlNode& p, const uint8_t* v, size_t n)
+    {
+        if (!n) return;
+        XmlNode& t = p.kid("Color");
+        if (n == 3) data_value(t, v, n);
+        else data_array(t, "uchar", v, n, "color");
+    }
+    void opacities(XmlNode& p, const uint8_t* v, size_t n)
+    {
+        if (!n) return;
+        XmlNode& t = p.kid("Opacity");
+        if (n == 1) data_value(t, v, n);
+        else data_array(t, "uchar", v, n, "opacity");
+    }
+    void normals(XmlNode& p, const float* v, size_t n)
+    {
+        if (!n) return;
+        XmlNode& t = p.kid("Normal");
+        if (n == 3) data_value(t, v, n);
+        else data_array(t, "float", v, n, "normal");
+    }
+    void connections(XmlNode& p, const uint32_t* v, size_t n)
+    {
+        if (n) data_array(p.kid("Connection"), "uint", v, n, "connect");
+    }
+};
+
+/* Host copies of a surface's arrays (a host surface's arrays are used as they are). */
+struct HostArrays {
+    const float* coords = nullptr;
+    const uint32_t* conn = nullptr;
+    const float* normals = nullptr;
+    const uint8_t* colors = nullptr;
+    std::vector<float> c, n;
+    std::vector<uint32_t> k;
+    std::vector<uint8_t> cl;
+};
+
+kmc_status host_arrays(const kmc_surface* s, hipStream_t st, HostArrays& h)
+{
+    const size_t nc = 3 * s->n_vertices, nk = s->n_connections, nn = 3 * s->n_normals;
+    const size_t ncl = s->colors ? 3 * s->n_colors : 0;
+    if (s->memory == KMC_MEMORY_HOST) {
+        h.coords = s->coords;
+        h.conn = s->connections;
+        h.normals = s->normals;
+        h.colors = ncl ? s->colors : nullptr;
+        return KMC_OK;
+    }
+    h.c.resize(nc);
+    h.k.resize(nk);
+    h.n.resize(nn);
+    h.cl.resize(ncl);
+    auto down = [&](void* dst, const void* src, size_t bytes) {
+        return !bytes || hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, st) == hipSuccess;
+    };
+    if (!down(h.c.data(), s->coords, nc * 4) || !down(h.k.data(), s->connections, nk * 4) ||
+        !down(h.n.data(), s->normals, nn * 4) || !down(h.cl.data(), s->colors, ncl) ||
+        hipStreamSynchronize(st) != hipSuccess) {
+        (void)hipGetLastError();
+        return KMC_ERR_DEVICE;
+    }
+    h.coords = h.c.data();
+    h.conn = h.k.data();
+    h.normals = h.n.data();
+    h.colors = ncl ? h.cl.data() : nullptr;
+    return KMC_OK;
+}
+
+/* PolygonExporter<KVSMLPolygonObject>::exec (Exporter/PolygonExporter.cpp:22-70)
+ * then KVSMLPolygonObject::write (KVSMLPolygonObject.cpp:349-495). */
+kmc_status export_kvsml(const kmc_surface* s, const uint8_t* color, int32_t opacity, const char* filename,
+                        int32_t writing, hipStream_t st)
+{
+    if (!s || !filename) return KMC_ERR_INVALID_ARGUMENT;
+    if (writing < KMC_KVSML_ASCII || writing > KMC_KVSML_EXTERNAL_BINARY || opacity > 255) return KMC_ERR_INVALID_ARGUMENT;
+    if ((s->polygon_type != 3 && s->polygon_type != 4) || (s->color_type != 0 && s->color_type != 1) ||
+        (s->normal_type != KMC_VERTEX_NORMAL && s->normal_type != KMC_POLYGON_NORMAL))
+        return KMC_ERR_INVALID_ARGUMENT;
+    HostArrays h;
+    kmc_status status = host_arrays(s, st, h);
+    if (status != KMC_OK) return status;
+
+    const size_t ncoords = 3 * s->n_vertices, nconn = s->n_connections, nnormals = 3 * s->n_normals;
+    const size_t ncolors = h.colors ? 3 * s->n_colors : (color ? 3 : 0);
+    const uint8_t* colors = h.colors ? h.colors : color;
+    const uint8_t op = (uint8_t)(opacity < 0 ? 0 : opacity);
+    const size_t nopac = opacity < 0 ? 0 : 1;
+    const bool vcolor = s->color_type == 0, vnormal = s->normal_type == KMC_VERTEX_NORMAL;
+
+    KvsmlWriter w;
+    w.type = writing;
+    split_path(filename, w.dir, w.base);
+    XmlNode root;
+    root.name = "KVSML";
+    XmlNode& object = root.kid("Object");
+    object.attrs.push_back({ "type", "PolygonObject" });
+    XmlNode& po = object.kid("PolygonObject");
+    po.attrs.push_back({ "polygon_type", s->polygon_type == 3 ? "triangle" : "quadrangle" });
+    po.attrs.push_back({ "color_type", vcolor ? "vertex" : "polygon" });
+    po.attrs.push_back({ "normal_type", vnormal ? "vertex" : "polygon" });
+    XmlNode& vertex = po.kid("Vertex");
+    vertex.attrs.push_back({ "nvertices", std::to_string(s->n_vertices) });
+    w.coords(vertex, h.coords, ncoords);
+    if (vcolor) {
+        w.colors(vertex, colors, ncolors);
+        w.opacities(vertex, &op, nopac);
+    }
+    if (vnormal) w.normals(vertex, h.normals, nnormals);
+    if (!vcolor || !vnormal || nopac || nconn) {
+        const size_t per = s->polygon_type == 3 ? 3 : 4;
+        XmlNode& polygon = po.kid("Polygon");
+        polygon.attrs.push_back({ "npolygons", std::to_string(nconn ? nconn / per : s->n_vertices / per) });
+        w.connections(polygon, h.conn, nconn);
+        if (!vcolor) {
+            w.colors(polygon, colors, ncolors);
+            w.opacities(polygon, &op, nopac);
+        }
+        if (!vnormal) w.normals(polygon, h.normals, nnormals);
+    }
+    if (!w.ok) return KMC_ERR_FILE;
+
+    std::string out = "<?xml version=\"1.0\" ?>\n<!-- Generated by kvs::KVSMLPolygonObject::write() -->\n";
+    xml_print(out, root, 0);
+    out += "\n";
+    return write_file(filename, out.data(), out.size()) ? KMC_OK : KMC_ERR_FILE;
+}
+
 }  // namespace
 
 }  // namespace kmc
@@ -203,6 +511,14 @@ extern "C" {
 kmc_status kmc_surface_expand(const kmc_surface* surface, int32_t out_memory, void* stream, kmc_surface* out)
 {
     const kmc_status s = kmc::expand_impl(surface, out_memory, (hipStream_t)stream, out);
+    kmc::report(s);
+    return s;
+}
+
+kmc_status kmc_export_kvsml(const kmc_surface* surface, const uint8_t* color, int32_t opacity, const char* filename,
+                            int32_t writing, void* stream)
+{
+    const kmc_status s = kmc::export_kvsml(surface, color, opacity, filename, writing, (hipStream_t)stream);
     kmc::report(s);
     return s;
 }

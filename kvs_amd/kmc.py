@@ -170,7 +170,7 @@ EXPORTED = [
     "kmc_set_timing", "kmc_last_timing", "kmc_extract_noalloc", "kmc_slice_plane",
     "kmc_extract_unstructured", "kmc_slice_plane_unstructured",
     "kmc_kvsml_read_volume", "kmc_avs_read_volume", "kmc_kvsml_volume_release",
-    "kmc_surface_expand", "kmc_export_stl",
+    "kmc_surface_expand", "kmc_export_stl", "kmc_export_kvsml",
 ]
 
 
@@ -236,6 +236,9 @@ def load():
     lib.kmc_surface_expand.restype = ctypes.c_int
     lib.kmc_export_stl.argtypes = [ctypes.POINTER(Surface), ctypes.c_char_p, ctypes.c_int32, ctypes.c_void_p]
     lib.kmc_export_stl.restype = ctypes.c_int
+    lib.kmc_export_kvsml.argtypes = [ctypes.POINTER(Surface), ctypes.POINTER(ctypes.c_uint8), ctypes.c_int32,
+                                     ctypes.c_char_p, ctypes.c_int32, ctypes.c_void_p]
+    lib.kmc_export_kvsml.restype = ctypes.c_int
     lib.kmc_surface_release.argtypes = [ctypes.POINTER(Surface)]
     lib.kmc_surface_release.restype = None
     lib.kmc_volume_min_max.argtypes = [ctypes.POINTER(Volume), ctypes.c_void_p,
@@ -499,7 +502,7 @@ def host_surface(d, normal_type, polygon_type=3):
     surface dict as extract returns); kept alive by the struct."""
     s = Surface()
     co = np.ascontiguousarray(d["coords"], dtype=np.float32)
-    cn = np.ascontiguousarray(d["connections"], dtype=np.uint32)
+    cn = np.ascontiguousarray(d.get("connections", np.zeros(0, np.uint32)), dtype=np.uint32)
     no = np.ascontiguousarray(d["normals"], dtype=np.float32)
     s._keep = (co, cn, no)
     s.coords = co.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
@@ -535,6 +538,27 @@ def export_stl(d, normal_type, filename, ascii=False):
     lib = load()
     s = host_surface(d, normal_type)
     st = lib.kmc_export_stl(ctypes.byref(s), str(filename).encode(), int(bool(ascii)), None)
+    if st != OK:
+        raise KmcError(st)
+
+
+KVSML_ASCII, KVSML_EXTERNAL_ASCII, KVSML_EXTERNAL_BINARY = range(3)
+
+
+def export_kvsml(d, normal_type, filename, writing=KVSML_ASCII, color=None, opacity=255):
+    """kmc_export_kvsml of a host surface dict.  A dict with per-vertex
+    "colors" (kmc.slice_plane's) is a VertexColor polygon, else a PolygonColor
+    one with the single `color` (3 bytes, or None)."""
+    lib = load()
+    s = host_surface(d, normal_type)
+    cl = np.ascontiguousarray(d.get("colors", np.zeros(0, np.uint8)), dtype=np.uint8)
+    if cl.size and cl.size == s.n_vertices * 3:
+        s._keep = s._keep + (cl,)
+        s.colors = cl.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))
+        s.n_colors = cl.size // 3
+        s.color_type = 0
+    c = (ctypes.c_uint8 * 3)(*color) if color is not None else None
+    st = lib.kmc_export_kvsml(ctypes.byref(s), c, int(opacity), str(filename).encode(), int(writing), None)
     if st != OK:
         raise KmcError(st)
 

@@ -186,6 +186,11 @@ def ref_lib():
         lib.ref_export_stl.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p, ctypes.c_uint64,
                                        ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int, ctypes.c_int, ctypes.c_char_p]
         lib.ref_export_stl.restype = ctypes.c_int
+        lib.ref_export_kvsml_polygon.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p, ctypes.c_uint64,
+                                                 ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p, ctypes.c_uint64,
+                                                 ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                                 ctypes.c_char_p]
+        lib.ref_export_kvsml_polygon.restype = ctypes.c_int
         _ref = lib
     return _ref
 
@@ -602,6 +607,25 @@ def ref_read_avs_volume(filename):
                 "max_ext": list(ext[3:])}
     finally:
         ref_lib().ref_read_volume_free(ctypes.byref(r))
+
+
+def ref_export_kvsml_polygon(mesh, filename, writing_type=0):
+    """PolygonExporter<kvs::KVSMLPolygonObject> + setWritingDataType(writing_type)
+    + write by the reference (oracle/_ref): 0 Ascii, 1 ExternalAscii, 2 ExternalBinary.
+    mesh: a dict as ref_marching_cubes / ref_slice_plane return (its colours,
+    opacities and types are the polygon's)."""
+    c = np.ascontiguousarray(mesh["coords"], dtype=np.float32)
+    k = np.ascontiguousarray(mesh["connections"], dtype=np.uint32)
+    n = np.ascontiguousarray(mesh["normals"], dtype=np.float32)
+    cl = np.ascontiguousarray(mesh["colors"], dtype=np.uint8)
+    op = np.asarray(mesh["opacities"], dtype=np.uint8)
+    rc = ref_lib().ref_export_kvsml_polygon(
+        c.ctypes.data if c.size else None, c.size, k.ctypes.data if k.size else None, k.size,
+        n.ctypes.data if n.size else None, n.size, cl.ctypes.data if cl.size else None, cl.size,
+        int(op[0]) if op.size else -1, int(mesh["color_type"]), int(mesh["normal_type"]), int(writing_type),
+        str(filename).encode())
+    if rc != 0:
+        raise RuntimeError("reference KVSML polygon export failed")
 
 
 def ref_export_stl(mesh, normal_type, filename, ascii=False):

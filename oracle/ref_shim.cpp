@@ -23,6 +23,7 @@
 #include <kvs/SlicePlane>
 #include <kvs/PolygonExporter>
 #include <kvs/Stl>
+#include <kvs/KVSMLPolygonObject>
 #include <kvs/UnstructuredVolumeObject>
 #include <kvs/MarchingTetrahedraTable>
 #include <kvs/MarchingHexahedraTable>
@@ -515,6 +516,28 @@ int ref_export_stl( const float* coords, uint64_t ncoords, const uint32_t* conn,
     p.setColor( kvs::RGBColor( 255, 255, 255 ) );
     kvs::PolygonExporter<kvs::Stl> e( &p );
     e.setFileType( ascii ? kvs::Stl::Ascii : kvs::Stl::Binary );
+    return e.write( filename ) ? 0 : 1;
+}
+
+// PolygonExporter<kvs::KVSMLPolygonObject>( polygon ), setWritingDataType( type )
+// and write( filename ) for a triangle polygon with the given arrays and types
+// (ncolors bytes of colour, opacity < 0: no opacity).
+int ref_export_kvsml_polygon( const float* coords, uint64_t ncoords, const uint32_t* conn, uint64_t nconn,
+                              const float* normals, uint64_t nnormals, const uint8_t* colors, uint64_t ncolors,
+                              int opacity, int color_type, int normal_type, int writing_type, const char* filename )
+{
+    kvs::PolygonObject p;
+    kvs::ValueArray<kvs::Real32> c( ncoords ); if ( ncoords ) std::memcpy( c.data(), coords, ncoords * 4 );
+    p.setCoords( c );
+    if ( nconn ) { kvs::ValueArray<kvs::UInt32> k( nconn ); std::memcpy( k.data(), conn, nconn * 4 ); p.setConnections( k ); }
+    if ( nnormals ) { kvs::ValueArray<kvs::Real32> n( nnormals ); std::memcpy( n.data(), normals, nnormals * 4 ); p.setNormals( n ); }
+    if ( ncolors ) { kvs::ValueArray<kvs::UInt8> k( ncolors ); std::memcpy( k.data(), colors, ncolors ); p.setColors( k ); }
+    if ( opacity >= 0 ) p.setOpacity( kvs::UInt8( opacity ) );
+    p.setPolygonType( kvs::PolygonObject::Triangle );
+    p.setColorType( kvs::PolygonObject::ColorType( color_type ) );
+    p.setNormalType( kvs::PolygonObject::NormalType( normal_type ) );
+    kvs::PolygonExporter<kvs::KVSMLPolygonObject> e( &p );
+    e.setWritingDataType( kvs::KVSMLPolygonObject::WritingDataType( writing_type ) );
     return e.write( filename ) ? 0 : 1;
 }
 

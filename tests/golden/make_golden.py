@@ -442,6 +442,30 @@ def main():
     pk.save(os.path.join(HERE, "avs.npz"))
     meta["avs"] = {"cases": cases, "refused": {"f32_rectilinear": refused}}
 
+    # 14. KVSML polygon files the reference's PolygonExporter<KVSMLPolygonObject>
+    #     + write make, in its three writing types: marching-cubes surfaces in the
+    #     three modes (one colour and opacity: <DataValue>s), an empty one, and a
+    #     slice plane (per-vertex colours)
+    kdir = os.path.join(HERE, "kvsml_polygon")
+    os.makedirs(kdir, exist_ok=True)
+    vol = datasets.blobs(10, seed=7, nblobs=2)
+    coef = [0.3, -0.5, 0.8, -3.0]
+    meshes = [(f"mc_n{n}_d{int(d)}", oracle.ref_marching_cubes(vol, 0.4, n, d), {"kind": "mc", "iso": 0.4,
+               "normal": n, "dup": d}) for n, d in MODES]
+    meshes.append(("mc_empty", oracle.ref_marching_cubes(vol, 5.0, 1, True),
+                   {"kind": "mc", "iso": 5.0, "normal": 1, "dup": True}))
+    meshes.append(("slice", oracle.ref_slice_plane(vol, coef), {"kind": "slice", "coef": coef, "tf_kind": 0}))
+    kp = []
+    for name, mesh, spec in meshes:
+        for wt in (0, 1, 2):
+            stem = f"{name}_w{wt}"
+            oracle.ref_export_kvsml_polygon(mesh, os.path.join(kdir, stem + ".kvsml"), wt)
+            files = sorted(f for f in os.listdir(kdir) if f == stem + ".kvsml" or f.startswith(stem + "_"))
+            kp.append({"file": stem + ".kvsml", "writing": wt, "files": files,
+                       "color": [int(x) for x in mesh["colors"][:3]] if mesh["colors"].size == 3 else None,
+                       "opacity": int(mesh["opacities"][0]) if mesh["opacities"].size else -1, **spec})
+    meta["kvsml_polygon"] = kp
+
     with open(os.path.join(HERE, "golden.json"), "w") as f:
         json.dump(meta, f, indent=1, sort_keys=True)
     print("wrote", sorted(os.listdir(HERE)))

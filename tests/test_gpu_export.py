@@ -1,7 +1,9 @@
 """The output side on the device: kmc_surface_expand (the triangle soup of
 PolygonExporter<kvs::Stl>) and kmc_export_stl against the STL files the
-reference's own exporter and writer produced (tests/golden/stl/), byte for
-byte."""
+reference's own exporter and writer produced (tests/golden/stl/), and
+kmc_export_kvsml against its KVSML polygon files in the three writing types
+(tests/golden/kvsml_polygon/), byte for byte."""
+import ctypes
 import os
 
 import numpy as np
@@ -53,3 +55,59 @@ def test_export_stl_matches_reference_files(tmp_path):
         kmc.export_stl(d, d["normal_type"], out, ascii=case["ascii"])
         want = open(os.path.join(SDIR, case["file"]), "rb").read()
         assert out.read_bytes() == want, case["file"]
+
+
+KPDIR = os.path.join(golden_io.GOLDEN, "kvsml_polygon")
+
+
+def same_files(out_dir, case):
+    stem = case["file"][:-len(".kvsml")]
+    ours = sorted(f for f in os.listdir(out_dir) if f == case["file"] or f.startswith(stem + "_"))
+    assert ours == case["files"], (case["file"], ours)
+    for f in ours:
+        with open(os.path.join(out_dir, f), "rb") as a, open(os.path.join(KPDIR, f), "rb") as b:
+            assert a.read() == b.read(), f
+
+
+def test_export_kvsml_equals_reference_files(tmp_path):
+    # PolygonExporter<KVSMLPolygonObject> + write: marching-cubes surfaces in
+    # the three modes and an empty one (host and device surfaces), a slice
+    # plane with per-vertex colours, each in Ascii, ExternalAscii and
+    # ExternalBinary
+    vol = datasets.blobs(10, seed=7, nblobs=2)
+    table = np.array(golden_io.meta()["colormap_tables"]["BrewerRdBu_256"], dtype=np.uint8)
+    lib = kmc.load()
+    for case in golden_io.meta()["kvsml_polygon"]:
+        print("case", case["file"], flush=True)
+        host, dev = tmp_path / ("host_" + case["file"]), tmp_path / ("dev_" + case["file"])
+        host.mkdir()
+        dev.mkdir()
+        if case["kind"] == "slice":
+            d = kmc.slice_plane(vol, case["coef"], table)
+            kmc.export_kvsml(d, kmc.POLYGON_NORMAL, host / case["file"], case["writing"], opacity=case["opacity"])
+            same_files(host, case)
+            continue
+        d = kmc.extract(vol, case["iso"], case["normal"], case["dup"])
+        kmc.export_kvsml(d, case["normal"], host / case["file"], case["writing"], color=case["color"],
+                         opacity=case["opacity"])
+        same_files(host, case)
+        v = kmc.make_volume(vol)
+        s = kmc.Surface()
+        assert lib.kmc_extract(ctypes.byref(v), case["iso"], case["normal"], int(case["dup"]), kmc.MEMORY_DEVICE,
+                               None, ctypes.byref(s)) == 0
+        try:
+            c = (ctypes.c_uint8 * 3)(*case["color"]) if case["color"] else None
+            assert lib.kmc_export_kvsml(ctypes.byref(s), c, case["opacity"], str(dev / case["file"]).encode(),
+                                        case["writing"], None) == 0
+        finally:
+            lib.kmc_surface_release(ctypes.byref(s))
+        same_files(dev, case)
+
+
+def test_export_kvsml_refusals(tmp_path):
+    d = kmc.extract(datasets.blobs(10, seed=7, nblobs=2), 0.4, kmc.POLYGON_NORMAL, True)
+    for writing, opacity in ((3, 255), (-1, 255), (0, 256)):
+        with pytest.raises(kmc.KmcError):
+            kmc.export_kvsml(d, kmc.POLYGON_NORMAL, tmp_path / "x.kvsml", writing, opacity=opacity)
+    with pytest.raises(kmc.KmcError):
+        kmc.export_kvsml(d, kmc.POLYGON_NORMAL, tmp_path / "missing_dir" / "x.kvsml", kmc.KVSML_ASCII)
