@@ -306,6 +306,22 @@ typedef enum kmc_kvsml_writing {
 kmc_status kmc_export_kvsml(const kmc_surface* surface, const uint8_t* color, int32_t opacity, const char* filename,
                             int32_t writing, void* stream);
 
+/*
+ * PolygonExporter<kvs::Ply>(polygon), setFileType(binary ? Binary : Ascii)
+ * and write(filename) (Exporter/PolygonExporter.cpp:178-363,
+ * FileFormat/PLY/Ply.cpp:389-489, PlyFile.cpp): ".ply" is appended to a name
+ * without it; the header and the vertex records are the reference's bytes --
+ * the single colour `color` (when the surface has no per-vertex colours; NULL:
+ * none) repeated at every vertex, PolygonNormal normals averaged over each
+ * vertex's triangles through the connections (zero without connections), as
+ * the exporter does.  Faces follow as the header declares them (a uchar count
+ * and int indices); the reference's own writer stops there, writing through a
+ * null index pointer (Ply.cpp:475-478).  A device surface's arrays are copied
+ * to the host on `stream` first.
+ */
+kmc_status kmc_export_ply(const kmc_surface* surface, const uint8_t* color, const char* filename, int32_t binary,
+                          void* stream);
+
 /* Release the arrays of a surface returned by any kmc_* extraction call. */
 void kmc_surface_release(kmc_surface* surface);
 

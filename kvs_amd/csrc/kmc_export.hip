@@ -14,7 +14,8 @@
  * written from the surface's arrays as the reference's exporter and writer
  * lay it out: the same elements in the same order, printed by the rules of
  * the XML library the reference bundles, numbers formatted as its streams
- * format them.
+ * format them; the PLY file (FileFormat/PLY/Ply.cpp:389-489) as its PLY
+ * writer lays it out.
  */
 #include <hip/hip_runtime.h>
 #include <algorithm>
@@ -502,6 +503,136 @@ kmc_status export_kvsml(const kmc_surface* s, const uint8_t* color, int32_t opac
     return write_file(filename, out.data(), out.size()) ? KMC_OK : KMC_ERR_FILE;
 }
 
+/* ------------------------------------------------------------------------- */
+/* PLY files                                                                   */
+/* ------------------------------------------------------------------------- */
+
+/* The vertex arrays PolygonExporter<kvs::Ply>::exec hands the writer
+ * (Exporter/PolygonExporter.cpp:178-363): a single colour repeated at every
+ * vertex, per-vertex colours as they are; VertexNormal normals as they are,
+ * PolygonNormal ones summed into each vertex of each triangle in triangle
+ * order and divided by the vertex's triangle count (zero without
+ * connections). */
+struct PlyVertices {
+    bool has_colors = false, has_normals = false;
+    std::vector<uint8_t> colors;
+    std::vector<float> normals;
+};
+
+bool ply_vertices(const kmc_surface* s, const HostArrays& h, const uint8_t* color, PlyVertices& pv)
+{
+    const uint64_t nv = s->n_vertices;
+    if (h.colors && s->n_colors && s->n_colors != nv) return false;   // per-vertex colours, one a vertex
+    if (h.colors && s->n_colors) {
+        pv.has_colors = true;
+        pv.colors.assign(h.colors, h.colors + 3 * s->n_colors);
+    } else if (color) {
+        pv.has_colors = true;
+        pv.colors.resize(3 * nv);
+        for (uint64_t i = 0; i < nv; ++i) std::memcpy(&pv.colors[3 * i], color, 3);
+    }
+    if (s->n_normals == 0) return true;
+    pv.has_normals = true;
+    if (s->normal_type == KMC_VERTEX_NORMAL) {
+        if (s->n_normals != nv) return false;
+        pv.normals.assign(h.normals, h.normals + 3 * s->n_normals);
+        return true;
+    }
+    pv.normals.assign(3 * nv, 0.0f);
+    std::vector<uint32_t> counter(nv, 0);
+    const uint64_t nt = s->n_connections / 3;
+    if (nt > s->n_normals) return false;   // PolygonNormal: one normal a triangle
+    for (uint64_t t = 0; t < nt; ++t) {
+        for (int j = 0; j < 3; ++j) {
+            const uint32_t v = h.conn[3 * t + j];
+            if (v >= nv) return false;
+            for (int k = 0; k < 3; ++k) pv.normals[3 * (uint64_t)v + k] += h.normals[3 * t + k];
+            counter[v] += 1;
+        }
+    }
+    for (uint64_t i = 0; i < nv; ++i) {
+        if (counter[i] == 0) continue;
+        for (int k = 0; k < 3; ++k) pv.normals[3 * i + k] /= (float)counter[i];
+    }
+    return true;
+}
+
+/* Ply::write (FileFormat/PLY/Ply.cpp:389-489) through PlyFile's writer
+ * (PlyFile.cpp:167-200 the name, 399-465 the header, 496-600 the records,
+ * 1897-2000 the items): "%g " / "%u " / "%d " items and a newline per record
+ * in ascii, packed native little-endian items in binary. */
+kmc_status export_ply(const kmc_surface* s, const uint8_t* color, const char* filename, int32_t binary, hipStream_t st)
+{
+    if (!s || !filename) return KMC_ERR_INVALID_ARGUMENT;
+    if (s->polygon_type != 3) return KMC_ERR_NOT_TRIANGLES;   // "Input object is not triangle polygon."
+    HostArrays h;
+    kmc_status status = host_arrays(s, st, h);
+    if (status != KMC_OK) return status;
+    PlyVertices pv;
+    if (!ply_vertices(s, h, color, pv)) return KMC_ERR_INVALID_ARGUMENT;
+    const uint64_t nv = s->n_vertices, nf = s->n_connections / 3;
+
+    std::string name = filename;
+    if (name.size() < 4 || name.compare(name.size() - 4, 4, ".ply") != 0) name += ".ply";
+    std::string head = std::string("ply\nformat ") + (binary ? "binary_little_endian" : "ascii") + " 1.0\n";
+    head += "element vertex " + std::to_string((int)nv) + "\n";
+    head += "property float x\nproperty float y\nproperty float z\n";
+    if (pv.has_colors) head += "property uchar red\nproperty uchar green\nproperty uchar blue\n";
+    if (pv.has_normals) head += "property float nx\nproperty float ny\nproperty float nz\n";
+    head += "element face " + std::to_string((int)nf) + "\n";
+    if (nf) head += "property list uchar int vertex_indices\n";
+    head += "end_header\n";
+
+    FILE* f = std::fopen(name.c_str(), "wb");
+    if (!f) return KMC_ERR_FILE;
+    bool ok = std::fwrite(head.data(), 1, head.size(), f) == head.size();
+    const size_t rec = 12 + (pv.has_colors ? 3 : 0) + (pv.has_normals ? 12 : 0);
+    constexpr uint64_t kBlock = 1u << 16;
+    std::string buf;
+    for (uint64_t i0 = 0; i0 < nv && ok; i0 += kBlock) {
+        const uint64_t i1 = std::min<uint64_t>(nv, i0 + kBlock);
+        buf.clear();
+        if (binary) {
+            buf.resize((i1 - i0) * rec);
+            char* o = &buf[0];
+            for (uint64_t i = i0; i < i1; ++i) {
+                std::memcpy(o, h.coords + 3 * i, 12);
+                o += 12;
+                if (pv.has_colors) { std::memcpy(o, &pv.colors[3 * i], 3); o += 3; }
+                if (pv.has_normals) { std::memcpy(o, &pv.normals[3 * i], 12); o += 12; }
+            }
+        } else {
+            for (uint64_t i = i0; i < i1; ++i) {
+                format_part(buf, h.coords + 3 * i, 3, " ");
+                if (pv.has_colors) format_part(buf, &pv.colors[3 * i], 3, " ");
+                if (pv.has_normals) format_part(buf, &pv.normals[3 * i], 3, " ");
+                buf += "\n";
+            }
+        }
+        ok = std::fwrite(buf.data(), 1, buf.size(), f) == buf.size();
+    }
+    for (uint64_t t0 = 0; t0 < nf && ok; t0 += kBlock) {
+        const uint64_t t1 = std::min<uint64_t>(nf, t0 + kBlock);
+        buf.clear();
+        for (uint64_t t = t0; t < t1; ++t) {
+            if (binary) {
+                const char three = 3;
+                const int32_t idx[3] = { (int32_t)h.conn[3 * t], (int32_t)h.conn[3 * t + 1], (int32_t)h.conn[3 * t + 2] };
+                buf.append(&three, 1);
+                buf.append((const char*)idx, 12);
+            } else {
+                char line[48];
+                const int n = std::snprintf(line, sizeof(line), "3 %d %d %d \n", (int32_t)h.conn[3 * t],
+                                            (int32_t)h.conn[3 * t + 1], (int32_t)h.conn[3 * t + 2]);
+                buf.append(line, (size_t)n);
+            }
+        }
+        ok = std::fwrite(buf.data(), 1, buf.size(), f) == buf.size();
+    }
+    ok = std::fclose(f) == 0 && ok;
+    return ok ? KMC_OK : KMC_ERR_FILE;
+}
+
 }  // namespace
 
 }  // namespace kmc
@@ -519,6 +650,14 @@ kmc_status kmc_export_kvsml(const kmc_surface* surface, const uint8_t* color, in
                             int32_t writing, void* stream)
 {
     const kmc_status s = kmc::export_kvsml(surface, color, opacity, filename, writing, (hipStream_t)stream);
+    kmc::report(s);
+    return s;
+}
+
+kmc_status kmc_export_ply(const kmc_surface* surface, const uint8_t* color, const char* filename, int32_t binary,
+                          void* stream)
+{
+    const kmc_status s = kmc::export_ply(surface, color, filename, binary, (hipStream_t)stream);
     kmc::report(s);
     return s;
 }

@@ -170,7 +170,7 @@ EXPORTED = [
     "kmc_set_timing", "kmc_last_timing", "kmc_extract_noalloc", "kmc_slice_plane",
     "kmc_extract_unstructured", "kmc_slice_plane_unstructured",
     "kmc_kvsml_read_volume", "kmc_avs_read_volume", "kmc_kvsml_volume_release",
-    "kmc_surface_expand", "kmc_export_stl", "kmc_export_kvsml",
+    "kmc_surface_expand", "kmc_export_stl", "kmc_export_kvsml", "kmc_export_ply",
 ]
 
 
@@ -239,6 +239,9 @@ def load():
     lib.kmc_export_kvsml.argtypes = [ctypes.POINTER(Surface), ctypes.POINTER(ctypes.c_uint8), ctypes.c_int32,
                                      ctypes.c_char_p, ctypes.c_int32, ctypes.c_void_p]
     lib.kmc_export_kvsml.restype = ctypes.c_int
+    lib.kmc_export_ply.argtypes = [ctypes.POINTER(Surface), ctypes.POINTER(ctypes.c_uint8), ctypes.c_char_p,
+                                   ctypes.c_int32, ctypes.c_void_p]
+    lib.kmc_export_ply.restype = ctypes.c_int
     lib.kmc_surface_release.argtypes = [ctypes.POINTER(Surface)]
     lib.kmc_surface_release.restype = None
     lib.kmc_volume_min_max.argtypes = [ctypes.POINTER(Volume), ctypes.c_void_p,
@@ -559,6 +562,23 @@ def export_kvsml(d, normal_type, filename, writing=KVSML_ASCII, color=None, opac
         s.color_type = 0
     c = (ctypes.c_uint8 * 3)(*color) if color is not None else None
     st = lib.kmc_export_kvsml(ctypes.byref(s), c, int(opacity), str(filename).encode(), int(writing), None)
+    if st != OK:
+        raise KmcError(st)
+
+
+def export_ply(d, normal_type, filename, binary=False, color=None):
+    """kmc_export_ply of a host surface dict (per-vertex "colors" when it has
+    them, else the single `color`)."""
+    lib = load()
+    s = host_surface(d, normal_type)
+    cl = np.ascontiguousarray(d.get("colors", np.zeros(0, np.uint8)), dtype=np.uint8)
+    if cl.size and cl.size == s.n_vertices * 3:
+        s._keep = s._keep + (cl,)
+        s.colors = cl.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))
+        s.n_colors = cl.size // 3
+        s.color_type = 0
+    c = (ctypes.c_uint8 * 3)(*color) if color is not None else None
+    st = lib.kmc_export_ply(ctypes.byref(s), c, str(filename).encode(), int(bool(binary)), None)
     if st != OK:
         raise KmcError(st)
 

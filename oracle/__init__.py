@@ -191,6 +191,12 @@ def ref_lib():
                                                  ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                                  ctypes.c_char_p]
         lib.ref_export_kvsml_polygon.restype = ctypes.c_int
+        lib.ref_export_ply.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_void_p,
+                                       ctypes.c_uint64, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int, ctypes.c_int,
+                                       ctypes.c_int, ctypes.c_char_p]
+        lib.ref_export_ply.restype = ctypes.c_int
+        lib.ref_read_ply.argtypes = [ctypes.c_char_p, ctypes.POINTER(_RefMesh)]
+        lib.ref_read_ply.restype = ctypes.c_int
         _ref = lib
     return _ref
 
@@ -626,6 +632,35 @@ def ref_export_kvsml_polygon(mesh, filename, writing_type=0):
         str(filename).encode())
     if rc != 0:
         raise RuntimeError("reference KVSML polygon export failed")
+
+
+def ref_export_ply(mesh, filename, binary=False):
+    """PolygonExporter<kvs::Ply> + setFileType + write by the reference
+    (oracle/_ref), for a mesh dict without connections (the reference's writer
+    fails on connections: it stores face indices through a null pointer)."""
+    c = np.ascontiguousarray(mesh["coords"], dtype=np.float32)
+    n = np.ascontiguousarray(mesh["normals"], dtype=np.float32)
+    cl = np.ascontiguousarray(mesh["colors"], dtype=np.uint8)
+    k = np.asarray(mesh.get("connections", np.zeros(0)))
+    rc = ref_lib().ref_export_ply(c.ctypes.data if c.size else None, c.size, k.size, n.ctypes.data if n.size else None,
+                                  n.size, cl.ctypes.data if cl.size else None, cl.size, int(mesh["color_type"]),
+                                  int(mesh["normal_type"]), int(bool(binary)), str(filename).encode())
+    if rc != 0:
+        raise RuntimeError(f"reference PLY export failed ({rc})")
+
+
+def ref_read_ply(filename):
+    """Ply::read by the reference: coords, colors, normals, connections and
+    whether the file had colours."""
+    m = _RefMesh()
+    if ref_lib().ref_read_ply(str(filename).encode(), ctypes.byref(m)) != 0:
+        raise RuntimeError("reference PLY read failed")
+    try:
+        return {"coords": _arr(m.coords, m.n_coords, np.float32), "colors": _arr(m.colors, m.n_colors, np.uint8),
+                "normals": _arr(m.normals, m.n_normals, np.float32),
+                "connections": _arr(m.connections, m.n_connections, np.uint32), "has_colors": bool(m.success)}
+    finally:
+        ref_lib().ref_mesh_free(ctypes.byref(m))
 
 
 def ref_export_stl(mesh, normal_type, filename, ascii=False):

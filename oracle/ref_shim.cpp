@@ -24,6 +24,7 @@
 #include <kvs/PolygonExporter>
 #include <kvs/Stl>
 #include <kvs/KVSMLPolygonObject>
+#include <kvs/Ply>
 #include <kvs/UnstructuredVolumeObject>
 #include <kvs/MarchingTetrahedraTable>
 #include <kvs/MarchingHexahedraTable>
@@ -539,6 +540,43 @@ int ref_export_kvsml_polygon( const float* coords, uint64_t ncoords, const uint3
     kvs::PolygonExporter<kvs::KVSMLPolygonObject> e( &p );
     e.setWritingDataType( kvs::KVSMLPolygonObject::WritingDataType( writing_type ) );
     return e.write( filename ) ? 0 : 1;
+}
+
+// PolygonExporter<kvs::Ply>( polygon ), setFileType and write( filename ) for
+// a triangle polygon without connections: with connections the reference's
+// writer stores face indices through a null pointer (Ply.cpp:475-478), so
+// this refuses them (returns 2).
+int ref_export_ply( const float* coords, uint64_t ncoords, uint64_t nconn, const float* normals, uint64_t nnormals,
+                    const uint8_t* colors, uint64_t ncolors, int color_type, int normal_type, int binary,
+                    const char* filename )
+{
+    if ( nconn ) return 2;
+    kvs::PolygonObject p;
+    kvs::ValueArray<kvs::Real32> c( ncoords ); if ( ncoords ) std::memcpy( c.data(), coords, ncoords * 4 );
+    p.setCoords( c );
+    if ( nnormals ) { kvs::ValueArray<kvs::Real32> n( nnormals ); std::memcpy( n.data(), normals, nnormals * 4 ); p.setNormals( n ); }
+    if ( ncolors ) { kvs::ValueArray<kvs::UInt8> k( ncolors ); std::memcpy( k.data(), colors, ncolors ); p.setColors( k ); }
+    p.setPolygonType( kvs::PolygonObject::Triangle );
+    p.setColorType( kvs::PolygonObject::ColorType( color_type ) );
+    p.setNormalType( kvs::PolygonObject::NormalType( normal_type ) );
+    kvs::PolygonExporter<kvs::Ply> e( &p );
+    e.setFileType( binary ? kvs::Ply::Binary : kvs::Ply::Ascii );
+    return e.write( filename ) ? 0 : 1;
+}
+
+// Ply::read( filename ): the coordinates, colours, normals and connections the
+// reference's reader makes of a PLY file (free with ref_mesh_free).
+int ref_read_ply( const char* filename, ref_mesh* out )
+{
+    std::memset( out, 0, sizeof( *out ) );
+    kvs::Ply ply;
+    if ( !ply.read( filename ) ) return 1;
+    out->coords = static_cast<float*>( CopyOut( ply.coords(), &out->n_coords ) );
+    out->connections = static_cast<uint32_t*>( CopyOut( ply.connections(), &out->n_connections ) );
+    out->normals = static_cast<float*>( CopyOut( ply.normals(), &out->n_normals ) );
+    out->colors = static_cast<uint8_t*>( CopyOut( ply.colors(), &out->n_colors ) );
+    out->success = ply.hasColors() ? 1 : 0;
+    return 0;
 }
 
 // MarchingCubesTable::TriangleID and VertexID, for pinning transcriptions.

@@ -14,6 +14,7 @@ against, since the reference is not available where they run.
 import hashlib
 import json
 import os
+import subprocess
 import sys
 
 import numpy as np
@@ -466,10 +467,53 @@ def main():
                        "opacity": int(mesh["opacities"][0]) if mesh["opacities"].size else -1, **spec})
     meta["kvsml_polygon"] = kp
 
+    # 15. PLY files the reference's PolygonExporter<kvs::Ply> + Ply::write make
+    #     of surfaces without connections, ascii and binary (with connections
+    #     its writer stores face indices through a null pointer, Ply.cpp:475-478).
+    #     Its ply_close also frees pointers the writing path never set
+    #     (PlyFile.cpp:1345-1380): it runs in a child process whose allocator
+    #     hands out zeroed blocks (no per-thread cache, perturb byte 255)
+    env = dict(os.environ, GLIBC_TUNABLES="glibc.malloc.tcache_count=0:glibc.malloc.perturb=255")
+    subprocess.run([sys.executable, os.path.abspath(__file__), "ply"], env=env, check=True)
+    with open(os.path.join(HERE, "ply", "cases.json")) as f:
+        meta["ply"] = json.load(f)
+    os.remove(os.path.join(HERE, "ply", "cases.json"))
+
     with open(os.path.join(HERE, "golden.json"), "w") as f:
         json.dump(meta, f, indent=1, sort_keys=True)
     print("wrote", sorted(os.listdir(HERE)))
 
+
+def ply_goldens():
+    """Section 15, run in its own process (see main)."""
+    pdir = os.path.join(HERE, "ply")
+    os.makedirs(pdir, exist_ok=True)
+    vol = datasets.blobs(10, seed=7, nblobs=2)
+    vn = oracle.ref_marching_cubes(vol, 0.4, 0, False)
+    vn_noconn = dict(vn, connections=np.zeros(0, np.uint32))
+    meshes = [("mc_n1_d1", oracle.ref_marching_cubes(vol, 0.4, 1, True), {"kind": "mc", "iso": 0.4, "normal": 1,
+                                                                          "dup": True}),
+              ("mc_n0_d0_noconn", vn_noconn, {"kind": "mc", "iso": 0.4, "normal": 0, "dup": False,
+                                              "drop_connections": True}),
+              ("mc_empty", oracle.ref_marching_cubes(vol, 5.0, 1, True), {"kind": "mc", "iso": 5.0, "normal": 1,
+                                                                          "dup": True}),
+              ("slice", oracle.ref_slice_plane(vol, [0.3, -0.5, 0.8, -3.0]), {"kind": "slice",
+                                                                             "coef": [0.3, -0.5, 0.8, -3.0]})]
+    cases = []
+    for name, mesh, spec in meshes:
+        for binary in (False, True):
+            fn = f"{name}_{'binary' if binary else 'ascii'}.ply"
+            oracle.ref_export_ply(mesh, os.path.join(pdir, fn), binary=binary)
+            cases.append({"file": fn, "binary": binary,
+                          "color": [int(x) for x in mesh["colors"]] if mesh["colors"].size == 3 else None,
+                          "sha256": hashlib.sha256(open(os.path.join(pdir, fn), "rb").read()).hexdigest(), **spec})
+    with open(os.path.join(pdir, "cases.json"), "w") as f:
+        json.dump(cases, f)
+
+
+if __name__ == "__main__" and len(sys.argv) > 1 and sys.argv[1] == "ply":
+    ply_goldens()
+    sys.exit(0)
 
 if __name__ == "__main__":
     main()
