@@ -156,24 +156,48 @@ kmc_status surface_to_host(kmc_surface* s, hipStream_t st)
 void run_batch(kmc_server* S, hipStream_t st, const std::vector<uint64_t>& ids)
 {
     const size_t n = ids.size();
-    std::vector<kmc_job> jobs(n);
-    std::vector<double> levels(n);
+    /* requests on one volume with one emit mode become one job whose
+     * isolevels are theirs in queue order: the volume is classified and
+     * counted once for all of them (fused levels, as kmc_extract_levels);
+     * surf[i] is request i's surface (job order, then level order) */
+    std::vector<kmc_job> jobs;
+    std::vector<std::vector<double>> levels;
+    std::vector<size_t> job_of(n), pos_of(n), surf(n);
     {
         std::lock_guard<std::mutex> lk(S->mu);
+        std::vector<uint64_t> keys;
         for (size_t i = 0; i < n; ++i) {
             const Request& r = S->reqs.at(ids[i]);
-            levels[i] = r.level;
-            jobs[i].volume = S->volumes[r.volume].v;
-            jobs[i].n_levels = 1;
-            jobs[i].normal_type = r.normal_type;
-            jobs[i].duplication = r.duplication;
+            const int dup = r.normal_type == KMC_VERTEX_NORMAL ? 0 : (r.duplication ? 1 : 0);   // MarchingCubes.cpp:69-73
+            const uint64_t key = (uint64_t)r.volume << 8 | (uint64_t)r.normal_type << 1 | (uint64_t)dup;
+            size_t j = 0;
+            while (j < keys.size() && keys[j] != key) ++j;
+            if (j == keys.size()) {
+                keys.push_back(key);
+                kmc_job job{};
+                job.volume = S->volumes[r.volume].v;
+                job.normal_type = r.normal_type;
+                job.duplication = dup;
+                jobs.push_back(job);
+                levels.emplace_back();
+            }
+            job_of[i] = j;
+            pos_of[i] = levels[j].size();
+            levels[j].push_back(r.level);
         }
     }
-    for (size_t i = 0; i < n; ++i) jobs[i].isolevels = &levels[i];
+    std::vector<size_t> first(jobs.size(), 0);
+    for (size_t j = 0, f = 0; j < jobs.size(); ++j) {
+        jobs[j].isolevels = levels[j].data();
+        jobs[j].n_levels = (uint32_t)levels[j].size();
+        first[j] = f;
+        f += levels[j].size();
+    }
+    for (size_t i = 0; i < n; ++i) surf[i] = first[job_of[i]] + pos_of[i];
 
     const Clock::time_point t0 = Clock::now();
     kmc_batch* b = nullptr;
-    kmc_status s = batch_plan(jobs.data(), (uint32_t)n, st, &b);
+    kmc_status s = batch_plan(jobs.data(), (uint32_t)jobs.size(), st, &b);
     const Clock::time_point t1 = Clock::now();
     if (s == KMC_OK && (batch_launch_front(b, st) != hipSuccess || hipStreamSynchronize(st) != hipSuccess)) {
         (void)hipGetLastError();
@@ -187,8 +211,8 @@ void run_batch(kmc_server* S, hipStream_t st, const std::vector<uint64_t>& ids)
     const bool front_ok = s == KMC_OK;
     if (s == KMC_OK) {
         for (size_t i = 0; i < n; ++i) {
-            batch_surface_sizes(b, (uint32_t)i, &out[i]);
-            obytes[i] = output_bytes(out[i]) + batch_record_bytes(b, (uint32_t)i);
+            batch_surface_sizes(b, (uint32_t)surf[i], &out[i]);
+            obytes[i] = output_bytes(out[i]) + batch_record_bytes(b, (uint32_t)surf[i]);
         }
         /* reserve outputs in queue order; the rest go back to the queue */
         std::lock_guard<std::mutex> lk(S->mu);
@@ -225,7 +249,7 @@ void run_batch(kmc_server* S, hipStream_t st, const std::vector<uint64_t>& ids)
     }
     if (s == KMC_OK) {
         for (size_t i = 0; i < n && s == KMC_OK; ++i) {
-            if (!admitted[i]) { batch_set_outputs(b, (uint32_t)i, nullptr, nullptr, nullptr); continue; }
+            if (!admitted[i]) { batch_set_outputs(b, (uint32_t)surf[i], nullptr, nullptr, nullptr); continue; }
             SurfaceStorage* store = new SurfaceStorage();
             store->memory = KMC_MEMORY_DEVICE;
             store->coords = store->connections = store->normals = nullptr;
@@ -242,7 +266,7 @@ void run_batch(kmc_server* S, hipStream_t st, const std::vector<uint64_t>& ids)
             out[i].coords = (float*)store->coords;
             out[i].normals = (float*)store->normals;
             out[i].connections = (uint32_t*)store->connections;
-            batch_set_outputs(b, (uint32_t)i, out[i].coords, out[i].normals, out[i].connections);
+            batch_set_outputs(b, (uint32_t)surf[i], out[i].coords, out[i].normals, out[i].connections);
         }
         t3 = Clock::now();
         if (s == KMC_OK && (batch_launch_emit(b, st) != hipSuccess || hipStreamSynchronize(st) != hipSuccess)) {
