@@ -519,7 +519,8 @@ def ct_leg(lib, kmc, datasets, torch, args):
     """512^3 uint8 CT-like volume (value noise + 20 ellipsoids), isolevel 100,
     VertexNormal: one kmc_extract on the HBM-resident volume (device outputs),
     and the PCIe-inclusive drop-in case (host volume in, host surface out);
-    the reference's own kvs::MarchingCubes on the same bytes, 1 thread."""
+    the reference's own kvs::MarchingCubes on the same bytes, 1 thread, its
+    coordinates, connections and normals compared with ours bit for bit."""
     import numpy as np
 
     import oracle
@@ -561,8 +562,14 @@ def ct_leg(lib, kmc, datasets, torch, args):
         print("[bench] CT leg: reference CPU path", file=sys.stderr, flush=True)
         r = oracle.ref_marching_cubes(host, iso, oracle.VERTEX_NORMAL, False, min_max_values=(mn.value, mx.value))
         assert r["connections"].size // 3 == nt, (r["connections"].size // 3, nt)
+        mine = kmc.extract(host, iso, kmc.VERTEX_NORMAL, False, min_max=(mn.value, mx.value))
+        same = all(np.array_equal(np.ascontiguousarray(mine[k]).view(np.uint32),
+                                  np.ascontiguousarray(r[k]).view(np.uint32)) for k in ("coords", "normals")) and \
+            np.array_equal(mine["connections"], r["connections"])
         cpu = {"ms": round(r["seconds"] * 1e3, 1), "cores": 1, "kind": "reference",
-               "speedup_device": round(r["seconds"] * 1e3 / dev_ms, 1)}
+               "speedup_device": round(r["seconds"] * 1e3 / dev_ms, 1),
+               "gpu_output_identical": bool(same)}
+        del mine, r
     return {
         "workload": "512^3 uint8 CT-like (32^3 value-noise lattice + 20 ellipsoids), isolevel 100, VertexNormal",
         "vertices": nv, "triangles": nt,
