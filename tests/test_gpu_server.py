@@ -172,3 +172,24 @@ def test_refused_request_in_a_batch_gets_no_device_memory():
         assert 0 < st["peak_batch_device_bytes"] <= budget
     finally:
         s.close()
+
+
+def test_many_requests_on_one_volume_share_passes():
+    # a burst of requests on one volume queues behind the first batch; the
+    # next batch gives each (volume, emit mode) one job with the requests'
+    # isolevels fused, more than the eight a pass holds in the PolygonNormal
+    # group; each result equals its plain call and the oracle
+    v = datasets.turbulence((64, 48, 40), seed=21)
+    lo, hi = np.percentile(v, [10, 90])
+    levels = [float(x) for x in np.linspace(lo, hi, 20)]
+    reqs = [(0, lv, kmc.POLYGON_NORMAL, True) for lv in levels]
+    reqs += [(0, lv, kmc.VERTEX_NORMAL, False) for lv in levels[::4]]
+    reqs += [(0, lv, kmc.POLYGON_NORMAL, False) for lv in levels[1::5]]
+    server = kmc.Server(budget_bytes=1 << 30, workers=1)
+    try:
+        check_mix(server, [v], reqs)
+        st = server.stats()
+        print("requests", len(reqs), "batches", st["batches"], flush=True)
+        assert st["completed"] == len(reqs)
+    finally:
+        server.close()
