@@ -164,6 +164,11 @@ __device__ __forceinline__ void wave_sync()
 
 __device__ __forceinline__ uint32_t uniform(uint32_t x) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)x); }
 
+__device__ __forceinline__ uint64_t uniform64(uint64_t x)
+{
+    return (uint64_t)uniform((uint32_t)(x >> 32)) << 32 | uniform((uint32_t)x);
+}
+
 __device__ __forceinline__ uint64_t readlane64(uint64_t x, int k)
 {
     const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)x, k);
@@ -1110,42 +1115,87 @@ struct RowChunk {
 };
 
 __device__ __forceinline__ void row_chunk(const JobArgs& a, const uint64_t* lev, uint32_t y, uint32_t z, uint32_t c,
-                                          bool cells, RowChunk& o)
+                                          RowChunk& o)
 {
     const uint64_t slice_words = (uint64_t)a.ny * a.NC;
-    const bool has_up = y + 1 < a.ny, has_next = z + 1 < a.nz, has_cells = cells && z >= 1 && has_up;
+    const bool has_up = y + 1 < a.ny, has_next = z + 1 < a.nz, has_cells = z >= 1 && has_up;
+    /* Every load is issued (no branch around it): a neighbour row outside the
+     * volume is read as the row itself, which makes its crossings (b ^ u,
+     * b ^ n0) zero, and the cells are masked out below.  The next chunk of a
+     * row's last chunk is read as the chunk itself: its bit 0 lands only on
+     * bit 63 of the shifted words and of `next`, i.e. on node / cell 64c + 63,
+     * which lies at or past nx - 1 in a row's last chunk, so valid_mask(c,
+     * nx - 1) clears it from x and act (and corner_pairs reads `next` only for
+     * an active cell 63). */
     const uint64_t* row = lev + (uint64_t)z * slice_words + (uint64_t)y * a.NC;
-    const uint64_t* up = row + a.NC;
-    const uint64_t* nz = row + slice_words;
-    const bool nxt = c + 1 < a.NC;
-    const uint64_t b = row[c];
-    const uint64_t b1 = nxt ? row[c + 1] : 0ull;
-    const uint64_t u = has_up ? up[c] : 0ull;
-    const uint64_t u1 = (has_up && nxt) ? up[c + 1] : 0ull;
-    const uint64_t n0 = has_next ? nz[c] : 0ull;
+    const uint64_t* up = has_up ? row + a.NC : row;
+    const uint64_t* nz = has_next ? row + slice_words : row;
+    const uint64_t* pr = has_cells ? row - slice_words : row;
+    const uint64_t* pu = has_cells ? pr + a.NC : row;
+    const uint32_t cn = c + 1 < a.NC ? c + 1 : c;
+    const uint64_t b = row[c], b1 = row[cn];
+    const uint64_t u = up[c], u1 = up[cn];
+    const uint64_t n0 = nz[c];
+    const uint64_t p = pr[c], p1 = pr[cn];
+    const uint64_t q = pu[c], q1 = pu[cn];
     const uint64_t c4 = b, c5 = (b >> 1) | (b1 << 63);
     const uint64_t c7 = u, c6 = (u >> 1) | (u1 << 63);
-    o.x = (b ^ c5) & valid_mask(c, a.nx - 1);
-    o.y = has_up ? ((b ^ u) & valid_mask(c, a.nx)) : 0ull;
-    o.z = has_next ? ((b ^ n0) & valid_mask(c, a.nx)) : 0ull;
-    o.act = 0ull;
-    o.k0 = o.k3 = o.k4 = o.k7 = 0ull;
-    o.next = 0u;
-    if (has_cells) {
-        const uint64_t* pr = row - slice_words;
-        const uint64_t* pu = pr + a.NC;
-        const uint64_t p = pr[c], p1 = nxt ? pr[c + 1] : 0ull;
-        const uint64_t q = pu[c], q1 = nxt ? pu[c + 1] : 0ull;
-        const uint64_t c1 = (p >> 1) | (p1 << 63), c2 = (q >> 1) | (q1 << 63);
-        const uint64_t or_ = p | c1 | c2 | q | c4 | c5 | c6 | c7;
-        const uint64_t an = p & c1 & c2 & q & c4 & c5 & c6 & c7;
-        o.act = (or_ & ~an) & valid_mask(c, a.nx - 1);
-        o.k0 = p;
-        o.k3 = q;
-        o.k4 = c4;
-        o.k7 = c7;
-        o.next = (uint32_t)(p1 & 1ull) | (uint32_t)(q1 & 1ull) << 1 | (uint32_t)(b1 & 1ull) << 2 |
-                 (uint32_t)(u1 & 1ull) << 3;
+    const uint64_t c1 = (p >> 1) | (p1 << 63), c2 = (q >> 1) | (q1 << 63);
+    const uint64_t vx = valid_mask(c, a.nx - 1), vn = valid_mask(c, a.nx);
+    o.x = (b ^ c5) & vx;
+    o.y = (b ^ u) & vn;
+    o.z = (b ^ n0) & vn;
+    const uint64_t or_ = p | c1 | c2 | q | c4 | c5 | c6 | c7;
+    const uint64_t an = p & c1 & c2 & q & c4 & c5 & c6 & c7;
+    o.act = has_cells ? (or_ & ~an) & vx : 0ull;
+    o.k0 = p;
+    o.k3 = q;
+    o.k4 = c4;
+    o.k7 = c7;
+    o.next = (uint32_t)(p1 & 1ull) | (uint32_t)(q1 & 1ull) << 1 | (uint32_t)(b1 & 1ull) << 2 |
+             (uint32_t)(u1 & 1ull) << 3;
+}
+
+/* (y, z) of node row g + sub of a wave that walks row groups g = g0, g0 +
+ * step, ... (g0 and step are the same on all of the wave's lanes): the
+ * group's first row advances by the step's quotient and remainder by ny,
+ * instead of a division of every row index. */
+struct RowWalk { uint32_t yg, zg, qs, rs; };
+
+__device__ __forceinline__ RowWalk row_walk(const JobArgs& a, uint64_t g0, uint64_t step)
+{
+    RowWalk w;
+    const uint64_t g = uniform64(g0), s = uniform64(step);
+    w.zg = (uint32_t)(g / a.ny);
+    w.yg = (uint32_t)(g - (uint64_t)w.zg * a.ny);
+    w.qs = (uint32_t)(s / a.ny);
+    w.rs = (uint32_t)(s - (uint64_t)w.qs * a.ny);
+    return w;
+}
+
+__device__ __forceinline__ void row_at(const JobArgs& a, const RowWalk& w, uint32_t sub, uint32_t& y, uint32_t& z)
+{
+    y = w.yg + sub;
+    z = w.zg;
+    if (y >= a.ny) {
+        /* sub < 64: one wrap when ny >= 64 */
+        if (a.ny >= 64u) {
+            y -= a.ny;
+            ++z;
+        } else {
+            z += y / a.ny;
+            y %= a.ny;
+        }
+    }
+}
+
+__device__ __forceinline__ void row_next(const JobArgs& a, RowWalk& w)
+{
+    w.yg += w.rs;
+    w.zg += w.qs;
+    if (w.yg >= a.ny) {
+        w.yg -= a.ny;
+        ++w.zg;
     }
 }
 
@@ -1214,11 +1264,12 @@ __device__ __forceinline__ void count_rows_t(const JobArgs& a, const uint8_t* nt
     const uint32_t G = 64u >> a.cshift, CS = 1u << a.cshift;
     const uint64_t level_words = (uint64_t)a.nz * a.ny * a.NC;
     const bool chunk_prefixes = a.vpc != nullptr;
-    for (uint64_t g = r0 + wave_first * G; g < r1; g += wave_step * G) {
+    RowWalk rw = row_walk(a, r0 + wave_first * G, wave_step * G);
+    for (uint64_t g = r0 + wave_first * G; g < r1; g += wave_step * G, row_next(a, rw)) {
         const uint64_t r = g + sub;
         const bool ok = r < r1;
-        const uint32_t z = ok ? (uint32_t)(r / a.ny) : 0u;
-        const uint32_t y = ok ? (uint32_t)(r - (uint64_t)z * a.ny) : 0u;
+        uint32_t y, z;
+        row_at(a, rw, sub, y, z);
         const uint32_t nl = NLEV ? (uint32_t)NLEV : a.nlev;
 #pragma unroll
         for (uint32_t l = 0; l < nl; ++l) {
@@ -1229,7 +1280,7 @@ __device__ __forceinline__ void count_rows_t(const JobArgs& a, const uint8_t* nt
                 uint32_t cv = 0, ct = 0;
                 if (ok && c < a.NC) {
                     RowChunk o;
-                    row_chunk(a, lev, y, z, c, true, o);
+                    row_chunk(a, lev, y, z, c, o);
                     cv = (uint32_t)(__popcll(o.x) + __popcll(o.y) + __popcll(o.z));
                     uint64_t m = o.act;
                     while (m) {
@@ -1300,9 +1351,10 @@ __device__ __forceinline__ void stage_ntri(uint8_t* ntri)
     __syncthreads();
 }
 
-/* eight waves per SIMD (64 VGPRs, a few bytes of spill) beat seven without:
- * the pass is bound by VALU issue and load latency */
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) void k_count(JobArgs a)
+/* seven waves per SIMD (at most 72 VGPRs): the pass is bound by VALU issue
+ * and load latency, and at eight waves (64 VGPRs) its spills cost more
+ * issue than the eighth wave hides */
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(7))) void k_count(JobArgs a)
 {
     __shared__ uint8_t ntri[256];
     stage_ntri(ntri);
@@ -1468,7 +1520,7 @@ __device__ __forceinline__ void compact_chunk(const JobArgs& a, const uint8_t* n
     const uint64_t* pre_v = a.prefix + (uint64_t)(l * 2 + 0) * a.nrows;
     const uint64_t* pre_t = a.prefix + (uint64_t)(l * 2 + 1) * a.nrows;
     RowChunk o;
-    row_chunk(a, a.bitvol + (uint64_t)l * level_words, y, z, c, true, o);
+    row_chunk(a, a.bitvol + (uint64_t)l * level_words, y, z, c, o);
     uint32_t pv, pt;
     chunk_prefix(a, w, pv, pt);
     if (!a.dup) {
@@ -1539,11 +1591,12 @@ __device__ __forceinline__ void compact_rows_t(const JobArgs& a, const uint8_t* 
     const uint64_t level_words = (uint64_t)a.nz * a.ny * a.NC;
     if (!a.trec) return;   // the records did not fit (batched form): nothing to write
     uint32_t qn = 0;
-    for (uint64_t g = r0 + wave_first * G; g < r1; g += wave_step * G) {
+    RowWalk rw = row_walk(a, r0 + wave_first * G, wave_step * G);
+    for (uint64_t g = r0 + wave_first * G; g < r1; g += wave_step * G, row_next(a, rw)) {
         const uint64_t r = g + sub;
         const bool ok = r < r1;
-        const uint32_t z = ok ? (uint32_t)(r / a.ny) : 0u;
-        const uint32_t y = ok ? (uint32_t)(r - (uint64_t)z * a.ny) : 0u;
+        uint32_t y, z;
+        row_at(a, rw, sub, y, z);
         for (uint32_t l0 = 0; l0 < a.nlev; l0 += NL) {
             for (uint32_t c0 = 0; c0 < a.NC; c0 += CS) {
                 const uint32_t c = c0 + cl;
@@ -2074,11 +2127,6 @@ __device__ __forceinline__ BlockPart block_part(uint64_t n)
     p.lo = (uint64_t)blockIdx.x * per < n ? (uint64_t)blockIdx.x * per : n;
     p.hi = p.lo + per < n ? p.lo + per : n;
     return p;
-}
-
-__device__ __forceinline__ uint64_t uniform64(uint64_t x)
-{
-    return (uint64_t)uniform((uint32_t)(x >> 32)) << 32 | uniform((uint32_t)x);
 }
 
 /* Calls f(jobs[j], lo, hi) for each job j with items in this workgroup's part,
