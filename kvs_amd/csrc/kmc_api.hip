@@ -190,6 +190,21 @@ void host_pinned_free(void* p)
     if (it != g_pin_size.end()) g_pin_free.emplace(it->second, p);
 }
 
+/* n bytes (totals, sizes, min/max) from device memory into host memory,
+ * through a pinned staging block, then a wait for the stream: a copy into
+ * pageable memory is staged by the runtime at a higher latency, and these
+ * copies sit between a pass's front and its emit. */
+hipError_t read_back(void* host, const void* dev, size_t n, hipStream_t st)
+{
+    void* pin = host_pinned_alloc(n);
+    if (!pin) return hipErrorOutOfMemory;
+    hipError_t e = hipMemcpyAsync(pin, dev, n, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e == hipSuccess) std::memcpy(host, pin, n);
+    host_pinned_free(pin);
+    return e;
+}
+
 /* ------------------------------------------------------------------------- */
 /* surface ownership                                                           */
 /* ------------------------------------------------------------------------- */
@@ -300,8 +315,7 @@ kmc_status min_max_on_device(const kmc_volume* v, const void* dvals, hipStream_t
     kmc_status s = KMC_OK;
     double host[2] = {0, 0};
     if (launch_minmax(v->type, dvals, n, scratch, dout, st) != hipSuccess ||
-        hipMemcpyAsync(host, dout, sizeof(host), hipMemcpyDeviceToHost, st) != hipSuccess ||
-        hipStreamSynchronize(st) != hipSuccess) {
+        read_back(host, dout, sizeof(host), st) != hipSuccess) {
         s = KMC_ERR_DEVICE;
     }
     device_free(scratch);
@@ -555,8 +569,7 @@ kmc_status run_pass(const kmc_volume* v, const void* dvals, const double* iso, u
         return KMC_ERR_DEVICE;
     }
     std::vector<uint64_t> tot(narr, 0);
-    if (hipMemcpyAsync(tot.data(), totals, narr * sizeof(uint64_t), hipMemcpyDeviceToHost, st) != hipSuccess ||
-        hipStreamSynchronize(st) != hipSuccess) {
+    if (read_back(tot.data(), totals, narr * sizeof(uint64_t), st) != hipSuccess) {
         cleanup();
         return KMC_ERR_DEVICE;
     }

@@ -84,7 +84,7 @@ kmc_status slice_by_thresholds(const kmc_volume* v, JobArgs& a, hipStream_t st, 
     if (hipMemcpyAsync(dtable, table, 3 * (size_t)res, hipMemcpyHostToDevice, st) != hipSuccess ||
         (ncr && (launch_slice_runs_front(a, thr, counts, st) != hipSuccess ||
                  launch_scan(counts, ncr, 1, aux, prefix, aux + nparts, st) != hipSuccess ||
-                 hipMemcpyAsync(&nt, aux + nparts, 8, hipMemcpyDeviceToHost, st) != hipSuccess)) ||
+                 read_back(&nt, aux + nparts, 8, st) != hipSuccess)) ||
         hipStreamSynchronize(st) != hipSuccess) {
         (void)hipGetLastError();
         return KMC_ERR_DEVICE;
@@ -211,8 +211,7 @@ kmc_status slice_plane_impl(const kmc_volume* v, const float coef[4], const uint
     uint64_t tot[2] = { 0, 0 };
     if (launch_classify_plane(a, st) != hipSuccess || launch_count(a, st) != hipSuccess ||
         launch_scan(counts, R, 2, partials, prefix, totals, st) != hipSuccess ||
-        hipMemcpyAsync(tot, totals, sizeof(tot), hipMemcpyDeviceToHost, st) != hipSuccess ||
-        hipStreamSynchronize(st) != hipSuccess) {
+        read_back(tot, totals, sizeof(tot), st) != hipSuccess) {
         (void)hipGetLastError();
         return finish(KMC_ERR_DEVICE);
     }

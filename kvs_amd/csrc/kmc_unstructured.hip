@@ -498,24 +498,31 @@ kmc_status run_cells(const kmc_unstructured_volume* v, bool slice, double iso, i
         double* dmm = (double*)(scratch + minmax_scratch_bytes());
         float* dcc = (float*)(scratch + minmax_scratch_bytes() + 16);
         CoordSpan* spans = (CoordSpan*)(scratch + minmax_scratch_bytes() + 64);
-        double hmm[2] = { mn, mx };
-        float hcc[6];
         bool ok = true;
-        if (need_values)
-            ok = launch_minmax_gather(v->type, dvals, dconn, nconn, scratch, dmm, st) == hipSuccess &&
-                 hipMemcpyAsync(hmm, dmm, 16, hipMemcpyDeviceToHost, st) == hipSuccess;
+        if (need_values) ok = launch_minmax_gather(v->type, dvals, dconn, nconn, scratch, dmm, st) == hipSuccess;
         if (ok && need_coords) {
             hipLaunchKernelGGL(k_coord_span_partial, dim3(kSpanParts), dim3(kBlock), 0, st, dcoords, dconn, nconn,
                                spans);
             hipLaunchKernelGGL(k_coord_span_final, dim3(1), dim3(kBlock), 0, st, spans, dcc);
-            ok = hipGetLastError() == hipSuccess &&
-                 hipMemcpyAsync(hcc, dcc, 24, hipMemcpyDeviceToHost, st) == hipSuccess;
+            ok = hipGetLastError() == hipSuccess;
         }
-        if (!ok || hipStreamSynchronize(st) != hipSuccess) { (void)hipGetLastError(); return finish(KMC_ERR_DEVICE); }
-        mn = hmm[0];
-        mx = hmm[1];
-        if (need_coords)
+        /* the min/max (16 bytes) and the coordinate span (24) lie together */
+        uint8_t back[40];
+        if (!ok || read_back(back, dmm, sizeof(back), st) != hipSuccess) {
+            (void)hipGetLastError();
+            return finish(KMC_ERR_DEVICE);
+        }
+        if (need_values) {
+            double hmm[2];
+            std::memcpy(hmm, back, 16);
+            mn = hmm[0];
+            mx = hmm[1];
+        }
+        if (need_coords) {
+            float hcc[6];
+            std::memcpy(hcc, back + 16, 24);
             for (int k = 0; k < 3; ++k) { minc[k] = hcc[k]; maxc[k] = hcc[3 + k]; }
+        }
     }
 
     /* the surface as the mapper leaves it */
@@ -577,8 +584,7 @@ kmc_status run_cells(const kmc_unstructured_volume* v, bool slice, double iso, i
     const hipError_t e1 = slice ? launch_cells<true>(v->type, kind, a, false, st)
                                 : launch_cells<false>(v->type, kind, a, false, st);
     if (e1 != hipSuccess || launch_scan(a.counts, ncells, 1, aux, prefix, aux + nparts, st) != hipSuccess ||
-        hipMemcpyAsync(&nt, aux + nparts, 8, hipMemcpyDeviceToHost, st) != hipSuccess ||
-        hipStreamSynchronize(st) != hipSuccess) {
+        read_back(&nt, aux + nparts, 8, st) != hipSuccess) {
         (void)hipGetLastError();
         return finish(KMC_ERR_DEVICE);
     }
