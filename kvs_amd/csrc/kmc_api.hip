@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "kmc.h"
+#include "kmc_guard.h"
 #include "kmc_job.h"
 #include "kmc_internal.h"
 
@@ -751,12 +752,14 @@ extern "C" {
 
 kmc_status kmc_init(int device)
 {
-    int n = 0;
-    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) { (void)hipGetLastError(); return KMC_ERR_NO_DEVICE; }
-    if (device < 0 || device >= n) return KMC_ERR_INVALID_ARGUMENT;
-    t_device = device;
-    t_device_set = true;
-    return hipSetDevice(device) == hipSuccess ? KMC_OK : KMC_ERR_DEVICE;
+    return kmc::abi_guard([&]() -> kmc_status {
+        int n = 0;
+        if (hipGetDeviceCount(&n) != hipSuccess || n == 0) { (void)hipGetLastError(); return KMC_ERR_NO_DEVICE; }
+        if (device < 0 || device >= n) return KMC_ERR_INVALID_ARGUMENT;
+        t_device = device;
+        t_device_set = true;
+        return hipSetDevice(device) == hipSuccess ? KMC_OK : KMC_ERR_DEVICE;
+    });
 }
 
 const char* kmc_status_string(kmc_status s)
@@ -785,19 +788,23 @@ const char* kmc_status_string(kmc_status s)
 kmc_status kmc_extract(const kmc_volume* volume, double isolevel, int32_t normal_type, int32_t duplication,
                        int32_t out_memory, void* stream, kmc_surface* out)
 {
-    const kmc_status s = extract_levels_impl(volume, &isolevel, 1, normal_type, duplication, out_memory, stream, out);
-    report(s);
-    return s;
+    return kmc::abi_guard([&]() -> kmc_status {
+        const kmc_status s = extract_levels_impl(volume, &isolevel, 1, normal_type, duplication, out_memory, stream, out);
+        report(s);
+        return s;
+    });
 }
 
 kmc_status kmc_extract_levels(const kmc_volume* volume, const double* isolevels, uint32_t n_levels,
                               int32_t normal_type, int32_t duplication, int32_t out_memory, void* stream,
                               kmc_surface* outs)
 {
-    const kmc_status s = extract_levels_impl(volume, isolevels, n_levels, normal_type, duplication, out_memory,
-                                             stream, outs);
-    report(s);
-    return s;
+    return kmc::abi_guard([&]() -> kmc_status {
+        const kmc_status s = extract_levels_impl(volume, isolevels, n_levels, normal_type, duplication, out_memory,
+                                                 stream, outs);
+        report(s);
+        return s;
+    });
 }
 
 void kmc_surface_release(kmc_surface* s)
@@ -816,25 +823,29 @@ void kmc_set_timing(int32_t enable) { t_timing = enable != 0; }
 
 kmc_status kmc_last_timing(kmc_timing* out)
 {
-    if (!out) return KMC_ERR_INVALID_ARGUMENT;
-    *out = t_last;
-    return KMC_OK;
+    return kmc::abi_guard([&]() -> kmc_status {
+        if (!out) return KMC_ERR_INVALID_ARGUMENT;
+        *out = t_last;
+        return KMC_OK;
+    });
 }
 
 kmc_status kmc_volume_min_max(const kmc_volume* v, void* stream, double* mn, double* mx)
 {
-    kmc_status s = validate_volume(v);
-    if (s == KMC_OK && (!mn || !mx)) s = KMC_ERR_INVALID_ARGUMENT;
-    if (s == KMC_OK) s = ensure_device();
-    if (s != KMC_OK) { report(s); return s; }
-    hipStream_t st = (hipStream_t)stream;
-    const void* dvals = nullptr;
-    void* owned = nullptr;
-    s = device_values(v, st, &dvals, &owned);
-    if (s == KMC_OK) s = min_max_on_device(v, dvals, st, mn, mx);
-    if (owned) { (void)hipStreamSynchronize(st); device_free(owned); }
-    report(s);
-    return s;
+    return kmc::abi_guard([&]() -> kmc_status {
+        kmc_status s = validate_volume(v);
+        if (s == KMC_OK && (!mn || !mx)) s = KMC_ERR_INVALID_ARGUMENT;
+        if (s == KMC_OK) s = ensure_device();
+        if (s != KMC_OK) { report(s); return s; }
+        hipStream_t st = (hipStream_t)stream;
+        const void* dvals = nullptr;
+        void* owned = nullptr;
+        s = device_values(v, st, &dvals, &owned);
+        if (s == KMC_OK) s = min_max_on_device(v, dvals, st, mn, mx);
+        if (owned) { (void)hipStreamSynchronize(st); device_free(owned); }
+        report(s);
+        return s;
+    });
 }
 
 }  // extern "C"

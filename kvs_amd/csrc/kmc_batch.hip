@@ -20,6 +20,7 @@
 #include <vector>
 
 #include "kmc.h"
+#include "kmc_guard.h"
 #include "kmc_internal.h"
 #include "kmc_job.h"
 
@@ -934,48 +935,56 @@ extern "C" {
 
 kmc_status kmc_batch_create(const kmc_job* jobs, uint32_t n_jobs, uint32_t flags, void* stream, kmc_batch** out)
 {
-    const kmc_status s = create_impl(jobs, n_jobs, flags, (hipStream_t)stream, out);
-    report(s);
-    return s;
+    return kmc::abi_guard([&]() -> kmc_status {
+        const kmc_status s = create_impl(jobs, n_jobs, flags, (hipStream_t)stream, out);
+        report(s);
+        return s;
+    });
 }
 
 kmc_status kmc_batch_run(kmc_batch* b, void* stream)
 {
-    if (!b) { report(KMC_ERR_INVALID_ARGUMENT); return KMC_ERR_INVALID_ARGUMENT; }
-    if (hipSetDevice(b->device) != hipSuccess) { report(KMC_ERR_DEVICE); return KMC_ERR_DEVICE; }
-    const kmc_status s = run(b, (hipStream_t)stream);
-    report(s);
-    return s;
+    return kmc::abi_guard([&]() -> kmc_status {
+        if (!b) { report(KMC_ERR_INVALID_ARGUMENT); return KMC_ERR_INVALID_ARGUMENT; }
+        if (hipSetDevice(b->device) != hipSuccess) { report(KMC_ERR_DEVICE); return KMC_ERR_DEVICE; }
+        const kmc_status s = run(b, (hipStream_t)stream);
+        report(s);
+        return s;
+    });
 }
 
 kmc_status kmc_batch_surfaces(const kmc_batch* b, kmc_surface* outs, uint32_t n_outs)
 {
-    if (!b || !outs || n_outs < b->n_surfaces) return KMC_ERR_INVALID_ARGUMENT;
-    if (!b->have_results) return KMC_ERR_NOT_READY;
-    for (uint32_t i = 0; i < b->n_surfaces; ++i) surface_view(b, i, &outs[i]);
-    return KMC_OK;
+    return kmc::abi_guard([&]() -> kmc_status {
+        if (!b || !outs || n_outs < b->n_surfaces) return KMC_ERR_INVALID_ARGUMENT;
+        if (!b->have_results) return KMC_ERR_NOT_READY;
+        for (uint32_t i = 0; i < b->n_surfaces; ++i) surface_view(b, i, &outs[i]);
+        return KMC_OK;
+    });
 }
 
 kmc_status kmc_batch_info_get(const kmc_batch* b, kmc_batch_info* info)
 {
-    if (!b || !info) return KMC_ERR_INVALID_ARGUMENT;
-    std::memset(info, 0, sizeof(*info));
-    info->n_jobs = (uint32_t)b->users.size();
-    info->n_surfaces = b->n_surfaces;
-    info->n_passes = (uint32_t)b->passes.size();
-    info->n_type_groups = (uint32_t)b->groups.size();
-    info->tile_rows = b->r_max;
-    info->queue_capacity = b->queue_cap;
-    info->graph_nodes = b->graph_nodes;
-    info->recaptures = b->recaptures;
-    for (const Pass& p : b->passes) info->nodes_classified += (uint64_t)p.a.nx * p.a.ny * p.a.nz * p.nlev;
-    if (b->have_results)
-        for (uint32_t s = 0; s < b->seg_pass_level.size(); ++s) info->n_triangles += b->segout_h[s].nt;
-    info->arena_bytes = 4 * (b->cap_c + b->cap_n + b->cap_k);
-    info->work_bytes = b->n_counts * 12 + b->bitvol_words * 8;
-    info->last_run_ms = b->last_ms;
-    info->flat_emit = b->flat ? 1u : 0u;
-    return KMC_OK;
+    return kmc::abi_guard([&]() -> kmc_status {
+        if (!b || !info) return KMC_ERR_INVALID_ARGUMENT;
+        std::memset(info, 0, sizeof(*info));
+        info->n_jobs = (uint32_t)b->users.size();
+        info->n_surfaces = b->n_surfaces;
+        info->n_passes = (uint32_t)b->passes.size();
+        info->n_type_groups = (uint32_t)b->groups.size();
+        info->tile_rows = b->r_max;
+        info->queue_capacity = b->queue_cap;
+        info->graph_nodes = b->graph_nodes;
+        info->recaptures = b->recaptures;
+        for (const Pass& p : b->passes) info->nodes_classified += (uint64_t)p.a.nx * p.a.ny * p.a.nz * p.nlev;
+        if (b->have_results)
+            for (uint32_t s = 0; s < b->seg_pass_level.size(); ++s) info->n_triangles += b->segout_h[s].nt;
+        info->arena_bytes = 4 * (b->cap_c + b->cap_n + b->cap_k);
+        info->work_bytes = b->n_counts * 12 + b->bitvol_words * 8;
+        info->last_run_ms = b->last_ms;
+        info->flat_emit = b->flat ? 1u : 0u;
+        return KMC_OK;
+    });
 }
 
 void kmc_batch_destroy(kmc_batch* b) { destroy(b); }
@@ -983,48 +992,50 @@ void kmc_batch_destroy(kmc_batch* b) { destroy(b); }
 kmc_status kmc_batch_extract(const kmc_job* jobs, uint32_t n_jobs, int32_t out_memory, void* stream,
                              kmc_surface* outs)
 {
-    if (out_memory != KMC_MEMORY_HOST && out_memory != KMC_MEMORY_DEVICE) {
-        report(KMC_ERR_INVALID_ARGUMENT);
-        return KMC_ERR_INVALID_ARGUMENT;
-    }
-    kmc_batch* b = nullptr;
-    kmc_status s = create_impl(jobs, n_jobs, 0, (hipStream_t)stream, &b);
-    if (s != KMC_OK) { report(s); return s; }
-    if (!outs && b->n_surfaces > 0) { destroy(b); report(KMC_ERR_INVALID_ARGUMENT); return KMC_ERR_INVALID_ARGUMENT; }
-    hipStream_t st = (hipStream_t)stream;
-    const hipMemcpyKind kind = out_memory == KMC_MEMORY_HOST ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice;
-    uint32_t done = 0;
-    for (uint32_t i = 0; i < b->n_surfaces && s == KMC_OK; ++i, ++done) {
-        kmc_surface v;
-        surface_view(b, i, &v);
-        SurfaceStorage* store = new SurfaceStorage();
-        store->memory = out_memory;
-        store->coords = store->connections = store->normals = nullptr;
-        const size_t bc = v.n_vertices * 12, bn = v.n_normals * 12, bk = v.n_connections * 4;
-        auto alloc = [&](void** p, size_t n) -> bool {
-            if (n == 0) return true;
-            if (out_memory == KMC_MEMORY_HOST) return hipHostMalloc(p, n, hipHostMallocDefault) == hipSuccess;
-            *p = device_alloc(n);
-            return *p != nullptr;
-        };
-        bool ok = alloc(&store->coords, bc) && alloc(&store->normals, bn) && alloc(&store->connections, bk);
-        if (ok && bc) ok = hipMemcpyAsync(store->coords, v.coords, bc, kind, st) == hipSuccess;
-        if (ok && bn) ok = hipMemcpyAsync(store->normals, v.normals, bn, kind, st) == hipSuccess;
-        if (ok && bk) ok = hipMemcpyAsync(store->connections, v.connections, bk, kind, st) == hipSuccess;
-        v.memory = out_memory;
-        v.coords = (float*)store->coords;
-        v.normals = (float*)store->normals;
-        v.connections = (uint32_t*)store->connections;
-        v.internal = store;
-        outs[i] = v;
-        if (!ok) { (void)hipGetLastError(); s = KMC_ERR_OUT_OF_MEMORY; }
-    }
-    if (s == KMC_OK && hipStreamSynchronize(st) != hipSuccess) s = KMC_ERR_DEVICE;
-    if (s != KMC_OK)
-        for (uint32_t i = 0; i < done; ++i) kmc_surface_release(&outs[i]);
-    destroy(b);
-    report(s);
-    return s;
+    return kmc::abi_guard([&]() -> kmc_status {
+        if (out_memory != KMC_MEMORY_HOST && out_memory != KMC_MEMORY_DEVICE) {
+            report(KMC_ERR_INVALID_ARGUMENT);
+            return KMC_ERR_INVALID_ARGUMENT;
+        }
+        kmc_batch* b = nullptr;
+        kmc_status s = create_impl(jobs, n_jobs, 0, (hipStream_t)stream, &b);
+        if (s != KMC_OK) { report(s); return s; }
+        if (!outs && b->n_surfaces > 0) { destroy(b); report(KMC_ERR_INVALID_ARGUMENT); return KMC_ERR_INVALID_ARGUMENT; }
+        hipStream_t st = (hipStream_t)stream;
+        const hipMemcpyKind kind = out_memory == KMC_MEMORY_HOST ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice;
+        uint32_t done = 0;
+        for (uint32_t i = 0; i < b->n_surfaces && s == KMC_OK; ++i, ++done) {
+            kmc_surface v;
+            surface_view(b, i, &v);
+            SurfaceStorage* store = new SurfaceStorage();
+            store->memory = out_memory;
+            store->coords = store->connections = store->normals = nullptr;
+            const size_t bc = v.n_vertices * 12, bn = v.n_normals * 12, bk = v.n_connections * 4;
+            auto alloc = [&](void** p, size_t n) -> bool {
+                if (n == 0) return true;
+                if (out_memory == KMC_MEMORY_HOST) return hipHostMalloc(p, n, hipHostMallocDefault) == hipSuccess;
+                *p = device_alloc(n);
+                return *p != nullptr;
+            };
+            bool ok = alloc(&store->coords, bc) && alloc(&store->normals, bn) && alloc(&store->connections, bk);
+            if (ok && bc) ok = hipMemcpyAsync(store->coords, v.coords, bc, kind, st) == hipSuccess;
+            if (ok && bn) ok = hipMemcpyAsync(store->normals, v.normals, bn, kind, st) == hipSuccess;
+            if (ok && bk) ok = hipMemcpyAsync(store->connections, v.connections, bk, kind, st) == hipSuccess;
+            v.memory = out_memory;
+            v.coords = (float*)store->coords;
+            v.normals = (float*)store->normals;
+            v.connections = (uint32_t*)store->connections;
+            v.internal = store;
+            outs[i] = v;
+            if (!ok) { (void)hipGetLastError(); s = KMC_ERR_OUT_OF_MEMORY; }
+        }
+        if (s == KMC_OK && hipStreamSynchronize(st) != hipSuccess) s = KMC_ERR_DEVICE;
+        if (s != KMC_OK)
+            for (uint32_t i = 0; i < done; ++i) kmc_surface_release(&outs[i]);
+        destroy(b);
+        report(s);
+        return s;
+    });
 }
 
 }  // extern "C"

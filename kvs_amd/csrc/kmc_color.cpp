@@ -23,6 +23,7 @@
 #include <vector>
 
 #include "kmc.h"
+#include "kmc_guard.h"
 
 namespace {
 
@@ -211,45 +212,49 @@ extern "C" {
 
 kmc_status kmc_colormap_table(int32_t colormap, uint32_t resolution, uint8_t* rgb_out)
 {
-    if (!rgb_out) return KMC_ERR_INVALID_ARGUMENT;
-    std::vector<uint8_t> t;
-    if (!create_table(colormap, resolution, t)) return KMC_ERR_INVALID_ARGUMENT;
-    for (size_t i = 0; i < t.size(); ++i) rgb_out[i] = t[i];
-    return KMC_OK;
+    return kmc::abi_guard([&]() -> kmc_status {
+        if (!rgb_out) return KMC_ERR_INVALID_ARGUMENT;
+        std::vector<uint8_t> t;
+        if (!create_table(colormap, resolution, t)) return KMC_ERR_INVALID_ARGUMENT;
+        for (size_t i = 0; i < t.size(); ++i) rgb_out[i] = t[i];
+        return KMC_OK;
+    });
 }
 
 kmc_status kmc_surface_color(int32_t colormap, uint32_t resolution, int32_t type, double volume_min,
                              double volume_max, double isolevel, uint8_t rgb[3])
 {
-    if (!rgb) return KMC_ERR_INVALID_ARGUMENT;
-    std::vector<uint8_t> t;
-    if (!create_table(colormap, resolution, t)) return KMC_ERR_INVALID_ARGUMENT;
-    /* MapperBase::setRange (MapperBase.cpp:54-83) on a transfer function
-     * without a range of its own. */
-    float lo, hi;
-    if (type == KMC_INT8) { lo = -128; hi = 127; }
-    else if (type == KMC_UINT8) { lo = 0; hi = 255; }
-    else {
-        lo = (float)volume_min;
-        hi = (float)volume_max;
-        if (equalf(lo, hi)) hi = hi + 1.0f;
-    }
-    /* ColorMap::at(float(isolevel)) (ColorMap.cpp:1043-1054) */
-    const float value = (float)isolevel;
-    const float v0 = clampf(value, lo, hi);
-    const float r = (float)(resolution - 1);
-    float v = (v0 - lo) / (hi - lo) * r;
-    /* when float(min) == float(max) and the +1 of setRange is absorbed
-     * (|min| >= 2^24), hi - lo is 0 and v is NaN: the reference converts NaN to
-     * size_t and reads outside its table; here the table's first colour */
-    if (!(v >= 0.0f)) v = 0.0f;
-    const size_t s0 = (size_t)v;
-    const size_t s1 = (s0 + 1 < (size_t)(resolution - 1)) ? s0 + 1 : (size_t)(resolution - 1);
-    const Rgb c0 = { t[3 * s0], t[3 * s0 + 1], t[3 * s0 + 2] };
-    const Rgb c1 = { t[3 * s1], t[3 * s1 + 1], t[3 * s1 + 2] };
-    const Rgb c = mix_rgb(c0, c1, v - (float)s0);
-    rgb[0] = c.r; rgb[1] = c.g; rgb[2] = c.b;
-    return KMC_OK;
+    return kmc::abi_guard([&]() -> kmc_status {
+        if (!rgb) return KMC_ERR_INVALID_ARGUMENT;
+        std::vector<uint8_t> t;
+        if (!create_table(colormap, resolution, t)) return KMC_ERR_INVALID_ARGUMENT;
+        /* MapperBase::setRange (MapperBase.cpp:54-83) on a transfer function
+         * without a range of its own. */
+        float lo, hi;
+        if (type == KMC_INT8) { lo = -128; hi = 127; }
+        else if (type == KMC_UINT8) { lo = 0; hi = 255; }
+        else {
+            lo = (float)volume_min;
+            hi = (float)volume_max;
+            if (equalf(lo, hi)) hi = hi + 1.0f;
+        }
+        /* ColorMap::at(float(isolevel)) (ColorMap.cpp:1043-1054) */
+        const float value = (float)isolevel;
+        const float v0 = clampf(value, lo, hi);
+        const float r = (float)(resolution - 1);
+        float v = (v0 - lo) / (hi - lo) * r;
+        /* when float(min) == float(max) and the +1 of setRange is absorbed
+         * (|min| >= 2^24), hi - lo is 0 and v is NaN: the reference converts NaN to
+         * size_t and reads outside its table; here the table's first colour */
+        if (!(v >= 0.0f)) v = 0.0f;
+        const size_t s0 = (size_t)v;
+        const size_t s1 = (s0 + 1 < (size_t)(resolution - 1)) ? s0 + 1 : (size_t)(resolution - 1);
+        const Rgb c0 = { t[3 * s0], t[3 * s0 + 1], t[3 * s0 + 2] };
+        const Rgb c1 = { t[3 * s1], t[3 * s1 + 1], t[3 * s1 + 2] };
+        const Rgb c = mix_rgb(c0, c1, v - (float)s0);
+        rgb[0] = c.r; rgb[1] = c.g; rgb[2] = c.b;
+        return KMC_OK;
+    });
 }
 
 }  // extern "C"

@@ -29,6 +29,7 @@
 #include <vector>
 
 #include "kmc.h"
+#include "kmc_guard.h"
 #include "kmc_internal.h"
 #include "kmc_job.h"
 
@@ -156,7 +157,13 @@ kmc_status expand_impl(const kmc_surface* in, int32_t out_memory, hipStream_t st
     }
     if (s == KMC_OK && hipStreamSynchronize(st) != hipSuccess) s = KMC_ERR_DEVICE;
     if (s == KMC_OK && out_memory == KMC_MEMORY_HOST) s = to_host(out, st);
-    if (s != KMC_OK) { (void)hipGetLastError(); kmc_surface_release(out); }
+    if (s != KMC_OK) {
+        /* nothing queued may still read or write the blocks that go back to
+         * the pool here (the outputs, and the device view's copies) */
+        (void)hipStreamSynchronize(st);
+        (void)hipGetLastError();
+        kmc_surface_release(out);
+    }
     return s;
 }
 
@@ -641,38 +648,46 @@ extern "C" {
 
 kmc_status kmc_surface_expand(const kmc_surface* surface, int32_t out_memory, void* stream, kmc_surface* out)
 {
-    const kmc_status s = kmc::expand_impl(surface, out_memory, (hipStream_t)stream, out);
-    kmc::report(s);
-    return s;
+    return kmc::abi_guard([&]() -> kmc_status {
+        const kmc_status s = kmc::expand_impl(surface, out_memory, (hipStream_t)stream, out);
+        kmc::report(s);
+        return s;
+    });
 }
 
 kmc_status kmc_export_kvsml(const kmc_surface* surface, const uint8_t* color, int32_t opacity, const char* filename,
                             int32_t writing, void* stream)
 {
-    const kmc_status s = kmc::export_kvsml(surface, color, opacity, filename, writing, (hipStream_t)stream);
-    kmc::report(s);
-    return s;
+    return kmc::abi_guard([&]() -> kmc_status {
+        const kmc_status s = kmc::export_kvsml(surface, color, opacity, filename, writing, (hipStream_t)stream);
+        kmc::report(s);
+        return s;
+    });
 }
 
 kmc_status kmc_export_ply(const kmc_surface* surface, const uint8_t* color, const char* filename, int32_t binary,
                           void* stream)
 {
-    const kmc_status s = kmc::export_ply(surface, color, filename, binary, (hipStream_t)stream);
-    kmc::report(s);
-    return s;
+    return kmc::abi_guard([&]() -> kmc_status {
+        const kmc_status s = kmc::export_ply(surface, color, filename, binary, (hipStream_t)stream);
+        kmc::report(s);
+        return s;
+    });
 }
 
 kmc_status kmc_export_stl(const kmc_surface* surface, const char* filename, int32_t ascii, void* stream)
 {
-    if (!filename) { kmc::report(KMC_ERR_INVALID_ARGUMENT); return KMC_ERR_INVALID_ARGUMENT; }
-    kmc_surface soup;
-    kmc_status s = kmc::expand_impl(surface, KMC_MEMORY_HOST, (hipStream_t)stream, &soup);
-    if (s == KMC_OK) {
-        s = kmc::write_stl(&soup, filename, ascii);
-        kmc_surface_release(&soup);
-    }
-    kmc::report(s);
-    return s;
+    return kmc::abi_guard([&]() -> kmc_status {
+        if (!filename) { kmc::report(KMC_ERR_INVALID_ARGUMENT); return KMC_ERR_INVALID_ARGUMENT; }
+        kmc_surface soup;
+        kmc_status s = kmc::expand_impl(surface, KMC_MEMORY_HOST, (hipStream_t)stream, &soup);
+        if (s == KMC_OK) {
+            s = kmc::write_stl(&soup, filename, ascii);
+            kmc_surface_release(&soup);
+        }
+        kmc::report(s);
+        return s;
+    });
 }
 
 }  // extern "C"

@@ -28,6 +28,7 @@
 #include <vector>
 
 #include "kmc.h"
+#include "kmc_guard.h"
 #include "kmc_internal.h"
 
 namespace kmc {
@@ -613,16 +614,20 @@ extern "C" {
 
 kmc_status kmc_avs_read_volume(const char* filename, int32_t memory, void* stream, kmc_volume* out)
 {
-    const kmc_status s = kmc::read_avs(filename, memory, (hipStream_t)stream, out);
-    kmc::report(s);
-    return s;
+    return kmc::abi_guard([&]() -> kmc_status {
+        const kmc_status s = kmc::read_avs(filename, memory, (hipStream_t)stream, out);
+        kmc::report(s);
+        return s;
+    });
 }
 
 kmc_status kmc_kvsml_read_volume(const char* filename, int32_t memory, void* stream, kmc_volume* out)
 {
-    const kmc_status s = kmc::read_volume(filename, memory, (hipStream_t)stream, out);
-    kmc::report(s);
-    return s;
+    return kmc::abi_guard([&]() -> kmc_status {
+        const kmc_status s = kmc::read_volume(filename, memory, (hipStream_t)stream, out);
+        kmc::report(s);
+        return s;
+    });
 }
 
 void kmc_kvsml_volume_release(kmc_volume* v)

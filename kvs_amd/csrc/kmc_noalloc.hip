@@ -21,6 +21,7 @@
 #include <mutex>
 
 #include "kmc.h"
+#include "kmc_guard.h"
 #include "kmc_internal.h"
 #include "kmc_job.h"
 
@@ -211,9 +212,11 @@ extern "C" kmc_status kmc_extract_noalloc(const kmc_volume* volume, double isole
                                           uint32_t* connections, uint64_t connections_capacity, float* normals,
                                           uint64_t normals_capacity, kmc_surface* out)
 {
-    const kmc_status s = kmc::extract_noalloc(volume, isolevel, normal_type, duplication, (hipStream_t)stream, coords,
-                                              coords_capacity, connections, connections_capacity, normals,
-                                              normals_capacity, out);
-    kmc::report(s);
-    return s;
+    return kmc::abi_guard([&]() -> kmc_status {
+        const kmc_status s = kmc::extract_noalloc(volume, isolevel, normal_type, duplication, (hipStream_t)stream, coords,
+                                                  coords_capacity, connections, connections_capacity, normals,
+                                                  normals_capacity, out);
+        kmc::report(s);
+        return s;
+    });
 }

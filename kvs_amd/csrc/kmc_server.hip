@@ -29,11 +29,13 @@
 #include <cstring>
 #include <deque>
 #include <mutex>
+#include <new>
 #include <thread>
 #include <unordered_map>
 #include <vector>
 
 #include "kmc.h"
+#include "kmc_guard.h"
 #include "kmc_internal.h"
 
 using namespace kmc;
@@ -153,7 +155,7 @@ kmc_status surface_to_host(kmc_surface* s, hipStream_t st)
     return KMC_OK;
 }
 
-void run_batch(kmc_server* S, hipStream_t st, const std::vector<uint64_t>& ids)
+void run_batch(kmc_server* S, hipStream_t st, const std::vector<uint64_t>& ids, bool& delivered)
 {
     const size_t n = ids.size();
     /* requests on one volume with one emit mode become one job whose
@@ -308,6 +310,7 @@ void run_batch(kmc_server* S, hipStream_t st, const std::vector<uint64_t>& ids)
         }
         --S->running;
         ++S->st.batches;
+        delivered = true;
     }
     S->cv_done.notify_all();
     S->cv_work.notify_all();
@@ -355,7 +358,29 @@ void worker_main(kmc_server* S)
             S->st.peak_inflight = std::max(S->st.peak_inflight, S->inflight);
             S->st.peak_budget_bytes = std::max<uint64_t>(S->st.peak_budget_bytes, S->used);
         }
-        run_batch(S, st, take);
+        bool delivered = false;
+        try {
+            run_batch(S, st, take, delivered);
+        } catch (...) {
+            /* host memory ran out inside the batch: before delivery its
+             * requests that still run fail, so that no waiter waits for them
+             * forever (their device blocks and output budget stay lost) */
+            if (!delivered) {
+                std::lock_guard<std::mutex> lk(S->mu);
+                for (uint64_t id : take) {
+                    auto it = S->reqs.find(id);
+                    if (it == S->reqs.end() || it->second.state != State::Running) continue;
+                    it->second.status = KMC_ERR_OUT_OF_MEMORY;
+                    it->second.state = State::Done;
+                    ++S->st.failed;
+                    S->used -= std::min(S->used, it->second.scratch);
+                    --S->inflight;
+                }
+                --S->running;
+            }
+            S->cv_done.notify_all();
+            S->cv_work.notify_all();
+        }
     }
     if (st) (void)hipStreamDestroy(st);
 }
@@ -392,125 +417,144 @@ kmc_status kmc_server_create(const kmc_server_config* config, kmc_server** out)
     if (cfg.out_memory != KMC_MEMORY_HOST && cfg.out_memory != KMC_MEMORY_DEVICE) return KMC_ERR_INVALID_ARGUMENT;
     kmc_status s = kmc_init(cfg.device);
     if (s != KMC_OK) { report(s); return s; }
-    kmc_server* S = new kmc_server();
+    kmc_server* S = new (std::nothrow) kmc_server();
+    if (!S) return KMC_ERR_OUT_OF_MEMORY;
     S->cfg = cfg;
     S->trace = std::getenv("KMC_SERVER_TRACE") != nullptr;
     std::memset(&S->st, 0, sizeof(S->st));
-    for (uint32_t i = 0; i < cfg.workers; ++i) S->workers.emplace_back(worker_main, S);
+    try {
+        for (uint32_t i = 0; i < cfg.workers; ++i) S->workers.emplace_back(worker_main, S);
+    } catch (...) {
+        /* a worker could not be started: stop the ones that were */
+        kmc_server_destroy(S);
+        return KMC_ERR_OUT_OF_MEMORY;
+    }
     *out = S;
     return KMC_OK;
 }
 
 kmc_status kmc_server_register_volume(kmc_server* S, const kmc_volume* v, uint32_t* volume_id)
 {
-    if (!S || !volume_id) return KMC_ERR_INVALID_ARGUMENT;
-    kmc_status s = validate_volume(v);
-    if (s == KMC_OK && levels_per_pass(v, 1) == 0) s = KMC_ERR_INVALID_ARGUMENT;   // rows too long for a tile
-    if (s == KMC_OK) s = kmc_init(S->cfg.device);
-    if (s != KMC_OK) { report(s); return s; }
-    Resident r;
-    r.v = *v;
-    const void* dvals = nullptr;
-    r.owned = nullptr;
-    s = device_values(v, nullptr, &dvals, &r.owned);
-    if (s == KMC_OK && !v->has_min_max_values) {
-        double mn = 0, mx = 0;
-        s = min_max_on_device(v, dvals, nullptr, &mn, &mx);
-        r.v.min_value = mn;
-        r.v.max_value = mx;
-    }
-    if (s == KMC_OK && hipStreamSynchronize(nullptr) != hipSuccess) s = KMC_ERR_DEVICE;
-    if (s != KMC_OK) { device_free(r.owned); report(s); return s; }
-    r.v.values = dvals;
-    r.v.memory = KMC_MEMORY_DEVICE;
-    r.v.has_min_max_values = 1;
-    std::lock_guard<std::mutex> lk(S->mu);
-    *volume_id = (uint32_t)S->volumes.size();
-    S->volumes.push_back(r);
-    return KMC_OK;
+    return kmc::abi_guard([&]() -> kmc_status {
+        if (!S || !volume_id) return KMC_ERR_INVALID_ARGUMENT;
+        kmc_status s = validate_volume(v);
+        if (s == KMC_OK && levels_per_pass(v, 1) == 0) s = KMC_ERR_INVALID_ARGUMENT;   // rows too long for a tile
+        if (s == KMC_OK) s = kmc_init(S->cfg.device);
+        if (s != KMC_OK) { report(s); return s; }
+        Resident r;
+        r.v = *v;
+        const void* dvals = nullptr;
+        r.owned = nullptr;
+        s = device_values(v, nullptr, &dvals, &r.owned);
+        if (s == KMC_OK && !v->has_min_max_values) {
+            double mn = 0, mx = 0;
+            s = min_max_on_device(v, dvals, nullptr, &mn, &mx);
+            r.v.min_value = mn;
+            r.v.max_value = mx;
+        }
+        if (s == KMC_OK && hipStreamSynchronize(nullptr) != hipSuccess) s = KMC_ERR_DEVICE;
+        if (s != KMC_OK) { device_free(r.owned); report(s); return s; }
+        r.v.values = dvals;
+        r.v.memory = KMC_MEMORY_DEVICE;
+        r.v.has_min_max_values = 1;
+        std::lock_guard<std::mutex> lk(S->mu);
+        *volume_id = (uint32_t)S->volumes.size();
+        S->volumes.push_back(r);
+        return KMC_OK;
+    });
 }
 
 kmc_status kmc_server_submit(kmc_server* S, uint32_t volume_id, double isolevel, int32_t normal_type,
                              int32_t duplication, uint64_t* request_id)
 {
-    if (!S || !request_id) return KMC_ERR_INVALID_ARGUMENT;
-    if (normal_type != KMC_VERTEX_NORMAL && normal_type != KMC_POLYGON_NORMAL) return KMC_ERR_INVALID_ARGUMENT;
-    std::lock_guard<std::mutex> lk(S->mu);
-    if (volume_id >= S->volumes.size() || S->stop) return KMC_ERR_INVALID_ARGUMENT;
-    Request r;
-    r.volume = volume_id;
-    r.level = isolevel;
-    r.normal_type = normal_type;
-    r.duplication = duplication ? 1 : 0;
-    r.scratch = request_scratch(S->volumes[volume_id].v);
-    r.t_submit = Clock::now();
-    std::memset(&r.result, 0, sizeof(r.result));
-    if (r.scratch > S->cfg.budget_bytes) return KMC_ERR_BUDGET;
-    const uint64_t id = S->next_id++;
-    if (!S->have_first) { S->have_first = true; S->first_submit = r.t_submit; }
-    ++S->st.submitted;
-    S->reqs.emplace(id, r);
-    S->pending.push_back(id);
-    *request_id = id;
-    S->cv_work.notify_one();
-    return KMC_OK;
+    return kmc::abi_guard([&]() -> kmc_status {
+        if (!S || !request_id) return KMC_ERR_INVALID_ARGUMENT;
+        if (normal_type != KMC_VERTEX_NORMAL && normal_type != KMC_POLYGON_NORMAL) return KMC_ERR_INVALID_ARGUMENT;
+        std::lock_guard<std::mutex> lk(S->mu);
+        if (volume_id >= S->volumes.size() || S->stop) return KMC_ERR_INVALID_ARGUMENT;
+        Request r;
+        r.volume = volume_id;
+        r.level = isolevel;
+        r.normal_type = normal_type;
+        r.duplication = duplication ? 1 : 0;
+        r.scratch = request_scratch(S->volumes[volume_id].v);
+        r.t_submit = Clock::now();
+        std::memset(&r.result, 0, sizeof(r.result));
+        if (r.scratch > S->cfg.budget_bytes) return KMC_ERR_BUDGET;
+        const uint64_t id = S->next_id++;
+        if (!S->have_first) { S->have_first = true; S->first_submit = r.t_submit; }
+        ++S->st.submitted;
+        S->reqs.emplace(id, r);
+        S->pending.push_back(id);
+        *request_id = id;
+        S->cv_work.notify_one();
+        return KMC_OK;
+    });
 }
 
 kmc_status kmc_server_poll(kmc_server* S, uint64_t id, kmc_surface* out)
 {
-    if (!S) return KMC_ERR_INVALID_ARGUMENT;
-    std::lock_guard<std::mutex> lk(S->mu);
-    return take_result(S, id, out);
+    return kmc::abi_guard([&]() -> kmc_status {
+        if (!S) return KMC_ERR_INVALID_ARGUMENT;
+        std::lock_guard<std::mutex> lk(S->mu);
+        return take_result(S, id, out);
+    });
 }
 
 kmc_status kmc_server_wait(kmc_server* S, uint64_t id, int64_t timeout_ms, kmc_surface* out)
 {
-    if (!S) return KMC_ERR_INVALID_ARGUMENT;
-    std::unique_lock<std::mutex> lk(S->mu);
-    auto ready = [&] {
-        auto it = S->reqs.find(id);
-        return it == S->reqs.end() || it->second.state == State::Done;
-    };
-    if (timeout_ms < 0) S->cv_done.wait(lk, ready);
-    else S->cv_done.wait_for(lk, std::chrono::milliseconds(timeout_ms), ready);
-    return take_result(S, id, out);
+    return kmc::abi_guard([&]() -> kmc_status {
+        if (!S) return KMC_ERR_INVALID_ARGUMENT;
+        std::unique_lock<std::mutex> lk(S->mu);
+        auto ready = [&] {
+            auto it = S->reqs.find(id);
+            return it == S->reqs.end() || it->second.state == State::Done;
+        };
+        if (timeout_ms < 0) S->cv_done.wait(lk, ready);
+        else S->cv_done.wait_for(lk, std::chrono::milliseconds(timeout_ms), ready);
+        return take_result(S, id, out);
+    });
 }
 
 kmc_status kmc_server_get_stats(kmc_server* S, kmc_server_stats* out)
 {
-    if (!S || !out) return KMC_ERR_INVALID_ARGUMENT;
-    std::lock_guard<std::mutex> lk(S->mu);
-    kmc_server_stats st = S->st;
-    std::vector<double> l = S->lat_ms;
-    if (!l.empty()) {
-        std::sort(l.begin(), l.end());
-        auto pct = [&](double p) {
-            const size_t k = (size_t)std::min<double>((double)l.size() - 1, std::ceil(p * l.size()) - 1);
-            return l[k];
-        };
-        st.p50_ms = pct(0.50);
-        st.p99_ms = pct(0.99);
-        st.max_ms = l.back();
-        double sum = 0;
-        for (double x : l) sum += x;
-        st.mean_ms = sum / l.size();
-    }
-    if (S->have_first && st.completed > 0) {
-        st.span_s = std::chrono::duration<double>(S->last_done - S->first_submit).count();
-        st.triangles_per_s = st.span_s > 0 ? st.triangles / st.span_s : 0.0;
-    }
-    *out = st;
-    return KMC_OK;
+    return kmc::abi_guard([&]() -> kmc_status {
+        if (!S || !out) return KMC_ERR_INVALID_ARGUMENT;
+        std::lock_guard<std::mutex> lk(S->mu);
+        kmc_server_stats st = S->st;
+        std::vector<double> l = S->lat_ms;
+        if (!l.empty()) {
+            std::sort(l.begin(), l.end());
+            auto pct = [&](double p) {
+                const size_t k = (size_t)std::min<double>((double)l.size() - 1, std::ceil(p * l.size()) - 1);
+                return l[k];
+            };
+            st.p50_ms = pct(0.50);
+            st.p99_ms = pct(0.99);
+            st.max_ms = l.back();
+            double sum = 0;
+            for (double x : l) sum += x;
+            st.mean_ms = sum / l.size();
+        }
+        if (S->have_first && st.completed > 0) {
+            st.span_s = std::chrono::duration<double>(S->last_done - S->first_submit).count();
+            st.triangles_per_s = st.span_s > 0 ? st.triangles / st.span_s : 0.0;
+        }
+        *out = st;
+        return KMC_OK;
+    });
 }
 
 kmc_status kmc_server_reset_stats(kmc_server* S)
 {
-    if (!S) return KMC_ERR_INVALID_ARGUMENT;
-    std::lock_guard<std::mutex> lk(S->mu);
-    std::memset(&S->st, 0, sizeof(S->st));
-    S->lat_ms.clear();
-    S->have_first = false;
-    return KMC_OK;
+    return kmc::abi_guard([&]() -> kmc_status {
+        if (!S) return KMC_ERR_INVALID_ARGUMENT;
+        std::lock_guard<std::mutex> lk(S->mu);
+        std::memset(&S->st, 0, sizeof(S->st));
+        S->lat_ms.clear();
+        S->have_first = false;
+        return KMC_OK;
+    });
 }
 
 void kmc_server_destroy(kmc_server* S)

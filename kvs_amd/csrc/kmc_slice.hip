@@ -17,6 +17,7 @@
 #include <vector>
 
 #include "kmc.h"
+#include "kmc_guard.h"
 #include "kmc_internal.h"
 #include "kmc_job.h"
 
@@ -272,8 +273,10 @@ extern "C" kmc_status kmc_slice_plane(const kmc_volume* volume, const float coef
                                       const uint8_t* color_table, uint32_t color_resolution,
                                       const float* color_range, int32_t out_memory, void* stream, kmc_surface* out)
 {
-    const kmc_status s = kmc::slice_plane_impl(volume, coefficients, color_table, color_resolution, color_range,
-                                               out_memory, (hipStream_t)stream, out);
-    kmc::report(s);
-    return s;
+    return kmc::abi_guard([&]() -> kmc_status {
+        const kmc_status s = kmc::slice_plane_impl(volume, coefficients, color_table, color_resolution, color_range,
+                                                   out_memory, (hipStream_t)stream, out);
+        kmc::report(s);
+        return s;
+    });
 }

@@ -21,6 +21,7 @@
 
 #include "cell_tables.h"
 #include "kmc.h"
+#include "kmc_guard.h"
 #include "kmc_internal.h"
 #include "kmc_job.h"
 
@@ -634,24 +635,28 @@ extern "C" {
 kmc_status kmc_extract_unstructured(const kmc_unstructured_volume* volume, double isolevel, int32_t normal_type,
                                     int32_t duplication, int32_t out_memory, void* stream, kmc_surface* out)
 {
-    kmc_status s = KMC_OK;
-    if (normal_type != KMC_VERTEX_NORMAL && normal_type != KMC_POLYGON_NORMAL) s = KMC_ERR_INVALID_ARGUMENT;
-    const int dup = (normal_type == KMC_VERTEX_NORMAL) ? 0 : (duplication ? 1 : 0);   // MarchingTetrahedra.cpp:78-81
-    if (s == KMC_OK)
-        s = kmc::run_cells(volume, false, isolevel, dup, nullptr, nullptr, 0, nullptr, out_memory, (hipStream_t)stream,
-                           out, normal_type);
-    kmc::report(s);
-    return s;
+    return kmc::abi_guard([&]() -> kmc_status {
+        kmc_status s = KMC_OK;
+        if (normal_type != KMC_VERTEX_NORMAL && normal_type != KMC_POLYGON_NORMAL) s = KMC_ERR_INVALID_ARGUMENT;
+        const int dup = (normal_type == KMC_VERTEX_NORMAL) ? 0 : (duplication ? 1 : 0);   // MarchingTetrahedra.cpp:78-81
+        if (s == KMC_OK)
+            s = kmc::run_cells(volume, false, isolevel, dup, nullptr, nullptr, 0, nullptr, out_memory, (hipStream_t)stream,
+                               out, normal_type);
+        kmc::report(s);
+        return s;
+    });
 }
 
 kmc_status kmc_slice_plane_unstructured(const kmc_unstructured_volume* volume, const float coefficients[4],
                                         const uint8_t* color_table, uint32_t color_resolution,
                                         const float* color_range, int32_t out_memory, void* stream, kmc_surface* out)
 {
-    const kmc_status s = kmc::run_cells(volume, true, 0.0, 1, coefficients, color_table, color_resolution, color_range,
-                                        out_memory, (hipStream_t)stream, out, KMC_POLYGON_NORMAL);
-    kmc::report(s);
-    return s;
+    return kmc::abi_guard([&]() -> kmc_status {
+        const kmc_status s = kmc::run_cells(volume, true, 0.0, 1, coefficients, color_table, color_resolution, color_range,
+                                            out_memory, (hipStream_t)stream, out, KMC_POLYGON_NORMAL);
+        kmc::report(s);
+        return s;
+    });
 }
 
 }  // extern "C"
