@@ -61,6 +61,8 @@ def main():
                     help="skip the SlicePlane / unstructured / KVSML / STL leg")
     ap.add_argument("--only-serving", action="store_true",
                     help="only the serving leg (its plain calls, then the server): for profiling")
+    ap.add_argument("--only-batched-ct", action="store_true",
+                    help="only the batched and the 512^3 CT legs (no CPU reference): for profiling")
     ap.add_argument("--only-main", action="store_true",
                     help="only the 1024^3 step (no CPU baseline, rows, batched, serving, CT or FP64 legs): for profiling")
     args = ap.parse_args()
@@ -81,6 +83,12 @@ def main():
     if args.only_serving:
         torch.cuda.init()
         print(json.dumps({"serving": serving_leg(lib, kmc, datasets, torch, args)}), flush=True)
+        return
+    if args.only_batched_ct:
+        torch.cuda.init()
+        args.no_cpu_baseline = True
+        print(json.dumps({"batched": batched_leg(lib, kmc, datasets, torch, args),
+                          "ct_512": ct_leg(lib, kmc, datasets, torch, args)}), flush=True)
         return
     n = args.size
     print(f"[bench] generating {n}^3 float32 turbulence field on the GPU", file=sys.stderr, flush=True)

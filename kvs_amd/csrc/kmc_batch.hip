@@ -272,6 +272,7 @@ struct Group {
     int mode;                    // 0 duplicated, 1 PolygonNormal, 2 VertexNormal
     uint32_t p0, p1;             // passes [p0, p1)
     uint64_t rows, units;
+    uint32_t max_levels;         // the most levels of one of its passes
     uint32_t emit_lds;
     uint32_t teams;                // emit teams per workgroup
     uint64_t* row_start_d;       // p1 - p0 + 1 entries
@@ -395,12 +396,14 @@ kmc_status plan_tiles(kmc_batch* b, uint32_t r_max, uint32_t queue_cap)
         std::vector<uint64_t> rs, ts;
         uint64_t r = 0, t = 0;
         uint32_t stride = 0, teams = kEmitTeams;
+        g.max_levels = 0;
         for (uint32_t i = g.p0; i < g.p1; ++i) {
             rs.push_back(r);
             ts.push_back(t);
             r += b->passes[i].a.nrows;
             t += b->passes[i].a.nunits;
             stride = std::max(stride, b->passes[i].a.team_stride);
+            g.max_levels = std::max(g.max_levels, b->passes[i].a.nlev);
         }
         /* one emit launch per group: every team region of a workgroup is as
          * large as the group's largest, and as many teams as 64 KiB hold */
@@ -460,7 +463,7 @@ hipError_t record_front(kmc_batch* b, hipStream_t st, bool copy_sizes)
         if (e != hipSuccess) return e;
     } else {
         for (const Group& g : b->groups) {
-            e = launch_classify_batch(g.dtype, b->jobs_d + g.p0, g.row_start_d, g.p1 - g.p0, g.rows, st);
+            e = launch_classify_batch(g.dtype, b->jobs_d + g.p0, g.row_start_d, g.p1 - g.p0, g.rows, g.max_levels, st);
             if (e != hipSuccess) return e;
         }
         for (const Group& g : b->groups) {

@@ -102,7 +102,7 @@ size_t minmax_scratch_bytes();
 hipError_t launch_minmax_gather(int dtype, const void* vol, const uint32_t* conn, uint64_t nconn, void* scratch,
                                 double* out, hipStream_t st);
 hipError_t launch_classify_batch(int dtype, const JobArgs* jobs, const uint64_t* row_start, uint32_t njobs,
-                                 uint64_t rows, hipStream_t st);
+                                 uint64_t rows, uint32_t max_levels, hipStream_t st);
 hipError_t launch_count_batch(const JobArgs* jobs, const uint64_t* row_start, uint32_t njobs, uint64_t rows,
                               hipStream_t st);
 /* mode: 0 duplicated vertices, 1 PolygonNormal, 2 VertexNormal (shared vertices) */

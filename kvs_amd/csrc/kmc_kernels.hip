@@ -1050,7 +1050,7 @@ template <typename T, int NLEV>
 __global__ __launch_bounds__(kBlock) void k_classify(JobArgs a)
 {
     const uint64_t nrows = (uint64_t)a.ny * a.nz;
-    const uint64_t wave_id = ((uint64_t)blockIdx.x * kBlock + threadIdx.x) >> 6;
+    const uint64_t wave_id = (uint64_t)blockIdx.x * kWaves + uniform(threadIdx.x >> 6);   // uniform: scalar
     const uint64_t nwaves = ((uint64_t)gridDim.x * kBlock) >> 6;
     for (uint64_t row = wave_id; row < nrows; row += nwaves) classify_row<T, NLEV>(a, row);
 }
@@ -1069,12 +1069,14 @@ __device__ __forceinline__ uint32_t find_job(const uint64_t* start, uint32_t n, 
 /* Classify pass of a batch of jobs sharing the value type T: the node rows of
  * all jobs form one sequence (row_start[j] = first row of job j), cut into one
  * contiguous range per wave so that the job changes rarely along a range; a
- * wave classifies its rows of a job as one run of chunks. */
-template <typename T>
+ * wave classifies its rows of a job as one run of chunks.  kWide: some job has
+ * more than four levels (the eight-level instance's registers would otherwise
+ * halve the waves per SIMD of every batch). */
+template <typename T, bool kWide>
 __global__ __launch_bounds__(kBlock) void k_classify_batch(const JobArgs* jobs, const uint64_t* row_start, uint32_t njobs)
 {
     const uint64_t total = row_start[njobs];
-    const uint64_t wave_id = ((uint64_t)blockIdx.x * kBlock + threadIdx.x) >> 6;
+    const uint64_t wave_id = (uint64_t)blockIdx.x * kWaves + uniform(threadIdx.x >> 6);   // uniform: scalar
     const uint64_t nwaves = ((uint64_t)gridDim.x * kBlock) >> 6;
     const uint64_t per = (total + nwaves - 1) / nwaves;
     uint64_t g = wave_id * per;
@@ -1091,7 +1093,9 @@ __global__ __launch_bounds__(kBlock) void k_classify_batch(const JobArgs* jobs, 
         case 2: classify_chunks<T, 2>(a, q0, q1); break;
         case 3: classify_chunks<T, 3>(a, q0, q1); break;
         case 4: classify_chunks<T, 4>(a, q0, q1); break;
-        default: classify_chunks<T, 8>(a, q0, q1); break;
+        default:
+            if constexpr (kWide) classify_chunks<T, 8>(a, q0, q1);
+            break;
         }
         g = je;
     }
@@ -1360,7 +1364,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(7))) voi
     stage_ntri(ntri);
     /* row groups of 64 >> cshift rows as the units */
     const uint64_t G = 64u >> a.cshift, groups = (a.nrows + G - 1) / G;
-    const Span sp = grid_span(0, groups, kWaves, threadIdx.x >> 6);
+    const Span sp = grid_span(0, groups, kWaves, uniform(threadIdx.x >> 6));
     const uint64_t r1 = sp.end * G < a.nrows ? sp.end * G : a.nrows;
     count_rows(a, ntri, sp.lo * G, r1, sp.first - sp.lo, sp.step);
 }
@@ -1372,7 +1376,7 @@ __global__ __launch_bounds__(kBlock) void k_count_batch(const JobArgs* jobs, con
     __shared__ uint8_t ntri[256];
     stage_ntri(ntri);
     const uint64_t total = row_start[njobs];
-    const uint64_t wave_id = ((uint64_t)blockIdx.x * kBlock + threadIdx.x) >> 6;
+    const uint64_t wave_id = (uint64_t)blockIdx.x * kWaves + uniform(threadIdx.x >> 6);   // uniform: scalar
     const uint64_t nwaves = ((uint64_t)gridDim.x * kBlock) >> 6;
     const uint64_t per = (total + nwaves - 1) / nwaves;
     uint64_t g = wave_id * per;
@@ -1661,7 +1665,7 @@ __global__ __launch_bounds__(kBlock) void k_compact(JobArgs a)
     __shared__ uint64_t queue[kWaves][kCompactQueue];
     stage_ntri(ntri);
     const uint64_t G = 64u >> a.cshift, groups = (a.nrows + G - 1) / G;
-    const Span sp = grid_span(0, groups, kWaves, threadIdx.x >> 6);
+    const Span sp = grid_span(0, groups, kWaves, uniform(threadIdx.x >> 6));
     const uint64_t r1 = sp.end * G < a.nrows ? sp.end * G : a.nrows;
     compact_rows(a, ntri, queue[threadIdx.x >> 6], sp.lo * G, r1, sp.first - sp.lo, sp.step);
 }
@@ -1675,7 +1679,7 @@ __global__ __launch_bounds__(kBlock) void k_compact_batch(const JobArgs* jobs, c
     stage_ntri(ntri);
     const uint64_t total = row_start[njobs];
     const uint64_t nwaves = ((uint64_t)gridDim.x * kBlock) >> 6;
-    const uint64_t wave_id = ((uint64_t)blockIdx.x * kBlock + threadIdx.x) >> 6;
+    const uint64_t wave_id = (uint64_t)blockIdx.x * kWaves + uniform(threadIdx.x >> 6);   // uniform: scalar
     const uint64_t per = (total + nwaves - 1) / nwaves;
     const uint64_t begin = wave_id * per;
     const uint64_t end = begin + per < total ? begin + per : total;
@@ -2204,7 +2208,7 @@ __global__ __launch_bounds__(kBlock) void k_classify_plane(JobArgs a)
 {
     const uint32_t lane = threadIdx.x & 63u;
     const uint64_t nrows = (uint64_t)a.ny * a.nz;
-    const uint64_t wave_id = ((uint64_t)blockIdx.x * kBlock + threadIdx.x) >> 6;
+    const uint64_t wave_id = (uint64_t)blockIdx.x * kWaves + uniform(threadIdx.x >> 6);   // uniform: scalar
     const uint64_t nwaves = ((uint64_t)gridDim.x * kBlock) >> 6;
     for (uint64_t row = wave_id; row < nrows; row += nwaves) {
         const uint64_t z = row / a.ny, y = row - z * a.ny;
@@ -3166,28 +3170,33 @@ hipError_t launch_emit(int dtype, const JobArgs& a, uint32_t lds_bytes, hipStrea
 
 template <typename T>
 hipError_t launch_classify_batch_t(const JobArgs* jobs, const uint64_t* row_start, uint32_t njobs, uint64_t rows,
-                                   hipStream_t st)
+                                   uint32_t max_levels, hipStream_t st)
 {
     uint64_t blocks = (rows + kWaves - 1) / kWaves;
     if (blocks > 65536) blocks = 65536;
-    hipLaunchKernelGGL((k_classify_batch<T>), dim3((uint32_t)blocks), dim3(kBlock), 0, st, jobs, row_start, njobs);
+    if (max_levels > 4)
+        hipLaunchKernelGGL((k_classify_batch<T, true>), dim3((uint32_t)blocks), dim3(kBlock), 0, st, jobs, row_start,
+                           njobs);
+    else
+        hipLaunchKernelGGL((k_classify_batch<T, false>), dim3((uint32_t)blocks), dim3(kBlock), 0, st, jobs, row_start,
+                           njobs);
     return hipGetLastError();
 }
 
 hipError_t launch_classify_batch(int dtype, const JobArgs* jobs, const uint64_t* row_start, uint32_t njobs,
-                                 uint64_t rows, hipStream_t st)
+                                 uint64_t rows, uint32_t max_levels, hipStream_t st)
 {
     switch (dtype) {
-    case 0: return launch_classify_batch_t<int8_t>(jobs, row_start, njobs, rows, st);
-    case 1: return launch_classify_batch_t<int16_t>(jobs, row_start, njobs, rows, st);
-    case 2: return launch_classify_batch_t<int32_t>(jobs, row_start, njobs, rows, st);
-    case 3: return launch_classify_batch_t<int64_t>(jobs, row_start, njobs, rows, st);
-    case 4: return launch_classify_batch_t<uint8_t>(jobs, row_start, njobs, rows, st);
-    case 5: return launch_classify_batch_t<uint16_t>(jobs, row_start, njobs, rows, st);
-    case 6: return launch_classify_batch_t<uint32_t>(jobs, row_start, njobs, rows, st);
-    case 7: return launch_classify_batch_t<uint64_t>(jobs, row_start, njobs, rows, st);
-    case 8: return launch_classify_batch_t<float>(jobs, row_start, njobs, rows, st);
-    default: return launch_classify_batch_t<double>(jobs, row_start, njobs, rows, st);
+    case 0: return launch_classify_batch_t<int8_t>(jobs, row_start, njobs, rows, max_levels, st);
+    case 1: return launch_classify_batch_t<int16_t>(jobs, row_start, njobs, rows, max_levels, st);
+    case 2: return launch_classify_batch_t<int32_t>(jobs, row_start, njobs, rows, max_levels, st);
+    case 3: return launch_classify_batch_t<int64_t>(jobs, row_start, njobs, rows, max_levels, st);
+    case 4: return launch_classify_batch_t<uint8_t>(jobs, row_start, njobs, rows, max_levels, st);
+    case 5: return launch_classify_batch_t<uint16_t>(jobs, row_start, njobs, rows, max_levels, st);
+    case 6: return launch_classify_batch_t<uint32_t>(jobs, row_start, njobs, rows, max_levels, st);
+    case 7: return launch_classify_batch_t<uint64_t>(jobs, row_start, njobs, rows, max_levels, st);
+    case 8: return launch_classify_batch_t<float>(jobs, row_start, njobs, rows, max_levels, st);
+    default: return launch_classify_batch_t<double>(jobs, row_start, njobs, rows, max_levels, st);
     }
 }
 
