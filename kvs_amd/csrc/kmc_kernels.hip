@@ -1055,6 +1055,78 @@ __global__ __launch_bounds__(kBlock) void k_classify(JobArgs a)
     for (uint64_t row = wave_id; row < nrows; row += nwaves) classify_row<T, NLEV>(a, row);
 }
 
+/* Node q of a dword of 8- or 16-bit values (little-endian: the first node
+ * in the low bits). */
+template <typename T>
+__device__ __forceinline__ T dword_node(uint32_t d, uint32_t q)
+{
+    if constexpr (sizeof(T) == 1) return (T)(uint8_t)(d >> (8u * q));
+    else return (T)(uint16_t)(d >> (16u * q));
+}
+
+/* Classification of an 8- or 16-bit volume whose rows are whole dwords (nx
+ * a multiple of P = 4 / sizeof(T), the values 4-byte aligned).  A lane loads
+ * one dword, P nodes, so that a wave load moves 256 bytes as a float row's
+ * does (one node a lane would move 64 or 128); the lane's P comparisons are
+ * bits (lane mod LPC) * P + q of its chunk's mask word, which an OR over the
+ * chunk's LPC = 64 / P lanes assembles.  Work items are (row, segment of 64
+ * dwords), 16 consecutive ones per wave iteration, their loads in flight
+ * together; the wave's item indices are scalars. */
+template <typename T, int NLEV>
+__global__ __launch_bounds__(kBlock) void k_classify_packed(JobArgs a)
+{
+    static_assert(sizeof(T) <= 2, "8- and 16-bit values");
+    constexpr uint32_t P = 4 / sizeof(T), LPC = 64 / P;
+    constexpr uint32_t kBatch = 16;
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t nd = a.nx / P;
+    const uint32_t nseg = (nd + 63) / 64;
+    const uint64_t items = (uint64_t)a.ny * a.nz * nseg;
+    const uint64_t level_words = (uint64_t)a.nz * a.ny * a.NC;
+    const uint64_t wave_id = (uint64_t)blockIdx.x * kWaves + uniform(threadIdx.x >> 6);
+    const uint64_t nwaves = (uint64_t)gridDim.x * kWaves;
+    const uint32_t* vals = (const uint32_t*)a.values;
+    for (uint64_t i0 = wave_id * kBatch; i0 < items; i0 += nwaves * kBatch) {
+        uint64_t row = i0 / nseg;
+        uint32_t seg = (uint32_t)(i0 - row * nseg);
+        uint32_t d[kBatch];
+#pragma unroll
+        for (uint32_t i = 0; i < kBatch; ++i) {
+            const uint32_t k = seg * 64u + lane;
+            d[i] = (i0 + i < items && k < nd) ? __builtin_nontemporal_load(vals + row * nd + k) : 0u;   // read once
+            if (++seg == nseg) { seg = 0; ++row; }
+        }
+        row = i0 / nseg;
+        seg = (uint32_t)(i0 - row * nseg);
+#pragma unroll
+        for (uint32_t i = 0; i < kBatch; ++i) {
+            if (i0 + i < items) {
+                const bool valid = seg * 64u + lane < nd;
+                const uint32_t c = seg * P + lane / LPC;
+                const uint32_t sh = (lane % LPC) * P;
+#pragma unroll
+                for (int l = 0; l < NLEV; ++l) {
+                    if (NLEV == 8 && l >= (int)a.nlev) break;   // the 8-level instance serves 5..7 levels
+                    uint32_t bits = 0;
+#pragma unroll
+                    for (uint32_t q = 0; q < P; ++q) bits |= (valid && above<T>(dword_node<T>(d[i], q), a, l) ? 1u : 0u) << q;
+                    /* OR over the chunk's lanes: each contributes its P bits */
+                    uint32_t lo = sh < 32u ? bits << sh : 0u, hi = sh >= 32u ? bits << (sh - 32u) : 0u;
+#pragma unroll
+                    for (uint32_t m = 1; m < LPC; m <<= 1) {
+                        lo |= (uint32_t)__shfl_xor((int)lo, (int)m);
+                        hi |= (uint32_t)__shfl_xor((int)hi, (int)m);
+                    }
+                    if (lane % LPC == 0 && c < a.NC)
+                        __builtin_nontemporal_store((uint64_t)hi << 32 | lo,
+                                                    &a.bitvol[(uint64_t)l * level_words + row * a.NC + c]);
+                }
+            }
+            if (++seg == nseg) { seg = 0; ++row; }
+        }
+    }
+}
+
 /* Index j with start[j] <= g < start[j+1] (start has n+1 ascending entries). */
 __device__ __forceinline__ uint32_t find_job(const uint64_t* start, uint32_t n, uint64_t g)
 {
@@ -2922,8 +2994,30 @@ hipError_t launch_emit_t(const JobArgs& a, uint32_t lds_bytes, hipStream_t st)
 }
 
 template <typename T>
+hipError_t launch_classify_packed_t(const JobArgs& a, hipStream_t st)
+{
+    constexpr uint32_t P = 4 / sizeof(T);
+    const uint64_t items = (uint64_t)a.ny * a.nz * ((a.nx / P + 63) / 64);
+    uint64_t blocks = (items + 16 * kWaves - 1) / (16 * kWaves);
+    if (blocks > (1u << 20)) blocks = 1u << 20;
+    const dim3 g((uint32_t)blocks), b(kBlock);
+    switch (a.nlev) {
+    case 1: hipLaunchKernelGGL((k_classify_packed<T, 1>), g, b, 0, st, a); break;
+    case 2: hipLaunchKernelGGL((k_classify_packed<T, 2>), g, b, 0, st, a); break;
+    case 3: hipLaunchKernelGGL((k_classify_packed<T, 3>), g, b, 0, st, a); break;
+    case 4: hipLaunchKernelGGL((k_classify_packed<T, 4>), g, b, 0, st, a); break;
+    default: hipLaunchKernelGGL((k_classify_packed<T, 8>), g, b, 0, st, a); break;
+    }
+    return hipGetLastError();
+}
+
+template <typename T>
 hipError_t launch_classify_t(const JobArgs& a, hipStream_t st)
 {
+    if constexpr (sizeof(T) <= 2) {
+        /* rows of whole dwords: one dword of nodes per lane */
+        if (a.nx % (4 / sizeof(T)) == 0 && ((uintptr_t)a.values & 3u) == 0) return launch_classify_packed_t<T>(a, st);
+    }
     /* one node row per wave: consecutive workgroups sweep the volume in
      * address order (measured faster than waves that take several rows) */
     const uint64_t rows = (uint64_t)a.ny * a.nz;
