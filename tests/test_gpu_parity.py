@@ -1,5 +1,7 @@
 """HIP path vs the oracle, bit for bit, through the C ABI (kmc_extract /
 kmc_extract_levels).  Seeded inputs at sizes the oracle finishes in seconds."""
+import ctypes
+
 import numpy as np
 import pytest
 
@@ -203,3 +205,43 @@ def test_tile_team_emit(monkeypatch):
         got = kmc.extract_levels(v, levels, normal, dup)
         for lv, g in zip(levels, got):
             assert_same_surface(g, oracle.extract(v, lv, normal, dup), f"team levels {lv} {normal} {dup}")
+
+
+@pytest.mark.parametrize("dtype", [np.int8, np.uint8, np.int16, np.uint16], ids=lambda d: np.dtype(d).name)
+def test_small_types_rows_of_whole_dwords_and_not(dtype):
+    """8- and 16-bit volumes classify one dword of nodes per lane when their
+    rows are whole dwords, and node by node otherwise: row lengths on both
+    sides of that rule and around the 64-dword segments and 64-node chunks."""
+    rng = np.random.default_rng(300 + np.dtype(dtype).itemsize * 2 + int(np.dtype(dtype).kind == "u"))
+    for nx in (2, 3, 4, 6, 60, 62, 64, 66, 68, 124, 126, 128, 130, 252, 256, 260, 510, 512, 516, 1030):
+        v, iso = random_volume(rng, dtype, (5, 7, nx))
+        for normal, dup in MODES:
+            check(v, iso, normal, dup)
+
+
+def test_small_types_values_off_dword_alignment():
+    """Device values one element past a 4-byte boundary (a view into a
+    larger array) classify node by node: the same surface as the oracle's."""
+    lib = kmc.load()
+    rng = np.random.default_rng(57)
+    for dtype in (np.uint8, np.int16):
+        v, iso = random_volume(rng, dtype, (6, 9, 64))
+        flat = np.concatenate([np.zeros(1, dtype), v.ravel()])
+        d = kmc.DeviceArray(flat)
+        try:
+            vol = kmc.make_volume(d.ptr.value + np.dtype(dtype).itemsize, dims=(64, 9, 6), memory=kmc.MEMORY_DEVICE,
+                                  dtype=dtype)
+            for normal, dup in MODES:
+                label = f"{np.dtype(dtype).name} off by one element, normal={normal} dup={dup}"
+                print("case", label, flush=True)
+                s = kmc.Surface()
+                st = lib.kmc_extract(ctypes.byref(vol), float(iso), int(normal), int(dup), kmc.MEMORY_HOST, None,
+                                     ctypes.byref(s))
+                assert st == kmc.OK, st
+                try:
+                    got = kmc.surface_to_numpy(s)
+                finally:
+                    lib.kmc_surface_release(ctypes.byref(s))
+                assert_same_surface(got, oracle.extract(v, iso, normal, dup), label)
+        finally:
+            d.free()
