@@ -1856,8 +1856,7 @@ __device__ __forceinline__ void stage_table_block(uint64_t* tri)
     __syncthreads();
 }
 
-/* Every triangle: with duplication its three vertices in table orientation and
- * its face normal (extract_surfaces_with_duplication, :155-243); otherwise its
+/* Every triangle of shared vertices (duplicated ones: dup_finish): its
  * connectivity (connect_isopoints, :439-521) and the face normal of the
  * shared vertices' coordinates -- the PolygonNormal output
  * (calculate_normals_on_polygon, :531-559) or, with VertexNormal, kept in
@@ -1865,7 +1864,6 @@ __device__ __forceinline__ void stage_table_block(uint64_t* tri)
 template <typename T, int M>
 __device__ __forceinline__ void triangle_item(const JobArgs& a, const uint64_t* tri, uint64_t g, uint64_t rec, uint32_t ci)
 {
-    const T* vol = (const T*)a.values;
     const uint64_t level_words = (uint64_t)a.nz * a.ny * a.NC;
     const uint32_t l = level_at(a.toff, a.nlev, g);
     const uint32_t x = (uint32_t)(rec & 0xFFFFFu), y = (uint32_t)((rec >> 20) & 0xFFFFFu);
@@ -1882,100 +1880,204 @@ __device__ __forceinline__ void triangle_item(const JobArgs& a, const uint64_t* 
     if (!coords) return;   // a level without outputs (not admitted)
     const uint64_t id = g - a.toff[l];
     float p[3][3], nn[3];
-    if constexpr (M == kModeDup) {
-        const double iso = iso_of(a, (int)l);
+    static_assert(M != kModeDup, "duplicated triangles: dup_finish");
+    const ChunkCross* cross_l = a.cross + (uint64_t)l * level_words;
+    /* the owner chunks' crossing records of all three edges are loaded
+     * before any is used, so the three loads are in flight together */
+    const uint64_t vbase = M == kModeVertex ? a.voff[l] : 0ull;
+    ChunkCross cc[3];
+    uint32_t bit[3], axs[3];
+#pragma unroll
+    for (int jj = 0; jj < 3; ++jj) {
+        const uint32_t o = edge_owner(eds[jj]);
+        const uint32_t vx = x + (o & 1u), vy = y + ((o >> 1) & 1u), vz = zc + ((o >> 2) & 1u);
+        bit[jj] = vx & 63u;
+        axs[jj] = o >> 3;
+        cc[jj] = cross_l[((uint64_t)vz * a.ny + vy) * a.NC + (vx >> 6)];
+    }
+    uint64_t vid[3];
+#pragma unroll
+    for (int jj = 0; jj < 3; ++jj) vid[jj] = cross_rank(cc[jj], bit[jj], axs[jj]);
+    /* the vertices' coordinates are loaded before this item's stores
+     * (vector memory operations complete in issue order) */
+#pragma unroll
+    for (int jj = 0; jj < 3; ++jj) {
+        p[jj][0] = coords[3 * vid[jj] + 0];
+        p[jj][1] = coords[3 * vid[jj] + 1];
+        p[jj][2] = coords[3 * vid[jj] + 2];
+    }
+    conn[3 * id + 0] = (uint32_t)vid[0];
+    conn[3 * id + 1] = (uint32_t)vid[1];
+    conn[3 * id + 2] = (uint32_t)vid[2];
+    if constexpr (M == kModeVertex) {
+        /* the first triangle of this cell that uses an edge tells the edge's
+         * vertex where the cell's records start; the cell sits at position q
+         * around the edge (around_cell) */
+        uint32_t* slots = a.vslot + 4 * vbase;
 #pragma unroll
         for (int jj = 0; jj < 3; ++jj) {
-            const uint32_t qq = edge_ends(eds[jj]);
-            interpolate<T>(a, vol, iso, x + (qq & 1u), y + ((qq >> 1) & 1u), zc + ((qq >> 2) & 1u),
-                           x + ((qq >> 4) & 1u), y + ((qq >> 5) & 1u), zc + ((qq >> 6) & 1u), p[jj]);
-        }
-        face_normal(p[0], p[1], p[2], nn);
-        float* co = coords + 9 * id;
-#pragma unroll
-        for (int jj = 0; jj < 3; ++jj) {
-            co[3 * jj + 0] = p[jj][0];
-            co[3 * jj + 1] = p[jj][1];
-            co[3 * jj + 2] = p[jj][2];
-        }
-        normals[3 * id + 0] = nn[0];
-        normals[3 * id + 1] = nn[1];
-        normals[3 * id + 2] = nn[2];
-    } else {
-        const ChunkCross* cross_l = a.cross + (uint64_t)l * level_words;
-        /* the owner chunks' crossing records of all three edges are loaded
-         * before any is used, so the three loads are in flight together */
-        const uint64_t vbase = M == kModeVertex ? a.voff[l] : 0ull;
-        ChunkCross cc[3];
-        uint32_t bit[3], axs[3];
-#pragma unroll
-        for (int jj = 0; jj < 3; ++jj) {
-            const uint32_t o = edge_owner(eds[jj]);
-            const uint32_t vx = x + (o & 1u), vy = y + ((o >> 1) & 1u), vz = zc + ((o >> 2) & 1u);
-            bit[jj] = vx & 63u;
-            axs[jj] = o >> 3;
-            cc[jj] = cross_l[((uint64_t)vz * a.ny + vy) * a.NC + (vx >> 6)];
-        }
-        uint64_t vid[3];
-#pragma unroll
-        for (int jj = 0; jj < 3; ++jj) vid[jj] = cross_rank(cc[jj], bit[jj], axs[jj]);
-        /* the vertices' coordinates are loaded before this item's stores
-         * (vector memory operations complete in issue order) */
-#pragma unroll
-        for (int jj = 0; jj < 3; ++jj) {
-            p[jj][0] = coords[3 * vid[jj] + 0];
-            p[jj][1] = coords[3 * vid[jj] + 1];
-            p[jj][2] = coords[3 * vid[jj] + 2];
-        }
-        conn[3 * id + 0] = (uint32_t)vid[0];
-        conn[3 * id + 1] = (uint32_t)vid[1];
-        conn[3 * id + 2] = (uint32_t)vid[2];
-        if constexpr (M == kModeVertex) {
-            /* the first triangle of this cell that uses an edge tells the edge's
-             * vertex where the cell's records start; the cell sits at position q
-             * around the edge (around_cell) */
-            uint32_t* slots = a.vslot + 4 * vbase;
-#pragma unroll
-            for (int jj = 0; jj < 3; ++jj) {
-                bool first = true;
-                for (uint32_t u = 0; u < t; ++u) {
-                    const uint32_t f0 = (uint32_t)((w >> (12 * u)) & 0xF), f1 = (uint32_t)((w >> (12 * u + 4)) & 0xF);
-                    const uint32_t f2 = (uint32_t)((w >> (12 * u + 8)) & 0xF);
-                    first = first && f0 != eds[jj] && f1 != eds[jj] && f2 != eds[jj];
-                }
-                if (!first) continue;
-                const uint32_t o = edge_owner(eds[jj]);
-                const uint32_t dx = o & 1u, dy = (o >> 1) & 1u, dz = (o >> 2) & 1u, ax = o >> 3;
-                const uint32_t q = ax == 0 ? ((dy ? 0u : 1u) | (dz ? 0u : 2u))
-                                 : ax == 1 ? ((dx ? 0u : 1u) | (dz ? 0u : 2u))
-                                           : ((dx ? 0u : 1u) | (dy ? 0u : 2u));
-                slots[4 * vid[jj] + q] = (uint32_t)(g - t);
+            bool first = true;
+            for (uint32_t u = 0; u < t; ++u) {
+                const uint32_t f0 = (uint32_t)((w >> (12 * u)) & 0xF), f1 = (uint32_t)((w >> (12 * u + 4)) & 0xF);
+                const uint32_t f2 = (uint32_t)((w >> (12 * u + 8)) & 0xF);
+                first = first && f0 != eds[jj] && f1 != eds[jj] && f2 != eds[jj];
             }
+            if (!first) continue;
+            const uint32_t o = edge_owner(eds[jj]);
+            const uint32_t dx = o & 1u, dy = (o >> 1) & 1u, dz = (o >> 2) & 1u, ax = o >> 3;
+            const uint32_t q = ax == 0 ? ((dy ? 0u : 1u) | (dz ? 0u : 2u))
+                             : ax == 1 ? ((dx ? 0u : 1u) | (dz ? 0u : 2u))
+                                       : ((dx ? 0u : 1u) | (dy ? 0u : 2u));
+            slots[4 * vid[jj] + q] = (uint32_t)(g - t);
         }
-        face_normal(p[0], p[1], p[2], nn);
-        float* dst = M == kModeVertex ? a.fn + 3 * g : normals + 3 * id;
-        dst[0] = nn[0];
-        dst[1] = nn[1];
-        dst[2] = nn[2];
+    }
+    face_normal(p[0], p[1], p[2], nn);
+    float* dst = M == kModeVertex ? a.fn + 3 * g : normals + 3 * id;
+    dst[0] = nn[0];
+    dst[1] = nn[1];
+    dst[2] = nn[2];
+}
+
+/* A duplicated triangle in flight: its record and cube index, and the node
+ * values at both ends of its three edges. */
+template <typename T>
+struct DupItem {
+    uint64_t rec;
+    uint32_t ci;
+    T va[3], vb[3];
+};
+
+/* The three edges of triangle t of cube index ci, in table order. */
+__device__ __forceinline__ void dup_edges(const uint64_t* tri, uint64_t rec, uint32_t ci, uint32_t eds[3])
+{
+    const uint32_t t = (uint32_t)(rec >> 60);
+    const uint64_t w = tri[ci];
+    eds[0] = (uint32_t)((w >> (12 * t)) & 0xF);
+    eds[1] = (uint32_t)((w >> (12 * t + 8)) & 0xF);
+    eds[2] = (uint32_t)((w >> (12 * t + 4)) & 0xF);
+}
+
+/* Issues the node-value loads of item d (its record and cube index set). */
+template <typename T>
+__device__ __forceinline__ void dup_load(const JobArgs& a, const uint64_t* tri, DupItem<T>& d)
+{
+    const T* vol = (const T*)a.values;
+    const uint64_t line = a.nx, slice = (uint64_t)a.nx * a.ny;
+    const uint64_t x = d.rec & 0xFFFFFu, y = (d.rec >> 20) & 0xFFFFFu, zc = (d.rec >> 40) & 0xFFFFFu;
+    uint32_t eds[3];
+    dup_edges(tri, d.rec, d.ci, eds);
+#pragma unroll
+    for (int jj = 0; jj < 3; ++jj) {
+        const uint32_t qq = edge_ends(eds[jj]);
+        const uint64_t ia = (x + (qq & 1u)) + (y + ((qq >> 1) & 1u)) * line + (zc + ((qq >> 2) & 1u)) * slice;
+        const uint64_t ib = (x + ((qq >> 4) & 1u)) + (y + ((qq >> 5) & 1u)) * line + (zc + ((qq >> 6) & 1u)) * slice;
+        d.va[jj] = vol[ia];
+        d.vb[jj] = vol[ib];
     }
 }
 
-/* Triangles g, g + step, ... < n; the next item's record is loaded while
- * this one is worked on. */
+/* Duplicated triangle g from its loaded item: its three vertices in table
+ * orientation and its face normal (extract_surfaces_with_duplication,
+ * :155-243), as triangle_item computes them. */
+template <typename T>
+__device__ __forceinline__ void dup_finish(const JobArgs& a, const uint64_t* tri, uint64_t g, const DupItem<T>& d)
+{
+    const uint32_t l = level_at(a.toff, a.nlev, g);
+    float* coords = nullptr;
+    float* normals = nullptr;
+#pragma unroll
+    for (int k = 0; k < kMaxLevels; ++k)
+        if (k == (int)l) { coords = a.coords[k]; normals = a.normals[k]; }
+    if (!coords) return;   // a level without outputs (not admitted)
+    const uint64_t id = g - a.toff[l];
+    const double iso = iso_of(a, (int)l);
+    const uint32_t x = (uint32_t)(d.rec & 0xFFFFFu), y = (uint32_t)((d.rec >> 20) & 0xFFFFFu);
+    const uint32_t zc = (uint32_t)((d.rec >> 40) & 0xFFFFFu);
+    uint32_t eds[3];
+    dup_edges(tri, d.rec, d.ci, eds);
+    float p[3][3], nn[3];
+#pragma unroll
+    for (int jj = 0; jj < 3; ++jj) {
+        const uint32_t qq = edge_ends(eds[jj]);
+        interpolate_values<T>(a, iso, d.va[jj], d.vb[jj], x + (qq & 1u), y + ((qq >> 1) & 1u), zc + ((qq >> 2) & 1u),
+                              x + ((qq >> 4) & 1u), y + ((qq >> 5) & 1u), zc + ((qq >> 6) & 1u), p[jj]);
+    }
+    face_normal(p[0], p[1], p[2], nn);
+    float* co = coords + 9 * id;
+#pragma unroll
+    for (int jj = 0; jj < 3; ++jj) {
+        co[3 * jj + 0] = p[jj][0];
+        co[3 * jj + 1] = p[jj][1];
+        co[3 * jj + 2] = p[jj][2];
+    }
+    normals[3 * id + 0] = nn[0];
+    normals[3 * id + 1] = nn[1];
+    normals[3 * id + 2] = nn[2];
+}
+
+/* Duplicated triangles g, g + step, ... < n, software-pipelined over two
+ * register sets that take turns (a copy of a loaded value would wait for its
+ * load): while item g is interpolated and stored, the node values of the
+ * next item and the record of the one after it are in flight (vector memory
+ * operations complete in issue order, so the wait for item g's values leaves
+ * the newer loads outstanding); indices past the range's end are clamped to
+ * its last item, so that every iteration issues the same loads. */
+template <typename T>
+__device__ __forceinline__ void triangles_range_dup(const JobArgs& a, const uint64_t* tri, uint64_t g, uint64_t n,
+                                                    uint64_t step)
+{
+    if (g >= n) return;
+    const uint64_t last = n - 1;
+    DupItem<T> d0, d1;
+    d0.rec = a.trec[g];
+    d0.ci = a.tci[g];
+    dup_load<T>(a, tri, d0);
+    uint64_t gn = g + step < last ? g + step : last;
+    uint64_t rn = a.trec[gn];
+    uint32_t cn = a.tci[gn];
+    for (;;) {
+        d1.rec = rn;
+        d1.ci = cn;
+        dup_load<T>(a, tri, d1);
+        gn = g + 2 * step < last ? g + 2 * step : last;
+        rn = a.trec[gn];
+        cn = a.tci[gn];
+        dup_finish<T>(a, tri, g, d0);
+        g += step;
+        if (g >= n) break;
+        d0.rec = rn;
+        d0.ci = cn;
+        dup_load<T>(a, tri, d0);
+        gn = g + 2 * step < last ? g + 2 * step : last;
+        rn = a.trec[gn];
+        cn = a.tci[gn];
+        dup_finish<T>(a, tri, g, d1);
+        g += step;
+        if (g >= n) break;
+    }
+}
+
+/* Triangles g, g + step, ... < n: duplicated ones through the two-stage
+ * pipeline above; with shared vertices the next item's record is loaded
+ * while this one is worked on. */
 template <typename T, int M>
 __device__ __forceinline__ void triangles_range(const JobArgs& a, const uint64_t* tri, uint64_t g, uint64_t n,
                                                 uint64_t step)
 {
-    uint64_t rec = g < n ? a.trec[g] : 0ull;
-    uint32_t ci = g < n ? a.tci[g] : 0u;
-    for (; g < n; g += step) {
-        const uint64_t cur = rec;
-        const uint32_t cur_ci = ci;
-        if (g + step < n) {
-            rec = a.trec[g + step];
-            ci = a.tci[g + step];
+    if constexpr (M == kModeDup) {
+        triangles_range_dup<T>(a, tri, g, n, step);
+    } else {
+        uint64_t rec = g < n ? a.trec[g] : 0ull;
+        uint32_t ci = g < n ? a.tci[g] : 0u;
+        for (; g < n; g += step) {
+            const uint64_t cur = rec;
+            const uint32_t cur_ci = ci;
+            if (g + step < n) {
+                rec = a.trec[g + step];
+                ci = a.tci[g + step];
+            }
+            triangle_item<T, M>(a, tri, g, cur, cur_ci);
         }
-        triangle_item<T, M>(a, tri, g, cur, cur_ci);
     }
 }
 
