@@ -11,6 +11,7 @@
 #include <cstring>
 #include <map>
 #include <mutex>
+#include <thread>
 #include <unordered_map>
 #include <vector>
 
@@ -154,6 +155,80 @@ void device_pool_trim()
     free_blocks(all);
 }
 
+/* Pinned host blocks for the surfaces returned in host memory (and host
+ * volumes read from files), cached like the device blocks: hipHostMalloc pins
+ * every page of a block and hipHostFree may wait for the device, each far
+ * slower than the copy into the block.  Freed blocks stay cached up to
+ * kHostKeep bytes (past that mark a free releases the largest ones), and all
+ * are released when a pinned allocation fails. */
+namespace {
+std::mutex g_host_mutex;
+std::multimap<size_t, void*> g_host_free;
+std::unordered_map<void*, size_t> g_host_size;
+size_t g_host_cached = 0;
+constexpr size_t kHostKeep = size_t(4) << 30;
+}
+
+void* host_block_alloc(size_t n)
+{
+    if (n == 0) n = 1;
+    const size_t sz = round_size(n);
+    {
+        std::lock_guard<std::mutex> lk(g_host_mutex);
+        auto it = g_host_free.lower_bound(sz);
+        if (it != g_host_free.end() && it->first <= sz * 2) {
+            void* p = it->second;
+            g_host_cached -= it->first;
+            g_host_free.erase(it);
+            return p;
+        }
+    }
+    void* p = nullptr;
+    if (hipHostMalloc(&p, sz, hipHostMallocDefault) != hipSuccess) {
+        (void)hipGetLastError();
+        std::vector<void*> all;
+        {
+            std::lock_guard<std::mutex> lk(g_host_mutex);
+            for (auto& kv : g_host_free) {
+                all.push_back(kv.second);
+                g_host_size.erase(kv.second);
+            }
+            g_host_free.clear();
+            g_host_cached = 0;
+        }
+        for (void* q : all) (void)hipHostFree(q);
+        p = nullptr;
+        if (hipHostMalloc(&p, sz, hipHostMallocDefault) != hipSuccess) {
+            (void)hipGetLastError();
+            return nullptr;
+        }
+    }
+    std::lock_guard<std::mutex> lk(g_host_mutex);
+    g_host_size[p] = sz;
+    return p;
+}
+
+void host_block_free(void* p)
+{
+    if (!p) return;
+    std::vector<void*> release;
+    {
+        std::lock_guard<std::mutex> lk(g_host_mutex);
+        auto it = g_host_size.find(p);
+        if (it == g_host_size.end()) return;
+        g_host_free.emplace(it->second, p);
+        g_host_cached += it->second;
+        while (g_host_cached > kHostKeep && !g_host_free.empty()) {
+            auto last = std::prev(g_host_free.end());
+            release.push_back(last->second);
+            g_host_cached -= last->first;
+            g_host_size.erase(last->second);
+            g_host_free.erase(last);
+        }
+    }
+    for (void* q : release) (void)hipHostFree(q);
+}
+
 /* Pinned host blocks for small staging buffers, by power-of-two size class;
  * reused for the life of the process (hipHostMalloc / hipHostFree are slow
  * and the latter may wait for the device). */
@@ -219,10 +294,10 @@ void free_storage(SurfaceStorage* s)
         device_free(s->normals);
         device_free(s->colors);
     } else {
-        if (s->coords) (void)hipHostFree(s->coords);
-        if (s->connections) (void)hipHostFree(s->connections);
-        if (s->normals) (void)hipHostFree(s->normals);
-        if (s->colors) (void)hipHostFree(s->colors);
+        host_block_free(s->coords);
+        host_block_free(s->connections);
+        host_block_free(s->normals);
+        host_block_free(s->colors);
     }
     delete s;
 }
@@ -291,6 +366,66 @@ kmc_status ensure_device()
 }
 
 /* Device copy of the volume values (or the caller's device pointer). */
+/* Pageable host memory to device memory.  The runtime stages a pageable
+ * copy itself at about half the pinned rate; large copies instead go through
+ * two pinned blocks in turn: a chunk is copied into one block by up to
+ * kUploadThreads threads while the other block's transfer runs, and a block
+ * is refilled only after its previous transfer has completed (its event). */
+constexpr size_t kUploadChunk = size_t(64) << 20;
+constexpr size_t kUploadPart = size_t(4) << 20;
+constexpr size_t kUploadThreads = 8;
+
+void copy_parallel(uint8_t* dst, const uint8_t* src, size_t n)
+{
+    const unsigned hw = std::thread::hardware_concurrency();
+    const size_t threads =
+        std::max<size_t>(1, std::min<size_t>({ kUploadThreads, hw ? (size_t)hw : 1, n / kUploadPart + 1 }));
+    const size_t part = (n + threads - 1) / threads;
+    auto work = [&](size_t t) {
+        const size_t b = t * part;
+        if (b < n) std::memcpy(dst + b, src + b, std::min(n, b + part) - b);
+    };
+    std::vector<std::thread> pool;
+    for (size_t t = 1; t < threads; ++t) pool.emplace_back(work, t);
+    work(0);
+    for (std::thread& th : pool) th.join();
+}
+
+hipError_t upload_host(void* dst, const void* src, size_t bytes, hipStream_t st)
+{
+    if (bytes < 2 * kUploadPart) return hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st);
+    /* at least four chunks where the volume allows, so that copies overlap */
+    const size_t chunk = std::min(kUploadChunk, std::max(2 * kUploadPart, (bytes / 4 + 4095) & ~size_t(4095)));
+    uint8_t* stage[2] = { (uint8_t*)host_pinned_alloc(kUploadChunk), (uint8_t*)host_pinned_alloc(kUploadChunk) };
+    hipEvent_t done[2] = { nullptr, nullptr };
+    bool used[2] = { false, false };
+    hipError_t e = hipSuccess;
+    if (!stage[0] || !stage[1] || hipEventCreateWithFlags(&done[0], hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&done[1], hipEventDisableTiming) != hipSuccess)
+        e = hipErrorOutOfMemory;
+    for (size_t off = 0, k = 0; e == hipSuccess && off < bytes; off += chunk, k ^= 1) {
+        const size_t n = std::min(chunk, bytes - off);
+        if (used[k] && (e = hipEventSynchronize(done[k])) != hipSuccess) break;
+        copy_parallel(stage[k], (const uint8_t*)src + off, n);
+        if ((e = hipMemcpyAsync((uint8_t*)dst + off, stage[k], n, hipMemcpyHostToDevice, st)) != hipSuccess ||
+            (e = hipEventRecord(done[k], st)) != hipSuccess)
+            break;
+        used[k] = true;
+    }
+    /* the blocks go back to the pool only when no transfer reads them */
+    for (int k = 0; k < 2; ++k)
+        if (used[k]) {
+            const hipError_t w = hipEventSynchronize(done[k]);
+            if (e == hipSuccess) e = w;
+        }
+    for (int k = 0; k < 2; ++k) {
+        if (done[k]) (void)hipEventDestroy(done[k]);
+        host_pinned_free(stage[k]);
+    }
+    if (e != hipSuccess) (void)hipGetLastError();
+    return e;
+}
+
 kmc_status device_values(const kmc_volume* v, hipStream_t st, const void** dvals, void** owned)
 {
     *owned = nullptr;
@@ -298,7 +433,7 @@ kmc_status device_values(const kmc_volume* v, hipStream_t st, const void** dvals
     const size_t bytes = (size_t)v->resolution[0] * v->resolution[1] * v->resolution[2] * type_size(v->type);
     void* p = device_alloc(bytes);
     if (!p) return KMC_ERR_OUT_OF_MEMORY;
-    if (hipMemcpyAsync(p, v->values, bytes, hipMemcpyHostToDevice, st) != hipSuccess) {
+    if (upload_host(p, v->values, bytes, st) != hipSuccess) {
         device_free(p);
         return KMC_ERR_DEVICE;
     }
@@ -663,10 +798,10 @@ kmc_status to_host(kmc_surface* s, hipStream_t st)
     const size_t bk = s->n_connections * sizeof(uint32_t);
     const size_t bl = s->colors ? s->n_colors * 3 : 0;
     bool ok = true;
-    if (bc) ok = ok && hipHostMalloc(&hs->coords, bc, hipHostMallocDefault) == hipSuccess;
-    if (bn) ok = ok && hipHostMalloc(&hs->normals, bn, hipHostMallocDefault) == hipSuccess;
-    if (bk) ok = ok && hipHostMalloc(&hs->connections, bk, hipHostMallocDefault) == hipSuccess;
-    if (bl) ok = ok && hipHostMalloc(&hs->colors, bl, hipHostMallocDefault) == hipSuccess;
+    if (bc) ok = ok && (hs->coords = host_block_alloc(bc)) != nullptr;
+    if (bn) ok = ok && (hs->normals = host_block_alloc(bn)) != nullptr;
+    if (bk) ok = ok && (hs->connections = host_block_alloc(bk)) != nullptr;
+    if (bl) ok = ok && (hs->colors = host_block_alloc(bl)) != nullptr;
     if (ok && bc) ok = hipMemcpyAsync(hs->coords, s->coords, bc, hipMemcpyDeviceToHost, st) == hipSuccess;
     if (ok && bn) ok = hipMemcpyAsync(hs->normals, s->normals, bn, hipMemcpyDeviceToHost, st) == hipSuccess;
     if (ok && bk) ok = hipMemcpyAsync(hs->connections, s->connections, bk, hipMemcpyDeviceToHost, st) == hipSuccess;

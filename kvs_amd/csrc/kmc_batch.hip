@@ -1016,7 +1016,7 @@ kmc_status kmc_batch_extract(const kmc_job* jobs, uint32_t n_jobs, int32_t out_m
             const size_t bc = v.n_vertices * 12, bn = v.n_normals * 12, bk = v.n_connections * 4;
             auto alloc = [&](void** p, size_t n) -> bool {
                 if (n == 0) return true;
-                if (out_memory == KMC_MEMORY_HOST) return hipHostMalloc(p, n, hipHostMallocDefault) == hipSuccess;
+                if (out_memory == KMC_MEMORY_HOST) return (*p = host_block_alloc(n)) != nullptr;
                 *p = device_alloc(n);
                 return *p != nullptr;
             };

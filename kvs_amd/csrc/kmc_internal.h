@@ -19,6 +19,8 @@ void device_pool_trim();
 /* Pooled pinned host memory for small staging buffers. */
 void* host_pinned_alloc(size_t bytes);
 void host_pinned_free(void* p);
+void* host_block_alloc(size_t bytes);
+void host_block_free(void* p);
 hipError_t read_back(void* host, const void* dev, size_t n, hipStream_t st);
 
 /* Arrays owned by a kmc_surface (its `internal` handle). */

@@ -420,11 +420,11 @@ kmc_status read_volume(const char* filename, int32_t memory, hipStream_t st, kmc
     const size_t bytes = n * type_size(type);
     void* dst = nullptr;
     if (memory == KMC_MEMORY_DEVICE) dst = device_alloc(bytes ? bytes : 1);
-    else if (hipHostMalloc(&dst, bytes ? bytes : 1, hipHostMallocDefault) != hipSuccess) dst = nullptr;
+    else dst = host_block_alloc(bytes ? bytes : 1);
     if (!dst) return KMC_ERR_OUT_OF_MEMORY;
     auto fail = [&](kmc_status e) {
         if (memory == KMC_MEMORY_DEVICE) device_free(dst);
-        else (void)hipHostFree(dst);
+        else host_block_free(dst);
         std::memset(out, 0, sizeof(*out));
         return e;
     };
@@ -568,11 +568,11 @@ kmc_status read_avs(const char* filename, int32_t memory, hipStream_t st, kmc_vo
     const size_t bytes = n * type_size(dtype);
     void* dst = nullptr;
     if (memory == KMC_MEMORY_DEVICE) dst = device_alloc(bytes ? bytes : 1);
-    else if (hipHostMalloc(&dst, bytes ? bytes : 1, hipHostMallocDefault) != hipSuccess) dst = nullptr;
+    else dst = host_block_alloc(bytes ? bytes : 1);
     if (!dst) return KMC_ERR_OUT_OF_MEMORY;
     auto fail = [&](kmc_status e) {
         if (memory == KMC_MEMORY_DEVICE) device_free(dst);
-        else (void)hipHostFree(dst);
+        else host_block_free(dst);
         std::memset(out, 0, sizeof(*out));
         return e;
     };
@@ -634,7 +634,7 @@ void kmc_kvsml_volume_release(kmc_volume* v)
 {
     if (!v || !v->values) return;
     if (v->memory == KMC_MEMORY_DEVICE) kmc::device_free((void*)v->values);
-    else (void)hipHostFree((void*)v->values);
+    else kmc::host_block_free((void*)v->values);
     v->values = nullptr;
 }
 

@@ -134,9 +134,9 @@ kmc_status surface_to_host(kmc_surface* s, hipStream_t st)
     h->coords = h->connections = h->normals = nullptr;
     const size_t bc = s->n_vertices * 12, bn = s->n_normals * 12, bk = s->n_connections * 4;
     bool ok = true;
-    if (bc) ok = ok && hipHostMalloc(&h->coords, bc, hipHostMallocDefault) == hipSuccess;
-    if (bn) ok = ok && hipHostMalloc(&h->normals, bn, hipHostMallocDefault) == hipSuccess;
-    if (bk) ok = ok && hipHostMalloc(&h->connections, bk, hipHostMallocDefault) == hipSuccess;
+    if (bc) ok = ok && (h->coords = host_block_alloc(bc)) != nullptr;
+    if (bn) ok = ok && (h->normals = host_block_alloc(bn)) != nullptr;
+    if (bk) ok = ok && (h->connections = host_block_alloc(bk)) != nullptr;
     if (ok && bc) ok = hipMemcpyAsync(h->coords, s->coords, bc, hipMemcpyDeviceToHost, st) == hipSuccess;
     if (ok && bn) ok = hipMemcpyAsync(h->normals, s->normals, bn, hipMemcpyDeviceToHost, st) == hipSuccess;
     if (ok && bk) ok = hipMemcpyAsync(h->connections, s->connections, bk, hipMemcpyDeviceToHost, st) == hipSuccess;
