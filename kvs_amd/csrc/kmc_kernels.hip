@@ -1072,8 +1072,11 @@ __device__ __forceinline__ T dword_node(uint32_t d, uint32_t q)
  * chunk's LPC = 64 / P lanes assembles.  Work items are (row, segment of 64
  * dwords), 16 consecutive ones per wave iteration, their loads in flight
  * together; the wave's item indices are scalars. */
+/* Items [i0, i1) of a job in the packed classification, item = row * nseg +
+ * segment: 16 consecutive ones at a time from i0 on, then every istep-th
+ * batch of 16 (i0, i1 and istep the same on all of the wave's lanes). */
 template <typename T, int NLEV>
-__global__ __launch_bounds__(kBlock) void k_classify_packed(JobArgs a)
+__device__ __forceinline__ void classify_packed_items(const JobArgs& a, uint64_t i0, uint64_t i1, uint64_t istep)
 {
     static_assert(sizeof(T) <= 2, "8- and 16-bit values");
     constexpr uint32_t P = 4 / sizeof(T), LPC = 64 / P;
@@ -1081,26 +1084,23 @@ __global__ __launch_bounds__(kBlock) void k_classify_packed(JobArgs a)
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t nd = a.nx / P;
     const uint32_t nseg = (nd + 63) / 64;
-    const uint64_t items = (uint64_t)a.ny * a.nz * nseg;
     const uint64_t level_words = (uint64_t)a.nz * a.ny * a.NC;
-    const uint64_t wave_id = (uint64_t)blockIdx.x * kWaves + uniform(threadIdx.x >> 6);
-    const uint64_t nwaves = (uint64_t)gridDim.x * kWaves;
     const uint32_t* vals = (const uint32_t*)a.values;
-    for (uint64_t i0 = wave_id * kBatch; i0 < items; i0 += nwaves * kBatch) {
+    for (; i0 < i1; i0 += istep) {
         uint64_t row = i0 / nseg;
         uint32_t seg = (uint32_t)(i0 - row * nseg);
         uint32_t d[kBatch];
 #pragma unroll
         for (uint32_t i = 0; i < kBatch; ++i) {
             const uint32_t k = seg * 64u + lane;
-            d[i] = (i0 + i < items && k < nd) ? __builtin_nontemporal_load(vals + row * nd + k) : 0u;   // read once
+            d[i] = (i0 + i < i1 && k < nd) ? __builtin_nontemporal_load(vals + row * nd + k) : 0u;   // read once
             if (++seg == nseg) { seg = 0; ++row; }
         }
         row = i0 / nseg;
         seg = (uint32_t)(i0 - row * nseg);
 #pragma unroll
         for (uint32_t i = 0; i < kBatch; ++i) {
-            if (i0 + i < items) {
+            if (i0 + i < i1) {
                 const bool valid = seg * 64u + lane < nd;
                 const uint32_t c = seg * P + lane / LPC;
                 const uint32_t sh = (lane % LPC) * P;
@@ -1125,6 +1125,24 @@ __global__ __launch_bounds__(kBlock) void k_classify_packed(JobArgs a)
             if (++seg == nseg) { seg = 0; ++row; }
         }
     }
+}
+
+/* Whether job a's values load a dword of nodes per lane: 8- or 16-bit
+ * values, rows of whole dwords, 4-byte aligned. */
+template <typename T>
+__device__ __host__ __forceinline__ bool packed_rows(const JobArgs& a)
+{
+    return sizeof(T) <= 2 && a.nx % (4 / sizeof(T)) == 0 && ((uintptr_t)a.values & 3u) == 0;
+}
+
+template <typename T, int NLEV>
+__global__ __launch_bounds__(kBlock) void k_classify_packed(JobArgs a)
+{
+    const uint64_t nseg = (a.nx / (4 / sizeof(T)) + 63) / 64;
+    const uint64_t items = (uint64_t)a.ny * a.nz * nseg;
+    const uint64_t wave_id = (uint64_t)blockIdx.x * kWaves + uniform(threadIdx.x >> 6);
+    const uint64_t nwaves = (uint64_t)gridDim.x * kWaves;
+    classify_packed_items<T, NLEV>(a, wave_id * 16, items, nwaves * 16);
 }
 
 /* Index j with start[j] <= g < start[j+1] (start has n+1 ascending entries). */
@@ -1159,6 +1177,24 @@ __global__ __launch_bounds__(kBlock) void k_classify_batch(const JobArgs* jobs, 
         while (g >= row_start[j + 1]) ++j;
         const JobArgs& a = jobs[j];
         const uint64_t je = row_start[j + 1] < end ? row_start[j + 1] : end;
+        if constexpr (sizeof(T) <= 2) {
+            if (packed_rows<T>(a)) {
+                /* the wave's rows as (row, segment) items, 16 at a time */
+                const uint64_t nseg = (a.nx / (4 / sizeof(T)) + 63) / 64;
+                const uint64_t i0 = (g - row_start[j]) * nseg, i1 = (je - row_start[j]) * nseg;
+                switch (a.nlev) {
+                case 1: classify_packed_items<T, 1>(a, i0, i1, 16); break;
+                case 2: classify_packed_items<T, 2>(a, i0, i1, 16); break;
+                case 3: classify_packed_items<T, 3>(a, i0, i1, 16); break;
+                case 4: classify_packed_items<T, 4>(a, i0, i1, 16); break;
+                default:
+                    if constexpr (kWide) classify_packed_items<T, 8>(a, i0, i1, 16);
+                    break;
+                }
+                g = je;
+                continue;
+            }
+        }
         const uint64_t q0 = (g - row_start[j]) * a.NC, q1 = (je - row_start[j]) * a.NC;
         switch (a.nlev) {
         case 1: classify_chunks<T, 1>(a, q0, q1); break;
@@ -3118,7 +3154,7 @@ hipError_t launch_classify_t(const JobArgs& a, hipStream_t st)
 {
     if constexpr (sizeof(T) <= 2) {
         /* rows of whole dwords: one dword of nodes per lane */
-        if (a.nx % (4 / sizeof(T)) == 0 && ((uintptr_t)a.values & 3u) == 0) return launch_classify_packed_t<T>(a, st);
+        if (packed_rows<T>(a)) return launch_classify_packed_t<T>(a, st);
     }
     /* one node row per wave: consecutive workgroups sweep the volume in
      * address order (measured faster than waves that take several rows) */

@@ -207,3 +207,23 @@ def test_batch_wide_and_short_rows():
     for j, ((v, levels, normal, dup), surfs) in enumerate(zip(specs, res)):
         for lv, got in zip(levels, surfs):
             assert_same_surface(got, oracle.extract(v, lv, normal, dup), f"wide job {j} level {lv}")
+
+
+def test_batch_small_types_rows_of_whole_dwords_against_oracle():
+    """8- and 16-bit jobs whose rows are whole dwords classify a dword of
+    nodes per lane inside the batch classification too (rows of one and of
+    several 64-dword segments, and rows that are not whole dwords beside
+    them); more than four levels take the wide instance."""
+    rng = np.random.default_rng(77)
+    specs = []
+    shapes = [(6, 9, 256), (5, 7, 260), (3, 4, 1024), (4, 6, 66), (5, 5, 63), (7, 3, 512)]
+    for k, shape in enumerate(shapes):
+        dtype = (np.uint8, np.int8, np.uint16, np.int16)[k % 4]
+        v = rng.integers(0, 90, shape).astype(dtype)
+        nlev = 6 if k == 2 else int(rng.integers(1, 4))
+        levels = [float(rng.integers(5, 80)) + 0.5 * float(rng.integers(0, 2)) for _ in range(nlev)]
+        specs.append((v, levels, *MODES[k % 3]))
+    res = kmc.batch_extract(specs)
+    for (v, levels, normal, dup), surfs in zip(specs, res):
+        for lv, got in zip(levels, surfs):
+            assert_same_surface(got, oracle.extract(v, lv, normal, dup), f"{v.dtype} {v.shape} {lv} {normal},{dup}")
