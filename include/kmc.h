@@ -322,7 +322,9 @@ kmc_status kmc_export_kvsml(const kmc_surface* surface, const uint8_t* color, in
 kmc_status kmc_export_ply(const kmc_surface* surface, const uint8_t* color, const char* filename, int32_t binary,
                           void* stream);
 
-/* Release the arrays of a surface returned by any kmc_* extraction call. */
+/* Release the arrays of a surface returned by any kmc_* extraction call.
+ * Device arrays go back to the library's device block cache, host arrays
+ * (pinned memory) to its pinned block cache (at most 4 GiB kept). */
 void kmc_surface_release(kmc_surface* surface);
 
 /*
