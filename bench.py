@@ -56,6 +56,8 @@ def main():
     ap.add_argument("--serve-volumes", type=int, default=32, help="resident volumes of the serving leg (0: skip it)")
     ap.add_argument("--serve-requests", type=int, default=300)
     ap.add_argument("--serve-workers", type=int, default=0, help="server worker threads (0: the library's default)")
+    ap.add_argument("--serve-max-batch", type=int, default=0,
+                    help="most requests one server batch takes (0: the library's default)")
     ap.add_argument("--no-ct", action="store_true", help="skip the 512^3 uint8 CT-like leg")
     ap.add_argument("--no-rows", action="store_true",
                     help="skip the SlicePlane / unstructured / KVSML / STL leg")
@@ -439,7 +441,8 @@ def serving_leg(lib, kmc, datasets, torch, args):
         vols.append(kmc.make_volume(t.data_ptr(), dims=(edge, edge, edge), memory=kmc.MEMORY_DEVICE, dtype=np.dtype(kind)))
         nbytes += t.numel() * t.element_size()
     torch.cuda.synchronize()
-    server = kmc.Server(budget_bytes=64 << 30, workers=args.serve_workers, out_memory=kmc.MEMORY_DEVICE)
+    server = kmc.Server(budget_bytes=64 << 30, workers=args.serve_workers, max_batch=args.serve_max_batch,
+                        out_memory=kmc.MEMORY_DEVICE)
     vids = [server.register(v) for v in vols]
     # min/max as registered (the plain calls below use them too)
     for v in vols:
