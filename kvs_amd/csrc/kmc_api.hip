@@ -270,6 +270,23 @@ void host_pinned_free(void* p)
  * through a pinned staging block, then a wait for the stream: a copy into
  * pageable memory is staged by the runtime at a higher latency, and these
  * copies sit between a pass's front and its emit. */
+hipError_t read_back_parts(const ReadPart* parts, int n, hipStream_t st)
+{
+    size_t total = 0;
+    for (int i = 0; i < n; ++i) total += (parts[i].bytes + 15) & ~size_t(15);
+    uint8_t* pin = (uint8_t*)host_pinned_alloc(total);
+    if (!pin) return hipErrorOutOfMemory;
+    hipError_t e = hipSuccess;
+    for (int i = 0, off = 0; i < n && e == hipSuccess; off += (int)((parts[i].bytes + 15) & ~size_t(15)), ++i)
+        e = hipMemcpyAsync(pin + off, parts[i].dev, parts[i].bytes, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e == hipSuccess)
+        for (int i = 0, off = 0; i < n; off += (int)((parts[i].bytes + 15) & ~size_t(15)), ++i)
+            std::memcpy(parts[i].host, pin + off, parts[i].bytes);
+    host_pinned_free(pin);
+    return e;
+}
+
 hipError_t read_back(void* host, const void* dev, size_t n, hipStream_t st)
 {
     void* pin = host_pinned_alloc(n);

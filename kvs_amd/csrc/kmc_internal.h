@@ -22,6 +22,13 @@ void host_pinned_free(void* p);
 void* host_block_alloc(size_t bytes);
 void host_block_free(void* p);
 hipError_t read_back(void* host, const void* dev, size_t n, hipStream_t st);
+/* Several device ranges into host memory with one wait for the stream. */
+struct ReadPart {
+    void* host;
+    const void* dev;
+    size_t bytes;
+};
+hipError_t read_back_parts(const ReadPart* parts, int n, hipStream_t st);
 
 /* Arrays owned by a kmc_surface (its `internal` handle). */
 struct SurfaceStorage {

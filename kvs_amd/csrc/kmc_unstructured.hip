@@ -493,6 +493,22 @@ kmc_status run_cells(const kmc_unstructured_volume* v, bool slice, double iso, i
     float minc[3], maxc[3];
     for (int k = 0; k < 3; ++k) { minc[k] = v->min_coord[k]; maxc[k] = v->max_coord[k]; }
     const bool need_values = !v->has_min_max_values, need_coords = !v->has_min_max_coords && nconn > 0;
+    /* the count pass does not depend on the ranges, so it runs before their
+     * read-back and one read-back brings ranges and triangle count together
+     * (with an empty value range its count is discarded) */
+    const bool count_now = ncells > 0 && (slice || dup);
+    UArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.values = dvals;
+    a.coords = dcoords;
+    a.conn = dconn;
+    a.ncells = ncells;
+    a.iso = iso;
+    if (slice)
+        for (int k = 0; k < 4; ++k) a.plane[k] = coef[k];
+    ReadPart parts[2];
+    int nparts_back = 0;
+    uint8_t ranges[40];
     if (need_values || need_coords) {
         uint8_t* scratch = (uint8_t*)alloc(minmax_scratch_bytes() + 64 + kSpanParts * sizeof(CoordSpan));
         if (!scratch) return finish(KMC_ERR_OUT_OF_MEMORY);
@@ -507,23 +523,46 @@ kmc_status run_cells(const kmc_unstructured_volume* v, bool slice, double iso, i
             hipLaunchKernelGGL(k_coord_span_final, dim3(1), dim3(kBlock), 0, st, spans, dcc);
             ok = hipGetLastError() == hipSuccess;
         }
-        /* the min/max (16 bytes) and the coordinate span (24) lie together */
-        uint8_t back[40];
-        if (!ok || read_back(back, dmm, sizeof(back), st) != hipSuccess) {
+        if (!ok) {
             (void)hipGetLastError();
             return finish(KMC_ERR_DEVICE);
         }
-        if (need_values) {
-            double hmm[2];
-            std::memcpy(hmm, back, 16);
-            mn = hmm[0];
-            mx = hmm[1];
+        /* the min/max (16 bytes) and the coordinate span (24) lie together */
+        parts[nparts_back++] = ReadPart{ ranges, dmm, sizeof(ranges) };
+    }
+    const uint32_t nparts = count_now ? scan_partials_needed(ncells, 1) : 0;
+    uint64_t* prefix = nullptr;
+    uint64_t* aux = nullptr;
+    uint64_t nt = 0;
+    if (count_now) {
+        a.counts = (uint32_t*)alloc(ncells * 4);
+        a.tidx = (uint8_t*)alloc(ncells);
+        prefix = (uint64_t*)alloc(ncells * 8);
+        aux = (uint64_t*)alloc((nparts + 1) * 8);
+        if (!a.counts || !a.tidx || !prefix || !aux) return finish(KMC_ERR_OUT_OF_MEMORY);
+        a.prefix = prefix;
+        const hipError_t e1 = slice ? launch_cells<true>(v->type, kind, a, false, st)
+                                    : launch_cells<false>(v->type, kind, a, false, st);
+        if (e1 != hipSuccess || launch_scan(a.counts, ncells, 1, aux, prefix, aux + nparts, st) != hipSuccess) {
+            (void)hipGetLastError();
+            return finish(KMC_ERR_DEVICE);
         }
-        if (need_coords) {
-            float hcc[6];
-            std::memcpy(hcc, back + 16, 24);
-            for (int k = 0; k < 3; ++k) { minc[k] = hcc[k]; maxc[k] = hcc[3 + k]; }
-        }
+        parts[nparts_back++] = ReadPart{ &nt, aux + nparts, 8 };
+    }
+    if (nparts_back && read_back_parts(parts, nparts_back, st) != hipSuccess) {
+        (void)hipGetLastError();
+        return finish(KMC_ERR_DEVICE);
+    }
+    if (need_values) {
+        double hmm[2];
+        std::memcpy(hmm, ranges, 16);
+        mn = hmm[0];
+        mx = hmm[1];
+    }
+    if (need_coords) {
+        float hcc[6];
+        std::memcpy(hcc, ranges + 16, 24);
+        for (int k = 0; k < 3; ++k) { minc[k] = hcc[k]; maxc[k] = hcc[3 + k]; }
     }
 
     /* the surface as the mapper leaves it */
@@ -548,17 +587,9 @@ kmc_status run_cells(const kmc_unstructured_volume* v, bool slice, double iso, i
     }
     if (ncells == 0) return finish(KMC_OK);
 
-    UArgs a;
-    std::memset(&a, 0, sizeof(a));
-    a.values = dvals;
-    a.coords = dcoords;
-    a.conn = dconn;
-    a.ncells = ncells;
-    a.iso = iso;
     a.minv = mn;
     a.maxv = mx;
     if (slice) {
-        for (int k = 0; k < 4; ++k) a.plane[k] = coef[k];
         a.cres = res;
         if (color_range) { a.crange[0] = color_range[0]; a.crange[1] = color_range[1]; }
         else if (v->type == KMC_INT8) { a.crange[0] = -128; a.crange[1] = 127; }
@@ -573,21 +604,6 @@ kmc_status run_cells(const kmc_unstructured_volume* v, bool slice, double iso, i
         if (!dt || hipMemcpyAsync(dt, table, 3 * (size_t)res, hipMemcpyHostToDevice, st) != hipSuccess)
             return finish(dt ? KMC_ERR_DEVICE : KMC_ERR_OUT_OF_MEMORY);
         a.ctable = dt;
-    }
-    const uint32_t nparts = scan_partials_needed(ncells, 1);
-    a.counts = (uint32_t*)alloc(ncells * 4);
-    a.tidx = (uint8_t*)alloc(ncells);
-    uint64_t* prefix = (uint64_t*)alloc(ncells * 8);
-    uint64_t* aux = (uint64_t*)alloc((nparts + 1) * 8);
-    if (!a.counts || !a.tidx || !prefix || !aux) return finish(KMC_ERR_OUT_OF_MEMORY);
-    a.prefix = prefix;
-    uint64_t nt = 0;
-    const hipError_t e1 = slice ? launch_cells<true>(v->type, kind, a, false, st)
-                                : launch_cells<false>(v->type, kind, a, false, st);
-    if (e1 != hipSuccess || launch_scan(a.counts, ncells, 1, aux, prefix, aux + nparts, st) != hipSuccess ||
-        read_back(&nt, aux + nparts, 8, st) != hipSuccess) {
-        (void)hipGetLastError();
-        return finish(KMC_ERR_DEVICE);
     }
     if (nt == 0) return finish(KMC_OK);
 
