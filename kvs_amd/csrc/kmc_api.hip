@@ -392,7 +392,7 @@ constexpr size_t kUploadChunk = size_t(64) << 20;
 constexpr size_t kUploadPart = size_t(4) << 20;
 constexpr size_t kUploadThreads = 8;
 
-void copy_parallel(uint8_t* dst, const uint8_t* src, size_t n)
+static void copy_parallel(uint8_t* dst, const uint8_t* src, size_t n)
 {
     const unsigned hw = std::thread::hardware_concurrency();
     const size_t threads =

@@ -92,7 +92,7 @@ kmc_status device_view(const kmc_surface* s, hipStream_t st, DeviceView& v)
         void* p = device_alloc(bytes);
         if (!p) return nullptr;
         v.owned.push_back(p);
-        return hipMemcpyAsync(p, src, bytes, hipMemcpyHostToDevice, st) == hipSuccess ? p : nullptr;
+        return upload_host(p, src, bytes, st) == hipSuccess ? p : nullptr;
     };
     const size_t bc = s->n_vertices * 12, bk = s->n_connections * 4, bn = s->n_normals * 12;
     v.coords = (const float*)up(s->coords, bc);

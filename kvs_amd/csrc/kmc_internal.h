@@ -22,6 +22,8 @@ void host_pinned_free(void* p);
 void* host_block_alloc(size_t bytes);
 void host_block_free(void* p);
 hipError_t read_back(void* host, const void* dev, size_t n, hipStream_t st);
+/* Pageable host memory to device memory (pinned staging for large copies). */
+hipError_t upload_host(void* dst, const void* src, size_t bytes, hipStream_t st);
 /* Several device ranges into host memory with one wait for the stream. */
 struct ReadPart {
     void* host;

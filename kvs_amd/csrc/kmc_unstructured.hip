@@ -479,9 +479,8 @@ kmc_status run_cells(const kmc_unstructured_volume* v, bool slice, double iso, i
         void* b = alloc(cb);
         void* c = alloc(kb);
         if (!a || !b || !c) return finish(KMC_ERR_OUT_OF_MEMORY);
-        if (hipMemcpyAsync(a, v->values, vb, hipMemcpyHostToDevice, st) != hipSuccess ||
-            hipMemcpyAsync(b, v->coords, cb, hipMemcpyHostToDevice, st) != hipSuccess ||
-            (kb && hipMemcpyAsync(c, v->connections, kb, hipMemcpyHostToDevice, st) != hipSuccess))
+        if (upload_host(a, v->values, vb, st) != hipSuccess || upload_host(b, v->coords, cb, st) != hipSuccess ||
+            (kb && upload_host(c, v->connections, kb, st) != hipSuccess))
             return finish(KMC_ERR_DEVICE);
         dvals = a;
         dcoords = (const float*)b;
