@@ -825,6 +825,8 @@ kmc_status to_host(kmc_surface* s, hipStream_t st)
     if (ok && bl) ok = hipMemcpyAsync(hs->colors, s->colors, bl, hipMemcpyDeviceToHost, st) == hipSuccess;
     if (ok) ok = hipStreamSynchronize(st) == hipSuccess;
     if (!ok) {
+        /* no copy still in flight may write into blocks that go back to the cache */
+        (void)hipStreamSynchronize(st);
         (void)hipGetLastError();
         free_storage(hs);
         return KMC_ERR_OUT_OF_MEMORY;

@@ -1033,8 +1033,12 @@ kmc_status kmc_batch_extract(const kmc_job* jobs, uint32_t n_jobs, int32_t out_m
             if (!ok) { (void)hipGetLastError(); s = KMC_ERR_OUT_OF_MEMORY; }
         }
         if (s == KMC_OK && hipStreamSynchronize(st) != hipSuccess) s = KMC_ERR_DEVICE;
-        if (s != KMC_OK)
+        if (s != KMC_OK) {
+            /* no copy still in flight may touch blocks that go back to the caches */
+            (void)hipStreamSynchronize(st);
+            (void)hipGetLastError();
             for (uint32_t i = 0; i < done; ++i) kmc_surface_release(&outs[i]);
+        }
         destroy(b);
         report(s);
         return s;

@@ -142,6 +142,8 @@ kmc_status surface_to_host(kmc_surface* s, hipStream_t st)
     if (ok && bk) ok = hipMemcpyAsync(h->connections, s->connections, bk, hipMemcpyDeviceToHost, st) == hipSuccess;
     if (ok) ok = hipStreamSynchronize(st) == hipSuccess;
     if (!ok) {
+        /* no copy still in flight may write into blocks that go back to the cache */
+        (void)hipStreamSynchronize(st);
         (void)hipGetLastError();
         free_storage(h);
         return KMC_ERR_OUT_OF_MEMORY;
