@@ -3197,6 +3197,14 @@ hipError_t launch_count(const JobArgs& a, hipStream_t st)
     return hipGetLastError();
 }
 
+/* The VertexNormal triangle kernel runs best at four waves per SIMD: more
+ * waves in flight evict the cache lines its gathers reuse (measured on the
+ * 1024^3 turbulence field: 0.609 ms at its register-bound five waves, 0.587
+ * at four, 0.617 at three).  A launch with this much dynamic LDS per
+ * 256-thread workgroup (the kernel uses none of it) leaves room for four
+ * workgroups in a CU's 160 KiB. */
+constexpr uint32_t kTriangleLdsPad = 38u << 10;
+
 /* Grid of a flat per-item kernel: up to 8 resident workgroups per CU. */
 static uint32_t flat_blocks(uint64_t items)
 {
@@ -3216,7 +3224,7 @@ hipError_t launch_flat_t(const JobArgs& a, hipStream_t st, hipEvent_t* marks)
         case kModePolygon:
             hipLaunchKernelGGL((k_triangles<T, kModePolygon>), dim3(flat_blocks(nt)), dim3(kBlock), 0, st, a);
             break;
-        default: hipLaunchKernelGGL((k_triangles<T, kModeVertex>), dim3(flat_blocks(nt)), dim3(kBlock), 0, st, a); break;
+        default: hipLaunchKernelGGL((k_triangles<T, kModeVertex>), dim3(flat_blocks(nt)), dim3(kBlock), kTriangleLdsPad, st, a); break;
         }
     }
     if (marks) (void)hipEventRecord(marks[2], st);
